@@ -1,0 +1,864 @@
+// fs_api.cpp -- the fs_* C ABI (include/fluidsim.h) over the gfx950 kernels.
+//
+// One handle owns one HIP stream and every device buffer of the step; nothing
+// is allocated per step.  Host work is limited to what the reference also does
+// on the host: scene/config/SDF parsing, the rigid-body integrator
+// (RigidBody::integrate utility.cuh:408-467) and file output.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/fluidsim.h"
+#include "fs_kernels.hpp"
+#include "fs_pressure.hpp"
+
+using fsd::f3;
+using fsd::mk;
+using namespace fsk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& m) {
+    g_err = m;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(FS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+struct HostBody {
+    bool active = false;
+    std::vector<float> sdf;
+    int sw = 0, sh = 0, sd = 0;
+    f3 file_origin{0, 0, 0}, sdf_origin{0, 0, 0}, local_com{0, 0, 0}, size{0, 0, 0};
+    float cs = 0;
+    f3 pos{0, 0, 0}, vel{0, 0, 0}, force{0, 0, 0}, omega{0, 0, 0}, torque{0, 0, 0};
+    float mass = 0, inertia = 0, inv_inertia = 0, theta = 0, psi = 0, phi = 0;
+    float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    f3 last_force{0, 0, 0}, last_torque{0, 0, 0};
+    float last_sub = 0;
+
+    // RigidBody::update_rotation_matrix utility.cuh:470-484
+    void update_rotation() {
+        double th = theta, ps = psi, ph = phi;
+        R[0] = (float)(std::cos(th) * std::cos(ps));
+        R[1] = (float)(std::sin(th) * std::sin(ph) - std::cos(th) * std::cos(ph) * std::sin(ps));
+        R[2] = (float)(std::cos(ph) * std::sin(th) + std::cos(th) * std::sin(ph) * std::sin(ps));
+        R[3] = (float)std::sin(ps);
+        R[4] = (float)(std::cos(ph) * std::cos(ps));
+        R[5] = (float)(-std::cos(ps) * std::sin(ph));
+        R[6] = (float)(-std::cos(ps) * std::sin(th));
+        R[7] = (float)(std::cos(th) * std::sin(ph) + std::cos(ph) * std::sin(th) * std::sin(ps));
+        R[8] = (float)(std::cos(th) * std::cos(ph) - std::sin(th) * std::sin(ph) * std::sin(ps));
+    }
+
+    // RigidBody::integrate utility.cuh:408-467 (semi-implicit Euler + Euler-angle rates)
+    void integrate(float dt) {
+        f3 accel = force / mass;
+        vel = vel + accel * dt;
+        pos = pos + vel * dt;
+        const float maxAngle = 3.14159265358979323846f;
+        theta = std::clamp(theta, -maxAngle, maxAngle);
+        update_rotation();
+        f3 tl = mk(R[0] * torque.x + R[3] * torque.y + R[6] * torque.z, R[1] * torque.x + R[4] * torque.y + R[7] * torque.z,
+                   R[2] * torque.x + R[5] * torque.y + R[8] * torque.z);
+        f3 aa = inv_inertia * tl;
+        omega = omega + aa * dt;
+        float dth = (float)(omega.y * std::cos((double)phi) - omega.z * std::sin((double)phi));
+        float dps = (float)((omega.y * std::sin((double)phi) + omega.z * std::cos((double)phi)) / std::cos((double)theta));
+        float dph = (float)(omega.x + dps * std::sin((double)theta));
+        theta += dth * dt;
+        psi += dps * dt;
+        phi += dph * dt;
+        f3 lo = file_origin;
+        f3 ro = mk(R[0] * lo.x + R[1] * lo.y + R[2] * lo.z, R[3] * lo.x + R[4] * lo.y + R[5] * lo.z,
+                   R[6] * lo.x + R[7] * lo.y + R[8] * lo.z);
+        sdf_origin = pos + ro;
+        force = mk(0, 0, 0);
+        torque = mk(0, 0, 0);
+    }
+
+    fsd::BodyParams params() const {
+        fsd::BodyParams b{};
+        b.active = active ? 1 : 0;
+        b.sw = sw; b.sh = sh; b.sd = sd;
+        b.cs = cs;
+        b.pos = pos; b.vel = vel; b.omega = omega;
+        b.file_origin = file_origin;
+        b.local_com = local_com;
+        for (int i = 0; i < 9; ++i) b.R[i] = R[i];
+        return b;
+    }
+};
+
+struct Solver {
+    fs_config cfg{};
+    int W = 0, H = 0, D = 0;
+    long long C = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[8] = {};
+    // grids
+    uint8_t *d_solid = nullptr, *d_labels = nullptr;
+    float *d_u = nullptr, *d_v = nullptr, *d_w = nullptr, *d_us = nullptr, *d_vs = nullptr, *d_ws = nullptr;
+    float *d_p = nullptr, *d_rhs = nullptr;
+    double* d_p64 = nullptr;
+    int *d_count = nullptr, *d_start = nullptr, *d_bsum = nullptr, *d_total = nullptr, *d_nfluid = nullptr;
+    // particles
+    long long np = 0;
+    ParticlesView PA{}, PB{};
+    int *d_key = nullptr, *d_rank = nullptr, *d_tmpkey = nullptr;
+    bool keys_valid = false;   // d_key/d_rank/d_count hold the counting pass of the current order
+    bool sorted_valid = false; // d_start matches the current particle order
+    // body
+    HostBody body;
+    float* d_sdf = nullptr;
+    double* d_bacc = nullptr;
+    MGPCG mg;
+    std::unique_ptr<Preconditioner> jacobi;
+    fs_solve_stats stats{};
+    double times[FS_NUM_STAGES] = {0};
+
+    long long NU() const { return (long long)(W + 1) * H * D; }
+    long long NV() const { return (long long)W * (H + 1) * D; }
+    long long NW() const { return (long long)W * H * (D + 1); }
+
+    ~Solver() {
+        free_grids();
+        free_particles();
+        dfree(d_sdf);
+        dfree(d_bacc);
+        for (auto& e : ev) if (e) hipEventDestroy(e);
+        if (st) hipStreamDestroy(st);
+    }
+    void free_grids() {
+        dfree(d_solid); dfree(d_labels); dfree(d_u); dfree(d_v); dfree(d_w); dfree(d_us); dfree(d_vs); dfree(d_ws);
+        dfree(d_p); dfree(d_rhs); dfree(d_p64); dfree(d_count); dfree(d_start); dfree(d_bsum); dfree(d_total);
+        dfree(d_nfluid);
+    }
+    static void free_view(ParticlesView& P) {
+        dfree(P.px); dfree(P.py); dfree(P.pz); dfree(P.vx); dfree(P.vy); dfree(P.vz); dfree(P.id);
+    }
+    void free_particles() {
+        free_view(PA);
+        free_view(PB);
+        dfree(d_key); dfree(d_rank); dfree(d_tmpkey);
+        np = 0;
+    }
+
+    int alloc_grids(int W_, int H_, int D_) {
+        free_grids();
+        W = W_; H = H_; D = D_;
+        C = (long long)W * H * D;
+        HIPCHK(hipMalloc(&d_solid, C));
+        HIPCHK(hipMalloc(&d_labels, C));
+        for (float** p : {&d_u, &d_us}) HIPCHK(hipMalloc(p, sizeof(float) * NU()));
+        for (float** p : {&d_v, &d_vs}) HIPCHK(hipMalloc(p, sizeof(float) * NV()));
+        for (float** p : {&d_w, &d_ws}) HIPCHK(hipMalloc(p, sizeof(float) * NW()));
+        HIPCHK(hipMalloc(&d_p, sizeof(float) * C));
+        HIPCHK(hipMalloc(&d_rhs, sizeof(float) * C));
+        HIPCHK(hipMalloc(&d_p64, sizeof(double) * C));
+        HIPCHK(hipMalloc(&d_count, sizeof(int) * (C + 4)));
+        HIPCHK(hipMalloc(&d_start, sizeof(int) * (C + 1)));
+        HIPCHK(hipMalloc(&d_bsum, sizeof(int) * (scan_blocks(C) + 1)));
+        HIPCHK(hipMalloc(&d_total, sizeof(int) * 2));
+        HIPCHK(hipMalloc(&d_nfluid, sizeof(int) * 2));
+        HIPCHK(hipMemset(d_count, 0, sizeof(int) * (C + 4)));
+        HIPCHK(hipMemset(d_start, 0, sizeof(int) * (C + 1)));
+        for (float* p : {d_u, d_us}) HIPCHK(hipMemset(p, 0, sizeof(float) * NU()));
+        for (float* p : {d_v, d_vs}) HIPCHK(hipMemset(p, 0, sizeof(float) * NV()));
+        for (float* p : {d_w, d_ws}) HIPCHK(hipMemset(p, 0, sizeof(float) * NW()));
+        HIPCHK(hipMemset(d_p, 0, sizeof(float) * C));
+        HIPCHK(hipMemset(d_rhs, 0, sizeof(float) * C));
+        HIPCHK(hipMemset(d_p64, 0, sizeof(double) * C));
+        if (mg.init(W, H, D) != 0) return fail(FS_ERR_DEVICE, "pressure solver allocation failed");
+        jacobi.reset(make_jacobi_preconditioner(&mg));
+        keys_valid = sorted_valid = false;
+        return FS_OK;
+    }
+
+    static int alloc_view(ParticlesView& P, long long n) {
+        size_t b = sizeof(float) * (size_t)std::max<long long>(n, 1);
+        for (float** p : {&P.px, &P.py, &P.pz, &P.vx, &P.vy, &P.vz})
+            if (hipMalloc(p, b) != hipSuccess) return fail(FS_ERR_DEVICE, "particle allocation failed");
+        if (hipMalloc(&P.id, sizeof(int) * (size_t)std::max<long long>(n, 1)) != hipSuccess)
+            return fail(FS_ERR_DEVICE, "particle allocation failed");
+        return FS_OK;
+    }
+
+    int alloc_particles(long long n) {
+        if (n > 0x7fffffffLL) return fail(FS_ERR_ARG, "particle count exceeds 2^31-1");
+        free_particles();
+        np = n;
+        int rc = alloc_view(PA, n);
+        if (rc) return rc;
+        rc = alloc_view(PB, n);
+        if (rc) return rc;
+        size_t b = sizeof(int) * (size_t)std::max<long long>(n, 1);
+        HIPCHK(hipMalloc(&d_key, b));
+        HIPCHK(hipMalloc(&d_rank, b));
+        keys_valid = sorted_valid = false;
+        return FS_OK;
+    }
+
+    LabelArgs label_args() {
+        LabelArgs la{};
+        la.W = W; la.H = H;
+        la.dx = cfg.dx;
+        la.body = body.params();
+        la.sdf = d_sdf;
+        la.n_fluid = d_nfluid;
+        return la;
+    }
+
+    GridArgs grid_args() {
+        GridArgs g{};
+        g.W = W; g.H = H; g.D = D;
+        g.dx = cfg.dx; g.dt = cfg.dt; g.density = cfg.density;
+        g.labels = d_labels;
+        g.u = d_u; g.v = d_v; g.w = d_w;
+        g.body = body.params();
+        return g;
+    }
+
+    // ---- stages -----------------------------------------------------------
+    int stage_sort_label() {
+        if (!keys_valid) {
+            HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int) * (C + 4), st));
+            launch_key_count(PA, (int)np, cfg.dx, W, H, D, d_key, d_rank, d_count, st);
+            keys_valid = true;
+        }
+        HIPCHK(hipMemsetAsync(d_nfluid, 0, sizeof(int), st));
+        LabelArgs la = label_args();
+        launch_scan_label(d_count, d_solid, d_labels, C, d_bsum, d_total, d_start, la, st);
+        launch_scatter(PA, PB, (int)np, d_key, d_rank, d_start, st);
+        std::swap(PA, PB);
+        keys_valid = false;  // counts were consumed by the scan
+        sorted_valid = true;
+        return FS_OK;
+    }
+
+    int stage_p2g() {
+        if (!sorted_valid) return fail(FS_ERR_STATE, "P2G needs the label/sort stage of this step");
+        P2GArgs a{};
+        a.P = PA;
+        a.start = d_start;
+        a.W = W; a.H = H; a.D = D;
+        a.dx = cfg.dx; a.dt = cfg.dt; a.gravity_y = cfg.gravity_y;
+        a.u = d_u; a.v = d_v; a.w = d_w; a.us = d_us; a.vs = d_vs; a.ws = d_ws;
+        launch_p2g(a, st);
+        return FS_OK;
+    }
+
+    int stage_rhs() {
+        launch_rhs(grid_args(), d_rhs, st);
+        return FS_OK;
+    }
+
+    int stage_pressure() {
+        Preconditioner* pre = nullptr;
+        if (cfg.precond == FS_PRECOND_JACOBI) pre = jacobi.get();
+        int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats);
+        if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
+        if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");
+        if (rc == -4) return fail(FS_ERR_NOT_CONVERGED, "pressure solve did not reach the tolerance");
+        return FS_OK;
+    }
+
+    int stage_project() {
+        launch_project(grid_args(), d_p, st);
+        return FS_OK;
+    }
+
+    // updateBody FluidSolver3D.cu:2793-2846 with drag/lift and buoyancy reduced on the GPU
+    int stage_body() {
+        if (!body.active) return FS_OK;
+        if (!sorted_valid) return fail(FS_ERR_STATE, "body stage needs the label/sort stage of this step");
+        HIPCHK(hipMemsetAsync(d_bacc, 0, sizeof(double) * 8, st));
+        launch_body_forces(grid_args(), d_sdf, d_start, PA, d_bacc, st);
+        double acc[8];
+        HIPCHK(hipMemcpyAsync(acc, d_bacc, sizeof(acc), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const f3 G = mk(0.0f, cfg.gravity_y, 0.0f);
+        f3 totalF = mk((float)acc[0], (float)acc[1], (float)acc[2]);
+        f3 totalT = mk((float)acc[3], (float)acc[4], (float)acc[5]);
+        body.force = body.force + body.mass * G;
+        body.force = body.force + totalF;
+        body.torque = body.torque + totalT;
+        float sub = (float)acc[6];
+        f3 Farch = (-cfg.density * sub) * G;
+        body.force = body.force + Farch;
+        body.last_force = body.force;
+        body.last_torque = body.torque;
+        body.last_sub = sub;
+        body.integrate(cfg.dt);
+        return FS_OK;
+    }
+
+    int stage_g2p_advect() {
+        G2PArgs a{};
+        a.P = PA;
+        a.n = (int)np;
+        a.W = W; a.H = H; a.D = D;
+        a.dx = cfg.dx; a.dt = cfg.dt; a.alpha = cfg.pic_weight; a.C = cfg.advect_c;
+        a.u = d_u; a.v = d_v; a.w = d_w; a.us = d_us; a.vs = d_vs; a.ws = d_ws;
+        a.nu = (int)NU(); a.nv = (int)NV(); a.nw = (int)NW();
+        a.labels = d_labels;
+        // the counts of the next label pass are accumulated here
+        HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int) * (C + 4), st));
+        a.next_count = d_count;
+        a.next_key = d_key;
+        a.next_rank = d_rank;
+        launch_g2p_advect(a, st);
+        keys_valid = true;
+        sorted_valid = false;
+        return FS_OK;
+    }
+
+    int step_once() {
+        if (C == 0) return fail(FS_ERR_STATE, "no scene loaded");
+        int rc;
+        HIPCHK(hipEventRecord(ev[0], st));
+        if ((rc = stage_sort_label())) return rc;
+        HIPCHK(hipEventRecord(ev[1], st));
+        if ((rc = stage_p2g())) return rc;
+        HIPCHK(hipEventRecord(ev[2], st));
+        stage_rhs();
+        rc = stage_pressure();
+        if (rc && rc != FS_ERR_NO_FLUID && rc != FS_ERR_NOT_CONVERGED) return rc;
+        HIPCHK(hipEventRecord(ev[3], st));
+        stage_project();
+        HIPCHK(hipEventRecord(ev[4], st));
+        if ((rc = stage_body())) return rc;
+        HIPCHK(hipEventRecord(ev[5], st));
+        if ((rc = stage_g2p_advect())) return rc;
+        HIPCHK(hipEventRecord(ev[6], st));
+        HIPCHK(hipEventSynchronize(ev[6]));
+        float ms;
+        const int slot[6] = {FS_STAGE_LABEL, FS_STAGE_P2G, FS_STAGE_PRESSURE, FS_STAGE_PROJECT, FS_STAGE_BODY,
+                             FS_STAGE_G2P};
+        for (int i = 0; i < 6; ++i) {
+            HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            times[slot[i]] += ms;
+        }
+        return FS_OK;
+    }
+};
+
+// ---------------- host IO --------------------------------------------------
+bool json_value(const std::string& t, const char* key, std::string& out) {
+    const std::string k = std::string("\"") + key + "\"";
+    size_t p = 0;
+    while ((p = t.find(k, p)) != std::string::npos) {
+        size_t q = p + k.size();
+        while (q < t.size() && std::isspace((unsigned char)t[q])) ++q;
+        if (q < t.size() && t[q] == ':') {
+            ++q;
+            while (q < t.size() && std::isspace((unsigned char)t[q])) ++q;
+            if (q < t.size() && t[q] == '"') {
+                size_t e = t.find('"', q + 1);
+                if (e == std::string::npos) return false;
+                out = t.substr(q + 1, e - q - 1);
+            } else {
+                size_t e = q;
+                while (e < t.size() && t[e] != ',' && t[e] != '}' && !std::isspace((unsigned char)t[e])) ++e;
+                out = t.substr(q, e - q);
+            }
+            return true;
+        }
+        p = q;
+    }
+    return false;
+}
+
+int read_label_file(const char* path, int& W, int& H, int& D, std::vector<uint8_t>& L) {
+    std::ifstream f(path);
+    if (!f.is_open()) return fail(FS_ERR_IO, std::string("Cannot open file: ") + path);
+    std::string header;
+    if (!std::getline(f, header)) return fail(FS_ERR_IO, std::string("empty label file: ") + path);
+    std::istringstream hs(header);
+    W = H = 0;
+    D = 1;
+    hs >> W >> H;
+    int d;
+    if (hs >> d) D = d;
+    if (W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_IO, std::string("bad label header in ") + path);
+    L.assign((size_t)W * H * D, FS_SOLID);
+    // characters are assigned in reverse order: the first is cell (W-1,H-1,D-1)
+    // (FluidSolver3D.cu:139-152); unknown characters leave SOLID
+    for (long long idx = (long long)W * H * D - 1; idx >= 0; --idx) {
+        char ch;
+        if (!(f >> ch)) break;
+        if (ch == 'S') L[(size_t)idx] = FS_SOLID;
+        else if (ch == 'F') L[(size_t)idx] = FS_FLUID;
+        else if (ch == 'A') L[(size_t)idx] = FS_AIR;
+    }
+    return FS_OK;
+}
+
+int upload_scene(Solver& s, int W, int H, int D, const std::vector<uint8_t>& L) {
+    int rc = s.alloc_grids(W, H, D);
+    if (rc) return rc;
+    std::vector<uint8_t> solid(L.size());
+    for (size_t c = 0; c < L.size(); ++c) solid[c] = (L[c] == FS_SOLID);
+    HIPCHK(hipMemcpy(s.d_labels, L.data(), L.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.d_solid, solid.data(), L.size(), hipMemcpyHostToDevice));
+    if (s.body.active) {
+        launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
+        HIPCHK(hipStreamSynchronize(s.st));
+    }
+    return FS_OK;
+}
+
+int write_particle_file(Solver& s, const char* path, int fmt) {
+    std::vector<float> pos((size_t)s.np * 3), vel;
+    {
+        std::vector<float> px(s.np), py(s.np), pz(s.np);
+        std::vector<int> id(s.np);
+        if (s.np) {
+            HIPCHK(hipMemcpy(px.data(), s.PA.px, sizeof(float) * s.np, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(py.data(), s.PA.py, sizeof(float) * s.np, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(pz.data(), s.PA.pz, sizeof(float) * s.np, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(id.data(), s.PA.id, sizeof(int) * s.np, hipMemcpyDeviceToHost));
+        }
+        for (long long i = 0; i < s.np; ++i) {
+            size_t d = (size_t)id[i];
+            pos[3 * d] = px[i]; pos[3 * d + 1] = py[i]; pos[3 * d + 2] = pz[i];
+        }
+    }
+    std::ofstream f(path);
+    if (!f.is_open()) return fail(FS_ERR_IO, std::string("Error opening file: ") + path);
+    if (fmt == FS_FMT_OFF) {
+        f << "OFF\n" << s.np << " 0 0\n";
+    } else {
+        f << "ply\nformat ascii 1.0\nelement vertex " << s.np
+          << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+    }
+    for (long long q = 0; q < s.np; ++q) f << pos[3 * q] << " " << pos[3 * q + 1] << " " << pos[3 * q + 2] << "\n";
+    return FS_OK;
+}
+
+// RigidBody::generateSurfacePoints + exportToPLY utility.cuh:341-400
+void write_body_ply(const Solver& s, const std::string& path) {
+    const HostBody& b = s.body;
+    std::vector<f3> pts;
+    const float density = 0.5f * s.cfg.dx;
+    for (int k = 0; k < b.sd; ++k)
+        for (int j = 0; j < b.sh; ++j)
+            for (int i = 0; i < b.sw; ++i) {
+                float v = b.sdf[(size_t)i + (size_t)j * b.sw + (size_t)k * b.sw * b.sh];
+                if (!(std::fabs(v) < density)) continue;
+                f3 pl = b.file_origin + mk(i * b.cs, j * b.cs, k * b.cs);
+                f3 pc = pl - b.local_com;
+                f3 pr = mk(b.R[0] * pc.x + b.R[1] * pc.y + b.R[2] * pc.z, b.R[3] * pc.x + b.R[4] * pc.y + b.R[5] * pc.z,
+                           b.R[6] * pc.x + b.R[7] * pc.y + b.R[8] * pc.z);
+                pts.push_back(b.pos + pr);
+            }
+    std::ofstream f(path);
+    if (!f.is_open()) return;
+    f << "ply\nformat ascii 1.0\nelement vertex " << pts.size()
+      << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+    for (auto& p : pts) f << p.x << " " << p.y << " " << p.z << "\n";
+}
+
+}  // namespace
+
+// ============================== C ABI ======================================
+extern "C" {
+
+const char* fs_last_error(void) { return g_err.c_str(); }
+const char* fs_backend(void) { return "hip"; }
+
+void fs_config_default(fs_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->dt = 0.02f;
+    c->dx = 0.1f;
+    c->pic_weight = 0.01f;
+    c->iter_per_frame = 1;
+    c->output_format = FS_FMT_PLY;
+    std::snprintf(c->output_template, sizeof(c->output_template), "%s", "InputData/particles_");
+    std::snprintf(c->input_label_data, sizeof(c->input_label_data), "%s", "InputData/labels.txt");
+    c->frame_amount = 100;
+    c->active_body = 0;
+    c->body_x = c->body_y = c->body_z = 0.5f;
+    c->body_mass = 50.0f;
+    std::snprintf(c->body_sdf_path, sizeof(c->body_sdf_path), "%s", "InputData/ball.sdf");
+    c->particles_per_cell = 4;
+    c->seed = 1;
+    c->gravity_y = -9.81f;
+    c->density = 1000.0f;
+    c->advect_c = 0.1f;
+    c->solver_tol = 1e-6;
+    c->solver_max_iter = 500;
+    c->precond = FS_PRECOND_MG;
+    c->stall_window = 5;
+}
+
+int fs_config_load_json(fs_config* c, const char* path) {
+    std::ifstream f(path);
+    if (!f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string t = ss.str();
+    std::string v;
+    try {
+        if (json_value(t, "dt", v)) c->dt = std::stof(v);
+        if (json_value(t, "dx", v)) c->dx = std::stof(v);
+        if (json_value(t, "PIC_WEIGHT", v)) c->pic_weight = std::stof(v);
+        if (json_value(t, "iterPerFrame", v)) c->iter_per_frame = std::stoi(v);
+        if (json_value(t, "outputFormat", v)) c->output_format = (v == "PLY") ? FS_FMT_PLY : FS_FMT_OFF;
+        if (json_value(t, "outputTemplate", v)) std::snprintf(c->output_template, 256, "%s", v.c_str());
+        if (json_value(t, "inputLabelData", v)) std::snprintf(c->input_label_data, 256, "%s", v.c_str());
+        if (json_value(t, "frameAmount", v)) c->frame_amount = std::stoi(v);
+        if (json_value(t, "activeBody", v)) c->active_body = (v == "true" || v == "1");
+        if (json_value(t, "body_x", v)) c->body_x = std::stof(v);
+        if (json_value(t, "body_y", v)) c->body_y = std::stof(v);
+        if (json_value(t, "body_z", v)) c->body_z = std::stof(v);
+        if (json_value(t, "body_mass", v)) c->body_mass = std::stof(v);
+        if (json_value(t, "bodySdf", v)) std::snprintf(c->body_sdf_path, 256, "%s", v.c_str());
+    } catch (const std::exception& e) {
+        return fail(FS_ERR_IO, std::string("bad value in ") + path + ": " + e.what());
+    }
+    return FS_OK;
+}
+
+int fs_create(const fs_config* cfg, void** out) {
+    if (!cfg || !out) return fail(FS_ERR_ARG, "null argument");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FS_ERR_DEVICE, "no HIP device");
+    std::unique_ptr<Solver> s(new Solver());
+    s->cfg = *cfg;
+    HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipMalloc(&s->d_bacc, sizeof(double) * 8));
+    *out = s.release();
+    return FS_OK;
+}
+
+void fs_destroy(void* h) { delete (Solver*)h; }
+
+int fs_load_labels_file(void* h, const char* path) {
+    Solver& s = *(Solver*)h;
+    int W, H, D;
+    std::vector<uint8_t> L;
+    int rc = read_label_file(path, W, H, D, L);
+    if (rc) return rc;
+    return upload_scene(s, W, H, D, L);
+}
+
+int fs_set_labels(void* h, const uint8_t* labels, int W, int H, int D) {
+    Solver& s = *(Solver*)h;
+    if (!labels || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad label grid");
+    std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
+    return upload_scene(s, W, H, D, L);
+}
+
+int fs_set_level_sets(void* h, const float* ps, const float* pf, int W, int H, int D) {
+    if (!ps || !pf || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad level sets");
+    std::vector<uint8_t> L((size_t)W * H * D);
+    for (size_t c = 0; c < L.size(); ++c) L[c] = ps[c] < 0 ? FS_SOLID : (pf[c] < 0 ? FS_FLUID : FS_AIR);
+    return fs_set_labels(h, L.data(), W, H, D);
+}
+
+int fs_get_dims(void* h, int* W, int* H, int* D) {
+    Solver& s = *(Solver*)h;
+    *W = s.W; *H = s.H; *D = s.D;
+    return FS_OK;
+}
+
+int fs_set_body_sdf(void* h, const float* sdf, int w, int hh, int d, float ox, float oy, float oz, float cs, float x,
+                    float y, float z, float mass) {
+    Solver& s = *(Solver*)h;
+    if (!sdf || w <= 0 || hh <= 0 || d <= 0 || !(cs > 0)) return fail(FS_ERR_ARG, "bad SDF grid");
+    HostBody& b = s.body;
+    b = HostBody();
+    b.active = true;
+    b.sdf.assign(sdf, sdf + (size_t)w * hh * d);
+    b.sw = w; b.sh = hh; b.sd = d;
+    b.file_origin = mk(ox, oy, oz);
+    b.cs = cs;
+    b.size = mk(w * cs, hh * cs, d * cs);
+    f3 center = b.file_origin + b.size * 0.5f;
+    f3 T = mk(x, y, z) - center;
+    b.pos = mk(x, y, z);
+    b.sdf_origin = b.file_origin + T;
+    b.local_com = center;
+    b.mass = mass;
+    float radius = b.size.x / 2.0f;
+    b.inertia = 0.4f * b.mass * radius * radius;
+    b.inv_inertia = 1.0f / b.inertia;
+    b.update_rotation();
+    dfree(s.d_sdf);
+    HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * b.sdf.size()));
+    HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
+    if (s.C > 0) {
+        launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
+        HIPCHK(hipStreamSynchronize(s.st));
+    }
+    return FS_OK;
+}
+
+// RigidBody::loadSDF utility.cuh:231-338: "w h d", origin, cell size, then k->j->i values
+int fs_load_body_sdf(void* h, const char* path, float x, float y, float z, float mass) {
+    std::ifstream f(path);
+    if (!f.is_open()) return fail(FS_ERR_IO, std::string("Cannot open SDF file: ") + path);
+    std::string line;
+    int w = 0, hh = 0, d = 0;
+    float ox = 0, oy = 0, oz = 0, cs = 0;
+    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing dimensions");
+    { std::istringstream is(line); is >> w >> hh >> d; }
+    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing origin");
+    { std::istringstream is(line); is >> ox >> oy >> oz; }
+    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing cell size");
+    try { cs = std::stof(line); } catch (...) { return fail(FS_ERR_IO, "SDF: bad cell size"); }
+    if (w <= 0 || hh <= 0 || d <= 0) return fail(FS_ERR_IO, "SDF: bad dimensions");
+    std::vector<float> data((size_t)w * hh * d);
+    for (auto& v : data)
+        if (!(f >> v)) return fail(FS_ERR_IO, "Unexpected end of SDF data");
+    return fs_set_body_sdf(h, data.data(), w, hh, d, ox, oy, oz, cs, x, y, z, mass);
+}
+
+int fs_get_body_state(void* h, fs_body_state* o) {
+    const HostBody& b = ((Solver*)h)->body;
+    std::memset(o, 0, sizeof(*o));
+    o->pos[0] = b.pos.x; o->pos[1] = b.pos.y; o->pos[2] = b.pos.z;
+    o->vel[0] = b.vel.x; o->vel[1] = b.vel.y; o->vel[2] = b.vel.z;
+    o->omega[0] = b.omega.x; o->omega[1] = b.omega.y; o->omega[2] = b.omega.z;
+    o->theta = b.theta; o->psi = b.psi; o->phi = b.phi;
+    o->force[0] = b.last_force.x; o->force[1] = b.last_force.y; o->force[2] = b.last_force.z;
+    o->torque[0] = b.last_torque.x; o->torque[1] = b.last_torque.y; o->torque[2] = b.last_torque.z;
+    o->submerged_volume = b.last_sub;
+    for (int i = 0; i < 9; ++i) o->rotation[i] = b.R[i];
+    return FS_OK;
+}
+
+int fs_set_body_state(void* h, const fs_body_state* in) {
+    HostBody& b = ((Solver*)h)->body;
+    b.pos = mk(in->pos[0], in->pos[1], in->pos[2]);
+    b.vel = mk(in->vel[0], in->vel[1], in->vel[2]);
+    b.omega = mk(in->omega[0], in->omega[1], in->omega[2]);
+    b.theta = in->theta; b.psi = in->psi; b.phi = in->phi;
+    b.update_rotation();
+    return FS_OK;
+}
+
+int fs_seed_particles(void* h, int ppc, uint64_t seed) {
+    Solver& s = *(Solver*)h;
+    if (s.C == 0) return fail(FS_ERR_STATE, "no scene loaded");
+    if (ppc <= 0 || ppc > 63) return fail(FS_ERR_ARG, "particles per cell must be in [1,63]");
+    // fluid-cell ranks by an exclusive scan over (label == FLUID) in k->j->i order
+    launch_scan_fluid(s.d_labels, s.C, s.d_bsum, s.d_total, s.d_start, s.st);
+    int nfluid = 0;
+    HIPCHK(hipMemcpyAsync(&nfluid, s.d_total, sizeof(int), hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    int rc = s.alloc_particles((long long)nfluid * ppc);
+    if (rc) return rc;
+    launch_seed(s.d_labels, s.d_start, s.C, s.W, s.H, s.cfg.dx, ppc, seed, s.PA, s.st);
+    HIPCHK(hipStreamSynchronize(s.st));
+    return FS_OK;
+}
+
+int fs_set_particles(void* h, const float* pos, const float* vel, int64_t n) {
+    Solver& s = *(Solver*)h;
+    if (n < 0 || (n > 0 && !pos)) return fail(FS_ERR_ARG, "bad particle arrays");
+    int rc = s.alloc_particles(n);
+    if (rc) return rc;
+    std::vector<float> a((size_t)n);
+    std::vector<int> id((size_t)n);
+    float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
+    for (int c = 0; c < 6; ++c) {
+        const float* src = c < 3 ? pos : vel;
+        for (int64_t i = 0; i < n; ++i) a[(size_t)i] = src ? src[3 * i + (c % 3)] : 0.0f;
+        if (n) HIPCHK(hipMemcpy(dst[c], a.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    for (int64_t i = 0; i < n; ++i) id[(size_t)i] = (int)i;
+    if (n) HIPCHK(hipMemcpy(s.PA.id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    return FS_OK;
+}
+
+int fs_get_particles(void* h, float* pos, float* vel, int64_t cap, int64_t* n) {
+    Solver& s = *(Solver*)h;
+    if (n) *n = s.np;
+    if (cap < s.np) return fail(FS_ERR_ARG, "buffer too small");
+    if (s.np == 0) return FS_OK;
+    HIPCHK(hipStreamSynchronize(s.st));
+    std::vector<float> a((size_t)s.np);
+    std::vector<int> id((size_t)s.np);
+    HIPCHK(hipMemcpy(id.data(), s.PA.id, sizeof(int) * s.np, hipMemcpyDeviceToHost));
+    const float* src[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
+    for (int c = 0; c < 6; ++c) {
+        float* out = c < 3 ? pos : vel;
+        if (!out) continue;
+        HIPCHK(hipMemcpy(a.data(), src[c], sizeof(float) * s.np, hipMemcpyDeviceToHost));
+        for (long long i = 0; i < s.np; ++i) out[3 * (size_t)id[i] + (c % 3)] = a[i];
+    }
+    return FS_OK;
+}
+
+int64_t fs_num_particles(void* h) { return ((Solver*)h)->np; }
+
+int fs_step(void* h, int nsteps) {
+    Solver& s = *(Solver*)h;
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = s.step_once();
+        if (rc) return rc;
+    }
+    return FS_OK;
+}
+
+int fs_run(void* h, int frames) {
+    Solver& s = *(Solver*)h;
+    const int ipf = std::max(1, s.cfg.iter_per_frame);
+    std::ofstream csv("OutputData/csvRes.csv");
+    if (csv.is_open()) csv << "frame,centerX1,centerX2,centerX3,angleX1,angleX2,angleX3\n";
+    for (int i = 0; i <= frames * ipf; ++i) {
+        int rc = s.step_once();
+        if (rc) return rc;
+        if (i % ipf == 0) {
+            std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
+                             (s.cfg.output_format == FS_FMT_OFF ? ".off" : ".ply");
+            if ((rc = write_particle_file(s, fn.c_str(), s.cfg.output_format))) return rc;
+            if (s.body.active) write_body_ply(s, "OutputData/body_" + std::to_string(i / ipf) + ".ply");
+            if (csv.is_open())
+                csv << i / ipf << "," << s.body.pos.x << "," << s.body.pos.y << "," << s.body.pos.z << ","
+                    << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";
+        }
+    }
+    return FS_OK;
+}
+
+#define STAGE_SYNC(call)                                                  \
+    do {                                                                  \
+        int rc_ = (call);                                                 \
+        if (rc_) return rc_;                                              \
+        HIPCHK(hipStreamSynchronize(s.st));                               \
+        return FS_OK;                                                     \
+    } while (0)
+
+int fs_stage_label(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_sort_label()); }
+int fs_stage_p2g(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_p2g()); }
+int fs_stage_rhs(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_rhs()); }
+int fs_stage_pressure(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_pressure()); }
+int fs_stage_project(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_project()); }
+int fs_stage_body(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_body()); }
+int fs_stage_g2p_advect(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_g2p_advect()); }
+
+int fs_get_grid(void* h, int which, void* out, int64_t count) {
+    Solver& s = *(Solver*)h;
+    HIPCHK(hipStreamSynchronize(s.st));
+    const void* src = nullptr;
+    long long n = 0;
+    size_t esz = 4;
+    switch (which) {
+        case FS_GRID_U: src = s.d_u; n = s.NU(); break;
+        case FS_GRID_V: src = s.d_v; n = s.NV(); break;
+        case FS_GRID_W: src = s.d_w; n = s.NW(); break;
+        case FS_GRID_USAVED: src = s.d_us; n = s.NU(); break;
+        case FS_GRID_VSAVED: src = s.d_vs; n = s.NV(); break;
+        case FS_GRID_WSAVED: src = s.d_ws; n = s.NW(); break;
+        case FS_GRID_P: src = s.d_p; n = s.C; break;
+        case FS_GRID_RHS: src = s.d_rhs; n = s.C; break;
+        case FS_GRID_LABELS: src = s.d_labels; n = s.C; esz = 1; break;
+        case FS_GRID_SOLID: src = s.d_solid; n = s.C; esz = 1; break;
+        case FS_GRID_P64: src = s.d_p64; n = s.C; esz = 8; break;
+        case FS_GRID_CELLKEY: {
+            n = s.np;
+            if (count < n) return fail(FS_ERR_ARG, "buffer too small");
+            if (n == 0) return FS_OK;
+            if (!s.d_tmpkey) HIPCHK(hipMalloc(&s.d_tmpkey, sizeof(int) * n));
+            launch_cell_keys(s.PA, (int)n, s.cfg.dx, s.W, s.H, s.D, s.d_tmpkey, s.st);
+            std::vector<int> k((size_t)n), id((size_t)n);
+            HIPCHK(hipMemcpyAsync(k.data(), s.d_tmpkey, sizeof(int) * n, hipMemcpyDeviceToHost, s.st));
+            HIPCHK(hipMemcpyAsync(id.data(), s.PA.id, sizeof(int) * n, hipMemcpyDeviceToHost, s.st));
+            HIPCHK(hipStreamSynchronize(s.st));
+            int* o = (int*)out;
+            for (long long i = 0; i < n; ++i) o[id[i]] = k[i];
+            return FS_OK;
+        }
+        default: return fail(FS_ERR_ARG, "unknown grid");
+    }
+    if (count < n) return fail(FS_ERR_ARG, "buffer too small");
+    if (n) HIPCHK(hipMemcpy(out, src, esz * n, hipMemcpyDeviceToHost));
+    return FS_OK;
+}
+
+int fs_set_grid(void* h, int which, const void* in, int64_t count) {
+    Solver& s = *(Solver*)h;
+    void* dst = nullptr;
+    long long n = 0;
+    size_t esz = 4;
+    switch (which) {
+        case FS_GRID_U: dst = s.d_u; n = s.NU(); break;
+        case FS_GRID_V: dst = s.d_v; n = s.NV(); break;
+        case FS_GRID_W: dst = s.d_w; n = s.NW(); break;
+        case FS_GRID_USAVED: dst = s.d_us; n = s.NU(); break;
+        case FS_GRID_VSAVED: dst = s.d_vs; n = s.NV(); break;
+        case FS_GRID_WSAVED: dst = s.d_ws; n = s.NW(); break;
+        case FS_GRID_P: dst = s.d_p; n = s.C; break;
+        case FS_GRID_RHS: dst = s.d_rhs; n = s.C; break;
+        case FS_GRID_LABELS: dst = s.d_labels; n = s.C; esz = 1; break;
+        case FS_GRID_P64: dst = s.d_p64; n = s.C; esz = 8; break;
+        default: return fail(FS_ERR_ARG, "grid not settable");
+    }
+    if (count != n) return fail(FS_ERR_ARG, "size mismatch");
+    HIPCHK(hipStreamSynchronize(s.st));
+    if (n) HIPCHK(hipMemcpy(dst, in, esz * n, hipMemcpyHostToDevice));
+    return FS_OK;
+}
+
+int fs_get_stage_times(void* h, double* ms, int n) {
+    Solver& s = *(Solver*)h;
+    for (int i = 0; i < n && i < FS_NUM_STAGES; ++i) ms[i] = s.times[i];
+    return FS_OK;
+}
+
+int fs_get_solve_stats(void* h, fs_solve_stats* o) {
+    *o = ((Solver*)h)->stats;
+    return FS_OK;
+}
+
+int fs_write_particles(void* h, const char* path, int fmt) { return write_particle_file(*(Solver*)h, path, fmt); }
+
+int fs_solve_pressure_system(void* h, const uint8_t* labels, const float* b, double* p, int W, int H, int D,
+                             double tol, fs_solve_stats* stats) {
+    Solver& s = *(Solver*)h;
+    if (!labels || !b || !p || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad system");
+    if (s.W != W || s.H != H || s.D != D) {
+        std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
+        int rc = upload_scene(s, W, H, D, L);
+        if (rc) return rc;
+    } else {
+        HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(s.d_rhs, b, sizeof(float) * (size_t)W * H * D, hipMemcpyHostToDevice));
+    double save = s.cfg.solver_tol;
+    s.cfg.solver_tol = tol;
+    int rc = s.stage_pressure();
+    s.cfg.solver_tol = save;
+    HIPCHK(hipStreamSynchronize(s.st));
+    HIPCHK(hipMemcpy(p, s.d_p64, sizeof(double) * (size_t)W * H * D, hipMemcpyDeviceToHost));
+    if (stats) *stats = s.stats;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+// fs_device.hpp -- device-side helpers shared by the gfx950 kernels.
+//
+// All FP32 arithmetic here keeps the reference's operation order
+// (src/FluidSim/FluidSolver3D.cu, utility.cu) and the library is compiled with
+// -ffp-contract=off, so per-element stages reproduce the reference's FP32 bits
+// wherever the reference's own result is defined.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fluidsim.h"
+
+#define FS_DEV __device__ __forceinline__
+#define FS_HD __host__ __device__ __forceinline__
+
+namespace fsd {
+
+struct f3 {
+    float x, y, z;
+};
+FS_HD f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+// utility.cu:3-25
+FS_HD f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+FS_HD f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+FS_HD f3 operator*(f3 a, float b) { return mk(a.x * b, a.y * b, a.z * b); }
+FS_HD f3 operator*(float b, f3 a) { return a * b; }
+FS_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+FS_HD f3 operator/(f3 a, float b) { return a * (1.0f / b); }
+FS_HD f3 cross(f3 a, f3 b) {  // utility.cu:208-215
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Particle -> cell index, MarkFluidCellsFunctor FluidSolver3D.cu:322-332.
+// IEEE division and floorf: bit-exact with the reference.
+FS_DEV int cell_key(float px, float py, float pz, float dx, int W, int H, int D) {
+    int i = (int)floorf(px / dx);
+    int j = (int)floorf(py / dx);
+    int k = (int)floorf(pz / dx);
+    i = i < 0 ? 0 : (i >= W ? W - 1 : i);
+    j = j < 0 ? 0 : (j >= H ? H - 1 : j);
+    k = k < 0 ? 0 : (k >= D ? D - 1 : k);
+    return i + j * W + k * (W * H);
+}
+
+FS_DEV float clamp01(float f) { return fminf(fmaxf(f, 0.0f), 1.0f); }
+
+// Base corner and fractions of one MAC component (FluidSolver3D.cu:476-571, :833-939).
+struct Base {
+    int i, j, k;
+    float fx, fy, fz;
+};
+template <int COMP>
+FS_DEV Base mac_base(float rx, float ry, float rz, int W, int H, int D) {
+    Base b;
+    if (COMP == 0) {
+        b.i = (int)floorf(rx); b.j = (int)floorf(ry - 0.5f); b.k = (int)floorf(rz - 0.5f);
+        b.i = min(max(b.i, 0), W); b.j = min(max(b.j, 0), H - 1); b.k = min(max(b.k, 0), D - 1);
+        b.fx = rx - b.i; b.fy = (ry - 0.5f) - b.j; b.fz = (rz - 0.5f) - b.k;
+    } else if (COMP == 1) {
+        b.i = (int)floorf(rx - 0.5f); b.j = (int)floorf(ry); b.k = (int)floorf(rz - 0.5f);
+        b.i = min(max(b.i, 0), W - 1); b.j = min(max(b.j, 0), H); b.k = min(max(b.k, 0), D - 1);
+        b.fx = (rx - 0.5f) - b.i; b.fy = ry - b.j; b.fz = (rz - 0.5f) - b.k;
+    } else {
+        b.i = (int)floorf(rx - 0.5f); b.j = (int)floorf(ry - 0.5f); b.k = (int)floorf(rz);
+        b.i = min(max(b.i, 0), W - 1); b.j = min(max(b.j, 0), H - 1); b.k = min(max(b.k, 0), D);
+        b.fx = (rx - 0.5f) - b.i; b.fy = (ry - 0.5f) - b.j; b.fz = rz - b.k;
+    }
+    b.fx = clamp01(b.fx); b.fy = clamp01(b.fy); b.fz = clamp01(b.fz);
+    return b;
+}
+
+// trilinearInterpolation FluidSolver3D.cu:784-801
+FS_DEV float trilin(float xd, float yd, float zd, float c000, float c100, float c001, float c101,
+                    float c010, float c110, float c011, float c111) {
+    float c00 = c000 * (1.0f - xd) + c100 * xd;
+    float c01 = c001 * (1.0f - xd) + c101 * xd;
+    float c10 = c010 * (1.0f - xd) + c110 * xd;
+    float c11 = c011 * (1.0f - xd) + c111 * xd;
+    float c0 = c00 * (1.0f - yd) + c10 * yd;
+    float c1 = c01 * (1.0f - yd) + c11 * yd;
+    return c0 * (1.0f - zd) + c1 * zd;
+}
+
+// Rigid-body pose used by the label pass and the face boundary conditions.
+struct BodyParams {
+    int active;
+    int sw, sh, sd;
+    float cs;
+    f3 pos, vel, omega;
+    f3 file_origin, local_com;
+    float R[9];
+};
+
+// Utility::contains utility.cu:163-206 (the reference passes fileOrigin as
+// sdf_origin, FluidSolver3D.cu:183,359).
+FS_DEV bool body_contains(const BodyParams& b, const float* __restrict__ sdf, f3 wp) {
+    f3 pr = wp - b.pos;
+    f3 plc;
+    plc.x = b.R[0] * pr.x + b.R[3] * pr.y + b.R[6] * pr.z;
+    plc.y = b.R[1] * pr.x + b.R[4] * pr.y + b.R[7] * pr.z;
+    plc.z = b.R[2] * pr.x + b.R[5] * pr.y + b.R[8] * pr.z;
+    f3 pl = plc + b.local_com;
+    f3 sc = mk((pl.x - b.file_origin.x) / b.cs, (pl.y - b.file_origin.y) / b.cs,
+               (pl.z - b.file_origin.z) / b.cs);
+    if (sc.x < 0 || sc.x >= b.sw || sc.y < 0 || sc.y >= b.sh || sc.z < 0 || sc.z >= b.sd) return false;
+    int i = min((int)sc.x, b.sw - 1);
+    int j = min((int)sc.y, b.sh - 1);
+    int k = min((int)sc.z, b.sd - 1);
+    return sdf[(size_t)i + (size_t)j * b.sw + (size_t)k * b.sw * b.sh] <= 0.0f;
+}
+
+// Rigid velocity V + Omega x r at a face (FluidSolver3D.cu:1386-1393, :2125-2135).
+FS_DEV f3 body_velocity(const BodyParams& b, f3 pos) {
+    f3 r = mk(pos.x - b.pos.x, pos.y - b.pos.y, pos.z - b.pos.z);
+    return mk(b.vel.x + b.omega.y * r.z - b.omega.z * r.y, b.vel.y + b.omega.z * r.x - b.omega.x * r.z,
+              b.vel.z + b.omega.x * r.y - b.omega.y * r.x);
+}
+
+// 64-bit SplitMix finaliser used by the counter-based particle seeding.
+FS_HD uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// Wave-level helpers (64-wide wavefronts).
+FS_DEV int lane_id() { return __lane_id(); }
+
+template <typename T>
+FS_DEV T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum of a double, result valid in thread 0.  `sm` needs blockDim/64 slots.
+FS_DEV double block_sum(double v, double* sm) {
+    v = wave_sum(v);
+    int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) sm[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0) {
+        int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) t += sm[i];
+    }
+    __syncthreads();
+    return t;
+}
+
+// Remap a linear block id so that consecutive tiles land on the same XCD
+// (block ids are dealt round-robin over the 8 XCDs of MI355X).
+FS_DEV int xcd_remap(int bid, int nblocks) {
+    const int nx = 8;
+    int per = nblocks / nx;
+    if (per == 0 || bid >= per * nx) return bid;
+    return (bid % nx) * per + bid / nx;
+}
+
+}  // namespace fsd
+
+namespace fsd {
+
+// Corner reads clamped into the array; the reference reads these corners
+// unguarded (FluidSolver3D.cu:851-858), which stays in range whenever positions
+// are clamped to (N-1)dx as advection does (:1298-1303).
+FS_DEV float ld_clamped(const float* __restrict__ f, int n, int idx) {
+    idx = idx < 0 ? 0 : (idx >= n ? n - 1 : idx);
+    return f[idx];
+}
+
+FS_DEV float sample_comp(const float* __restrict__ f, int n, const Base& b, int sx, int sxy) {
+    int i0 = b.i + b.j * sx + b.k * sxy;
+    return trilin(b.fx, b.fy, b.fz, ld_clamped(f, n, i0), ld_clamped(f, n, i0 + 1), ld_clamped(f, n, i0 + sxy),
+                  ld_clamped(f, n, i0 + 1 + sxy), ld_clamped(f, n, i0 + sx), ld_clamped(f, n, i0 + 1 + sx),
+                  ld_clamped(f, n, i0 + sx + sxy), ld_clamped(f, n, i0 + 1 + sx + sxy));
+}
+
+// interpVelDevice3D FluidSolver3D.cu:1051-1157
+FS_DEV f3 interp_vel3(const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ w, int nu,
+                      int nv, int nw, int W, int H, int D, float dx, f3 pos) {
+    float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
+    Base bu = mac_base<0>(rx, ry, rz, W, H, D);
+    Base bv = mac_base<1>(rx, ry, rz, W, H, D);
+    Base bw = mac_base<2>(rx, ry, rz, W, H, D);
+    return mk(sample_comp(u, nu, bu, W + 1, (W + 1) * H), sample_comp(v, nv, bv, W, W * (H + 1)),
+              sample_comp(w, nw, bw, W, W * H));
+}
+
+}  // namespace fsd

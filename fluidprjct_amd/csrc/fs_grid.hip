@@ -1,0 +1,293 @@
+// fs_grid.hip -- grid-side kernels of the PIC/FLIP step (gfx950).
+//
+//   k_rhs           RHSCalculator3D        FluidSolver3D.cu:1360-1474 (scale :1520)
+//   k_project       U/V/WFunctor           FluidSolver3D.cu:2072-2336 (scale :2341)
+//   k_drag_lift     DragLiftForceCalculator FluidSolver3D.cu:2674-2791
+//   k_buoyancy      applyBuoyancy           FluidSolver3D.cu:2917-3026 (moved to the GPU)
+//
+// Label reads that the reference performs past a row/slab end (:1406-1455,
+// :2109-2110) use the same wrapped linear index; an index outside the array
+// reads SOLID with p = 0.
+#include "fs_kernels.hpp"
+
+using namespace fsd;
+
+namespace fsk {
+
+FS_DEV int label_at(const uint8_t* __restrict__ L, long long idx, long long C) {
+    return (idx < 0 || idx >= C) ? FS_SOLID : (int)L[idx];
+}
+FS_DEV float p_at(const float* __restrict__ p, long long idx, long long C) {
+    return (idx < 0 || idx >= C) ? 0.0f : p[idx];
+}
+
+__global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs) {
+    const long long C = (long long)a.W * a.H * a.D;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= C) return;
+    if (a.labels[idx] != FS_FLUID) {
+        rhs[idx] = 0.0f;
+        return;
+    }
+    const int W = a.W, H = a.H;
+    int i = (int)(idx % W), j = (int)((idx / W) % H), k = (int)(idx / ((long long)W * H));
+    const float dx = a.dx;
+    const float scale = (a.density * dx) / a.dt;
+    size_t ui0 = (size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H;
+    size_t vi0 = (size_t)i + (size_t)j * W + (size_t)k * W * (H + 1);
+    size_t wi0 = (size_t)i + (size_t)j * W + (size_t)k * W * H;
+    const float *u = a.u, *v = a.v, *w = a.w;
+    float div = u[ui0 + 1] - u[ui0] + v[vi0 + W] - v[vi0] + w[wi0 + (size_t)W * H] - w[wi0];
+    float r = -scale * div;
+    if (i - 1 >= 0) {
+        int L = label_at(a.labels, idx - 1, C);
+        if (L == FS_SOLID || L == FS_BODY) {
+            float s = (L == FS_BODY) ? body_velocity(a.body, mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
+            r -= scale * (u[ui0] - s);
+        }
+    }
+    {
+        int L = label_at(a.labels, idx + 1, C);
+        if (L == FS_SOLID || L == FS_BODY) {
+            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
+            r += scale * (u[ui0 + 1] - s);
+        }
+    }
+    if (j - 1 >= 0) {
+        int L = label_at(a.labels, idx - W, C);
+        if (L == FS_SOLID || L == FS_BODY) {
+            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx)).y : 0.0f;
+            r -= scale * (v[vi0] - s);
+        }
+    }
+    {
+        int L = label_at(a.labels, idx + W, C);
+        if (L == FS_SOLID || L == FS_BODY) {
+            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx)).y : 0.0f;
+            r += scale * (v[vi0 + W] - s);
+        }
+    }
+    if (k - 1 >= 0) {
+        int L = label_at(a.labels, idx - (long long)W * H, C);
+        if (L == FS_SOLID || L == FS_BODY) {
+            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx)).z : 0.0f;
+            r -= scale * (w[wi0] - s);
+        }
+    }
+    {
+        int L = label_at(a.labels, idx + (long long)W * H, C);
+        if (L == FS_SOLID || L == FS_BODY) {
+            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx)).z : 0.0f;
+            r += scale * (w[wi0 + (size_t)W * H] - s);
+        }
+    }
+    rhs[idx] = r;
+}
+
+// One launch over the concatenated u, v, w faces.
+__global__ __launch_bounds__(256) void k_project(GridArgs a, const float* __restrict__ p) {
+    const int W = a.W, H = a.H, D = a.D;
+    const long long C = (long long)W * H * D;
+    const long long NU = (long long)(W + 1) * H * D, NV = (long long)W * (H + 1) * D, NW = (long long)W * H * (D + 1);
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NU + NV + NW) return;
+    const float dx = a.dx;
+    int comp;
+    long long idx, left, right;
+    f3 fp;
+    float* G;
+    if (g < NU) {
+        comp = 0; idx = g; G = a.u;
+        long long slice = (long long)(W + 1) * H;
+        int fk = (int)(idx / slice); long long rem = idx % slice;
+        int fj = (int)(rem / (W + 1)), fi = (int)(rem % (W + 1));
+        left = (long long)(fi - 1) + (long long)fj * W + (long long)fk * W * H;
+        right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
+        fp = mk(fi * dx, (fj + 0.5f) * dx, (fk + 0.5f) * dx);
+    } else if (g < NU + NV) {
+        comp = 1; idx = g - NU; G = a.v;
+        long long slice = (long long)W * (H + 1);
+        int fk = (int)(idx / slice); long long rem = idx % slice;
+        int fj = (int)(rem / W), fi = (int)(rem % W);
+        left = (long long)fi + (long long)(fj - 1) * W + (long long)fk * W * H;
+        right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
+        fp = mk((fi + 0.5f) * dx, fj * dx, (fk + 0.5f) * dx);
+    } else {
+        comp = 2; idx = g - NU - NV; G = a.w;
+        long long slice = (long long)W * H;
+        int fk = (int)(idx / slice); long long rem = idx % slice;
+        int fj = (int)(rem / W), fi = (int)(rem % W);
+        left = (long long)fi + (long long)fj * W + (long long)(fk - 1) * W * H;
+        right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
+        fp = mk((fi + 0.5f) * dx, (fj + 0.5f) * dx, fk * dx);
+    }
+    const float scale = a.dt / (a.density * dx);
+    const float invScale = 1.0f / scale;
+    int Ll = label_at(a.labels, left, C), Lr = label_at(a.labels, right, C);
+    float gv = G[idx];
+    float p0, p1;
+    if ((Ll == FS_FLUID || Ll == FS_AIR) && (Lr == FS_FLUID || Lr == FS_AIR)) {
+        p0 = p_at(p, left, C);
+        p1 = p_at(p, right, C);
+    } else if (Ll == FS_SOLID || Ll == FS_BODY) {
+        float sol = 0.0f;
+        if (Ll == FS_BODY) { f3 vb = body_velocity(a.body, fp); sol = comp == 0 ? vb.x : comp == 1 ? vb.y : vb.z; }
+        p0 = p_at(p, right, C) - invScale * (gv - sol);
+        p1 = p_at(p, right, C);
+    } else if (Lr == FS_SOLID || Lr == FS_BODY) {
+        float sol = 0.0f;
+        if (Lr == FS_BODY) { f3 vb = body_velocity(a.body, fp); sol = comp == 0 ? vb.x : comp == 1 ? vb.y : vb.z; }
+        p0 = p_at(p, left, C);
+        p1 = p_at(p, left, C) + invScale * (gv - sol);
+    } else {
+        p0 = p_at(p, left, C);
+        p1 = p_at(p, right, C);
+    }
+    G[idx] = gv - scale * (p1 - p0);
+}
+
+// Drag and lift on BODY-FLUID faces.  The lift term is skipped when
+// |n x u_rel| == 0, where the reference divides by zero (:2774-2775).
+__global__ __launch_bounds__(256) void k_drag_lift(GridArgs a, double* __restrict__ out) {
+    __shared__ double sm[4];
+    const int W = a.W, H = a.H, D = a.D;
+    const long long C = (long long)W * H * D;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (idx < C && a.labels[idx] == FS_BODY) {
+        const float dx = a.dx, rho = a.density;
+        int ci = (int)(idx % W), cj = (int)((idx / W) % H), ck = (int)(idx / ((long long)W * H));
+        f3 cc = mk((ci + 0.5f) * dx, (cj + 0.5f) * dx, (ck + 0.5f) * dx);
+        const float area = dx * dx;
+        const int NUc = (W + 1) * H * D, NVc = W * (H + 1) * D, NWc = W * H * (D + 1);
+        for (int dir = 0; dir < 6; ++dir) {
+            int ax = dir >> 1;
+            int sgn = (dir & 1) ? -1 : 1;
+            int ni = ci + (ax == 0 ? sgn : 0), nj = cj + (ax == 1 ? sgn : 0), nk = ck + (ax == 2 ? sgn : 0);
+            if (ni < 0 || ni >= W || nj < 0 || nj >= H || nk < 0 || nk >= D) continue;
+            if (a.labels[(size_t)ni + (size_t)nj * W + (size_t)nk * W * H] != FS_FLUID) continue;
+            f3 n = mk(ax == 0 ? (float)sgn : 0.0f, ax == 1 ? (float)sgn : 0.0f, ax == 2 ? (float)sgn : 0.0f);
+            f3 fo = mk(ax == 0 ? 0.5f * sgn : 0.0f, ax == 1 ? 0.5f * sgn : 0.0f, ax == 2 ? 0.5f * sgn : 0.0f);
+            f3 fc = cc + fo * dx;
+            f3 r = fc - a.body.pos;
+            f3 up = interp_vel3(a.u, a.v, a.w, NUc, NVc, NWc, W, H, D, dx, fc);
+            f3 urel = a.body.vel - up;
+            float ura = sqrtf(dot(urel, urel));
+            const float alpha_s = 0.5f;
+            float Aeff = 0.0f;
+            float nur = dot(n, urel);
+            if (nur > 0) Aeff = (nur / ura * alpha_s + (1 - alpha_s)) * area;
+            const float Cd = 0.1f, Cl = 0.1f;
+            float sd = (float)(-0.5 * (double)rho * (double)Cd * (double)Aeff * (double)ura);
+            f3 Fd = sd * urel;
+            f3 Td = cross(r, Fd);
+            f3 nxu = cross(n, urel);
+            float nxa = sqrtf(dot(nxu, nxu));
+            f3 Fl = mk(0, 0, 0), Tl = mk(0, 0, 0);
+            if (nxa != 0.0f) {
+                Fl = -0.5f * rho * Cl * Aeff * ura * cross(urel, nxu / nxa);
+                Tl = cross(r, Fl);
+            }
+            f3 Ft = Fd + Fl, Tt = Td + Tl;
+            acc[0] += Ft.x; acc[1] += Ft.y; acc[2] += Ft.z;
+            acc[3] += Tt.x; acc[4] += Tt.y; acc[5] += Tt.z;
+        }
+    }
+    for (int q = 0; q < 6; ++q) {
+        double t = block_sum(acc[q], sm);
+        if (threadIdx.x == 0 && t != 0.0) atomicAdd(&out[q], t);
+    }
+}
+
+// Submerged SDF volume: a voxel with phi <= 0 counts (partially when phi > -cs)
+// if its cell is FLUID or a particle binned into that cell lies within 2dx.
+__global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __restrict__ sdf,
+                                                  const int* __restrict__ start, ParticlesView P,
+                                                  double* __restrict__ out) {
+    __shared__ double sm[4];
+    const BodyParams& b = a.body;
+    long long nvox = (long long)b.sw * b.sh * b.sd;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double vol = 0.0;
+    if (q < nvox) {
+        float sv = sdf[q];
+        if (sv <= 0.0f) {
+            int i = (int)(q % b.sw), j = (int)((q / b.sw) % b.sh), k = (int)(q / ((long long)b.sw * b.sh));
+            const float cs = b.cs;
+            f3 pl = b.file_origin + mk(i * cs, j * cs, k * cs);
+            f3 pc = pl - b.local_com;
+            f3 pr = mk(b.R[0] * pc.x + b.R[1] * pc.y + b.R[2] * pc.z, b.R[3] * pc.x + b.R[4] * pc.y + b.R[5] * pc.z,
+                       b.R[6] * pc.x + b.R[7] * pc.y + b.R[8] * pc.z);
+            f3 vc = b.pos + pr;
+            float v = cs * cs * cs;
+            if (sv > -cs) v *= 0.5f - 0.5f * sv / cs;
+            const float dx = a.dx;
+            int ii = min(max((int)(vc.x / dx), 0), a.W - 1);
+            int jj = min(max((int)(vc.y / dx), 0), a.H - 1);
+            int kk = min(max((int)(vc.z / dx), 0), a.D - 1);
+            size_t gi = (size_t)ii + (size_t)jj * a.W + (size_t)kk * a.W * a.H;
+            bool subm = a.labels[gi] == FS_FLUID;
+            if (!subm) {
+                const float search = 2.0f * dx;
+                for (int e = start[gi]; e < start[gi + 1]; ++e) {
+                    float ddx = P.px[e] - vc.x, ddy = P.py[e] - vc.y, ddz = P.pz[e] - vc.z;
+                    if (sqrtf(ddx * ddx + ddy * ddy + ddz * ddz) < search) { subm = true; break; }
+                }
+            }
+            if (subm) vol = v;
+        }
+    }
+    double t = block_sum(vol, sm);
+    if (threadIdx.x == 0 && t != 0.0) atomicAdd(&out[6], t);
+}
+
+void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st) {
+    long long C = (long long)a.W * a.H * a.D;
+    hipLaunchKernelGGL(k_rhs, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, a, rhs);
+}
+
+void launch_project(const GridArgs& a, const float* p, hipStream_t st) {
+    long long n = (long long)(a.W + 1) * a.H * a.D + (long long)a.W * (a.H + 1) * a.D + (long long)a.W * a.H * (a.D + 1);
+    hipLaunchKernelGGL(k_project, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, p);
+}
+
+void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P, double* out,
+                        hipStream_t st) {
+    long long C = (long long)a.W * a.H * a.D;
+    hipLaunchKernelGGL(k_drag_lift, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, a, out);
+    long long nv = (long long)a.body.sw * a.body.sh * a.body.sd;
+    hipLaunchKernelGGL(k_buoyancy, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, a, sdf, start, P, out);
+}
+
+}  // namespace fsk
+
+namespace fsk {
+
+// MarkBodyCellsFunctor FluidSolver3D.cu:45-78 (scene set-up: BODY where the SDF
+// is <= 0 at the cell centre, unless SOLID)
+__global__ __launch_bounds__(256) void k_mark_body(LabelArgs la, uint8_t* __restrict__ labels, long long C) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    int i = (int)(c % la.W), j = (int)((c / la.W) % la.H), k = (int)(c / ((long long)la.W * la.H));
+    f3 cc = mk((i + 0.5f) * la.dx, (j + 0.5f) * la.dx, (k + 0.5f) * la.dx);
+    if (labels[c] != FS_SOLID && body_contains(la.body, la.sdf, cc)) labels[c] = FS_BODY;
+}
+
+// per-particle cell index of the current positions (read-back only)
+__global__ __launch_bounds__(256) void k_cell_keys(ParticlesView P, int n, float dx, int W, int H, int D,
+                                                   int* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = cell_key(P.px[i], P.py[i], P.pz[i], dx, W, H, D);
+}
+
+void launch_mark_body(const LabelArgs& la, uint8_t* labels, long long C, hipStream_t st) {
+    hipLaunchKernelGGL(k_mark_body, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, la, labels, C);
+}
+
+void launch_cell_keys(const ParticlesView& P, int n, float dx, int W, int H, int D, int* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_keys, dim3((n + 255) / 256), dim3(256), 0, st, P, n, dx, W, H, D, out);
+}
+
+}  // namespace fsk

@@ -1,0 +1,92 @@
+// fs_kernels.hpp -- argument blocks and host launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fs_device.hpp"
+
+// P2G LDS tile (cells); one 256-thread block per tile.
+#ifndef P2G_TX
+#define P2G_TX 16
+#endif
+#ifndef P2G_TY
+#define P2G_TY 8
+#endif
+#ifndef P2G_TZ
+#define P2G_TZ 8
+#endif
+
+namespace fsk {
+
+struct ParticlesView {
+    float* px;
+    float* py;
+    float* pz;
+    float* vx;
+    float* vy;
+    float* vz;
+    int* id;
+};
+
+struct LabelArgs {
+    int W, H;
+    float dx;
+    fsd::BodyParams body;
+    const float* sdf;
+    int* n_fluid;
+};
+
+struct P2GArgs {
+    ParticlesView P;
+    const int* start;
+    int W, H, D;
+    float dx, dt, gravity_y;
+    float *u, *v, *w, *us, *vs, *ws;
+};
+
+struct G2PArgs {
+    ParticlesView P;
+    int n;
+    int W, H, D;
+    float dx, dt, alpha, C;
+    const float *u, *v, *w, *us, *vs, *ws;
+    int nu, nv, nw;
+    const uint8_t* labels;
+    int* next_count;  // null: do not compute next-step keys
+    int* next_key;
+    int* next_rank;
+};
+
+struct GridArgs {
+    int W, H, D;
+    float dx, dt, density;
+    const uint8_t* labels;
+    float *u, *v, *w;
+    fsd::BodyParams body;
+};
+
+// particles
+void launch_key_count(const ParticlesView& P, int n, float dx, int W, int H, int D, int* key, int* rank, int* count,
+                      hipStream_t st);
+int scan_blocks(long long C);
+void launch_scan_label(int* count, const uint8_t* solid, uint8_t* labels, long long C, int* bsum, int* total,
+                       int* start, const LabelArgs& la, hipStream_t st);
+void launch_scan_fluid(const uint8_t* labels, long long C, int* bsum, int* total, int* start, hipStream_t st);
+void launch_seed(const uint8_t* labels, const int* frank, long long C, int W, int H, float dx, int ppc, uint64_t seed,
+                 const ParticlesView& P, hipStream_t st);
+void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, const int* key, const int* rank,
+                    const int* start, hipStream_t st);
+void launch_p2g(const P2GArgs& a, hipStream_t st);
+void launch_g2p_advect(const G2PArgs& a, hipStream_t st);
+
+// grid stages
+void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st);
+void launch_project(const GridArgs& a, const float* p, hipStream_t st);
+// drag/lift + buoyancy partial sums: out[0..2] force, [3..5] torque, [6] submerged volume (FP64)
+void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P,
+                        double* out, hipStream_t st);
+void launch_mark_body(const LabelArgs& la, uint8_t* labels, long long C, hipStream_t st);
+void launch_cell_keys(const ParticlesView& P, int n, float dx, int W, int H, int D, int* out, hipStream_t st);
+
+}  // namespace fsk

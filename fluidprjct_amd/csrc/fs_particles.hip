@@ -1,0 +1,530 @@
+// fs_particles.hip -- particle-side kernels of the PIC/FLIP step (gfx950).
+//
+//   k_seed              seedParticles           FluidSolver3D.cu:213-297
+//   k_key_count         MarkFluidCellsFunctor   FluidSolver3D.cu:306-337 (key part)
+//   k_scan_*            cell-count scan fused with labelGrid :340-397
+//   k_scatter           counting-sort scatter (no reference counterpart)
+//   k_p2g_tile          particlesToGrid :607-685 + saveVelocities :399-415 + applyForces :700-761
+//   k_g2p_advect        gridToParticles :983-1039 + advectParticles :1334-1357
+//
+// Particles are SoA (px,py,pz,vx,vy,vz,id), kept sorted by cell index so that
+// each P2G tile reads contiguous rows and G2P gathers hit neighbouring faces.
+#include "fs_kernels.hpp"
+
+using namespace fsd;
+
+namespace fsk {
+
+// ---------------------------------------------------------------------------
+// wave-aggregated per-cell counting: lanes holding the same key (particles are
+// sorted, so they are adjacent) issue one atomic per run instead of one each.
+FS_DEV int wave_agg_rank(int key, bool act, int* __restrict__ count) {
+    const int lane = __lane_id();
+    int prev = __shfl_up(key, 1, 64);
+    bool head = act && (lane == 0 || prev != key);
+    uint64_t hm = __ballot(head);
+    uint64_t am = __ballot(act);
+    int nact = __popcll(am);  // active lanes form a prefix of the wave
+    int base = 0, h = 0;
+    if (act) {
+        uint64_t below = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+        h = 63 - __clzll(hm & below);
+        uint64_t above = hm & ~below;
+        int next = above ? (__ffsll((unsigned long long)above) - 1) : 64;
+        int end = next < nact ? next : nact;
+        if (head) base = atomicAdd(&count[key], end - lane);
+    }
+    base = __shfl(base, h, 64);
+    return base + (lane - h);
+}
+
+__global__ __launch_bounds__(256) void k_key_count(const float* __restrict__ px, const float* __restrict__ py,
+                                                   const float* __restrict__ pz, int n, float dx, int W, int H,
+                                                   int D, int* __restrict__ key, int* __restrict__ rank,
+                                                   int* __restrict__ count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = i < n;
+    int k = act ? cell_key(px[i], py[i], pz[i], dx, W, H, D) : -1;
+    int r = wave_agg_rank(k, act, count);
+    if (act) {
+        key[i] = k;
+        rank[i] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan over cells, 4 cells per thread (int4), 256 threads per block.
+constexpr int SCAN_T = 256;
+constexpr int SCAN_PER_BLOCK = SCAN_T * 4;
+
+FS_DEV int block_excl_scan(int v, int* sm, int* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sm[w] = x;
+    __syncthreads();
+    int wpre = 0, tot = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+        int s = sm[q];
+        if (q < w) wpre += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + x - v;
+}
+
+// mode 0: values are cell counts; mode 1: values are (label == FLUID)
+template <int MODE>
+FS_DEV int4 scan_load(const int* __restrict__ cnt, const uint8_t* __restrict__ lab, long long c4, long long C) {
+    int4 v = make_int4(0, 0, 0, 0);
+    if (MODE == 0) {
+        if (c4 + 3 < C) {
+            v = *reinterpret_cast<const int4*>(cnt + c4);
+        } else {
+            if (c4 < C) v.x = cnt[c4];
+            if (c4 + 1 < C) v.y = cnt[c4 + 1];
+            if (c4 + 2 < C) v.z = cnt[c4 + 2];
+        }
+    } else {
+        if (c4 < C) v.x = lab[c4] == FS_FLUID;
+        if (c4 + 1 < C) v.y = lab[c4 + 1] == FS_FLUID;
+        if (c4 + 2 < C) v.z = lab[c4 + 2] == FS_FLUID;
+        if (c4 + 3 < C) v.w = lab[c4 + 3] == FS_FLUID;
+    }
+    return v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int* __restrict__ cnt, const uint8_t* __restrict__ lab,
+                                                        long long C, int* __restrict__ bsum) {
+    __shared__ int sm[SCAN_T / 64];
+    long long c4 = (long long)blockIdx.x * SCAN_PER_BLOCK + threadIdx.x * 4;
+    int4 v = scan_load<MODE>(cnt, lab, c4, C);
+    int s = v.x + v.y + v.z + v.w;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int q = 0; q < SCAN_T / 64; ++q) t += sm[q];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+// single block: exclusive scan of nb block sums in place; total -> *total_out
+__global__ __launch_bounds__(1024) void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out) {
+    __shared__ int sm[16];
+    const int T = blockDim.x;
+    int per = (nb + T - 1) / T;
+    int b0 = threadIdx.x * per, b1 = min(b0 + per, nb);
+    int local = 0;
+    for (int b = b0; b < b1; ++b) local += bsum[b];
+    int tot;
+    int pre = block_excl_scan(local, sm, &tot);
+    int run = pre;
+    for (int b = b0; b < b1; ++b) {
+        int v = bsum[b];
+        bsum[b] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) *total_out = tot;
+}
+
+// Final pass.  MODE 0 (cell counts): writes start[], labels (labelGrid :340-397
+// order: SOLID kept, BODY marked, then FLUID where particles are), clears the
+// count for the next step and accumulates the fluid-cell count.
+// MODE 1 (fluid flags for seeding): writes start[] only.
+template <int MODE>
+__global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, const uint8_t* __restrict__ solid,
+                                                       uint8_t* __restrict__ labels, long long C,
+                                                       const int* __restrict__ bsum, int* __restrict__ start,
+                                                       LabelArgs la) {
+    __shared__ int sm[SCAN_T / 64];
+    long long c4 = (long long)blockIdx.x * SCAN_PER_BLOCK + threadIdx.x * 4;
+    int4 v = scan_load<MODE>(cnt, labels, c4, C);
+    int s = v.x + v.y + v.z + v.w;
+    int tot;
+    int pre = block_excl_scan(s, sm, &tot) + bsum[blockIdx.x];
+    int e[4] = {pre, pre + v.x, pre + v.x + v.y, pre + v.x + v.y + v.z};
+    int vv[4] = {v.x, v.y, v.z, v.w};
+    int nfl = 0;
+    for (int q = 0; q < 4; ++q) {
+        long long c = c4 + q;
+        if (c >= C) break;
+        start[c] = e[q];
+        if (MODE == 0) {
+            uint8_t L;
+            if (solid[c]) {
+                L = FS_SOLID;
+            } else {
+                L = FS_AIR;
+                if (la.body.active) {
+                    int i = (int)(c % la.W), j = (int)((c / la.W) % la.H), k = (int)(c / ((long long)la.W * la.H));
+                    f3 cc = mk((i + 0.5f) * la.dx, (j + 0.5f) * la.dx, (k + 0.5f) * la.dx);
+                    if (body_contains(la.body, la.sdf, cc)) L = FS_BODY;
+                }
+                if (L == FS_AIR && vv[q] > 0) L = FS_FLUID;
+            }
+            labels[c] = L;
+            nfl += (L == FS_FLUID);
+        }
+    }
+    if (MODE == 0) {
+        if (c4 + 3 < C) {
+            *reinterpret_cast<int4*>(cnt + c4) = make_int4(0, 0, 0, 0);
+        } else {
+            for (int q = 0; q < 4; ++q)
+                if (c4 + q < C) cnt[c4 + q] = 0;
+        }
+        nfl = wave_sum(nfl);
+        if ((threadIdx.x & 63) == 0 && nfl) atomicAdd(la.n_fluid, nfl);
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = bsum[blockIdx.x] + tot;
+}
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_scatter(ParticlesView src, ParticlesView dst, int n,
+                                                 const int* __restrict__ key, const int* __restrict__ rank,
+                                                 const int* __restrict__ start) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int d = start[key[i]] + rank[i];
+    dst.px[d] = src.px[i];
+    dst.py[d] = src.py[i];
+    dst.pz[d] = src.pz[i];
+    dst.vx[d] = src.vx[i];
+    dst.vy[d] = src.vy[i];
+    dst.vz[d] = src.vz[i];
+    dst.id[d] = src.id[i];
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based seeding: cell c (FLUID) with fluid rank r gets particles
+// r*ppc .. r*ppc+ppc-1, each from draws (seed, c, slot, d).
+__global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels, const int* __restrict__ frank,
+                                              long long C, int W, int H, float dx, int ppc, uint64_t seed,
+                                              ParticlesView P) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C || labels[c] != FS_FLUID) return;
+    int i = (int)(c % W), j = (int)((c / W) % H), k = (int)(c / ((long long)W * H));
+    const double lo = (double)(-0.24f), hi = (double)(0.24f);
+    f3 cc = mk(((float)i + 0.5f) * dx, ((float)j + 0.5f) * dx, ((float)k + 0.5f) * dx);
+    const float off = 0.25f * dx;
+    const uint64_t key = splitmix64(seed);
+    long long base = (long long)frank[c] * ppc;
+    for (int q = 0; q < ppc; ++q) {
+        uint64_t ctr = ((uint64_t)c << 8) | ((uint64_t)q << 2);
+        int oct = (int)(splitmix64(key ^ ctr) >> 61);
+        f3 sc = mk(cc.x + ((oct & 1) ? off : -off), cc.y + ((oct & 2) ? off : -off), cc.z + ((oct & 4) ? off : -off));
+        float jit[3];
+        for (int a = 0; a < 3; ++a) {
+            double uu = (double)(splitmix64(key ^ (ctr | (uint64_t)(1 + a))) >> 11) * (1.0 / 9007199254740992.0);
+            double val = lo + (hi - lo) * uu;
+            jit[a] = (float)(val * (double)dx);
+        }
+        f3 p = mk(sc.x + jit[0], sc.y + jit[1], sc.z + jit[2]);
+        float lx = i * dx, hx = (i + 1) * dx, ly = j * dx, hy = (j + 1) * dx, lz = k * dx, hz = (k + 1) * dx;
+        p.x = p.x < lx ? lx : (hx < p.x ? hx : p.x);
+        p.y = p.y < ly ? ly : (hy < p.y ? hy : p.y);
+        p.z = p.z < lz ? lz : (hz < p.z ? hz : p.z);
+        long long d = base + q;
+        P.px[d] = p.x; P.py[d] = p.y; P.pz[d] = p.z;
+        P.vx[d] = 0.f; P.vy[d] = 0.f; P.vz[d] = 0.f;
+        P.id[d] = (int)d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2G over an LDS tile.  A block owns the faces of a TX x TY x TZ cell tile and
+// reads the particles of the tile plus a one-cell halo (the only cells whose
+// particles reach the owned faces).  Weights and products follow
+// AccumulateVelocities :468-594; sums are accumulated with LDS float atomics,
+// then each owned face is normalised (ComputeVelocityFunc :598-605), saved
+// (saveVelocities) and given gravity (ApplyScalarForce :687-698) in one write.
+template <int COMP, int TX, int TY, int TZ>
+FS_DEV void p2g_scatter_comp(float rx, float ry, float rz, float vc, const P2GArgs& a, int x0, int y0, int z0,
+                             float* __restrict__ snum, float* __restrict__ sden) {
+    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
+    Base b = mac_base<COMP>(rx, ry, rz, a.W, a.H, a.D);
+    // owned global index range of this component (inclusive)
+    const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
+    const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
+    const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
+    float wxs[2] = {1.0f - b.fx, b.fx}, wys[2] = {1.0f - b.fy, b.fy}, wzs[2] = {1.0f - b.fz, b.fz};
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz) {
+        int k = b.k + dz;
+        if (k < z0 || k > hiz) continue;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy) {
+            int j = b.j + dy;
+            if (j < y0 || j > hiy) continue;
+#pragma unroll
+            for (int dxi = 0; dxi < 2; ++dxi) {
+                int i = b.i + dxi;
+                if (i < x0 || i > hix) continue;
+                float wgt = wxs[dxi] * wys[dy] * wzs[dz];
+                int l = (i - x0) + LX * ((j - y0) + LY * (k - z0));
+                atomicAdd(&snum[l], vc * wgt);
+                atomicAdd(&sden[l], wgt);
+            }
+        }
+    }
+}
+
+template <int COMP, int TX, int TY, int TZ>
+FS_DEV void p2g_write_comp(const P2GArgs& a, int x0, int y0, int z0, const float* __restrict__ snum,
+                           const float* __restrict__ sden) {
+    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1), LZ = TZ + (COMP == 2);
+    const int nx = a.W + (COMP == 0), ny = a.H + (COMP == 1);
+    const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
+    const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
+    const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
+    float* __restrict__ g = COMP == 0 ? a.u : COMP == 1 ? a.v : a.w;
+    float* __restrict__ gs = COMP == 0 ? a.us : COMP == 1 ? a.vs : a.ws;
+    const float acc = COMP == 1 ? a.gravity_y : 0.0f;
+    const float vel_unknown = -2147483648.0f;  // (float)INT_MIN, utility.cuh:56
+    for (int l = threadIdx.x; l < LX * LY * LZ; l += blockDim.x) {
+        int li = l % LX, lj = (l / LX) % LY, lk = l / (LX * LY);
+        int i = x0 + li, j = y0 + lj, k = z0 + lk;
+        if (i > hix || j > hiy || k > hiz) continue;
+        float num = snum[l], den = sden[l];
+        float val = (den > 1e-8f) ? num / den : 0.0f;
+        size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
+        gs[idx] = val;
+        g[idx] = (val > vel_unknown) ? val + a.dt * acc : val;
+    }
+}
+
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
+    constexpr int NU = (TX + 1) * TY * TZ, NV = TX * (TY + 1) * TZ, NW = TX * TY * (TZ + 1);
+    constexpr int MAXROWS = (TY + 2) * (TZ + 2);
+    __shared__ float s_un[NU], s_ud[NU], s_vn[NV], s_vd[NV], s_wn[NW], s_wd[NW];
+    __shared__ int s_pref[MAXROWS + 1];
+    __shared__ int s_beg[MAXROWS];
+
+    const int ntx = (a.W + TX - 1) / TX, nty = (a.H + TY - 1) / TY;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
+    const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+
+    for (int l = threadIdx.x; l < NU; l += blockDim.x) { s_un[l] = 0.f; s_ud[l] = 0.f; }
+    for (int l = threadIdx.x; l < NV; l += blockDim.x) { s_vn[l] = 0.f; s_vd[l] = 0.f; }
+    for (int l = threadIdx.x; l < NW; l += blockDim.x) { s_wn[l] = 0.f; s_wd[l] = 0.f; }
+
+    const int hx0 = max(x0 - 1, 0), hx1 = min(x0 + TX, a.W - 1);
+    const int hy0 = max(y0 - 1, 0), hy1 = min(y0 + TY, a.H - 1);
+    const int hz0 = max(z0 - 1, 0), hz1 = min(z0 + TZ, a.D - 1);
+    const int nyr = hy1 - hy0 + 1, nzr = hz1 - hz0 + 1, nrows = nyr * nzr;
+    int cnt = 0;
+    if ((int)threadIdx.x < nrows) {
+        int r = threadIdx.x;
+        int cy = hy0 + r % nyr, cz = hz0 + r / nyr;
+        long long rowc = (long long)cy * a.W + (long long)cz * a.W * a.H;
+        int beg = a.start[rowc + hx0], end = a.start[rowc + hx1 + 1];
+        s_beg[r] = beg;
+        cnt = end - beg;
+    }
+    // MAXROWS <= blockDim: one row per thread, block-wide exclusive scan of counts
+    __shared__ int s_scan[4];
+    int tot;
+    int pre = fsk::block_excl_scan(cnt, s_scan, &tot);
+    if ((int)threadIdx.x < nrows) s_pref[threadIdx.x] = pre;
+    if (threadIdx.x == 0) s_pref[nrows] = tot;
+    __syncthreads();
+
+    const float dx = a.dx;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        int lo = 0, hi = nrows - 1;
+        while (lo < hi) {  // largest r with s_pref[r] <= t
+            int mid = (lo + hi + 1) >> 1;
+            if (s_pref[mid] <= t) lo = mid; else hi = mid - 1;
+        }
+        int p = s_beg[lo] + (t - s_pref[lo]);
+        float px = a.P.px[p], py = a.P.py[p], pz = a.P.pz[p];
+        float rx = px / dx, ry = py / dx, rz = pz / dx;
+        p2g_scatter_comp<0, TX, TY, TZ>(rx, ry, rz, a.P.vx[p], a, x0, y0, z0, s_un, s_ud);
+        p2g_scatter_comp<1, TX, TY, TZ>(rx, ry, rz, a.P.vy[p], a, x0, y0, z0, s_vn, s_vd);
+        p2g_scatter_comp<2, TX, TY, TZ>(rx, ry, rz, a.P.vz[p], a, x0, y0, z0, s_wn, s_wd);
+    }
+    __syncthreads();
+    p2g_write_comp<0, TX, TY, TZ>(a, x0, y0, z0, s_un, s_ud);
+    p2g_write_comp<1, TX, TY, TZ>(a, x0, y0, z0, s_vn, s_vd);
+    p2g_write_comp<2, TX, TY, TZ>(a, x0, y0, z0, s_wn, s_wd);
+}
+
+// ---------------------------------------------------------------------------
+// G2P + advection, one thread per particle.
+
+// PIC and FLIP samples of one component: du = u - uSaved per corner (:990-1012)
+template <int COMP>
+FS_DEV void sample_pic_flip(const float* __restrict__ f, const float* __restrict__ fs, int n, const Base& b, int sx,
+                            int sxy, float& pic, float& flip) {
+    int i0 = b.i + b.j * sx + b.k * sxy;
+    int ix[8] = {i0, i0 + 1, i0 + sxy, i0 + 1 + sxy, i0 + sx, i0 + 1 + sx, i0 + sx + sxy, i0 + 1 + sx + sxy};
+    float c[8], d[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        int id = ix[q] < 0 ? 0 : (ix[q] >= n ? n - 1 : ix[q]);
+        c[q] = f[id];
+        d[q] = f[id] - fs[id];
+    }
+    pic = trilin(b.fx, b.fy, b.fz, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]);
+    flip = trilin(b.fx, b.fy, b.fz, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
+}
+
+FS_DEV f3 interp_vel(const G2PArgs& a, f3 pos) {
+    return interp_vel3(a.u, a.v, a.w, a.nu, a.nv, a.nw, a.W, a.H, a.D, a.dx, pos);
+}
+
+__constant__ signed char c_proj_off[34][3] = {
+    {1, 0, 0},    {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},   {0, 0, 1},  {0, 0, -1},   {1, 1, 0},
+    {-1, 1, 0},   {1, -1, 0},  {-1, -1, 0}, {1, 0, 1},    {-1, 0, 1}, {1, 0, -1},   {-1, 0, -1},
+    {0, 1, 1},    {0, -1, 1},  {0, 1, -1},  {0, -1, -1},  {1, 1, 1},  {-1, 1, 1},   {1, -1, 1},
+    {1, 1, -1},   {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1},  {-1, -1, -1}, {-2, -1, 0}, {0, -1, -2},
+    {-2, -2, 0},  {0, -2, -2}, {-2, -2, -2}, {-2, 0, -2}, {2, -1, 0},  {0, -1, 2}};
+
+// projectParticleDevice3D :1161-1239 (FLUID pass only, as the reference's loop runs once)
+FS_DEV bool project_particle(f3& pos, const uint8_t* __restrict__ labels, int W, int H, int D, float dx) {
+    int cx = (int)floorf(pos.x / dx), cy = (int)floorf(pos.y / dx), cz = (int)floorf(pos.z / dx);
+    float bestD = 1e10f;
+    int best = -1;
+    for (int n = 0; n < 34; ++n) {
+        int nx = cx + c_proj_off[n][0], ny = cy + c_proj_off[n][1], nz = cz + c_proj_off[n][2];
+        if (nx < 0 || nx >= W || ny < 0 || ny >= H || nz < 0 || nz >= D) continue;
+        if (labels[(size_t)nx + (size_t)ny * W + (size_t)nz * W * H] != FS_FLUID) continue;
+        f3 cc = mk((nx + 0.5f) * dx, (ny + 0.5f) * dx, (nz + 0.5f) * dx);
+        float d = (cc.x - pos.x) * (cc.x - pos.x) + (cc.y - pos.y) * (cc.y - pos.y) + (cc.z - pos.z) * (cc.z - pos.z);
+        if (d < bestD) { bestD = d; best = n; }
+    }
+    if (best < 0) return false;
+    pos = pos + mk((float)c_proj_off[best][0], (float)c_proj_off[best][1], (float)c_proj_off[best][2]) * 0.5f * dx;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = i < a.n;
+    int key = -1;
+    if (act) {
+        const int W = a.W, H = a.H, D = a.D;
+        const float dx = a.dx;
+        f3 pos = mk(a.P.px[i], a.P.py[i], a.P.pz[i]);
+        f3 vel = mk(a.P.vx[i], a.P.vy[i], a.P.vz[i]);
+        // ---- G2P (GridToParticleFunctor :822-979)
+        float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
+        Base bu = mac_base<0>(rx, ry, rz, W, H, D);
+        Base bv = mac_base<1>(rx, ry, rz, W, H, D);
+        Base bw = mac_base<2>(rx, ry, rz, W, H, D);
+        float uP, uF, vP, vF, wP, wF;
+        sample_pic_flip<0>(a.u, a.us, a.nu, bu, W + 1, (W + 1) * H, uP, uF);
+        sample_pic_flip<1>(a.v, a.vs, a.nv, bv, W, W * (H + 1), vP, vF);
+        sample_pic_flip<2>(a.w, a.ws, a.nw, bw, W, W * H, wP, wF);
+        const float al = a.alpha;
+        vel.x = al * uP + (1.0f - al) * (vel.x + uF);
+        vel.y = al * vP + (1.0f - al) * (vel.y + vF);
+        vel.z = al * wP + (1.0f - al) * (vel.z + wF);
+        // ---- advection (AdvectParticlesFunctor :1260-1330)
+        const float dt = a.dt, C = a.C;
+        const float vfloor = sqrtf(5.0f * dx * 9.8f);
+        const float xmax = (W - 1) * dx, ymax = (H - 1) * dx, zmax = (D - 1) * dx;
+        float subT = 0.0f;
+        bool finished = false;
+        int iter = 0;
+        while (!finished && iter++ < 1000) {
+            f3 k1 = interp_vel(a, pos);
+            float speed = sqrtf(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z) + 1e-37f + vfloor;
+            float dT = (C * dx) / speed;
+            if (subT + dT >= dt) {
+                dT = dt - subT;
+                finished = true;
+            } else if (subT + 2 * dT >= dt) {
+                dT *= 0.5f;
+            }
+            f3 k2 = interp_vel(a, pos + k1 * 0.5f * dT);
+            f3 k3 = interp_vel(a, pos + k2 * 0.75f * dT);
+            pos = pos + (2.0f / 9.0f) * dT * k1 + (3.0f / 9.0f) * dT * k2 + (4.0f / 9.0f) * dT * k3;
+            subT += dT;
+            pos.x = fmaxf(pos.x, 0.0f); pos.x = fminf(pos.x, xmax);
+            pos.y = fmaxf(pos.y, 0.0f); pos.y = fminf(pos.y, ymax);
+            pos.z = fmaxf(pos.z, 0.0f); pos.z = fminf(pos.z, zmax);
+            int cx = (int)floorf(pos.x / dx), cy = (int)floorf(pos.y / dx), cz = (int)floorf(pos.z / dx);
+            if (cx >= 0 && cx < W && cy >= 0 && cy < H && cz >= 0 && cz < D) {
+                uint8_t L = a.labels[(size_t)cx + (size_t)cy * W + (size_t)cz * W * H];
+                if (L == FS_SOLID || L == FS_BODY) {
+                    if (!project_particle(pos, a.labels, W, H, D, dx)) break;
+                }
+            }
+            if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || isnan(pos.x) || isnan(pos.y) || isnan(pos.z)) break;
+        }
+        a.P.px[i] = pos.x; a.P.py[i] = pos.y; a.P.pz[i] = pos.z;
+        a.P.vx[i] = vel.x; a.P.vy[i] = vel.y; a.P.vz[i] = vel.z;
+        if (a.next_count) key = cell_key(pos.x, pos.y, pos.z, dx, W, H, D);
+    }
+    if (a.next_count) {
+        int r = wave_agg_rank(key, act, a.next_count);
+        if (act) {
+            a.next_key[i] = key;
+            a.next_rank[i] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+void launch_key_count(const ParticlesView& P, int n, float dx, int W, int H, int D, int* key, int* rank, int* count,
+                      hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_key_count, dim3((n + 255) / 256), dim3(256), 0, st, P.px, P.py, P.pz, n, dx, W, H, D, key,
+                       rank, count);
+}
+
+int scan_blocks(long long C) { return (int)((C + SCAN_PER_BLOCK - 1) / SCAN_PER_BLOCK); }
+
+void launch_scan_label(int* count, const uint8_t* solid, uint8_t* labels, long long C, int* bsum, int* total,
+                       int* start, const LabelArgs& la, hipStream_t st) {
+    int nb = scan_blocks(C);
+    hipLaunchKernelGGL(k_scan_reduce<0>, dim3(nb), dim3(SCAN_T), 0, st, count, labels, C, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_final<0>, dim3(nb), dim3(SCAN_T), 0, st, count, solid, labels, C, bsum, start, la);
+}
+
+void launch_scan_fluid(const uint8_t* labels, long long C, int* bsum, int* total, int* start, hipStream_t st) {
+    int nb = scan_blocks(C);
+    LabelArgs la{};
+    hipLaunchKernelGGL(k_scan_reduce<1>, dim3(nb), dim3(SCAN_T), 0, st, nullptr, labels, C, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_final<1>, dim3(nb), dim3(SCAN_T), 0, st, nullptr, nullptr, const_cast<uint8_t*>(labels),
+                       C, bsum, start, la);
+}
+
+void launch_seed(const uint8_t* labels, const int* frank, long long C, int W, int H, float dx, int ppc, uint64_t seed,
+                 const ParticlesView& P, hipStream_t st) {
+    hipLaunchKernelGGL(k_seed, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, labels, frank, C, W, H, dx, ppc,
+                       seed, P);
+}
+
+void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, const int* key, const int* rank,
+                    const int* start, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n, key, rank, start);
+}
+
+void launch_p2g(const P2GArgs& a, hipStream_t st) {
+    constexpr int TX = P2G_TX, TY = P2G_TY, TZ = P2G_TZ;
+    static_assert((TY + 2) * (TZ + 2) <= 256, "one halo row per thread");
+    int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
+    hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(nb), dim3(256), 0, st, a);
+}
+
+void launch_g2p_advect(const G2PArgs& a, hipStream_t st) {
+    if (a.n <= 0) return;
+    hipLaunchKernelGGL(k_g2p_advect, dim3((a.n + 255) / 256), dim3(256), 0, st, a);
+}
+
+}  // namespace fsk

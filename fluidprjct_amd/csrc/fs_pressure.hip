@@ -1,0 +1,629 @@
+// fs_pressure.hip -- matrix-free MGPCG with FP64 dots and FP64 iterative
+// refinement for the reference's pressure system (see fs_pressure.hpp).
+//
+// Multigrid follows McAdams, Sifakis & Teran 2010 ("A parallel multigrid
+// Poisson solver for fluids simulation on large grids"): cell-centred levels
+// coarsened 2x per axis, a coarse cell is Dirichlet if any child is Dirichlet,
+// else an unknown if any child is an unknown, else inactive; the 7-point
+// operator is re-discretised on every level with unit coefficients; transfers
+// are cell-centred trilinear prolongation P and restriction R = P^T / 2 (the
+// factor 4 of the unit-coefficient coarse operator times the 1/8 average);
+// damped Jacobi (omega = 6/7) pre/post smoothing and a fixed number of Jacobi
+// sweeps on the coarsest level keep the V-cycle a symmetric positive definite
+// linear operator, as CG requires.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fs_kernels.hpp"
+#include "fs_pressure.hpp"
+
+using namespace fsd;
+
+namespace fsk {
+
+constexpr int RED_BLOCKS = 4096;
+constexpr uint32_t SELF = 0x8000u;
+
+enum { FIN_BNORM = 0, FIN_OUTER, FIN_RZ, FIN_PQ, FIN_RR };
+
+FS_DEV float stencil_Ax(const float* __restrict__ x, long long c, uint32_t code, int W, long long WH) {
+    float s = (float)((code >> 8) & 7u) * x[c];
+    if (code & 1u) s -= x[c + 1];
+    if (code & 2u) s -= x[c - 1];
+    if (code & 4u) s -= x[c + W];
+    if (code & 8u) s -= x[c - W];
+    if (code & 16u) s -= x[c + WH];
+    if (code & 32u) s -= x[c - WH];
+    return s;
+}
+
+FS_DEV double stencil_Ax64(const double* __restrict__ x, long long c, uint32_t code, int W, long long WH) {
+    double s = (double)((code >> 8) & 7u) * x[c];
+    if (code & 1u) s -= x[c + 1];
+    if (code & 2u) s -= x[c - 1];
+    if (code & 4u) s -= x[c + W];
+    if (code & 8u) s -= x[c - W];
+    if (code & 16u) s -= x[c + WH];
+    if (code & 32u) s -= x[c - WH];
+    return s;
+}
+
+// ---------------------------------------------------------------- level setup
+__global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ labels, uint8_t* __restrict__ type,
+                                               long long n) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    uint8_t L = labels[c];
+    type[c] = L == FS_FLUID ? 1 : (L == FS_AIR ? 2 : 0);
+}
+
+__global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Cc.n) return;
+    int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
+    int any1 = 0, any2 = 0;
+    for (int dk = 0; dk < 2; ++dk) {
+        int k = 2 * K + dk;
+        if (k >= F.D) continue;
+        for (int dj = 0; dj < 2; ++dj) {
+            int j = 2 * J + dj;
+            if (j >= F.H) continue;
+            for (int di = 0; di < 2; ++di) {
+                int i = 2 * I + di;
+                if (i >= F.W) continue;
+                uint8_t t = F.type[(size_t)i + (size_t)j * F.W + (size_t)k * F.W * F.H];
+                any1 |= (t == 1);
+                any2 |= (t == 2);
+            }
+        }
+    }
+    Cc.type[c] = any2 ? 2 : (any1 ? 1 : 0);
+}
+
+__global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ counters) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int unk = 0, dir = 0;
+    if (c < L.n) {
+        int i = (int)(c % L.W), j = (int)((c / L.W) % L.H), k = (int)(c / ((long long)L.W * L.H));
+        uint32_t code = 0;
+        if (L.type[c] == 1) {
+            const long long WH = (long long)L.W * L.H;
+            int nbr[6] = {i + 1 < L.W ? 1 : 0, i > 0 ? 1 : 0, j + 1 < L.H ? 1 : 0, j > 0 ? 1 : 0, k + 1 < L.D ? 1 : 0,
+                          k > 0 ? 1 : 0};
+            long long off[6] = {1, -1, L.W, -(long long)L.W, WH, -WH};
+            uint32_t diag = 0;
+            for (int q = 0; q < 6; ++q) {
+                if (!nbr[q]) continue;
+                uint8_t t = L.type[c + off[q]];
+                if (t == 1) { code |= (1u << q); ++diag; }
+                else if (t == 2) { ++diag; dir = 1; }
+            }
+            code |= SELF | (diag << 8);
+            unk = 1;
+        }
+        L.code[c] = (uint16_t)code;
+    }
+    unk = wave_sum(unk);
+    dir = wave_sum(dir);
+    if ((threadIdx.x & 63) == 0) {
+        if (unk) atomicAdd(&counters[0], unk);
+        if (dir) atomicOr(&counters[1], 1);
+    }
+}
+
+// ---------------------------------------------------------------- smoothing
+template <bool FIRST, bool DOT>
+__global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restrict__ b, const float* __restrict__ xin,
+                                                float* __restrict__ xout, float omega, double* __restrict__ partial,
+                                                const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[4];
+    const long long WH = (long long)L.W * L.H;
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        uint32_t code = L.code[c];
+        float xn = 0.0f;
+        if (code & SELF) {
+            uint32_t d = (code >> 8) & 7u;
+            if (d) {
+                float bc = b[c];
+                if (FIRST) xn = omega * bc / (float)d;
+                else xn = xin[c] + omega * (bc - stencil_Ax(xin, c, code, L.W, WH)) / (float)d;
+                if (DOT) acc += (double)bc * (double)xn;
+            }
+        }
+        xout[c] = xn;
+    }
+    if (DOT) {
+        double t = block_sum(acc, sm);
+        if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_residual(MGLevel L, const float* __restrict__ b, const float* __restrict__ x,
+                                                  float* __restrict__ r, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L.n) return;
+    uint32_t code = L.code[c];
+    r[c] = (code & SELF) ? b[c] - stencil_Ax(x, c, code, L.W, (long long)L.W * L.H) : 0.0f;
+}
+
+// coarse b = 1/2 * P^T r (trilinear weights 1/4,3/4,3/4,1/4 per axis)
+__global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const float* __restrict__ rf,
+                                                  float* __restrict__ bc, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Cc.n) return;
+    if (!(Cc.code[c] & SELF)) { bc[c] = 0.0f; return; }
+    int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
+    const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+    float s = 0.0f;
+    for (int dk = 0; dk < 4; ++dk) {
+        int k = 2 * K - 1 + dk;
+        if (k < 0 || k >= F.D) continue;
+        for (int dj = 0; dj < 4; ++dj) {
+            int j = 2 * J - 1 + dj;
+            if (j < 0 || j >= F.H) continue;
+            float wjk = wt[dj] * wt[dk];
+            const float* row = rf + (size_t)j * F.W + (size_t)k * F.W * F.H;
+            for (int di = 0; di < 4; ++di) {
+                int i = 2 * I - 1 + di;
+                if (i < 0 || i >= F.W) continue;
+                s += wt[di] * wjk * row[i];
+            }
+        }
+    }
+    bc[c] = 0.5f * s;
+}
+
+// fine x += P xc
+__global__ __launch_bounds__(256) void k_prolong(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
+                                                 float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= F.n) return;
+    if (!(F.code[c] & SELF)) return;
+    int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / ((long long)F.W * F.H));
+    int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+    int Ia = (i & 1) ? I0 + 1 : I0 - 1, Ja = (j & 1) ? J0 + 1 : J0 - 1, Ka = (k & 1) ? K0 + 1 : K0 - 1;
+    int Is[2] = {I0, Ia}, Js[2] = {J0, Ja}, Ks[2] = {K0, Ka};
+    const float w[2] = {0.75f, 0.25f};
+    float s = 0.0f;
+    for (int a = 0; a < 2; ++a) {
+        if (Ks[a] < 0 || Ks[a] >= Cc.D) continue;
+        for (int bb = 0; bb < 2; ++bb) {
+            if (Js[bb] < 0 || Js[bb] >= Cc.H) continue;
+            for (int cc = 0; cc < 2; ++cc) {
+                if (Is[cc] < 0 || Is[cc] >= Cc.W) continue;
+                s += w[cc] * w[bb] * w[a] * xc[(size_t)Is[cc] + (size_t)Js[bb] * Cc.W + (size_t)Ks[a] * Cc.W * Cc.H];
+            }
+        }
+    }
+    xf[c] += s;
+}
+
+// coarsest level: fixed number of damped-Jacobi sweeps from zero in LDS
+constexpr int COARSE_MAX = 4096;
+__global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* __restrict__ b, float* __restrict__ x,
+                                                       int sweeps, float omega, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ float sx[2][COARSE_MAX];
+    __shared__ float sb[COARSE_MAX];
+    __shared__ uint16_t scode[COARSE_MAX];
+    const int n = (int)L.n;
+    const long long WH = (long long)L.W * L.H;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        sb[c] = b[c];
+        scode[c] = L.code[c];
+        sx[0][c] = 0.0f;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int s = 0; s < sweeps; ++s) {
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            uint32_t code = scode[c];
+            float xn = 0.0f;
+            if (code & SELF) {
+                uint32_t d = (code >> 8) & 7u;
+                if (d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) / (float)d;
+            }
+            sx[cur ^ 1][c] = xn;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = sx[cur][c];
+}
+
+// ---------------------------------------------------------------- CG kernels
+// p_new = z + beta p_old ; q = A p_new ; partial p_new.q
+__global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __restrict__ z,
+                                                      const float* __restrict__ pold, float* __restrict__ pnew,
+                                                      float* __restrict__ q, double* __restrict__ partial,
+                                                      const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    const float beta = (float)sc->beta;
+    const long long WH = (long long)L.W * L.H;
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        uint32_t code = L.code[c];
+        float pn = 0.0f, qc = 0.0f;
+        if (code & SELF) {
+            pn = z[c] + beta * pold[c];
+            qc = (float)((code >> 8) & 7u) * pn;
+            if (code & 1u) qc -= z[c + 1] + beta * pold[c + 1];
+            if (code & 2u) qc -= z[c - 1] + beta * pold[c - 1];
+            if (code & 4u) qc -= z[c + L.W] + beta * pold[c + L.W];
+            if (code & 8u) qc -= z[c - L.W] + beta * pold[c - L.W];
+            if (code & 16u) qc -= z[c + WH] + beta * pold[c + WH];
+            if (code & 32u) qc -= z[c - WH] + beta * pold[c - WH];
+            acc += (double)pn * (double)qc;
+        }
+        pnew[c] = pn;
+        q[c] = qc;
+    }
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// e += alpha p ; r -= alpha q ; partial r.r
+__global__ __launch_bounds__(256) void k_update(long long n, const float* __restrict__ p, const float* __restrict__ q,
+                                                float* __restrict__ e, float* __restrict__ r,
+                                                double* __restrict__ partial, const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    const float alpha = (float)sc->alpha;
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        float rc = r[c] - alpha * q[c];
+        e[c] += alpha * p[c];
+        r[c] = rc;
+        acc += (double)rc * (double)rc;
+    }
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// sum of b^2 and b over unknowns
+__global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restrict__ b, double* __restrict__ p1,
+                                               double* __restrict__ p2) {
+    __shared__ double sm[4];
+    double a1 = 0.0, a2 = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        if (L.code[c] & SELF) {
+            double v = b[c];
+            a1 += v * v;
+            a2 += v;
+        }
+    }
+    double t1 = block_sum(a1, sm);
+    double t2 = block_sum(a2, sm);
+    if (threadIdx.x == 0) { p1[blockIdx.x] = t1; p2[blockIdx.x] = t2; }
+}
+
+// x64 += e ; e = 0
+__global__ __launch_bounds__(256) void k_axpy64(long long n, double* __restrict__ x64, float* __restrict__ e) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    x64[c] += (double)e[c];
+    e[c] = 0.0f;
+}
+
+// r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partial r.r (FP64)
+__global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, const double* __restrict__ x64,
+                                                 double shift, float* __restrict__ r, double* __restrict__ partial) {
+    __shared__ double sm[4];
+    const long long WH = (long long)L.W * L.H;
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        uint32_t code = L.code[c];
+        float rf = 0.0f;
+        if (code & SELF) {
+            double rv = ((double)b[c] - shift) - stencil_Ax64(x64, c, code, L.W, WH);
+            acc += rv * rv;
+            rf = (float)rv;
+        }
+        r[c] = rf;
+    }
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64, float* __restrict__ e) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    x64[c] = 0.0;
+    e[c] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_to_float(long long n, const uint8_t* __restrict__ labels,
+                                                  const double* __restrict__ x64, float* __restrict__ p) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    p[c] = labels[c] == FS_FLUID ? (float)x64[c] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_finalize(int mode, const double* __restrict__ p1,
+                                                  const double* __restrict__ p2, int np, SolveScalars* sc,
+                                                  double tol2b) {
+    __shared__ double sm[4];
+    if (mode != FIN_BNORM && mode != FIN_OUTER && sc->done) return;
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+        a1 += p1[i];
+        if (p2) a2 += p2[i];
+    }
+    double s1 = block_sum(a1, sm);
+    double s2 = block_sum(a2, sm);
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+        case FIN_BNORM:
+            sc->bnorm2 = s1;
+            sc->bsum = s2;
+            break;
+        case FIN_OUTER:
+            sc->rr_true = s1;
+            sc->tol2b = tol2b;
+            if (s1 <= tol2b) {
+                sc->done = 2;
+            } else {
+                sc->done = 0;
+                sc->first = 1;
+                sc->rr = s1;
+                sc->rr_start = s1;
+                sc->target = fmax(sc->inner_rtol2 * s1, 0.25 * tol2b);
+            }
+            break;
+        case FIN_RZ:
+            if (!(s1 > 0.0) || isnan(s1)) { sc->done = 3; break; }
+            sc->beta = sc->first ? 0.0 : s1 / sc->rz;
+            sc->rz = s1;
+            break;
+        case FIN_PQ:
+            if (!(s1 > 0.0) || isnan(s1)) { sc->done = 3; break; }
+            sc->alpha = sc->rz / s1;
+            break;
+        case FIN_RR:
+            sc->rr = s1;
+            sc->iters += 1;
+            sc->first = 0;
+            if (s1 <= sc->target) sc->done = 1;
+            else if (sc->iters >= sc->maxit) sc->done = 4;
+            break;
+    }
+}
+
+// ---------------------------------------------------------------- host side
+static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+
+MGPCG::~MGPCG() { free_all(); }
+
+void MGPCG::free_all() {
+    for (auto& L : lv) {
+        hipFree(L.code); hipFree(L.type);
+        if (&L != &lv[0]) { hipFree(L.b); hipFree(L.x); }
+        hipFree(L.xt); hipFree(L.r);
+    }
+    lv.clear();
+    hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(e);
+    hipFree(partial); hipFree(partial2); hipFree(sc);
+    if (sc_host) hipHostFree(sc_host);
+    r = z = pa = pb = q = e = nullptr;
+    partial = partial2 = nullptr;
+    sc = nullptr;
+    sc_host = nullptr;
+}
+
+int MGPCG::init(int W_, int H_, int D_) {
+    free_all();
+    W = W_; H = H_; D = D_;
+    N = (long long)W * H * D;
+    int w = W, h = H, d = D;
+    while (true) {
+        MGLevel L{};
+        L.W = w; L.H = h; L.D = d;
+        L.n = (long long)w * h * d;
+        lv.push_back(L);
+        if (L.n <= coarse_max) break;
+        if (w == 1 && h == 1 && d == 1) break;
+        w = (w + 1) / 2; h = (h + 1) / 2; d = (d + 1) / 2;
+    }
+    bool ok = true;
+    auto A = [&](void** p, size_t bytes) { if (hipMalloc(p, bytes) != hipSuccess) ok = false; };
+    for (size_t l = 0; l < lv.size(); ++l) {
+        MGLevel& L = lv[l];
+        A((void**)&L.code, sizeof(uint16_t) * L.n);
+        A((void**)&L.type, L.n);
+        if (l > 0) {
+            A((void**)&L.b, sizeof(float) * L.n);
+            A((void**)&L.x, sizeof(float) * L.n);
+        }
+        A((void**)&L.xt, sizeof(float) * L.n);
+        A((void**)&L.r, sizeof(float) * L.n);
+    }
+    A((void**)&r, sizeof(float) * N);
+    A((void**)&z, sizeof(float) * N);
+    A((void**)&pa, sizeof(float) * N);
+    A((void**)&pb, sizeof(float) * N);
+    A((void**)&q, sizeof(float) * N);
+    A((void**)&e, sizeof(float) * N);
+    A((void**)&partial, sizeof(double) * RED_BLOCKS);
+    A((void**)&partial2, sizeof(double) * RED_BLOCKS);
+    A((void**)&sc, sizeof(SolveScalars) * 2);
+    if (hipHostMalloc((void**)&sc_host, sizeof(SolveScalars)) != hipSuccess) ok = false;
+    nred = (int)std::min<long long>(RED_BLOCKS, (N + 255) / 256);
+    if (!ok) { free_all(); return -1; }
+    // CG direction buffers start finite: the first update multiplies them by beta = 0
+    hipMemset(pa, 0, sizeof(float) * N);
+    hipMemset(pb, 0, sizeof(float) * N);
+    hipMemset(e, 0, sizeof(float) * N);
+    // the coarsest level must fit the single-block LDS solve
+    if (lv.back().n > COARSE_MAX) { free_all(); return -2; }
+    return 0;
+}
+
+void MGPCG::setup(const uint8_t* labels, hipStream_t st) {
+    int* counters = reinterpret_cast<int*>(sc + 1);  // scratch: second SolveScalars slot
+    hipMemsetAsync(counters, 0, 2 * sizeof(int), st);
+    hipLaunchKernelGGL(k_type0, dim3(nblk(N)), dim3(256), 0, st, labels, lv[0].type, N);
+    for (size_t l = 0; l + 1 < lv.size(); ++l)
+        hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
+    for (size_t l = 0; l < lv.size(); ++l) {
+        int* cnt = (l == 0) ? counters : reinterpret_cast<int*>(sc + 1) + 2;  // coarse counts are not used
+        hipLaunchKernelGGL(k_code, dim3(nblk(lv[l].n)), dim3(256), 0, st, lv[l], cnt);
+    }
+}
+
+void MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, const float* dot_with, SolveScalars* s) {
+    MGLevel& L = lv[l];
+    if (l == (int)lv.size() - 1) {
+        hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st, L, L.b, L.x, coarse_sweeps, omega, s);
+        if (l == 0 && pdot) {
+            // degenerate single-level hierarchy: recompute the r.z partials
+            hipLaunchKernelGGL((k_jacobi<false, true>), dim3(nred), dim3(256), 0, st, L, L.b, L.x, L.xt, 0.0f, pdot,
+                               s);
+            hipMemcpyAsync(L.x, L.xt, sizeof(float) * L.n, hipMemcpyDeviceToDevice, st);
+        }
+        return;
+    }
+    const unsigned nb = nblk(L.n);
+    // pre-smoothing from x = 0
+    float* cur = L.x;
+    float* oth = L.xt;
+    hipLaunchKernelGGL((k_jacobi<true, false>), dim3(nb), dim3(256), 0, st, L, L.b, (const float*)nullptr, cur, omega,
+                       (double*)nullptr, s);
+    for (int k = 1; k < nu_pre; ++k) {
+        hipLaunchKernelGGL((k_jacobi<false, false>), dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega,
+                           (double*)nullptr, s);
+        std::swap(cur, oth);
+    }
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
+    MGLevel& Cn = lv[l + 1];
+    hipLaunchKernelGGL(k_restrict, dim3(nblk(Cn.n)), dim3(256), 0, st, L, Cn, L.r, Cn.b, s);
+    vcycle_level(l + 1, st, nullptr, nullptr, s);
+    hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, Cn.x, cur, s);
+    for (int k = 0; k < nu_post; ++k) {
+        bool last = (k == nu_post - 1);
+        if (last && pdot) {
+            hipLaunchKernelGGL((k_jacobi<false, true>), dim3(nred), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
+        } else {
+            hipLaunchKernelGGL((k_jacobi<false, false>), dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega,
+                               (double*)nullptr, s);
+        }
+        std::swap(cur, oth);
+    }
+    if (cur != L.x) hipMemcpyAsync(L.x, cur, sizeof(float) * L.n, hipMemcpyDeviceToDevice, st);
+}
+
+void MGPCG::vcycle(const float* rin, float* zout, double* pdot, SolveScalars* s, hipStream_t st) {
+    MGLevel& L0 = lv[0];
+    L0.b = const_cast<float*>(rin);
+    L0.x = zout;
+    vcycle_level(0, st, pdot, rin, s);
+}
+
+void MGPCG::apply_A(const float* x, float* y, hipStream_t st) {
+    MGLevel& L = lv[0];
+    hipLaunchKernelGGL(k_residual, dim3(nblk(L.n)), dim3(256), 0, st, L, (const float*)nullptr, x, y,
+                       (const SolveScalars*)nullptr);
+}
+
+class JacobiPre : public Preconditioner {
+public:
+    explicit JacobiPre(MGPCG* s) : s_(s) {}
+    void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) override {
+        MGLevel L = s_->lv[0];
+        hipLaunchKernelGGL((k_jacobi<true, true>), dim3(s_->nred), dim3(256), 0, st, L, r, (const float*)nullptr, z,
+                           1.0f, partial, sc);
+    }
+
+private:
+    MGPCG* s_;
+};
+
+Preconditioner* make_jacobi_preconditioner(MGPCG* s) { return new JacobiPre(s); }
+
+int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
+                 Preconditioner* pre, hipStream_t st, fs_solve_stats* stats) {
+    MGPreconditioner mgp(this);
+    if (!pre) pre = &mgp;
+    MGLevel& L0 = lv[0];
+    setup(labels, st);
+    // norms of b over unknowns, counters
+    SolveScalars init{};
+    init.maxit = maxit;
+    init.inner_rtol2 = inner_rtol * inner_rtol;
+    hipMemcpyAsync(sc, &init, sizeof(SolveScalars), hipMemcpyHostToDevice, st);
+    hipLaunchKernelGGL(k_bnorm, dim3(nred), dim3(256), 0, st, L0, b, partial, partial2);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_BNORM, partial, partial2, nred, sc, 0.0);
+    int counters[2];
+    hipMemcpyAsync(counters, reinterpret_cast<int*>(sc + 1), sizeof(counters), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return -5;
+    const int nunk = counters[0];
+    const bool singular = (nunk > 0) && !counters[1];
+    fs_solve_stats loc{};
+    loc.fluid_cells = nunk;
+    loc.singular = singular ? 1 : 0;
+    hipLaunchKernelGGL(k_fill0, dim3(nblk(N)), dim3(256), 0, st, N, x64, e);
+    if (nunk == 0) {
+        hipLaunchKernelGGL(k_to_float, dim3(nblk(N)), dim3(256), 0, st, N, labels, x64, p);
+        if (stats) *stats = loc;
+        return -3;
+    }
+    double shift = singular ? sc_host->bsum / nunk : 0.0;
+    double bn2 = singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
+    double tol2b = tol * tol * bn2;
+    int total_iters = 0, outer = 0;
+    bool converged = false;
+    double rr_true = 0.0;
+    for (outer = 0;; ++outer) {
+        if (outer > 0) hipLaunchKernelGGL(k_axpy64, dim3(nblk(N)), dim3(256), 0, st, N, x64, e);
+        hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, b, x64, shift, r, partial);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, partial, (const double*)nullptr,
+                           nred, sc, tol2b);
+        hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
+        if (hipStreamSynchronize(st) != hipSuccess) return -5;
+        rr_true = sc_host->rr_true;
+        total_iters = sc_host->iters;
+        if (sc_host->done == 2 || bn2 == 0.0) { converged = true; break; }
+        if (outer >= max_outer || total_iters >= maxit) break;
+        float* pold = pa;
+        float* pnew = pb;
+        while (true) {
+            for (int it = 0; it < batch; ++it) {
+                pre->apply(r, z, partial, sc, st);
+                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_RZ, partial,
+                                   (const double*)nullptr, nred, sc, 0.0);
+                hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, partial, sc);
+                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_PQ, partial,
+                                   (const double*)nullptr, nred, sc, 0.0);
+                hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, N, pnew, q, e, r, partial, sc);
+                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_RR, partial,
+                                   (const double*)nullptr, nred, sc, 0.0);
+                std::swap(pold, pnew);
+            }
+            hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) return -5;
+            if (sc_host->done) break;
+        }
+    }
+    hipLaunchKernelGGL(k_to_float, dim3(nblk(N)), dim3(256), 0, st, N, labels, x64, p);
+    last_iters = total_iters;
+    last_outer = outer + 1;
+    loc.iterations = total_iters;
+    loc.outer = outer + 1;
+    loc.converged = converged ? 1 : 0;
+    loc.rel_residual = bn2 > 0 ? std::sqrt(rr_true / bn2) : 0.0;
+    if (stats) *stats = loc;
+    return converged ? 0 : -4;
+}
+
+}  // namespace fsk

@@ -1,0 +1,97 @@
+// fs_pressure.hpp -- matrix-free multigrid-preconditioned CG for A p = b.
+//
+// A is the reference's pressure matrix (constructA FluidSolver3D.cu:1718-1777):
+// on FLUID cells, diag = #in-domain FLUID/AIR neighbours, -1 per FLUID
+// neighbour, SOLID/BODY neighbours Neumann, AIR Dirichlet (p = 0).  It replaces
+// the cuDSS FP32 direct solve (:1930-2008).
+//
+// Precision: stencil, smoother and transfers in FP32; dot products in FP64;
+// the solution is accumulated in FP64 and the residual b - A x is recomputed in
+// FP64 by an outer iterative-refinement loop until ||r|| <= tol ||b||.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace fsk {
+
+struct SolveScalars {
+    double rz, alpha, beta, rr, target, rr_true, bnorm2, bsum, tol2b, inner_rtol2, rr_start;
+    double zsum_unused;
+    int done, iters, first, maxit, n_unknown, dirichlet, stall, pad;
+};
+
+struct MGLevel {
+    int W, H, D;
+    long long n;
+    uint16_t* code;   // bits 0-5 neighbour unknown (+x,-x,+y,-y,+z,-z), 8-10 diag, 15 self unknown
+    uint8_t* type;    // 0 none, 1 unknown, 2 Dirichlet
+    float* b;         // right-hand side (level 0: aliases the CG residual)
+    float* x;         // current iterate
+    float* xt;        // Jacobi ping-pong partner
+    float* r;         // residual scratch
+};
+
+// A preconditioner that maps r -> z on the fine grid.  `apply` must leave z = 0
+// on non-unknown cells and add the partial sums of r.z into `partial`.
+struct Preconditioner {
+    virtual ~Preconditioner() {}
+    virtual void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) = 0;
+};
+
+class MGPCG {
+public:
+    MGPCG() {}
+    ~MGPCG();
+    // allocate the hierarchy for a W x H x D grid
+    int init(int W, int H, int D);
+    // Solve A x = b (b: full-grid float, 0 off fluid).  x64 (full grid, double)
+    // and p (float, 0 off fluid) are written.  Returns 0 on convergence.
+    int solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
+              Preconditioner* pre, hipStream_t st, struct fs_solve_stats* stats);
+    // Set up the level codes for `labels` (also used by the learned path).
+    void setup(const uint8_t* labels, hipStream_t st);
+    // One V-cycle z = M r on level 0, accumulating r.z partial sums.
+    void vcycle(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st);
+    // y = A x on the fine grid (FP32), for tests and the learned path.
+    void apply_A(const float* x, float* y, hipStream_t st);
+
+    int W = 0, H = 0, D = 0;
+    long long N = 0;
+    std::vector<MGLevel> lv;
+    int nu_pre = 2, nu_post = 2, coarse_sweeps = 64;
+    long long coarse_max = 512;   // coarsen until a level has at most this many cells
+    float omega = 6.0f / 7.0f;
+    int batch = 4;              // inner iterations launched between host checks
+    double inner_rtol = 1e-4;   // inner relative reduction per refinement pass
+    int max_outer = 12;
+    // fine-grid CG vectors
+    float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *e = nullptr;
+    double* partial = nullptr;
+    double* partial2 = nullptr;
+    SolveScalars* sc = nullptr;   // device
+    SolveScalars* sc_host = nullptr;  // pinned
+    int nred = 0;   // number of partial-sum slots used by reducing kernels
+    int last_iters = 0, last_outer = 0;
+
+private:
+    void free_all();
+    void vcycle_level(int l, hipStream_t st, double* partial_dot, const float* dot_with, SolveScalars* sc);
+};
+
+Preconditioner* make_jacobi_preconditioner(MGPCG* s);
+
+class MGPreconditioner : public Preconditioner {
+public:
+    explicit MGPreconditioner(MGPCG* s) : s_(s) {}
+    void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) override {
+        s_->vcycle(r, z, partial, sc, st);
+    }
+
+private:
+    MGPCG* s_;
+};
+
+}  // namespace fsk

@@ -1,0 +1,192 @@
+"""HIP path vs the CPU oracle, stage by stage and over whole steps.
+
+Bars:
+  - bit-exact: seeding, particle -> cell keys, labels, RHS, pressure
+    projection, G2P + advection (same inputs on both sides; both sides keep the
+    reference's FP32 operation order without contraction);
+  - P2G: relative 2e-6 of the face magnitude (FP32 LDS atomics vs the
+    oracle's FP64 sums, both rounded once);
+  - pressure: ||p - p_ref|| / ||p_ref|| <= 1e-5 against FP64 direct solves
+    (scipy spsolve golden vectors and the oracle's band LDL^T);
+  - whole steps: particle positions within 2e-5 * domain after 3 steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fluidprjct_amd._abi import (FS_GRID_CELLKEY, FS_GRID_LABELS, FS_GRID_P, FS_GRID_P64, FS_GRID_RHS, FS_GRID_U,
+                                 FS_GRID_USAVED, FS_GRID_V, FS_GRID_VSAVED, FS_GRID_W, FS_GRID_WSAVED)
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def tank(N, block=(0.25, 0.5, 1.0), shape=None):
+    D, H, W = shape or (N, N, N)
+    lab = np.full((D, H, W), 2, np.uint8)
+    lab[0], lab[-1] = 0, 0
+    lab[:, 0], lab[:, -1] = 0, 0
+    lab[:, :, 0], lab[:, :, -1] = 0, 0
+    bx, by, bz = block
+    lab[1:max(2, int(bz * D)) - (1 if bz >= 1 else 0), 1:max(2, int(by * H)), 1:max(2, int(bx * W))] = 1
+    return lab
+
+
+def pair(hip, oracle, **cfg):
+    base = dict(dt=1.0 / 120, dx=1.0 / 32, pic_weight=0.05, solver_tol=1e-6)
+    base.update(cfg)
+    return hip.solver(**base), oracle.solver(**base)
+
+
+def random_particles(rng, n, W, H, D, dx, margin=0.0):
+    lo = np.array([-margin, -margin, -margin]) * dx
+    hi = np.array([W + margin, H + margin, D + margin]) * dx
+    pos = (lo + (hi - lo) * rng.random((n, 3))).astype(np.float32)
+    vel = rng.standard_normal((n, 3)).astype(np.float32)
+    return pos, vel
+
+
+def test_seed_bitexact(hip, oracle):
+    lab = tank(20)
+    h, o = pair(hip, oracle, dx=0.05)
+    for s in (h, o):
+        s.set_labels(lab)
+        s.seed_particles(8, 1234)
+    ph, vh = h.particles()
+    po, vo = o.particles()
+    assert ph.shape == po.shape and ph.shape[0] == 8 * int((lab == 1).sum())
+    assert np.array_equal(ph.view(np.uint32), po.view(np.uint32))
+    assert not vh.any() and not vo.any()
+
+
+def test_keys_and_labels_bitexact(hip, oracle):
+    rng = np.random.default_rng(1)
+    W, H, D = 19, 23, 17
+    lab = np.full((D, H, W), 2, np.uint8)
+    lab[0], lab[:, :, 0], lab[:, -1] = 0, 0, 0
+    lab[5:9, 4:8, 6:10] = 0
+    dx = 0.037
+    pos, vel = random_particles(rng, 20000, W, H, D, dx, margin=0.3)
+    # particles exactly on cell faces and on the domain faces
+    pos[:500] = (rng.integers(0, 18, (500, 3)) * np.float32(dx)).astype(np.float32)
+    h, o = pair(hip, oracle, dx=dx)
+    for s in (h, o):
+        s.set_labels(lab)
+        s.set_particles(pos, vel)
+        s.stage("label")
+    assert np.array_equal(h.get_grid(FS_GRID_LABELS), o.get_grid(FS_GRID_LABELS))
+    assert np.array_equal(h.get_grid(FS_GRID_CELLKEY), o.get_grid(FS_GRID_CELLKEY))
+    # sorting must not lose or alter particles
+    ph, vh = h.particles()
+    assert np.array_equal(ph, pos) and np.array_equal(vh, vel)
+
+
+def test_p2g_matches_oracle(hip, oracle):
+    rng = np.random.default_rng(2)
+    W, H, D = 37, 21, 18   # not multiples of the LDS tile
+    dx = 0.05
+    lab = tank(0, shape=(D, H, W))
+    pos, vel = random_particles(rng, 60000, W, H, D, dx, margin=0.2)
+    h, o = pair(hip, oracle, dx=dx, dt=0.01)
+    for s in (h, o):
+        s.set_labels(lab)
+        s.set_particles(pos, vel)
+        s.stage("label")
+        s.stage("p2g")
+    for g in (FS_GRID_U, FS_GRID_V, FS_GRID_W, FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED):
+        a, b = h.get_grid(g), o.get_grid(g)
+        scale = max(1.0, float(np.abs(b).max()))
+        assert np.abs(a - b).max() <= 2e-6 * scale, (g, np.abs(a - b).max())
+
+
+def test_rhs_and_project_bitexact(hip, oracle):
+    rng = np.random.default_rng(3)
+    W, H, D = 16, 14, 12
+    lab = rng.choice(np.array([0, 1, 1, 1, 2], np.uint8), size=(D, H, W))
+    h, o = pair(hip, oracle, dx=0.1, dt=0.02)
+    shapes = {FS_GRID_U: (D, H, W + 1), FS_GRID_V: (D, H + 1, W), FS_GRID_W: (D + 1, H, W)}
+    fields = {g: rng.standard_normal(s).astype(np.float32) for g, s in shapes.items()}
+    p = (rng.standard_normal((D, H, W)) * 50).astype(np.float32) * (lab == 1)
+    for s in (h, o):
+        s.set_labels(lab)
+        for g, a in fields.items():
+            s.set_grid(g, a)
+        s.stage("rhs")
+    assert np.array_equal(h.get_grid(FS_GRID_RHS).view(np.uint32), o.get_grid(FS_GRID_RHS).view(np.uint32))
+    for s in (h, o):
+        s.set_grid(FS_GRID_P, p)
+        s.stage("project")
+    for g in shapes:
+        assert np.array_equal(h.get_grid(g).view(np.uint32), o.get_grid(g).view(np.uint32)), g
+
+
+def test_g2p_advect_bitexact(hip, oracle):
+    rng = np.random.default_rng(4)
+    N = 20
+    dx = 1.0 / N
+    lab = tank(N, block=(0.5, 0.5, 1.0))
+    lab[8:11, 3:6, 8:12] = 0   # an interior obstacle: exercises the projection
+    pos, vel = random_particles(rng, 30000, N - 2, N - 2, N - 2, dx)
+    pos += dx
+    h, o = pair(hip, oracle, dx=dx, dt=1.0 / 60, pic_weight=0.05)
+    shapes = {FS_GRID_U: (N, N, N + 1), FS_GRID_V: (N, N + 1, N), FS_GRID_W: (N + 1, N, N)}
+    f = {g: (rng.standard_normal(s) * 0.7).astype(np.float32) for g, s in shapes.items()}
+    fs = {g: (a + rng.standard_normal(a.shape).astype(np.float32) * 0.1) for g, a in f.items()}
+    saved = {FS_GRID_U: FS_GRID_USAVED, FS_GRID_V: FS_GRID_VSAVED, FS_GRID_W: FS_GRID_WSAVED}
+    for s in (h, o):
+        s.set_labels(lab)
+        s.set_particles(pos, vel)
+        for g in shapes:
+            s.set_grid(g, f[g])
+            s.set_grid(saved[g], fs[g])
+        s.stage("g2p_advect")
+    ph, vh = h.particles()
+    po, vo = o.particles()
+    assert np.array_equal(vh.view(np.uint32), vo.view(np.uint32))
+    bad = np.flatnonzero((ph.view(np.uint32) != po.view(np.uint32)).any(1))
+    assert bad.size == 0, (bad[:10], ph[bad[:3]], po[bad[:3]])
+
+
+@pytest.mark.parametrize("name", ["dam", "mix"])
+def test_pressure_vs_golden(hip, name):
+    g = np.load(os.path.join(GOLDEN, "pressure_golden.npz"))
+    lab, b, pref = g[name + "_labels"], g[name + "_b"], g[name + "_p"]
+    h = hip.solver()
+    p, st = h.solve_pressure_system(lab, b, 1e-6)
+    rel = np.linalg.norm(p - pref) / np.linalg.norm(pref)
+    assert st.converged == 1 and st.rel_residual <= 1e-6
+    assert rel <= 1e-5, rel
+    p, st = h.solve_pressure_system(lab, b, 1e-12)
+    assert np.linalg.norm(p - pref) / np.linalg.norm(pref) <= 1e-9
+
+
+def test_pressure_vs_oracle_direct(hip, oracle):
+    rng = np.random.default_rng(5)
+    N = 40
+    lab = tank(N, block=(0.6, 0.4, 1.0))
+    b = (rng.standard_normal(lab.shape) * 1e3).astype(np.float32) * (lab == 1)
+    o = oracle.solver()
+    po, so = o.solve_pressure_system(lab, b, 1e-13)
+    h = hip.solver()
+    ph, sh = h.solve_pressure_system(lab, b, 1e-6)
+    assert sh.converged == 1
+    assert np.linalg.norm(ph - po) / np.linalg.norm(po) <= 1e-5
+
+
+def test_full_steps_close(hip, oracle):
+    N = 32
+    lab = tank(N)
+    h, o = pair(hip, oracle, dx=1.0 / N, dt=1.0 / 120, solver_tol=1e-8)
+    for s in (h, o):
+        s.set_labels(lab)
+        s.seed_particles(8, 3)
+    for step in range(3):
+        for s in (h, o):
+            s.step(1)
+        ph, vh = h.particles()
+        po, vo = o.particles()
+        assert np.abs(ph - po).max() <= 2e-5, (step, np.abs(ph - po).max())
+        assert np.abs(vh - vo).max() <= 2e-3 * max(1.0, np.abs(vo).max()), (step, np.abs(vh - vo).max())
+    lh, lo = h.get_grid(FS_GRID_LABELS), o.get_grid(FS_GRID_LABELS)
+    assert (lh != lo).sum() <= 2
