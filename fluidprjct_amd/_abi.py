@@ -31,6 +31,7 @@ class FsConfig(C.Structure):
         ("particles_per_cell", C.c_int), ("seed", C.c_uint64), ("gravity_y", C.c_float),
         ("density", C.c_float), ("advect_c", C.c_float), ("solver_tol", C.c_double),
         ("solver_max_iter", C.c_int), ("precond", C.c_int), ("stall_window", C.c_int),
+        ("cfl", C.c_float),
     ]
 
 
@@ -91,6 +92,7 @@ _SIGS = {
     "fs_set_grid": (C.c_int, [_vp, C.c_int, _vp, C.c_int64]),
     "fs_get_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.c_int]),
     "fs_get_solve_stats": (C.c_int, [_vp, C.POINTER(FsSolveStats)]),
+    "fs_get_clock": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int64)]),
     "fs_write_particles": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "fs_solve_pressure_system": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double,
                                            C.POINTER(FsSolveStats)]),
@@ -262,6 +264,11 @@ class Solver:
         buf = (C.c_double * len(STAGES))()
         self.fl.check(self.lib.fs_get_stage_times(self.h, buf, len(STAGES)))
         return dict(zip(STAGES, list(buf)))
+
+    def clock(self):
+        t, dt, n = C.c_double(), C.c_float(), C.c_int64()
+        self.fl.check(self.lib.fs_get_clock(self.h, C.byref(t), C.byref(dt), C.byref(n)))
+        return t.value, dt.value, n.value
 
     def solve_stats(self) -> FsSolveStats:
         st = FsSolveStats()

@@ -138,6 +138,11 @@ struct Solver {
     std::unique_ptr<Preconditioner> jacobi;
     fs_solve_stats stats{};
     double times[FS_NUM_STAGES] = {0};
+    float dt_cur = 0.0f;        // dt of the current step
+    double sim_time = 0.0;
+    int64_t nsteps = 0;
+    unsigned* d_umax = nullptr; // max |v| of the particle velocities (float bits)
+    bool umax_valid = false;
 
     long long NU() const { return (long long)(W + 1) * H * D; }
     long long NV() const { return (long long)W * (H + 1) * D; }
@@ -148,6 +153,7 @@ struct Solver {
         free_particles();
         dfree(d_sdf);
         dfree(d_bacc);
+        dfree(d_umax);
         for (auto& e : ev) if (e) hipEventDestroy(e);
         if (st) hipStreamDestroy(st);
     }
@@ -217,7 +223,24 @@ struct Solver {
         size_t b = sizeof(int) * (size_t)std::max<long long>(n, 1);
         HIPCHK(hipMalloc(&d_key, b));
         HIPCHK(hipMalloc(&d_rank, b));
-        keys_valid = sorted_valid = false;
+        keys_valid = sorted_valid = umax_valid = false;
+        return FS_OK;
+    }
+
+    // CFL-limited dt (no reference counterpart): dt = min(dt_max, cfl*dx/max|v|)
+    int choose_dt() {
+        dt_cur = cfg.dt;
+        if (!(cfg.cfl > 0.0f) || np == 0) return FS_OK;
+        if (!umax_valid) {
+            HIPCHK(hipMemsetAsync(d_umax, 0, sizeof(unsigned), st));
+            launch_max_speed(PA, (int)np, d_umax, st);
+        }
+        unsigned bits = 0;
+        HIPCHK(hipMemcpyAsync(&bits, d_umax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        float umax;
+        std::memcpy(&umax, &bits, sizeof(float));
+        if (umax > 0.0f) dt_cur = std::fmin(cfg.dt, (cfg.cfl * cfg.dx) / umax);
         return FS_OK;
     }
 
@@ -234,7 +257,7 @@ struct Solver {
     GridArgs grid_args() {
         GridArgs g{};
         g.W = W; g.H = H; g.D = D;
-        g.dx = cfg.dx; g.dt = cfg.dt; g.density = cfg.density;
+        g.dx = cfg.dx; g.dt = dt_cur; g.density = cfg.density;
         g.labels = d_labels;
         g.u = d_u; g.v = d_v; g.w = d_w;
         g.body = body.params();
@@ -264,7 +287,7 @@ struct Solver {
         a.P = PA;
         a.start = d_start;
         a.W = W; a.H = H; a.D = D;
-        a.dx = cfg.dx; a.dt = cfg.dt; a.gravity_y = cfg.gravity_y;
+        a.dx = cfg.dx; a.dt = dt_cur; a.gravity_y = cfg.gravity_y;
         a.u = d_u; a.v = d_v; a.w = d_w; a.us = d_us; a.vs = d_vs; a.ws = d_ws;
         launch_p2g(a, st);
         return FS_OK;
@@ -311,7 +334,7 @@ struct Solver {
         body.last_force = body.force;
         body.last_torque = body.torque;
         body.last_sub = sub;
-        body.integrate(cfg.dt);
+        body.integrate(dt_cur);
         return FS_OK;
     }
 
@@ -320,7 +343,7 @@ struct Solver {
         a.P = PA;
         a.n = (int)np;
         a.W = W; a.H = H; a.D = D;
-        a.dx = cfg.dx; a.dt = cfg.dt; a.alpha = cfg.pic_weight; a.C = cfg.advect_c;
+        a.dx = cfg.dx; a.dt = dt_cur; a.alpha = cfg.pic_weight; a.C = cfg.advect_c;
         a.u = d_u; a.v = d_v; a.w = d_w; a.us = d_us; a.vs = d_vs; a.ws = d_ws;
         a.nu = (int)NU(); a.nv = (int)NV(); a.nw = (int)NW();
         a.labels = d_labels;
@@ -329,7 +352,10 @@ struct Solver {
         a.next_count = d_count;
         a.next_key = d_key;
         a.next_rank = d_rank;
+        HIPCHK(hipMemsetAsync(d_umax, 0, sizeof(unsigned), st));
+        a.umax = d_umax;
         launch_g2p_advect(a, st);
+        umax_valid = true;
         keys_valid = true;
         sorted_valid = false;
         return FS_OK;
@@ -338,6 +364,7 @@ struct Solver {
     int step_once() {
         if (C == 0) return fail(FS_ERR_STATE, "no scene loaded");
         int rc;
+        if ((rc = choose_dt())) return rc;
         HIPCHK(hipEventRecord(ev[0], st));
         if ((rc = stage_sort_label())) return rc;
         HIPCHK(hipEventRecord(ev[1], st));
@@ -361,6 +388,8 @@ struct Solver {
             HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
             times[slot[i]] += ms;
         }
+        sim_time += dt_cur;
+        nsteps += 1;
         return FS_OK;
     }
 };
@@ -512,6 +541,7 @@ void fs_config_default(fs_config* c) {
     c->solver_max_iter = 500;
     c->precond = FS_PRECOND_MG;
     c->stall_window = 5;
+    c->cfl = 0.0f;
 }
 
 int fs_config_load_json(fs_config* c, const char* path) {
@@ -551,6 +581,8 @@ int fs_create(const fs_config* cfg, void** out) {
     HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipMalloc(&s->d_bacc, sizeof(double) * 8));
+    HIPCHK(hipMalloc(&s->d_umax, sizeof(unsigned)));
+    s->dt_cur = cfg->dt;
     *out = s.release();
     return FS_OK;
 }
@@ -829,6 +861,14 @@ int fs_set_grid(void* h, int which, const void* in, int64_t count) {
 int fs_get_stage_times(void* h, double* ms, int n) {
     Solver& s = *(Solver*)h;
     for (int i = 0; i < n && i < FS_NUM_STAGES; ++i) ms[i] = s.times[i];
+    return FS_OK;
+}
+
+int fs_get_clock(void* h, double* t, float* dt, int64_t* n) {
+    Solver& s = *(Solver*)h;
+    if (t) *t = s.sim_time;
+    if (dt) *dt = s.dt_cur;
+    if (n) *n = s.nsteps;
     return FS_OK;
 }
 

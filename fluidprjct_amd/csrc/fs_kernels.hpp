@@ -56,6 +56,7 @@ struct G2PArgs {
     int* next_count;  // null: do not compute next-step keys
     int* next_key;
     int* next_rank;
+    unsigned* umax;   // null, or max |v| over the new particle velocities (float bits)
 };
 
 struct GridArgs {
@@ -79,6 +80,7 @@ void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, c
                     const int* start, hipStream_t st);
 void launch_p2g(const P2GArgs& a, hipStream_t st);
 void launch_g2p_advect(const G2PArgs& a, hipStream_t st);
+void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st);
 
 // grid stages
 void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st);

@@ -407,10 +407,29 @@ FS_DEV bool project_particle(f3& pos, const uint8_t* __restrict__ labels, int W,
     return true;
 }
 
+// max |v| over a wave, folded into a device scalar (float bits of a
+// non-negative value order like unsigned integers)
+FS_DEV void wave_max_speed(float sp, unsigned* umax) {
+    for (int off = 32; off > 0; off >>= 1) sp = fmaxf(sp, __shfl_xor(sp, off, 64));
+    if (__lane_id() == 0 && sp > 0.0f) atomicMax(umax, __float_as_uint(sp));
+}
+
+__global__ __launch_bounds__(256) void k_max_speed(ParticlesView P, int n, unsigned* umax) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float sp = 0.0f;
+    if (i < n) {
+        float vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
+        sp = sqrtf(vx * vx + vy * vy + vz * vz);
+        if (isnan(sp)) sp = 0.0f;
+    }
+    wave_max_speed(sp, umax);
+}
+
 __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool act = i < a.n;
     int key = -1;
+    float speed_out = 0.0f;
     if (act) {
         const int W = a.W, H = a.H, D = a.D;
         const float dx = a.dx;
@@ -429,6 +448,8 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
         vel.x = al * uP + (1.0f - al) * (vel.x + uF);
         vel.y = al * vP + (1.0f - al) * (vel.y + vF);
         vel.z = al * wP + (1.0f - al) * (vel.z + wF);
+        speed_out = sqrtf(vel.x * vel.x + vel.y * vel.y + vel.z * vel.z);
+        if (isnan(speed_out)) speed_out = 0.0f;
         // ---- advection (AdvectParticlesFunctor :1260-1330)
         const float dt = a.dt, C = a.C;
         const float vfloor = sqrtf(5.0f * dx * 9.8f);
@@ -466,6 +487,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
         a.P.vx[i] = vel.x; a.P.vy[i] = vel.y; a.P.vz[i] = vel.z;
         if (a.next_count) key = cell_key(pos.x, pos.y, pos.z, dx, W, H, D);
     }
+    if (a.umax) wave_max_speed(speed_out, a.umax);
     if (a.next_count) {
         int r = wave_agg_rank(key, act, a.next_count);
         if (act) {
@@ -520,6 +542,11 @@ void launch_p2g(const P2GArgs& a, hipStream_t st) {
     static_assert((TY + 2) * (TZ + 2) <= 256, "one halo row per thread");
     int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
     hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(nb), dim3(256), 0, st, a);
+}
+
+void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_max_speed, dim3((n + 255) / 256), dim3(256), 0, st, P, n, umax);
 }
 
 void launch_g2p_advect(const G2PArgs& a, hipStream_t st) {

@@ -63,7 +63,7 @@ extern "C" {
 #define FS_GRID_LABELS   7   /* uint8 W*H*D, FS_SOLID/FLUID/AIR/BODY                 */
 #define FS_GRID_RHS      8   /* float W*H*D, b of A p = b (0 off fluid)              */
 #define FS_GRID_P64      9   /* double W*H*D, pressure solution before rounding      */
-#define FS_GRID_CELLKEY 10   /* int32 Np, per-particle cell index of the last label pass */
+#define FS_GRID_CELLKEY 10   /* int32 Np, cell index of each particle's current position (id order) */
 #define FS_GRID_SOLID   11   /* uint8 W*H*D, static SOLID mask (1 = SOLID)            */
 
 /* per-stage timer slots, in the reference's order (FluidSolver3D.cu:3030-3098) */
@@ -103,6 +103,8 @@ typedef struct fs_config {
     int      solver_max_iter;    /* default 500                                      */
     int      precond;            /* FS_PRECOND_*                                     */
     int      stall_window;       /* learned-FPCG fallback after this many non-decreasing iterations (5) */
+    float    cfl;                /* > 0: dt = min(dt, cfl*dx/max|v_particle|) each step; 0: fixed dt
+                                    (no reference counterpart: the reference's dt is fixed) */
 } fs_config;
 
 typedef struct fs_body_state {
@@ -186,6 +188,8 @@ int fs_stage_g2p_advect(void* s);   /* gridToParticles :983-1039 + advectParticl
 int fs_get_grid(void* s, int which, void* out, int64_t count);
 int fs_set_grid(void* s, int which, const void* in, int64_t count);
 int fs_get_stage_times(void* s, double* ms, int n);   /* accumulated ms per FS_STAGE_* */
+/* simulated time, the dt of the last step and the number of steps taken */
+int fs_get_clock(void* s, double* sim_time, float* last_dt, int64_t* steps);
 int fs_get_solve_stats(void* s, fs_solve_stats* out);
 
 /* ---- output (save3dParticlesToPLY / OFF, utility.cu:91-127) ------------------ */

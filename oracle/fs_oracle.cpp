@@ -157,6 +157,9 @@ struct Solver {
     Body body;
     fs_solve_stats stats{};
     double times[FS_NUM_STAGES] = {0};
+    float dt_cur = 0.0f;     /* dt of the current step */
+    double sim_time = 0.0;
+    int64_t nsteps = 0;
 
     size_t C() const { return (size_t)W * H * D; }
     size_t NU() const { return (size_t)(W + 1) * H * D; }
@@ -332,16 +335,16 @@ void stage_p2g(Solver& s) {
             float fn = (float)(*num[c])[idx], fd = (float)(*den[c])[idx];
             float val = (fd > 1e-8f) ? fn / fd : 0.0f;
             (*sv[c])[idx] = val;                                          /* saveVelocities */
-            G[idx] = (val > vel_unknown) ? val + s.cfg.dt * acc[c] : val;  /* ApplyScalarForce :695-697 */
+            G[idx] = (val > vel_unknown) ? val + s.dt_cur * acc[c] : val;  /* ApplyScalarForce :695-697 */
         }
     }
 }
 
 /* RHSCalculator3D FluidSolver3D.cu:1360-1474 with scale = rho*dx/dt (:1520) */
 void stage_rhs(Solver& s) {
-    const int W = s.W, H = s.H, D = s.D;
+    const int W = s.W, H = s.H;
     const float dx = s.cfg.dx;
-    const float scale = (s.cfg.density * dx) / s.cfg.dt;
+    const float scale = (s.cfg.density * dx) / s.dt_cur;
     const f3 vel_com = s.body.vel, omega = s.body.omega, com = s.body.pos;
     auto body_vel = [&](f3 pos) -> f3 {
         f3 r = mk(pos.x - com.x, pos.y - com.y, pos.z - com.z);
@@ -667,9 +670,9 @@ int stage_pressure(Solver& s) {
 
 /* UFunctor/VFunctor/WFunctor FluidSolver3D.cu:2072-2336, scale = dt/(rho*dx) (:2341) */
 void stage_project(Solver& s) {
-    const int W = s.W, H = s.H, D = s.D;
+    const int W = s.W, H = s.H;
     const float dx = s.cfg.dx;
-    const float scale = s.cfg.dt / (s.cfg.density * dx);
+    const float scale = s.dt_cur / (s.cfg.density * dx);
     const f3 vel_com = s.body.vel, omega = s.body.omega, com = s.body.pos;
     for (int comp = 0; comp < 3; ++comp) {
         std::vector<float>& G = comp == 0 ? s.u : comp == 1 ? s.v : s.w;
@@ -862,7 +865,7 @@ void stage_g2p_advect(Solver& s) {
         float nv = alpha * pic.y + (1.0f - alpha) * (s.vel[3 * q + 1] + flip.y);
         float nw = alpha * pic.z + (1.0f - alpha) * (s.vel[3 * q + 2] + flip.z);
         s.vel[3 * q] = nu; s.vel[3 * q + 1] = nv; s.vel[3 * q + 2] = nw;
-        f3 np = advect_one(F, s.labels.data(), pos, s.cfg.dt, s.cfg.advect_c);
+        f3 np = advect_one(F, s.labels.data(), pos, s.dt_cur, s.cfg.advect_c);
         s.pos[3 * q] = np.x; s.pos[3 * q + 1] = np.y; s.pos[3 * q + 2] = np.z;
     }
 }
@@ -973,7 +976,7 @@ void stage_body(Solver& s) {
     b.last_force = b.force;
     b.last_torque = b.torque;
     b.last_sub = subf;
-    b.integrate(s.cfg.dt);
+    b.integrate(s.dt_cur);
 }
 
 /* ---------------- scene IO ------------------------------------------------------ */
@@ -1037,10 +1040,6 @@ void init_body(Solver& s, const float* sdf, int w, int h, int d, f3 origin, floa
     if (!s.labels.empty()) mark_body(s);
 }
 
-void write_points(std::ofstream& f, const std::vector<f3>& pts) {
-    for (auto& p : pts) f << p.x << " " << p.y << " " << p.z << "\n";
-}
-
 int write_particles(const Solver& s, const char* path, int fmt) {
     std::ofstream f(path);
     if (!f.is_open()) return fail(FS_ERR_IO, std::string("Error opening file: ") + path);
@@ -1055,7 +1054,21 @@ int write_particles(const Solver& s, const char* path, int fmt) {
     return FS_OK;
 }
 
+/* CFL-limited dt (no reference counterpart): dt = min(dt_max, cfl*dx/max|v|) */
+void choose_dt(Solver& s) {
+    s.dt_cur = s.cfg.dt;
+    if (!(s.cfg.cfl > 0.0f)) return;
+    float umax = 0.0f;
+    for (int64_t q = 0; q < s.np(); ++q) {
+        float vx = s.vel[3 * q], vy = s.vel[3 * q + 1], vz = s.vel[3 * q + 2];
+        float sp = sqrtf(vx * vx + vy * vy + vz * vz);
+        if (sp > umax) umax = sp;
+    }
+    if (umax > 0.0f) s.dt_cur = fminf(s.cfg.dt, (s.cfg.cfl * s.cfg.dx) / umax);
+}
+
 int step_once(Solver& s) {
+    choose_dt(s);
     auto t = clk::now();
     stage_label(s);
     s.times[FS_STAGE_LABEL] += ms_since(t);
@@ -1076,6 +1089,8 @@ int step_once(Solver& s) {
     t = clk::now();
     stage_g2p_advect(s);
     s.times[FS_STAGE_G2P] += ms_since(t);
+    s.sim_time += s.dt_cur;
+    s.nsteps += 1;
     return FS_OK;
 }
 
@@ -1130,6 +1145,7 @@ void fs_config_default(fs_config* c) {
     c->solver_max_iter = 500;
     c->precond = FS_PRECOND_MG;
     c->stall_window = 5;
+    c->cfl = 0.0f;
 }
 
 int fs_config_load_json(fs_config* c, const char* path) {
@@ -1158,6 +1174,7 @@ int fs_create(const fs_config* cfg, void** out) {
     if (!cfg || !out) return fail(FS_ERR_ARG, "null argument");
     Solver* s = new Solver();
     s->cfg = *cfg;
+    s->dt_cur = cfg->dt;
     *out = s;
     return FS_OK;
 }
@@ -1356,7 +1373,12 @@ int fs_get_grid(void* h, int which, void* out, int64_t count) {
         case FS_GRID_LABELS: return cp(s.labels.data(), s.labels.size(), 1);
         case FS_GRID_RHS: return cp(s.rhs.data(), s.rhs.size(), 4);
         case FS_GRID_P64: return cp(s.p64.data(), s.p64.size(), 8);
-        case FS_GRID_CELLKEY: return cp(s.key.data(), s.key.size(), 4);
+        case FS_GRID_CELLKEY: {
+            std::vector<int> k((size_t)s.np());
+            for (int64_t q = 0; q < s.np(); ++q)
+                k[(size_t)q] = cell_key(s.pos[3 * q], s.pos[3 * q + 1], s.pos[3 * q + 2], s.cfg.dx, s.W, s.H, s.D);
+            return cp(k.data(), k.size(), 4);
+        }
         case FS_GRID_SOLID: return cp(s.solid.data(), s.solid.size(), 1);
         default: return fail(FS_ERR_ARG, "unknown grid");
     }
@@ -1387,6 +1409,14 @@ int fs_set_grid(void* h, int which, const void* in, int64_t count) {
 int fs_get_stage_times(void* h, double* ms, int n) {
     Solver& s = *(Solver*)h;
     for (int i = 0; i < n && i < FS_NUM_STAGES; ++i) ms[i] = s.times[i];
+    return FS_OK;
+}
+
+int fs_get_clock(void* h, double* t, float* dt, int64_t* n) {
+    Solver& s = *(Solver*)h;
+    if (t) *t = s.sim_time;
+    if (dt) *dt = s.dt_cur;
+    if (n) *n = s.nsteps;
     return FS_OK;
 }
 

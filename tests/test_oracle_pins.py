@@ -1,0 +1,381 @@
+"""Pins of the CPU oracle: fixtures of the reference, closed forms, hand-computed
+tiny cases and an independent FP64 solve (scipy).  Runs without a GPU."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from fluidprjct_amd._abi import (FS_GRID_LABELS, FS_GRID_P, FS_GRID_P64, FS_GRID_RHS, FS_GRID_U, FS_GRID_USAVED,
+                                 FS_GRID_V, FS_GRID_VSAVED, FS_GRID_W, FS_GRID_WSAVED)
+from fluidprjct_amd import scenes
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+INPUTS = os.path.join(GOLDEN, "inputdata")
+
+
+# ---------------------------------------------------------------- scene input
+def test_label_file_counts(oracle):
+    s = oracle.solver()
+    s.load_labels_file(os.path.join(INPUTS, "labels_3d_sphere.txt"))
+    L = s.get_grid(FS_GRID_LABELS)
+    assert s.dims == (35, 35, 35)
+    assert [(L == v).sum() for v in (1, 0, 2)] == [7570, 20276, 15029]
+    s.load_labels_file(os.path.join(INPUTS, "labels_bathymetry.txt"))
+    L = s.get_grid(FS_GRID_LABELS)
+    assert s.dims == (100, 40, 30)
+    assert [(L == v).sum() for v in (1, 0, 2)] == [21944, 43612, 54444]
+
+
+def test_label_file_reverse_order(oracle, tmp_path):
+    # the first character is cell (W-1, H-1, D-1) (FluidSolver3D.cu:139-152); '?' leaves SOLID
+    p = tmp_path / "l.txt"
+    p.write_text("3 2 2\nF A S\nA A A\nS S ?\nA F A\n")
+    s = oracle.solver()
+    s.load_labels_file(str(p))
+    L = s.get_grid(FS_GRID_LABELS)   # (D, H, W)
+    seq = "FASAAASS?AFA"
+    m = {"F": 1, "A": 2, "S": 0, "?": 0}
+    for n, ch in enumerate(seq):
+        idx = 3 * 2 * 2 - 1 - n
+        k, rem = divmod(idx, 6)
+        j, i = divmod(rem, 3)
+        assert L[k, j, i] == m[ch], (n, ch)
+    # the sphere tank's pool lies at the bottom (j = 4..13) only with reversed reading
+    s.load_labels_file(os.path.join(INPUTS, "labels_3d_sphere.txt"))
+    L = s.get_grid(FS_GRID_LABELS)
+    js = np.nonzero((L == 1).any(axis=(0, 2)))[0]
+    assert js.min() == 4 and js.max() == 22
+
+
+def test_label_file_2d_reads_as_depth_one(oracle):
+    s = oracle.solver()
+    s.load_labels_file(os.path.join(INPUTS, "labels_simple.txt"))
+    assert s.dims == (5, 4, 1)
+    assert (s.get_grid(FS_GRID_LABELS) == 1).sum() == 6
+
+
+def test_config_json_keys(oracle):
+    cfg = oracle.default_config()
+    assert (cfg.dt, cfg.dx, cfg.frame_amount, cfg.body_mass) == (pytest.approx(0.02), pytest.approx(0.1), 100, 50.0)
+    assert cfg.pic_weight == pytest.approx(0.01) and cfg.iter_per_frame == 1 and cfg.active_body == 0
+    rc = oracle.lib.fs_config_load_json(C.byref(cfg), os.path.join(INPUTS, "config.json").encode())
+    assert rc == 0
+    assert cfg.dt == pytest.approx(0.005) and cfg.dx == pytest.approx(0.02)
+    assert cfg.pic_weight == pytest.approx(0.001) and cfg.frame_amount == 280 and cfg.active_body == 1
+    assert cfg.input_label_data == b"InputData/labels_gaussian.txt"
+    assert (cfg.body_x, cfg.body_y, cfg.body_z) == (pytest.approx(0.5), pytest.approx(0.2), pytest.approx(0.5))
+    assert cfg.body_mass == pytest.approx(0.03)
+
+
+# ---------------------------------------------------------------- seeding
+def test_seeding_matches_reference_init_dump(oracle):
+    """InputData/particles_-1.ply (FluidSolver3D.cu:201): 4 particles per FLUID
+    cell, k->j->i cell order, each within +-0.49 dx of its cell centre (octant
+    offset 0.25 dx + jitter U(-0.24, 0.24) dx)."""
+    ref = np.load(os.path.join(GOLDEN, "particles_init.npz"))["pos"]
+    dx = 0.02
+    assert ref.shape == (44144, 3)
+    cells = np.floor(ref / dx).astype(int)
+    uniq, first, counts = np.unique(cells[:, ::-1], axis=0, return_index=True, return_counts=True)
+    assert uniq.shape[0] == 11036 and (counts == 4).all()
+    D, H, W = 50, 12, 50
+    lab = np.zeros((D, H, W), np.uint8)
+    lab[uniq[:, 0], uniq[:, 1], uniq[:, 2]] = 1
+    s = oracle.solver(dx=dx)
+    s.set_labels(lab)
+    s.seed_particles(4, 99)
+    pos, vel = s.particles()
+    assert pos.shape == ref.shape and not vel.any()
+    mine = np.floor(pos / dx).astype(int)
+    # same cells, same order (k -> j -> i, 4 consecutive per cell)
+    assert np.array_equal(mine, cells)
+    for p in (pos, ref):
+        off = p / dx - (np.floor(p / dx) + 0.5)
+        assert np.abs(off).max() <= 0.49 + 1e-4
+        assert np.abs(off).min() >= 0.01 - 1e-4
+        # octants are drawn uniformly
+        octant = (off > 0).astype(int) @ np.array([1, 2, 4])
+        frac = np.bincount(octant, minlength=8) / octant.size
+        assert np.abs(frac - 0.125).max() < 0.01
+
+
+def test_seeding_is_deterministic_and_seed_dependent(oracle):
+    lab = scenes.dam_break(12)
+    a, b, c = (oracle.solver(dx=1 / 12) for _ in range(3))
+    for s, seed in ((a, 5), (b, 5), (c, 6)):
+        s.set_labels(lab)
+        s.seed_particles(8, seed)
+    pa, pb, pc = a.particles()[0], b.particles()[0], c.particles()[0]
+    assert np.array_equal(pa, pb) and not np.array_equal(pa, pc)
+
+
+# ---------------------------------------------------------------- pressure system
+def test_cudss_5x5_known_answer(oracle):
+    """cuDSStest (test.cu:262-290): SPD upper CSR, exact x = 1..5, tol 2e-15."""
+    f = oracle.lib.oracle_spd_solve_upper_csr
+    f.argtypes = [C.c_int] + [C.c_void_p] * 5
+    off = np.array([0, 2, 4, 6, 7, 8], np.int32)
+    col = np.array([0, 2, 1, 2, 2, 4, 3, 4], np.int32)
+    val = np.array([4, 1, 3, 2, 5, 1, 1, 2], np.float64)
+    b = np.array([7, 12, 25, 4, 13], np.float64)
+    x = np.zeros(5)
+    assert f(5, off.ctypes.data, col.ctypes.data, val.ctypes.data, b.ctypes.data, x.ctypes.data) == 0
+    assert np.abs(x - np.arange(1, 6)).max() <= 2e-15
+
+
+def test_reference_csr_of_enclosed_toy(oracle):
+    """labels_simple3d.txt: 3x2 enclosed fluid (no AIR): unit-coefficient SPD
+    matrix of MatrixFiller3D (FluidSolver3D.cu:1656-1716), upper triangle,
+    diagonal first then neighbours with col > row in +x,-x,+y,-y,+z,-z order."""
+    s = oracle.solver()
+    s.load_labels_file(os.path.join(INPUTS, "labels_simple3d.txt"))
+    lab = s.get_grid(FS_GRID_LABELS)
+    f = oracle.lib.oracle_assemble_upper_csr
+    f.restype = C.c_int64
+    f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    nnz = f(lab.ctypes.data, 5, 4, 1, None, None, None)
+    off = np.zeros(7, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    f(lab.ctypes.data, 5, 4, 1, off.ctypes.data, col.ctypes.data, val.ctypes.data)
+    # fluid numbering: (1,1)=0 (2,1)=1 (3,1)=2 (1,2)=3 (2,2)=4 (3,2)=5
+    assert off.tolist() == [0, 3, 6, 8, 10, 12, 13]
+    assert col.tolist() == [0, 1, 3, 1, 2, 4, 2, 5, 3, 4, 4, 5, 5]
+    assert val.tolist() == [2, -1, -1, 3, -1, -1, 2, -1, 2, -1, 3, -1, 2]
+
+
+@pytest.mark.parametrize("name", ["dam", "mix"])
+def test_direct_solve_vs_scipy(oracle, name):
+    g = np.load(os.path.join(GOLDEN, "pressure_golden.npz"))
+    lab, b, pref = g[name + "_labels"], g[name + "_b"], g[name + "_p"]
+    p, st = oracle.solver().solve_pressure_system(lab, b, 1e-13)
+    assert np.abs(p - pref).max() <= 1e-10 * np.abs(pref).max()
+
+
+def test_pcg_vs_direct(oracle):
+    rng = np.random.default_rng(8)
+    lab = scenes.dam_break(40, (0.6, 0.4, 1.0))
+    b = (rng.standard_normal(lab.shape) * 1e3).astype(np.float32) * (lab == 1)
+    s = oracle.solver()
+    p_direct, st = s.solve_pressure_system(lab, b, 1e-13)
+    assert st.iterations == 0           # band LDL^T path
+    big = scenes.dam_break(44, (0.6, 0.8, 1.0))   # too large for the direct path
+    b2 = (rng.standard_normal(big.shape) * 1e3).astype(np.float32) * (big == 1)
+    p_cg, st2 = s.solve_pressure_system(big, b2, 1e-12)
+    assert st2.iterations > 0 and st2.rel_residual <= 1e-12
+    from scipy.sparse.linalg import spsolve
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("mf", os.path.join(GOLDEN, "make_fixtures.py"))
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    A, fl = mf.assemble(big)
+    x = spsolve(A.tocsc(), b2.reshape(-1)[fl].astype(np.float64))
+    assert np.abs(p_cg.reshape(-1)[fl] - x).max() <= 1e-9 * np.abs(x).max()
+
+
+def test_singular_enclosed_system_is_projected(oracle):
+    s = oracle.solver()
+    s.load_labels_file(os.path.join(INPUTS, "labels_simple3d.txt"))
+    lab = s.get_grid(FS_GRID_LABELS)
+    b = np.zeros(lab.shape, np.float32)
+    b[lab == 1] = [1, 2, 3, -1, 0, 4]
+    p, st = s.solve_pressure_system(lab, b, 1e-12)
+    assert st.singular == 1 and st.converged == 1
+    # A p = b - mean(b) on the fluid cells
+    f = lab == 1
+    bb = b[f] - b[f].mean()
+    pp = p[f]
+    A = np.array([[2, -1, 0, -1, 0, 0], [-1, 3, -1, 0, -1, 0], [0, -1, 2, 0, 0, -1],
+                  [-1, 0, 0, 2, -1, 0], [0, -1, 0, -1, 3, -1], [0, 0, -1, 0, -1, 2]], float)
+    assert np.abs(A @ pp - bb).max() <= 1e-9
+
+
+def test_hydrostatic_pool(oracle):
+    """Fluid layer of m cells under air at rest: b = rho g dx in the bottom
+    layer, p_j = rho g dx (m - j + 1), and the projected v vanishes."""
+    N, m = 12, 6
+    lab = scenes.bordered(N, N, N)
+    lab[1:-1, 1:1 + m, 1:-1] = 1
+    dx, dt = 1.0 / N, 0.01
+    s = oracle.solver(dx=dx, dt=dt, solver_tol=1e-12)
+    s.set_labels(lab)
+    s.seed_particles(8, 1)
+    for st in ("label", "p2g", "rhs"):
+        s.stage(st)
+    b = s.get_grid(FS_GRID_RHS)
+    rho_g_dx = 1000.0 * 9.81 * dx
+    assert np.allclose(b[1:-1, 1, 1:-1], rho_g_dx, rtol=1e-5)
+    assert np.abs(b[1:-1, 2:1 + m, 1:-1]).max() <= 1e-3
+    s.stage("pressure")
+    p = s.get_grid(FS_GRID_P64)
+    for j in range(1, m + 1):
+        assert np.allclose(p[1:-1, j, 1:-1], rho_g_dx * (m - j + 1), rtol=1e-6)
+    s.stage("project")
+    v = s.get_grid(FS_GRID_V)
+    assert np.abs(v[1:-1, 1:m + 2, 1:-1]).max() <= 1e-5
+
+
+# ---------------------------------------------------------------- transfers
+def test_p2g_single_particle_weights(oracle):
+    """One particle at (1.3, 2.6, 1.9) dx in a 4^3 grid: u-face base (1,2,1) with
+    fractions (0.3, 0.1, 0.4); v-face base (0,2,1) (0.8, 0.6, 0.4); w-face base
+    (0,2,1) (0.8, 0.1, 0.9) (MAC offsets of FluidSolver3D.cu:482-571)."""
+    dx = 0.25
+    s = oracle.solver(dx=dx, dt=0.01)
+    s.set_labels(np.full((4, 4, 4), 2, np.uint8))
+    s.set_particles(np.array([[1.3 * dx, 2.6 * dx, 1.9 * dx]], np.float32), np.array([[1.0, 2.0, 3.0]], np.float32))
+    s.stage("label")
+    s.stage("p2g")
+
+    def den_pattern(base, frac, shape):
+        out = np.zeros(shape)
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dxx in (0, 1):
+                    w = ((frac[0] if dxx else 1 - frac[0]) * (frac[1] if dy else 1 - frac[1]) *
+                         (frac[2] if dz else 1 - frac[2]))
+                    out[base[2] + dz, base[1] + dy, base[0] + dxx] = w
+        return out
+
+    us = s.get_grid(FS_GRID_USAVED)
+    vs = s.get_grid(FS_GRID_VSAVED)
+    ws = s.get_grid(FS_GRID_WSAVED)
+    # a single particle reproduces its own velocity on every face it touches
+    for g, val, base, frac in ((us, 1.0, (1, 2, 1), (0.3, 0.1, 0.4)), (vs, 2.0, (0, 2, 1), (0.8, 0.6, 0.4)),
+                               (ws, 3.0, (0, 2, 1), (0.8, 0.1, 0.9))):
+        pat = den_pattern(base, frac, g.shape)
+        touched = pat > 1e-8
+        assert touched.sum() == 8
+        assert np.allclose(g[touched], val, rtol=1e-6) and not g[~touched].any()
+        assert pat.sum() == pytest.approx(1.0)
+    # gravity on every v face after the save (ApplyScalarForce, FluidSolver3D.cu:687-698)
+    v = s.get_grid(FS_GRID_V)
+    assert np.allclose(v - vs, np.float32(0.01) * np.float32(-9.81), rtol=1e-6)
+    assert np.array_equal(s.get_grid(FS_GRID_U), us)
+
+
+def test_p2g_uniform_velocity_is_reproduced(oracle):
+    rng = np.random.default_rng(3)
+    N, dx = 10, 0.1
+    s = oracle.solver(dx=dx)
+    s.set_labels(scenes.bordered(N, N, N))
+    pos = (rng.random((5000, 3)) * N * dx).astype(np.float32)
+    s.set_particles(pos, np.tile(np.array([[0.7, -1.25, 2.5]], np.float32), (5000, 1)))
+    s.stage("label")
+    s.stage("p2g")
+    for g, val in ((FS_GRID_USAVED, 0.7), (FS_GRID_VSAVED, -1.25), (FS_GRID_WSAVED, 2.5)):
+        a = s.get_grid(g)
+        assert np.allclose(a[a != 0], val, rtol=2e-6)
+
+
+def _uniform_fields(s, N, U, dU):
+    for g, gs, val, dv in ((FS_GRID_U, FS_GRID_USAVED, U[0], dU[0]), (FS_GRID_V, FS_GRID_VSAVED, U[1], dU[1]),
+                           (FS_GRID_W, FS_GRID_WSAVED, U[2], dU[2])):
+        shape, _ = s.grid_shape(g)
+        s.set_grid(g, np.full(shape, val, np.float32))
+        s.set_grid(gs, np.full(shape, val - dv, np.float32))
+
+
+def test_g2p_uniform_fields_and_advection(oracle):
+    """Uniform u = U with saved u = U - dU: v_p <- a U + (1 - a)(v_p + dU)
+    (FluidSolver3D.cu:972-974); constant-field RK3 moves x by U dt exactly up
+    to rounding (Ralston weights 2/9 + 3/9 + 4/9 = 1)."""
+    N, dx, dt, alpha = 16, 1.0 / 16, 0.01, 0.05
+    s = oracle.solver(dx=dx, dt=dt, pic_weight=alpha)
+    s.set_labels(np.full((N, N, N), 2, np.uint8))
+    rng = np.random.default_rng(4)
+    pos = (0.3 + 0.4 * rng.random((200, 3))).astype(np.float32)
+    vel = rng.standard_normal((200, 3)).astype(np.float32)
+    s.set_particles(pos, vel)
+    U, dU = (0.8, -0.5, 0.25), (0.1, 0.2, -0.3)
+    _uniform_fields(s, N, U, dU)
+    s.stage("g2p_advect")
+    p2, v2 = s.particles()
+    expect = alpha * np.array(U) + (1 - alpha) * (vel + np.array(dU))
+    assert np.allclose(v2, expect, atol=1e-6)
+    assert np.allclose(p2, pos + np.array(U) * dt, atol=2e-6)
+
+
+def test_advection_clamps_to_domain(oracle):
+    N, dx, dt = 8, 0.125, 0.1
+    s = oracle.solver(dx=dx, dt=dt, pic_weight=1.0)
+    s.set_labels(np.full((N, N, N), 2, np.uint8))
+    s.set_particles(np.array([[0.8, 0.05, 0.5]], np.float32))
+    _uniform_fields(s, N, (3.0, -3.0, 0.0), (0, 0, 0))
+    s.stage("g2p_advect")
+    p, _ = s.particles()
+    assert p[0, 0] == np.float32((N - 1) * dx) and p[0, 1] == 0.0 and p[0, 2] == pytest.approx(0.5)
+
+
+def test_advection_projects_out_of_solid(oracle):
+    """A particle carried into a SOLID cell is moved by 0.5 dx toward the
+    nearest FLUID cell among the 34 offsets (FluidSolver3D.cu:1161-1239)."""
+    N, dx = 8, 0.125
+    lab = np.full((N, N, N), 2, np.uint8)
+    lab[:, :, 5:] = 0      # solid for i >= 5
+    lab[4, 4, 4] = 1       # fluid cell next to the wall
+    s = oracle.solver(dx=dx, dt=0.01, pic_weight=1.0)
+    s.set_labels(lab)
+    x0 = np.array([[4.98 * dx, 4.5 * dx, 4.5 * dx]], np.float32)
+    s.set_particles(x0)
+    _uniform_fields(s, N, (1.0, 0.0, 0.0), (0, 0, 0))
+    s.stage("g2p_advect")
+    p, _ = s.particles()
+    # it crossed into cell i = 5 (SOLID) and was pushed back by -0.5 dx in x
+    assert np.floor(p[0, 0] / dx) == 4
+    assert p[0, 0] == pytest.approx(x0[0, 0] + 0.01 - 0.5 * dx, abs=1e-5)
+
+
+# ---------------------------------------------------------------- rigid body
+def test_body_sphere_labels_and_buoyancy(oracle):
+    """Sphere in a flooded tank with one particle at every cell centre.  A
+    voxel with phi <= 0 counts as submerged when a particle of its cell lies
+    within 2 dx (FluidSolver3D.cu:2993-3004); every cell holds one here, so the
+    submerged volume is the sphere's volume and the vertical force is
+    (rho V - m) g (drag vanishes for a body at rest in still fluid)."""
+    N, r = 40, 0.2
+    dx = 1.0 / N
+    lab = scenes.bordered(N, N, N)
+    lab[1:-1, 1:-1, 1:-1] = 1
+    sdf, origin, cs = scenes.sphere_sdf(r, dx / 2)
+    mass = scenes.sphere_mass(r, 500.0)
+    s = oracle.solver(dx=dx, dt=1e-3)
+    s.set_labels(lab)
+    s.set_body_sdf(sdf, origin, cs, (0.5, 0.5, 0.5), mass)
+    L = s.get_grid(FS_GRID_LABELS)
+    nb = int((L == 3).sum())
+    vol = 4 / 3 * np.pi * r ** 3
+    assert abs(nb * dx ** 3 / vol - 1) < 0.05
+    k, j, i = np.nonzero(lab == 1)
+    pos = ((np.stack([i, j, k], 1) + 0.5) * dx).astype(np.float32)
+    s.set_particles(pos)
+    s.step(1)
+    st = s.body_state()
+    # every voxel with phi <= 0 counts, with the partial weight 0.5 - 0.5 phi/cs
+    # inside the surface band (FluidSolver3D.cu:2979-2983)
+    inside = sdf[sdf <= 0].astype(np.float64)
+    w = np.where(inside > -cs, 0.5 - 0.5 * inside / cs, 1.0)
+    assert st.submerged_volume == pytest.approx(cs ** 3 * w.sum(), rel=1e-4)
+    assert abs(st.submerged_volume / vol - 1) < 0.06
+    expect = (1000.0 * st.submerged_volume - mass) * 9.81
+    assert st.force[1] == pytest.approx(expect, rel=1e-3)
+    assert abs(st.force[0]) < 1e-3 * expect and abs(st.force[2]) < 1e-3 * expect
+
+
+def test_body_free_fall_in_air(oracle):
+    N = 24
+    dx, dt = 1.0 / N, 0.01
+    lab = scenes.bordered(N, N, N)
+    lab[1:-1, 1:3, 1:-1] = 1
+    sdf, origin, cs = scenes.sphere_sdf(0.1, dx)
+    s = oracle.solver(dx=dx, dt=dt)
+    s.set_labels(lab)
+    s.set_body_sdf(sdf, origin, cs, (0.5, 0.7, 0.5), 2.0)
+    s.seed_particles(2, 1)
+    s.step(3)
+    st = s.body_state()
+    # semi-implicit Euler: v_n = -n g dt, y_n = y0 - g dt^2 n(n+1)/2
+    assert st.vel[1] == pytest.approx(-3 * 9.81 * dt, rel=1e-5)
+    assert st.pos[1] == pytest.approx(0.7 - 9.81 * dt * dt * 6, rel=1e-5)
+    assert st.submerged_volume == 0.0

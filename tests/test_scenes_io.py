@@ -1,0 +1,58 @@
+"""Scene generators and the reference's file formats (label files, SDF files,
+PLY/OFF output), checked through the oracle's readers and writers.  No GPU."""
+import os
+
+import numpy as np
+import pytest
+
+from fluidprjct_amd import scenes
+from fluidprjct_amd._abi import FS_GRID_LABELS
+
+
+def test_label_writer_roundtrip(oracle, tmp_path):
+    lab = scenes.shallow_basin(24, seed=3)
+    p = tmp_path / "labels_gaussian.txt"
+    scenes.write_labels(str(p), lab)
+    s = oracle.solver()
+    s.load_labels_file(str(p))
+    assert s.dims == (24, 24, 24)
+    assert np.array_equal(s.get_grid(FS_GRID_LABELS), lab)
+
+
+def test_sdf_writer_roundtrip(oracle, tmp_path):
+    sdf, origin, cs = scenes.sphere_sdf(0.1, 0.02)
+    p = tmp_path / "ball.sdf"
+    scenes.write_sdf(str(p), sdf, origin, cs)
+    lab = scenes.bordered(20, 20, 20)
+    a, b = oracle.solver(dx=0.05), oracle.solver(dx=0.05)
+    for s in (a, b):
+        s.set_labels(lab)
+    a.load_body_sdf(str(p), (0.5, 0.5, 0.5), 1.0)
+    b.set_body_sdf(sdf, origin, cs, (0.5, 0.5, 0.5), 1.0)
+    la, lb = a.get_grid(FS_GRID_LABELS), b.get_grid(FS_GRID_LABELS)
+    assert np.array_equal(la, lb) and (la == 3).sum() > 0
+
+
+def test_scene_shapes():
+    lab = scenes.dam_break(32)
+    # cell centres x<=0.25, y<=0.5, z<=1 inside the one-cell border
+    assert (lab == 1).sum() == 7 * 15 * 30
+    lab = scenes.water_column(40, 0.4)
+    assert (lab == 1).sum() == 15 * 38 * 38
+    assert scenes.sphere_mass(0.08, 500.0) == pytest.approx(1.0723, abs=1e-4)
+
+
+def test_ply_and_off_output(oracle, tmp_path):
+    s = oracle.solver(dx=0.1)
+    s.set_labels(scenes.bordered(6, 6, 6))
+    pos = np.array([[0.15, 0.25, 0.35], [0.123456789, 0.3, 0.45]], np.float32)
+    s.set_particles(pos)
+    ply, off = tmp_path / "p.ply", tmp_path / "p.off"
+    s.write_particles(str(ply), 0)
+    s.write_particles(str(off), 1)
+    lines = ply.read_text().splitlines()
+    # save3dParticlesToPLY (utility.cu:91-109): 7-line header, default 6 significant digits
+    assert lines[:7] == ["ply", "format ascii 1.0", "element vertex 2", "property float x", "property float y",
+                         "property float z", "end_header"]
+    assert lines[7] == "0.15 0.25 0.35" and lines[8] == "0.123457 0.3 0.45"
+    assert off.read_text().splitlines()[:2] == ["OFF", "2 0 0"]
