@@ -130,6 +130,7 @@ struct Solver {
     int *d_key = nullptr, *d_rank = nullptr, *d_tmpkey = nullptr;
     bool keys_valid = false;   // d_key/d_rank/d_count hold the counting pass of the current order
     bool sorted_valid = false; // d_start matches the current particle order
+    bool count_clean = true;   // d_count is all zero
     // body
     HostBody body;
     float* d_sdf = nullptr;
@@ -190,6 +191,7 @@ struct Solver {
         HIPCHK(hipMalloc(&d_total, sizeof(int) * 2));
         HIPCHK(hipMalloc(&d_nfluid, sizeof(int) * 2));
         HIPCHK(hipMemset(d_count, 0, sizeof(int) * (C + 4)));
+        count_clean = true;
         HIPCHK(hipMemset(d_start, 0, sizeof(int) * (C + 1)));
         for (float* p : {d_u, d_us}) HIPCHK(hipMemset(p, 0, sizeof(float) * NU()));
         for (float* p : {d_v, d_vs}) HIPCHK(hipMemset(p, 0, sizeof(float) * NV()));
@@ -224,6 +226,10 @@ struct Solver {
         HIPCHK(hipMalloc(&d_key, b));
         HIPCHK(hipMalloc(&d_rank, b));
         keys_valid = sorted_valid = umax_valid = false;
+        if (C > 0) {
+            HIPCHK(hipMemset(d_count, 0, sizeof(int) * (C + 4)));
+            count_clean = true;
+        }
         return FS_OK;
     }
 
@@ -266,19 +272,39 @@ struct Solver {
 
     // ---- stages -----------------------------------------------------------
     int stage_sort_label() {
-        if (!keys_valid) {
-            HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int) * (C + 4), st));
-            launch_key_count(PA, (int)np, cfg.dx, W, H, D, d_key, d_rank, d_count, st);
-            keys_valid = true;
-        }
+        int rc = ensure_keys();
+        if (rc) return rc;
         HIPCHK(hipMemsetAsync(d_nfluid, 0, sizeof(int), st));
         LabelArgs la = label_args();
         launch_scan_label(d_count, d_solid, d_labels, C, d_bsum, d_total, d_start, la, st);
+        return finish_sort();
+    }
+
+    // the counting pass of the current particle order (keys, ranks, counts)
+    int ensure_keys() {
+        if (keys_valid) return FS_OK;
+        if (!count_clean) HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int) * (C + 4), st));
+        launch_key_count(PA, (int)np, cfg.dx, W, H, D, d_key, d_rank, d_count, st);
+        keys_valid = true;
+        count_clean = false;
+        return FS_OK;
+    }
+
+    int finish_sort() {
         launch_scatter(PA, PB, (int)np, d_key, d_rank, d_start, st);
         std::swap(PA, PB);
-        keys_valid = false;  // counts were consumed by the scan
+        keys_valid = false;  // counts were consumed (and cleared) by the scan
+        count_clean = true;
         sorted_valid = true;
         return FS_OK;
+    }
+
+    // sort by cell without relabelling (for a G2P/advection stage called on its own)
+    int sort_only() {
+        int rc = ensure_keys();
+        if (rc) return rc;
+        launch_scan_sort(d_count, C, d_bsum, d_total, d_start, st);
+        return finish_sort();
     }
 
     int stage_p2g() {
@@ -339,6 +365,10 @@ struct Solver {
     }
 
     int stage_g2p_advect() {
+        if (!sorted_valid) {
+            int rc = sort_only();
+            if (rc) return rc;
+        }
         G2PArgs a{};
         a.P = PA;
         a.n = (int)np;
@@ -348,15 +378,16 @@ struct Solver {
         a.nu = (int)NU(); a.nv = (int)NV(); a.nw = (int)NW();
         a.labels = d_labels;
         // the counts of the next label pass are accumulated here
-        HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int) * (C + 4), st));
+        if (!count_clean) HIPCHK(hipMemsetAsync(d_count, 0, sizeof(int) * (C + 4), st));
         a.next_count = d_count;
         a.next_key = d_key;
         a.next_rank = d_rank;
         HIPCHK(hipMemsetAsync(d_umax, 0, sizeof(unsigned), st));
         a.umax = d_umax;
-        launch_g2p_advect(a, st);
+        launch_g2p_advect(a, d_start, st);
         umax_valid = true;
         keys_valid = true;
+        count_clean = false;
         sorted_valid = false;
         return FS_OK;
     }

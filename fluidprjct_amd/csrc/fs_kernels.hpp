@@ -6,6 +6,20 @@
 
 #include "fs_device.hpp"
 
+// G2P/advection LDS tile (cells) and staged margin; one 256-thread block per tile.
+#ifndef ADV_TX
+#define ADV_TX 32
+#endif
+#ifndef ADV_TY
+#define ADV_TY 4
+#endif
+#ifndef ADV_TZ
+#define ADV_TZ 4
+#endif
+#ifndef ADV_MARGIN
+#define ADV_MARGIN 2
+#endif
+
 // P2G LDS tile (cells); one 256-thread block per tile.
 #ifndef P2G_TX
 #define P2G_TX 16
@@ -73,13 +87,14 @@ void launch_key_count(const ParticlesView& P, int n, float dx, int W, int H, int
 int scan_blocks(long long C);
 void launch_scan_label(int* count, const uint8_t* solid, uint8_t* labels, long long C, int* bsum, int* total,
                        int* start, const LabelArgs& la, hipStream_t st);
+void launch_scan_sort(int* count, long long C, int* bsum, int* total, int* start, hipStream_t st);
 void launch_scan_fluid(const uint8_t* labels, long long C, int* bsum, int* total, int* start, hipStream_t st);
 void launch_seed(const uint8_t* labels, const int* frank, long long C, int W, int H, float dx, int ppc, uint64_t seed,
                  const ParticlesView& P, hipStream_t st);
 void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, const int* key, const int* rank,
                     const int* start, hipStream_t st);
 void launch_p2g(const P2GArgs& a, hipStream_t st);
-void launch_g2p_advect(const G2PArgs& a, hipStream_t st);
+void launch_g2p_advect(const G2PArgs& a, const int* start, hipStream_t st);
 void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st);
 
 // grid stages

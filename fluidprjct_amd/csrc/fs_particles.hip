@@ -5,7 +5,7 @@
 //   k_scan_*            cell-count scan fused with labelGrid :340-397
 //   k_scatter           counting-sort scatter (no reference counterpart)
 //   k_p2g_tile          particlesToGrid :607-685 + saveVelocities :399-415 + applyForces :700-761
-//   k_g2p_advect        gridToParticles :983-1039 + advectParticles :1334-1357
+//   k_g2p_advect_tile   gridToParticles :983-1039 + advectParticles :1334-1357
 //
 // Particles are SoA (px,py,pz,vx,vy,vz,id), kept sorted by cell index so that
 // each P2G tile reads contiguous rows and G2P gathers hit neighbouring faces.
@@ -81,7 +81,7 @@ FS_DEV int block_excl_scan(int v, int* sm, int* total) {
 template <int MODE>
 FS_DEV int4 scan_load(const int* __restrict__ cnt, const uint8_t* __restrict__ lab, long long c4, long long C) {
     int4 v = make_int4(0, 0, 0, 0);
-    if (MODE == 0) {
+    if (MODE != 1) {
         if (c4 + 3 < C) {
             v = *reinterpret_cast<const int4*>(cnt + c4);
         } else {
@@ -138,6 +138,7 @@ __global__ __launch_bounds__(1024) void k_scan_top(int* __restrict__ bsum, int n
 // order: SOLID kept, BODY marked, then FLUID where particles are), clears the
 // count for the next step and accumulates the fluid-cell count.
 // MODE 1 (fluid flags for seeding): writes start[] only.
+// MODE 2 (cell counts, re-sort without relabelling): start[] and cleared counts.
 template <int MODE>
 __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, const uint8_t* __restrict__ solid,
                                                        uint8_t* __restrict__ labels, long long C,
@@ -173,15 +174,23 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, co
             nfl += (L == FS_FLUID);
         }
     }
-    if (MODE == 0) {
+    if (MODE != 1) {
         if (c4 + 3 < C) {
             *reinterpret_cast<int4*>(cnt + c4) = make_int4(0, 0, 0, 0);
         } else {
             for (int q = 0; q < 4; ++q)
                 if (c4 + q < C) cnt[c4 + q] = 0;
         }
+    }
+    if (MODE == 0) {
+        // one atomic per block
+        __shared__ int s_nfl;
+        if (threadIdx.x == 0) s_nfl = 0;
+        __syncthreads();
         nfl = wave_sum(nfl);
-        if ((threadIdx.x & 63) == 0 && nfl) atomicAdd(la.n_fluid, nfl);
+        if ((threadIdx.x & 63) == 0 && nfl) atomicAdd(&s_nfl, nfl);
+        __syncthreads();
+        if (threadIdx.x == 0 && s_nfl) atomicAdd(la.n_fluid, s_nfl);
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = bsum[blockIdx.x] + tot;
 }
@@ -339,7 +348,13 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     __syncthreads();
 
     const float dx = a.dx;
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    for (int base = 0; base < tot; base += blockDim.x) {
+        // In a full chunk, consecutive lanes take particles 8 apart: with ~8
+        // particles per cell they land in different cells, so the LDS atomics
+        // of a wave hit different faces instead of serialising on one.
+        int t = (base + (int)blockDim.x <= tot) ? base + (int)((threadIdx.x & 31) * 8 + (threadIdx.x >> 5))
+                                                 : base + (int)threadIdx.x;
+        if (t >= tot) break;
         int lo = 0, hi = nrows - 1;
         while (lo < hi) {  // largest r with s_pref[r] <= t
             int mid = (lo + hi + 1) >> 1;
@@ -425,74 +440,229 @@ __global__ __launch_bounds__(256) void k_max_speed(ParticlesView P, int n, unsig
     wave_max_speed(sp, umax);
 }
 
-__global__ __launch_bounds__(256) void k_g2p_advect(G2PArgs a) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool act = i < a.n;
-    int key = -1;
-    float speed_out = 0.0f;
-    if (act) {
-        const int W = a.W, H = a.H, D = a.D;
-        const float dx = a.dx;
-        f3 pos = mk(a.P.px[i], a.P.py[i], a.P.pz[i]);
-        f3 vel = mk(a.P.vx[i], a.P.vy[i], a.P.vz[i]);
-        // ---- G2P (GridToParticleFunctor :822-979)
-        float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
-        Base bu = mac_base<0>(rx, ry, rz, W, H, D);
-        Base bv = mac_base<1>(rx, ry, rz, W, H, D);
-        Base bw = mac_base<2>(rx, ry, rz, W, H, D);
-        float uP, uF, vP, vF, wP, wF;
-        sample_pic_flip<0>(a.u, a.us, a.nu, bu, W + 1, (W + 1) * H, uP, uF);
-        sample_pic_flip<1>(a.v, a.vs, a.nv, bv, W, W * (H + 1), vP, vF);
-        sample_pic_flip<2>(a.w, a.ws, a.nw, bw, W, W * H, wP, wF);
-        const float al = a.alpha;
-        vel.x = al * uP + (1.0f - al) * (vel.x + uF);
-        vel.y = al * vP + (1.0f - al) * (vel.y + vF);
-        vel.z = al * wP + (1.0f - al) * (vel.z + wF);
-        speed_out = sqrtf(vel.x * vel.x + vel.y * vel.y + vel.z * vel.z);
-        if (isnan(speed_out)) speed_out = 0.0f;
-        // ---- advection (AdvectParticlesFunctor :1260-1330)
-        const float dt = a.dt, C = a.C;
-        const float vfloor = sqrtf(5.0f * dx * 9.8f);
-        const float xmax = (W - 1) * dx, ymax = (H - 1) * dx, zmax = (D - 1) * dx;
-        float subT = 0.0f;
-        bool finished = false;
-        int iter = 0;
-        while (!finished && iter++ < 1000) {
-            f3 k1 = interp_vel(a, pos);
-            float speed = sqrtf(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z) + 1e-37f + vfloor;
-            float dT = (C * dx) / speed;
-            if (subT + dT >= dt) {
-                dT = dt - subT;
-                finished = true;
-            } else if (subT + 2 * dT >= dt) {
-                dT *= 0.5f;
-            }
-            f3 k2 = interp_vel(a, pos + k1 * 0.5f * dT);
-            f3 k3 = interp_vel(a, pos + k2 * 0.75f * dT);
-            pos = pos + (2.0f / 9.0f) * dT * k1 + (3.0f / 9.0f) * dT * k2 + (4.0f / 9.0f) * dT * k3;
-            subT += dT;
-            pos.x = fmaxf(pos.x, 0.0f); pos.x = fminf(pos.x, xmax);
-            pos.y = fmaxf(pos.y, 0.0f); pos.y = fminf(pos.y, ymax);
-            pos.z = fmaxf(pos.z, 0.0f); pos.z = fminf(pos.z, zmax);
-            int cx = (int)floorf(pos.x / dx), cy = (int)floorf(pos.y / dx), cz = (int)floorf(pos.z / dx);
-            if (cx >= 0 && cx < W && cy >= 0 && cy < H && cz >= 0 && cz < D) {
-                uint8_t L = a.labels[(size_t)cx + (size_t)cy * W + (size_t)cz * W * H];
-                if (L == FS_SOLID || L == FS_BODY) {
-                    if (!project_particle(pos, a.labels, W, H, D, dx)) break;
-                }
-            }
-            if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || isnan(pos.x) || isnan(pos.y) || isnan(pos.z)) break;
-        }
-        a.P.px[i] = pos.x; a.P.py[i] = pos.y; a.P.pz[i] = pos.z;
-        a.P.vx[i] = vel.x; a.P.vy[i] = vel.y; a.P.vz[i] = vel.z;
-        if (a.next_count) key = cell_key(pos.x, pos.y, pos.z, dx, W, H, D);
+// G2P + advection over a tile of TX x TY x TZ cells.  The particles of a tile
+// are the rows of the cell-sorted arrays; u, v, w and the labels of the tile
+// plus a margin of M cells are staged in LDS once, and every interpolation of
+// the RK3 substeps reads LDS.  A sample whose 8 corners are not all inside the
+// staged box (a particle that left the margin) reads global memory instead,
+// with the same indices, so both paths return the same values.
+template <int BX, int BY, int BZ>
+struct TileFields {
+    int X0, Y0, Z0;
+    const float* su;
+    const float* sv;
+    const float* sw;
+    const uint8_t* sl;
+    const G2PArgs* a;
+
+    template <int COMP>
+    FS_DEV bool in_box(const Base& b, int& i0) const {
+        constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1), LZ = BZ + (COMP == 2);
+        const int nx = a->W + (COMP == 0), ny = a->H + (COMP == 1), nz = a->D + (COMP == 2);
+        int li = b.i - X0, lj = b.j - Y0, lk = b.k - Z0;
+        bool ok = li >= 0 && li + 1 < LX && lj >= 0 && lj + 1 < LY && lk >= 0 && lk + 1 < LZ && b.i + 1 < nx &&
+                  b.j + 1 < ny && b.k + 1 < nz;
+        i0 = li + LX * (lj + LY * lk);
+        return ok;
     }
-    if (a.umax) wave_max_speed(speed_out, a.umax);
-    if (a.next_count) {
-        int r = wave_agg_rank(key, act, a.next_count);
+
+    template <int COMP>
+    FS_DEV float sample(const Base& b) const {
+        constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1);
+        int i0;
+        if (in_box<COMP>(b, i0)) {
+            const float* s = COMP == 0 ? su : (COMP == 1 ? sv : sw);
+            constexpr int SXY = LX * LY;
+            return trilin(b.fx, b.fy, b.fz, s[i0], s[i0 + 1], s[i0 + SXY], s[i0 + 1 + SXY], s[i0 + LX],
+                          s[i0 + 1 + LX], s[i0 + LX + SXY], s[i0 + 1 + LX + SXY]);
+        }
+        const int W = a->W, H = a->H;
+        if (COMP == 0) return sample_comp(a->u, a->nu, b, W + 1, (W + 1) * H);
+        if (COMP == 1) return sample_comp(a->v, a->nv, b, W, W * (H + 1));
+        return sample_comp(a->w, a->nw, b, W, W * H);
+    }
+
+    // PIC value and FLIP delta (u - uSaved per corner, FluidSolver3D.cu:990-1012)
+    template <int COMP>
+    FS_DEV void pic_flip(const Base& b, float& pic, float& flip) const {
+        constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1);
+        const int W = a->W, H = a->H;
+        const int sx = COMP == 0 ? W + 1 : W;
+        const int sxy = COMP == 0 ? (W + 1) * H : (COMP == 1 ? W * (H + 1) : W * H);
+        const float* fs = COMP == 0 ? a->us : (COMP == 1 ? a->vs : a->ws);
+        const int n = COMP == 0 ? a->nu : (COMP == 1 ? a->nv : a->nw);
+        int i0;
+        if (in_box<COMP>(b, i0)) {
+            const float* s = COMP == 0 ? su : (COMP == 1 ? sv : sw);
+            constexpr int SXY = LX * LY;
+            const int lo[8] = {0, 1, SXY, 1 + SXY, LX, 1 + LX, LX + SXY, 1 + LX + SXY};
+            const int g0 = b.i + b.j * sx + b.k * sxy;
+            const int go[8] = {0, 1, sxy, 1 + sxy, sx, 1 + sx, sx + sxy, 1 + sx + sxy};
+            float c[8], d[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                c[q] = s[i0 + lo[q]];
+                d[q] = c[q] - fs[g0 + go[q]];
+            }
+            pic = trilin(b.fx, b.fy, b.fz, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]);
+            flip = trilin(b.fx, b.fy, b.fz, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
+            return;
+        }
+        const float* f = COMP == 0 ? a->u : (COMP == 1 ? a->v : a->w);
+        sample_pic_flip<COMP>(f, fs, n, b, sx, sxy, pic, flip);
+    }
+
+    FS_DEV f3 interp(f3 pos) const {
+        const float dx = a->dx;
+        float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
+        Base bu = mac_base<0>(rx, ry, rz, a->W, a->H, a->D);
+        Base bv = mac_base<1>(rx, ry, rz, a->W, a->H, a->D);
+        Base bw = mac_base<2>(rx, ry, rz, a->W, a->H, a->D);
+        return mk(sample<0>(bu), sample<1>(bv), sample<2>(bw));
+    }
+
+    // label of an in-domain cell
+    FS_DEV uint8_t label(int cx, int cy, int cz) const {
+        int li = cx - X0, lj = cy - Y0, lk = cz - Z0;
+        if (li >= 0 && li < BX && lj >= 0 && lj < BY && lk >= 0 && lk < BZ) return sl[li + BX * (lj + BY * lk)];
+        return a->labels[(size_t)cx + (size_t)cy * a->W + (size_t)cz * a->W * a->H];
+    }
+};
+
+template <int TX, int TY, int TZ, int M>
+__global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* __restrict__ start) {
+    constexpr int BX = TX + 2 * M, BY = TY + 2 * M, BZ = TZ + 2 * M;
+    constexpr int NU = (BX + 1) * BY * BZ, NV = BX * (BY + 1) * BZ, NW = BX * BY * (BZ + 1), NL = BX * BY * BZ;
+    constexpr int ROWS = TY * TZ;
+    __shared__ float su[NU], sv[NV], sw[NW];
+    __shared__ uint8_t sl[NL];
+    __shared__ int s_beg[ROWS], s_pref[ROWS + 1], s_scan[4];
+
+    const int W = a.W, H = a.H, D = a.D;
+    const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int x0 = (bid % ntx) * TX, y0 = ((bid / ntx) % nty) * TY, z0 = (bid / (ntx * nty)) * TZ;
+
+    int cnt = 0;
+    if ((int)threadIdx.x < ROWS) {
+        int r = threadIdx.x;
+        int cy = y0 + r % TY, cz = z0 + r / TY;
+        if (cy < H && cz < D) {
+            long long rowc = (long long)cy * W + (long long)cz * W * H;
+            int beg = start[rowc + x0], end = start[rowc + min(x0 + TX, W)];
+            s_beg[r] = beg;
+            cnt = end - beg;
+        } else {
+            s_beg[r] = 0;
+        }
+    }
+    int tot;
+    int pre = block_excl_scan(cnt, s_scan, &tot);
+    if (tot == 0) return;   // uniform over the block
+    if ((int)threadIdx.x < ROWS) s_pref[threadIdx.x] = pre;
+    if (threadIdx.x == 0) s_pref[ROWS] = tot;
+
+    // stage the velocity faces and labels of the tile + margin
+    const int X0 = x0 - M, Y0 = y0 - M, Z0 = z0 - M;
+    for (int l = threadIdx.x; l < NU; l += blockDim.x) {
+        int i = X0 + l % (BX + 1), j = Y0 + (l / (BX + 1)) % BY, k = Z0 + l / ((BX + 1) * BY);
+        su[l] = (i >= 0 && i <= W && j >= 0 && j < H && k >= 0 && k < D)
+                    ? a.u[(size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H] : 0.0f;
+    }
+    for (int l = threadIdx.x; l < NV; l += blockDim.x) {
+        int i = X0 + l % BX, j = Y0 + (l / BX) % (BY + 1), k = Z0 + l / (BX * (BY + 1));
+        sv[l] = (i >= 0 && i < W && j >= 0 && j <= H && k >= 0 && k < D)
+                    ? a.v[(size_t)i + (size_t)j * W + (size_t)k * W * (H + 1)] : 0.0f;
+    }
+    for (int l = threadIdx.x; l < NW; l += blockDim.x) {
+        int i = X0 + l % BX, j = Y0 + (l / BX) % BY, k = Z0 + l / (BX * BY);
+        sw[l] = (i >= 0 && i < W && j >= 0 && j < H && k >= 0 && k <= D)
+                    ? a.w[(size_t)i + (size_t)j * W + (size_t)k * W * H] : 0.0f;
+    }
+    for (int l = threadIdx.x; l < NL; l += blockDim.x) {
+        int i = X0 + l % BX, j = Y0 + (l / BX) % BY, k = Z0 + l / (BX * BY);
+        sl[l] = (i >= 0 && i < W && j >= 0 && j < H && k >= 0 && k < D)
+                    ? a.labels[(size_t)i + (size_t)j * W + (size_t)k * W * H] : (uint8_t)FS_SOLID;
+    }
+    __syncthreads();
+
+    TileFields<BX, BY, BZ> F{X0, Y0, Z0, su, sv, sw, sl, &a};
+    const float dx = a.dx, dt = a.dt, C = a.C, al = a.alpha;
+    const float vfloor = sqrtf(5.0f * dx * 9.8f);
+    const float xmax = (W - 1) * dx, ymax = (H - 1) * dx, zmax = (D - 1) * dx;
+
+    for (int base = 0; base < tot; base += blockDim.x) {
+        const int t = base + threadIdx.x;
+        const bool act = t < tot;
+        int key = -1, p = 0;
+        float speed_out = 0.0f;
         if (act) {
-            a.next_key[i] = key;
-            a.next_rank[i] = r;
+            int lo = 0, hi = ROWS - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (s_pref[mid] <= t) lo = mid; else hi = mid - 1;
+            }
+            p = s_beg[lo] + (t - s_pref[lo]);
+            f3 pos = mk(a.P.px[p], a.P.py[p], a.P.pz[p]);
+            f3 vel = mk(a.P.vx[p], a.P.vy[p], a.P.vz[p]);
+            // ---- G2P (GridToParticleFunctor :822-979)
+            {
+                float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
+                Base bu = mac_base<0>(rx, ry, rz, W, H, D);
+                Base bv = mac_base<1>(rx, ry, rz, W, H, D);
+                Base bw = mac_base<2>(rx, ry, rz, W, H, D);
+                float uP, uF, vP, vF, wP, wF;
+                F.template pic_flip<0>(bu, uP, uF);
+                F.template pic_flip<1>(bv, vP, vF);
+                F.template pic_flip<2>(bw, wP, wF);
+                vel.x = al * uP + (1.0f - al) * (vel.x + uF);
+                vel.y = al * vP + (1.0f - al) * (vel.y + vF);
+                vel.z = al * wP + (1.0f - al) * (vel.z + wF);
+            }
+            speed_out = sqrtf(vel.x * vel.x + vel.y * vel.y + vel.z * vel.z);
+            if (isnan(speed_out)) speed_out = 0.0f;
+            // ---- advection (AdvectParticlesFunctor :1260-1330)
+            float subT = 0.0f;
+            bool finished = false;
+            int iter = 0;
+            while (!finished && iter++ < 1000) {
+                f3 k1 = F.interp(pos);
+                float speed = sqrtf(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z) + 1e-37f + vfloor;
+                float dT = (C * dx) / speed;
+                if (subT + dT >= dt) {
+                    dT = dt - subT;
+                    finished = true;
+                } else if (subT + 2 * dT >= dt) {
+                    dT *= 0.5f;
+                }
+                f3 k2 = F.interp(pos + k1 * 0.5f * dT);
+                f3 k3 = F.interp(pos + k2 * 0.75f * dT);
+                pos = pos + (2.0f / 9.0f) * dT * k1 + (3.0f / 9.0f) * dT * k2 + (4.0f / 9.0f) * dT * k3;
+                subT += dT;
+                pos.x = fmaxf(pos.x, 0.0f); pos.x = fminf(pos.x, xmax);
+                pos.y = fmaxf(pos.y, 0.0f); pos.y = fminf(pos.y, ymax);
+                pos.z = fmaxf(pos.z, 0.0f); pos.z = fminf(pos.z, zmax);
+                int cx = (int)floorf(pos.x / dx), cy = (int)floorf(pos.y / dx), cz = (int)floorf(pos.z / dx);
+                if (cx >= 0 && cx < W && cy >= 0 && cy < H && cz >= 0 && cz < D) {
+                    uint8_t L = F.label(cx, cy, cz);
+                    if (L == FS_SOLID || L == FS_BODY) {
+                        if (!project_particle(pos, a.labels, W, H, D, dx)) break;
+                    }
+                }
+                if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || isnan(pos.x) || isnan(pos.y) || isnan(pos.z))
+                    break;
+            }
+            a.P.px[p] = pos.x; a.P.py[p] = pos.y; a.P.pz[p] = pos.z;
+            a.P.vx[p] = vel.x; a.P.vy[p] = vel.y; a.P.vz[p] = vel.z;
+            if (a.next_count) key = cell_key(pos.x, pos.y, pos.z, dx, W, H, D);
+        }
+        if (a.umax) wave_max_speed(speed_out, a.umax);
+        if (a.next_count) {
+            int r = wave_agg_rank(key, act, a.next_count);
+            if (act) {
+                a.next_key[p] = key;
+                a.next_rank[p] = r;
+            }
         }
     }
 }
@@ -514,6 +684,14 @@ void launch_scan_label(int* count, const uint8_t* solid, uint8_t* labels, long l
     hipLaunchKernelGGL(k_scan_reduce<0>, dim3(nb), dim3(SCAN_T), 0, st, count, labels, C, bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nb, total);
     hipLaunchKernelGGL(k_scan_final<0>, dim3(nb), dim3(SCAN_T), 0, st, count, solid, labels, C, bsum, start, la);
+}
+
+void launch_scan_sort(int* count, long long C, int* bsum, int* total, int* start, hipStream_t st) {
+    int nb = scan_blocks(C);
+    LabelArgs la{};
+    hipLaunchKernelGGL(k_scan_reduce<2>, dim3(nb), dim3(SCAN_T), 0, st, count, nullptr, C, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+    hipLaunchKernelGGL(k_scan_final<2>, dim3(nb), dim3(SCAN_T), 0, st, count, nullptr, nullptr, C, bsum, start, la);
 }
 
 void launch_scan_fluid(const uint8_t* labels, long long C, int* bsum, int* total, int* start, hipStream_t st) {
@@ -540,6 +718,7 @@ void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, c
 void launch_p2g(const P2GArgs& a, hipStream_t st) {
     constexpr int TX = P2G_TX, TY = P2G_TY, TZ = P2G_TZ;
     static_assert((TY + 2) * (TZ + 2) <= 256, "one halo row per thread");
+    static_assert(256 % 32 == 0, "lane spreading assumes 256-thread blocks");
     int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
     hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(nb), dim3(256), 0, st, a);
 }
@@ -549,9 +728,12 @@ void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t
     hipLaunchKernelGGL(k_max_speed, dim3((n + 255) / 256), dim3(256), 0, st, P, n, umax);
 }
 
-void launch_g2p_advect(const G2PArgs& a, hipStream_t st) {
+void launch_g2p_advect(const G2PArgs& a, const int* start, hipStream_t st) {
     if (a.n <= 0) return;
-    hipLaunchKernelGGL(k_g2p_advect, dim3((a.n + 255) / 256), dim3(256), 0, st, a);
+    constexpr int TX = ADV_TX, TY = ADV_TY, TZ = ADV_TZ;
+    static_assert(TY * TZ <= 256, "one tile row per thread");
+    int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
+    hipLaunchKernelGGL((k_g2p_advect_tile<TX, TY, TZ, ADV_MARGIN>), dim3(nb), dim3(256), 0, st, a, start);
 }
 
 }  // namespace fsk

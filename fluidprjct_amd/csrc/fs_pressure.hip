@@ -50,12 +50,65 @@ FS_DEV double stencil_Ax64(const double* __restrict__ x, long long c, uint32_t c
 }
 
 // ---------------------------------------------------------------- level setup
-__global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ labels, uint8_t* __restrict__ type,
-                                               long long n) {
+// fluid bounding box (min/max i, j, k of FLUID cells) -> box[0..5]
+__global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels, int W, int H, long long n,
+                                              int* __restrict__ box) {
+    __shared__ int sb[6];
+    if (threadIdx.x < 6) sb[threadIdx.x] = (threadIdx.x & 1) ? -1 : 0x7fffffff;
+    __syncthreads();
+    int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        if (labels[c] != FS_FLUID) continue;
+        int v[3] = {(int)(c % W), (int)((c / W) % H), (int)(c / ((long long)W * H))};
+        for (int q = 0; q < 3; ++q) { mn[q] = min(mn[q], v[q]); mx[q] = max(mx[q], v[q]); }
+    }
+    for (int q = 0; q < 3; ++q) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[q] = min(mn[q], __shfl_xor(mn[q], off, 64));
+            mx[q] = max(mx[q], __shfl_xor(mx[q], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int q = 0; q < 3; ++q) { atomicMin(&sb[2 * q], mn[q]); atomicMax(&sb[2 * q + 1], mx[q]); }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        if (threadIdx.x & 1) atomicMax(&box[threadIdx.x], sb[threadIdx.x]);
+        else atomicMin(&box[threadIdx.x], sb[threadIdx.x]);
+    }
+}
+
+// level-0 types of the solve box (offset ox, oy, oz in the full W x H grid)
+__global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ labels, MGLevel L, int W, int H, int ox,
+                                               int oy, int oz) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L.n) return;
+    int i = (int)(c % L.W) + ox, j = (int)((c / L.W) % L.H) + oy, k = (int)(c / ((long long)L.W * L.H)) + oz;
+    uint8_t lab = labels[(size_t)i + (size_t)j * W + (size_t)k * W * H];
+    L.type[c] = lab == FS_FLUID ? 1 : (lab == FS_AIR ? 2 : 0);
+}
+
+// b of the solve box from the full-grid right-hand side
+__global__ __launch_bounds__(256) void k_gather_b(const float* __restrict__ bfull, MGLevel L, int W, int H, int ox,
+                                                  int oy, int oz, float* __restrict__ bbox) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L.n) return;
+    int i = (int)(c % L.W) + ox, j = (int)((c / L.W) % L.H) + oy, k = (int)(c / ((long long)L.W * L.H)) + oz;
+    bbox[c] = bfull[(size_t)i + (size_t)j * W + (size_t)k * W * H];
+}
+
+// full-grid pressure (FP64 and FP32, 0 off fluid) from the solve box
+__global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__ labels, long long n, int W, int H,
+                                                     MGLevel L, int ox, int oy, int oz, const double* __restrict__ xb,
+                                                     double* __restrict__ p64, float* __restrict__ p) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    uint8_t L = labels[c];
-    type[c] = L == FS_FLUID ? 1 : (L == FS_AIR ? 2 : 0);
+    int i = (int)(c % W) - ox, j = (int)((c / W) % H) - oy, k = (int)(c / ((long long)W * H)) - oz;
+    double v = 0.0;
+    if (labels[c] == FS_FLUID && i >= 0 && i < L.W && j >= 0 && j < L.H && k >= 0 && k < L.D)
+        v = xb[(size_t)i + (size_t)j * L.W + (size_t)k * L.W * L.H];
+    p64[c] = v;
+    p[c] = (float)v;
 }
 
 __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
@@ -104,11 +157,19 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
         }
         L.code[c] = (uint16_t)code;
     }
+    __shared__ int s_cnt[2];
+    if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
+    __syncthreads();
     unk = wave_sum(unk);
     dir = wave_sum(dir);
     if ((threadIdx.x & 63) == 0) {
-        if (unk) atomicAdd(&counters[0], unk);
-        if (dir) atomicOr(&counters[1], 1);
+        if (unk) atomicAdd(&s_cnt[0], unk);
+        if (dir) atomicOr(&s_cnt[1], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_cnt[0]) atomicAdd(&counters[0], s_cnt[0]);
+        if (s_cnt[1]) atomicOr(&counters[1], 1);
     }
 }
 
@@ -406,39 +467,53 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 MGPCG::~MGPCG() { free_all(); }
 
 void MGPCG::free_all() {
-    for (auto& L : lv) {
+    for (size_t l = 0; l < lv.size(); ++l) {
+        MGLevel& L = lv[l];
         hipFree(L.code); hipFree(L.type);
-        if (&L != &lv[0]) { hipFree(L.b); hipFree(L.x); }
+        if (l > 0) { hipFree(L.b); hipFree(L.x); }
         hipFree(L.xt); hipFree(L.r);
     }
     lv.clear();
-    hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(e);
-    hipFree(partial); hipFree(partial2); hipFree(sc);
+    nlev = 0;
+    hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(e); hipFree(bb); hipFree(xb);
+    hipFree(partial); hipFree(partial2); hipFree(sc); hipFree(counters);
     if (sc_host) hipHostFree(sc_host);
-    r = z = pa = pb = q = e = nullptr;
+    if (host_counters) hipHostFree(host_counters);
+    r = z = pa = pb = q = e = bb = nullptr;
+    xb = nullptr;
     partial = partial2 = nullptr;
     sc = nullptr;
     sc_host = nullptr;
+    counters = host_counters = nullptr;
+}
+
+// Level dimensions for a W x H x D box: halve (rounding up) until a level has at
+// most coarse_max cells.
+static int level_dims(int W, int H, int D, long long coarse_max, int dims[][3], int cap) {
+    int n = 0, w = W, h = H, d = D;
+    while (n < cap) {
+        dims[n][0] = w; dims[n][1] = h; dims[n][2] = d;
+        ++n;
+        if ((long long)w * h * d <= coarse_max || (w == 1 && h == 1 && d == 1)) break;
+        w = (w + 1) / 2; h = (h + 1) / 2; d = (d + 1) / 2;
+    }
+    return n;
 }
 
 int MGPCG::init(int W_, int H_, int D_) {
     free_all();
     W = W_; H = H_; D = D_;
     N = (long long)W * H * D;
-    int w = W, h = H, d = D;
-    while (true) {
-        MGLevel L{};
-        L.W = w; L.H = h; L.D = d;
-        L.n = (long long)w * h * d;
-        lv.push_back(L);
-        if (L.n <= coarse_max) break;
-        if (w == 1 && h == 1 && d == 1) break;
-        w = (w + 1) / 2; h = (h + 1) / 2; d = (d + 1) / 2;
-    }
+    int dims[40][3];
+    int n = level_dims(W, H, D, coarse_max, dims, 40);
     bool ok = true;
     auto A = [&](void** p, size_t bytes) { if (hipMalloc(p, bytes) != hipSuccess) ok = false; };
-    for (size_t l = 0; l < lv.size(); ++l) {
+    lv.resize(n);
+    for (int l = 0; l < n; ++l) {
         MGLevel& L = lv[l];
+        L = MGLevel{};
+        L.W = dims[l][0]; L.H = dims[l][1]; L.D = dims[l][2];
+        L.n = (long long)L.W * L.H * L.D;
         A((void**)&L.code, sizeof(uint16_t) * L.n);
         A((void**)&L.type, L.n);
         if (l > 0) {
@@ -448,55 +523,62 @@ int MGPCG::init(int W_, int H_, int D_) {
         A((void**)&L.xt, sizeof(float) * L.n);
         A((void**)&L.r, sizeof(float) * L.n);
     }
-    A((void**)&r, sizeof(float) * N);
-    A((void**)&z, sizeof(float) * N);
-    A((void**)&pa, sizeof(float) * N);
-    A((void**)&pb, sizeof(float) * N);
-    A((void**)&q, sizeof(float) * N);
-    A((void**)&e, sizeof(float) * N);
+    for (float** p : {&r, &z, &pa, &pb, &q, &e, &bb}) A((void**)p, sizeof(float) * N);
+    A((void**)&xb, sizeof(double) * N);
     A((void**)&partial, sizeof(double) * RED_BLOCKS);
     A((void**)&partial2, sizeof(double) * RED_BLOCKS);
-    A((void**)&sc, sizeof(SolveScalars) * 2);
+    A((void**)&sc, sizeof(SolveScalars));
+    A((void**)&counters, sizeof(int) * 16);
     if (hipHostMalloc((void**)&sc_host, sizeof(SolveScalars)) != hipSuccess) ok = false;
-    nred = (int)std::min<long long>(RED_BLOCKS, (N + 255) / 256);
+    if (hipHostMalloc((void**)&host_counters, sizeof(int) * 16) != hipSuccess) ok = false;
     if (!ok) { free_all(); return -1; }
+    if (lv.back().n > COARSE_MAX) { free_all(); return -2; }
     // CG direction buffers start finite: the first update multiplies them by beta = 0
     hipMemset(pa, 0, sizeof(float) * N);
     hipMemset(pb, 0, sizeof(float) * N);
     hipMemset(e, 0, sizeof(float) * N);
-    // the coarsest level must fit the single-block LDS solve
-    if (lv.back().n > COARSE_MAX) { free_all(); return -2; }
+    configure(W, H, D);
     return 0;
 }
 
-void MGPCG::setup(const uint8_t* labels, hipStream_t st) {
-    int* counters = reinterpret_cast<int*>(sc + 1);  // scratch: second SolveScalars slot
-    hipMemsetAsync(counters, 0, 2 * sizeof(int), st);
-    hipLaunchKernelGGL(k_type0, dim3(nblk(N)), dim3(256), 0, st, labels, lv[0].type, N);
-    for (size_t l = 0; l + 1 < lv.size(); ++l)
-        hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
-    for (size_t l = 0; l < lv.size(); ++l) {
-        int* cnt = (l == 0) ? counters : reinterpret_cast<int*>(sc + 1) + 2;  // coarse counts are not used
-        hipLaunchKernelGGL(k_code, dim3(nblk(lv[l].n)), dim3(256), 0, st, lv[l], cnt);
+void MGPCG::configure(int Wb, int Hb, int Db) {
+    int dims[40][3];
+    int n = level_dims(Wb, Hb, Db, coarse_max, dims, (int)lv.size());
+    for (int l = 0; l < n; ++l) {
+        lv[l].W = dims[l][0]; lv[l].H = dims[l][1]; lv[l].D = dims[l][2];
+        lv[l].n = (long long)lv[l].W * lv[l].H * lv[l].D;
     }
+    nlev = n;
+    n0 = lv[0].n;
+    nred = (int)std::min<long long>(RED_BLOCKS, (n0 + 255) / 256);
 }
 
-void MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, const float* dot_with, SolveScalars* s) {
+void MGPCG::setup(const uint8_t* labels, hipStream_t st) {
+    hipLaunchKernelGGL(k_type0, dim3(nblk(n0)), dim3(256), 0, st, labels, lv[0], W, H, box[0], box[1], box[2]);
+    for (int l = 0; l + 1 < nlev; ++l)
+        hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
+    for (int l = 0; l < nlev; ++l)
+        hipLaunchKernelGGL(k_code, dim3(nblk(lv[l].n)), dim3(256), 0, st, lv[l], counters + (l == 0 ? 8 : 10));
+}
+
+MGLevel MGPCG::fine() const { return lv[0]; }
+
+float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s) {
     MGLevel& L = lv[l];
-    if (l == (int)lv.size() - 1) {
+    if (l == nlev - 1) {
         hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st, L, L.b, L.x, coarse_sweeps, omega, s);
         if (l == 0 && pdot) {
-            // degenerate single-level hierarchy: recompute the r.z partials
-            hipLaunchKernelGGL((k_jacobi<false, true>), dim3(nred), dim3(256), 0, st, L, L.b, L.x, L.xt, 0.0f, pdot,
-                               s);
-            hipMemcpyAsync(L.x, L.xt, sizeof(float) * L.n, hipMemcpyDeviceToDevice, st);
+            // single-level hierarchy: r.z partials from the coarse result (omega = 0 keeps x)
+            hipLaunchKernelGGL((k_jacobi<false, true>), dim3(nred), dim3(256), 0, st, L, L.b, L.x, L.xt, 0.0f,
+                               pdot, s);
+            return L.xt;
         }
-        return;
+        return L.x;
     }
     const unsigned nb = nblk(L.n);
-    // pre-smoothing from x = 0
-    float* cur = L.x;
-    float* oth = L.xt;
+    // the buffer written first is chosen so that the last smoothing sweep lands in L.x
+    float* cur = ((nu_pre + nu_post - 1) & 1) ? L.xt : L.x;
+    float* oth = (cur == L.x) ? L.xt : L.x;
     hipLaunchKernelGGL((k_jacobi<true, false>), dim3(nb), dim3(256), 0, st, L, L.b, (const float*)nullptr, cur, omega,
                        (double*)nullptr, s);
     for (int k = 1; k < nu_pre; ++k) {
@@ -507,8 +589,8 @@ void MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, const float* dot_w
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
     MGLevel& Cn = lv[l + 1];
     hipLaunchKernelGGL(k_restrict, dim3(nblk(Cn.n)), dim3(256), 0, st, L, Cn, L.r, Cn.b, s);
-    vcycle_level(l + 1, st, nullptr, nullptr, s);
-    hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, Cn.x, cur, s);
+    float* xc = vcycle_level(l + 1, st, nullptr, s);
+    hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, xc, cur, s);
     for (int k = 0; k < nu_post; ++k) {
         bool last = (k == nu_post - 1);
         if (last && pdot) {
@@ -519,14 +601,15 @@ void MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, const float* dot_w
         }
         std::swap(cur, oth);
     }
-    if (cur != L.x) hipMemcpyAsync(L.x, cur, sizeof(float) * L.n, hipMemcpyDeviceToDevice, st);
+    return cur;
 }
 
 void MGPCG::vcycle(const float* rin, float* zout, double* pdot, SolveScalars* s, hipStream_t st) {
     MGLevel& L0 = lv[0];
     L0.b = const_cast<float*>(rin);
     L0.x = zout;
-    vcycle_level(0, st, pdot, rin, s);
+    float* res = vcycle_level(0, st, pdot, s);
+    if (res != zout) hipMemcpyAsync(zout, res, sizeof(float) * L0.n, hipMemcpyDeviceToDevice, st);
 }
 
 void MGPCG::apply_A(const float* x, float* y, hipStream_t st) {
@@ -550,34 +633,61 @@ private:
 
 Preconditioner* make_jacobi_preconditioner(MGPCG* s) { return new JacobiPre(s); }
 
+int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
+    // fluid bounding box, expanded by one cell so that every neighbour of a
+    // FLUID cell (AIR/SOLID/BODY or out of domain) is inside the solve box
+    int init_box[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
+    hipMemcpyAsync(counters, init_box, sizeof(init_box), hipMemcpyHostToDevice, st);
+    hipLaunchKernelGGL(k_bbox, dim3((unsigned)std::min<long long>(RED_BLOCKS, (N + 255) / 256)), dim3(256), 0, st,
+                       labels, W, H, N, counters);
+    hipMemcpyAsync(host_counters, counters, sizeof(int) * 6, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return -5;
+    const int* hb = host_counters;
+    if (hb[1] < 0) return -3;   // no fluid
+    int lo[3], hi[3];
+    const int dimsz[3] = {W, H, D};
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = std::max(0, hb[2 * q] - 1);
+        hi[q] = std::min(dimsz[q] - 1, hb[2 * q + 1] + 1);
+    }
+    box[0] = lo[0]; box[1] = lo[1]; box[2] = lo[2];
+    configure(hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1);
+    hipMemsetAsync(counters + 8, 0, sizeof(int) * 4, st);
+    setup(labels, st);
+    hipLaunchKernelGGL(k_gather_b, dim3(nblk(n0)), dim3(256), 0, st, b, lv[0], W, H, box[0], box[1], box[2], bb);
+    return 0;
+}
+
 int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
                  Preconditioner* pre, hipStream_t st, fs_solve_stats* stats) {
     MGPreconditioner mgp(this);
     if (!pre) pre = &mgp;
+    fs_solve_stats loc{};
+    int rc = prepare(labels, b, st);
+    if (rc == -5) return -5;
+    if (rc == -3) {
+        box[0] = box[1] = box[2] = 0;
+        configure(W, H, D);
+        hipLaunchKernelGGL(k_fill0, dim3(nblk(N)), dim3(256), 0, st, N, xb, e);
+        hipLaunchKernelGGL(k_scatter_out, dim3(nblk(N)), dim3(256), 0, st, labels, N, W, H, lv[0], 0, 0, 0, xb, x64, p);
+        if (stats) *stats = loc;
+        return -3;
+    }
     MGLevel& L0 = lv[0];
-    setup(labels, st);
-    // norms of b over unknowns, counters
     SolveScalars init{};
     init.maxit = maxit;
     init.inner_rtol2 = inner_rtol * inner_rtol;
     hipMemcpyAsync(sc, &init, sizeof(SolveScalars), hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_bnorm, dim3(nred), dim3(256), 0, st, L0, b, partial, partial2);
+    hipLaunchKernelGGL(k_bnorm, dim3(nred), dim3(256), 0, st, L0, bb, partial, partial2);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_BNORM, partial, partial2, nred, sc, 0.0);
-    int counters[2];
-    hipMemcpyAsync(counters, reinterpret_cast<int*>(sc + 1), sizeof(counters), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(host_counters + 8, counters + 8, sizeof(int) * 2, hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
-    const int nunk = counters[0];
-    const bool singular = (nunk > 0) && !counters[1];
-    fs_solve_stats loc{};
+    const int nunk = host_counters[8];
+    const bool singular = (nunk > 0) && !host_counters[9];
     loc.fluid_cells = nunk;
     loc.singular = singular ? 1 : 0;
-    hipLaunchKernelGGL(k_fill0, dim3(nblk(N)), dim3(256), 0, st, N, x64, e);
-    if (nunk == 0) {
-        hipLaunchKernelGGL(k_to_float, dim3(nblk(N)), dim3(256), 0, st, N, labels, x64, p);
-        if (stats) *stats = loc;
-        return -3;
-    }
+    hipLaunchKernelGGL(k_fill0, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
     double shift = singular ? sc_host->bsum / nunk : 0.0;
     double bn2 = singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
     double tol2b = tol * tol * bn2;
@@ -585,8 +695,8 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     bool converged = false;
     double rr_true = 0.0;
     for (outer = 0;; ++outer) {
-        if (outer > 0) hipLaunchKernelGGL(k_axpy64, dim3(nblk(N)), dim3(256), 0, st, N, x64, e);
-        hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, b, x64, shift, r, partial);
+        if (outer > 0) hipLaunchKernelGGL(k_axpy64, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
+        hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, partial);
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, partial, (const double*)nullptr,
                            nred, sc, tol2b);
         hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
@@ -605,7 +715,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
                 hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, partial, sc);
                 hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_PQ, partial,
                                    (const double*)nullptr, nred, sc, 0.0);
-                hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, N, pnew, q, e, r, partial, sc);
+                hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r, partial, sc);
                 hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_RR, partial,
                                    (const double*)nullptr, nred, sc, 0.0);
                 std::swap(pold, pnew);
@@ -615,7 +725,8 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             if (sc_host->done) break;
         }
     }
-    hipLaunchKernelGGL(k_to_float, dim3(nblk(N)), dim3(256), 0, st, N, labels, x64, p);
+    hipLaunchKernelGGL(k_scatter_out, dim3(nblk(N)), dim3(256), 0, st, labels, N, W, H, L0, box[0], box[1], box[2], xb,
+                       x64, p);
     last_iters = total_iters;
     last_outer = outer + 1;
     loc.iterations = total_iters;

@@ -51,34 +51,47 @@ public:
     // and p (float, 0 off fluid) are written.  Returns 0 on convergence.
     int solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
               Preconditioner* pre, hipStream_t st, struct fs_solve_stats* stats);
-    // Set up the level codes for `labels` (also used by the learned path).
+    // Fluid bounding box of `labels` (+1 cell), level set-up for that box and
+    // the gathered right-hand side bb.  Returns -3 when there is no fluid.
+    int prepare(const uint8_t* labels, const float* b, hipStream_t st);
+    // level dimensions for a solve box of Wb x Hb x Db cells
+    void configure(int Wb, int Hb, int Db);
+    // level codes of the configured box
     void setup(const uint8_t* labels, hipStream_t st);
+    MGLevel fine() const;
     // One V-cycle z = M r on level 0, accumulating r.z partial sums.
     void vcycle(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st);
     // y = A x on the fine grid (FP32), for tests and the learned path.
     void apply_A(const float* x, float* y, hipStream_t st);
 
     int W = 0, H = 0, D = 0;
-    long long N = 0;
-    std::vector<MGLevel> lv;
+    long long N = 0;               // full grid cells
+    std::vector<MGLevel> lv;       // allocated for the full grid; the first nlev are in use
+    int nlev = 0;
+    long long n0 = 0;              // cells of the solve box (level 0)
+    int box[3] = {0, 0, 0};        // solve-box origin in the full grid
     int nu_pre = 2, nu_post = 2, coarse_sweeps = 64;
     long long coarse_max = 512;   // coarsen until a level has at most this many cells
     float omega = 6.0f / 7.0f;
     int batch = 4;              // inner iterations launched between host checks
-    double inner_rtol = 1e-4;   // inner relative reduction per refinement pass
+    double inner_rtol = 1e-5;   // inner relative reduction per refinement pass
     int max_outer = 12;
     // fine-grid CG vectors
     float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *e = nullptr;
+    float* bb = nullptr;           // b on the solve box
+    double* xb = nullptr;          // FP64 solution on the solve box
+    int* counters = nullptr;       // device: bbox [0..5], unknown count [8], Dirichlet flag [9]
+    int* host_counters = nullptr;  // pinned mirror
     double* partial = nullptr;
     double* partial2 = nullptr;
-    SolveScalars* sc = nullptr;   // device
+    SolveScalars* sc = nullptr;       // device
     SolveScalars* sc_host = nullptr;  // pinned
     int nred = 0;   // number of partial-sum slots used by reducing kernels
     int last_iters = 0, last_outer = 0;
 
 private:
     void free_all();
-    void vcycle_level(int l, hipStream_t st, double* partial_dot, const float* dot_with, SolveScalars* sc);
+    float* vcycle_level(int l, hipStream_t st, double* partial_dot, SolveScalars* sc);
 };
 
 Preconditioner* make_jacobi_preconditioner(MGPCG* s);

@@ -121,7 +121,8 @@ def test_rhs_and_project_bitexact(hip, oracle):
         assert np.array_equal(h.get_grid(g).view(np.uint32), o.get_grid(g).view(np.uint32)), g
 
 
-def test_g2p_advect_bitexact(hip, oracle):
+@pytest.mark.parametrize("speed", [0.7, 12.0])   # 12 m/s leaves the staged LDS margin (global path)
+def test_g2p_advect_bitexact(hip, oracle, speed):
     rng = np.random.default_rng(4)
     N = 20
     dx = 1.0 / N
@@ -131,7 +132,7 @@ def test_g2p_advect_bitexact(hip, oracle):
     pos += dx
     h, o = pair(hip, oracle, dx=dx, dt=1.0 / 60, pic_weight=0.05)
     shapes = {FS_GRID_U: (N, N, N + 1), FS_GRID_V: (N, N + 1, N), FS_GRID_W: (N + 1, N, N)}
-    f = {g: (rng.standard_normal(s) * 0.7).astype(np.float32) for g, s in shapes.items()}
+    f = {g: (rng.standard_normal(s) * speed).astype(np.float32) for g, s in shapes.items()}
     fs = {g: (a + rng.standard_normal(a.shape).astype(np.float32) * 0.1) for g, a in f.items()}
     saved = {FS_GRID_U: FS_GRID_USAVED, FS_GRID_V: FS_GRID_VSAVED, FS_GRID_W: FS_GRID_WSAVED}
     for s in (h, o):
