@@ -249,40 +249,73 @@ __global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels
 
 // ---------------------------------------------------------------------------
 // P2G over an LDS tile.  A block owns the faces of a TX x TY x TZ cell tile and
-// reads the particles of the tile plus a one-cell halo (the only cells whose
-// particles reach the owned faces).  Weights and products follow
-// AccumulateVelocities :468-594; sums are accumulated with LDS float atomics,
-// then each owned face is normalised (ComputeVelocityFunc :598-605), saved
-// (saveVelocities) and given gravity (ApplyScalarForce :687-698) in one write.
+// visits the cells of the tile plus a one-cell halo (the only cells whose
+// particles reach the owned faces), one thread per cell.  A particle of cell
+// (ci,cj,ck) reaches, per component, a 2x3x3 window of faces around its cell
+// (u: i in {ci,ci+1}, j in {cj-1..cj+1}, k in {ck-1..ck+1}; v and w alike with
+// the 2-wide axis along y or z), so the thread sums the weights and weighted
+// velocities of all its particles into that window in registers (weights and
+// products as AccumulateVelocities :468-594) and then adds the window to the
+// owned faces with one LDS atomic per face.  Each owned face is finally
+// normalised (ComputeVelocityFunc :598-605), saved (saveVelocities) and given
+// gravity (ApplyScalarForce :687-698) in one write.
+template <int COMP>
+FS_DEV void window_weights(const Base& b, int ci, int cj, int ck, float* wx, float* wy, float* wz) {
+    // window origin: the 2-wide axis starts at the cell, the 3-wide axes one below
+    const int ox = (COMP == 0) ? ci : ci - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
+    const int rx = b.i - ox, ry = b.j - oy, rz = b.k - oz;   // each is 0 or 1
+    const float ax0 = 1.0f - b.fx, ax1 = b.fx, ay0 = 1.0f - b.fy, ay1 = b.fy, az0 = 1.0f - b.fz, az1 = b.fz;
+    // 2-wide axis: slot 1 receives the (1-f) weight when the base sits on slot 1;
+    // its f weight then targets a face beyond the domain, which the reference skips
+    if (COMP == 0) { wx[0] = rx ? 0.0f : ax0; wx[1] = rx ? ax0 : ax1; }
+    else { wx[0] = rx ? 0.0f : ax0; wx[1] = rx ? ax0 : ax1; wx[2] = rx ? ax1 : 0.0f; }
+    if (COMP == 1) { wy[0] = ry ? 0.0f : ay0; wy[1] = ry ? ay0 : ay1; }
+    else { wy[0] = ry ? 0.0f : ay0; wy[1] = ry ? ay0 : ay1; wy[2] = ry ? ay1 : 0.0f; }
+    if (COMP == 2) { wz[0] = rz ? 0.0f : az0; wz[1] = rz ? az0 : az1; }
+    else { wz[0] = rz ? 0.0f : az0; wz[1] = rz ? az0 : az1; wz[2] = rz ? az1 : 0.0f; }
+}
+
+template <int COMP>
+FS_DEV void window_accumulate(const Base& b, int ci, int cj, int ck, float vc, float* num, float* den) {
+    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
+    float wx[3], wy[3], wz[3];
+    window_weights<COMP>(b, ci, cj, ck, wx, wy, wz);
+#pragma unroll
+    for (int z = 0; z < NZ; ++z)
+#pragma unroll
+        for (int y = 0; y < NY; ++y)
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const int s = x + NX * (y + NY * z);
+                float wgt = wx[x] * wy[y] * wz[z];
+                num[s] += vc * wgt;
+                den[s] += wgt;
+            }
+}
+
 template <int COMP, int TX, int TY, int TZ>
-FS_DEV void p2g_scatter_comp(float rx, float ry, float rz, float vc, const P2GArgs& a, int x0, int y0, int z0,
-                             float* __restrict__ snum, float* __restrict__ sden) {
+FS_DEV void window_flush(const P2GArgs& a, int x0, int y0, int z0, int ci, int cj, int ck, const float* num,
+                         const float* den, float* __restrict__ snum, float* __restrict__ sden) {
+    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
     constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
-    Base b = mac_base<COMP>(rx, ry, rz, a.W, a.H, a.D);
-    // owned global index range of this component (inclusive)
     const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
     const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
     const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
-    float wxs[2] = {1.0f - b.fx, b.fx}, wys[2] = {1.0f - b.fy, b.fy}, wzs[2] = {1.0f - b.fz, b.fz};
+    const int ox = (COMP == 0) ? ci : ci - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
 #pragma unroll
-    for (int dz = 0; dz < 2; ++dz) {
-        int k = b.k + dz;
-        if (k < z0 || k > hiz) continue;
+    for (int z = 0; z < NZ; ++z)
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy) {
-            int j = b.j + dy;
-            if (j < y0 || j > hiy) continue;
+        for (int y = 0; y < NY; ++y)
 #pragma unroll
-            for (int dxi = 0; dxi < 2; ++dxi) {
-                int i = b.i + dxi;
-                if (i < x0 || i > hix) continue;
-                float wgt = wxs[dxi] * wys[dy] * wzs[dz];
-                int l = (i - x0) + LX * ((j - y0) + LY * (k - z0));
-                atomicAdd(&snum[l], vc * wgt);
-                atomicAdd(&sden[l], wgt);
+            for (int x = 0; x < NX; ++x) {
+                const int s = x + NX * (y + NY * z);
+                const int i = ox + x, j = oy + y, k = oz + z;
+                if (i < x0 || i > hix || j < y0 || j > hiy || k < z0 || k > hiz) continue;
+                if (den[s] == 0.0f) continue;
+                const int l = (i - x0) + LX * ((j - y0) + LY * (k - z0));
+                atomicAdd(&snum[l], num[s]);
+                atomicAdd(&sden[l], den[s]);
             }
-        }
-    }
 }
 
 template <int COMP, int TX, int TY, int TZ>
@@ -312,12 +345,10 @@ FS_DEV void p2g_write_comp(const P2GArgs& a, int x0, int y0, int z0, const float
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     constexpr int NU = (TX + 1) * TY * TZ, NV = TX * (TY + 1) * TZ, NW = TX * TY * (TZ + 1);
-    constexpr int MAXROWS = (TY + 2) * (TZ + 2);
     __shared__ float s_un[NU], s_ud[NU], s_vn[NV], s_vd[NV], s_wn[NW], s_wd[NW];
-    __shared__ int s_pref[MAXROWS + 1];
-    __shared__ int s_beg[MAXROWS];
 
-    const int ntx = (a.W + TX - 1) / TX, nty = (a.H + TY - 1) / TY;
+    const int W = a.W, H = a.H, D = a.D;
+    const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
@@ -325,47 +356,31 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     for (int l = threadIdx.x; l < NU; l += blockDim.x) { s_un[l] = 0.f; s_ud[l] = 0.f; }
     for (int l = threadIdx.x; l < NV; l += blockDim.x) { s_vn[l] = 0.f; s_vd[l] = 0.f; }
     for (int l = threadIdx.x; l < NW; l += blockDim.x) { s_wn[l] = 0.f; s_wd[l] = 0.f; }
-
-    const int hx0 = max(x0 - 1, 0), hx1 = min(x0 + TX, a.W - 1);
-    const int hy0 = max(y0 - 1, 0), hy1 = min(y0 + TY, a.H - 1);
-    const int hz0 = max(z0 - 1, 0), hz1 = min(z0 + TZ, a.D - 1);
-    const int nyr = hy1 - hy0 + 1, nzr = hz1 - hz0 + 1, nrows = nyr * nzr;
-    int cnt = 0;
-    if ((int)threadIdx.x < nrows) {
-        int r = threadIdx.x;
-        int cy = hy0 + r % nyr, cz = hz0 + r / nyr;
-        long long rowc = (long long)cy * a.W + (long long)cz * a.W * a.H;
-        int beg = a.start[rowc + hx0], end = a.start[rowc + hx1 + 1];
-        s_beg[r] = beg;
-        cnt = end - beg;
-    }
-    // MAXROWS <= blockDim: one row per thread, block-wide exclusive scan of counts
-    __shared__ int s_scan[4];
-    int tot;
-    int pre = fsk::block_excl_scan(cnt, s_scan, &tot);
-    if ((int)threadIdx.x < nrows) s_pref[threadIdx.x] = pre;
-    if (threadIdx.x == 0) s_pref[nrows] = tot;
     __syncthreads();
 
+    const int hx0 = max(x0 - 1, 0), hx1 = min(x0 + TX, W - 1);
+    const int hy0 = max(y0 - 1, 0), hy1 = min(y0 + TY, H - 1);
+    const int hz0 = max(z0 - 1, 0), hz1 = min(z0 + TZ, D - 1);
+    const int nxh = hx1 - hx0 + 1, nyh = hy1 - hy0 + 1, nzh = hz1 - hz0 + 1;
+    const int ncell = nxh * nyh * nzh;
     const float dx = a.dx;
-    for (int base = 0; base < tot; base += blockDim.x) {
-        // In a full chunk, consecutive lanes take particles 8 apart: with ~8
-        // particles per cell they land in different cells, so the LDS atomics
-        // of a wave hit different faces instead of serialising on one.
-        int t = (base + (int)blockDim.x <= tot) ? base + (int)((threadIdx.x & 31) * 8 + (threadIdx.x >> 5))
-                                                 : base + (int)threadIdx.x;
-        if (t >= tot) break;
-        int lo = 0, hi = nrows - 1;
-        while (lo < hi) {  // largest r with s_pref[r] <= t
-            int mid = (lo + hi + 1) >> 1;
-            if (s_pref[mid] <= t) lo = mid; else hi = mid - 1;
+    for (int cl = threadIdx.x; cl < ncell; cl += blockDim.x) {
+        const int ci = hx0 + cl % nxh, cj = hy0 + (cl / nxh) % nyh, ck = hz0 + cl / (nxh * nyh);
+        const long long c = (long long)ci + (long long)cj * W + (long long)ck * W * H;
+        const int beg = a.start[c], end = a.start[c + 1];
+        if (beg == end) continue;
+        float un[18], ud[18], vn[18], vd[18], wn[18], wd[18];
+#pragma unroll
+        for (int q = 0; q < 18; ++q) { un[q] = 0.f; ud[q] = 0.f; vn[q] = 0.f; vd[q] = 0.f; wn[q] = 0.f; wd[q] = 0.f; }
+        for (int p = beg; p < end; ++p) {
+            const float rx = a.P.px[p] / dx, ry = a.P.py[p] / dx, rz = a.P.pz[p] / dx;
+            window_accumulate<0>(mac_base<0>(rx, ry, rz, W, H, D), ci, cj, ck, a.P.vx[p], un, ud);
+            window_accumulate<1>(mac_base<1>(rx, ry, rz, W, H, D), ci, cj, ck, a.P.vy[p], vn, vd);
+            window_accumulate<2>(mac_base<2>(rx, ry, rz, W, H, D), ci, cj, ck, a.P.vz[p], wn, wd);
         }
-        int p = s_beg[lo] + (t - s_pref[lo]);
-        float px = a.P.px[p], py = a.P.py[p], pz = a.P.pz[p];
-        float rx = px / dx, ry = py / dx, rz = pz / dx;
-        p2g_scatter_comp<0, TX, TY, TZ>(rx, ry, rz, a.P.vx[p], a, x0, y0, z0, s_un, s_ud);
-        p2g_scatter_comp<1, TX, TY, TZ>(rx, ry, rz, a.P.vy[p], a, x0, y0, z0, s_vn, s_vd);
-        p2g_scatter_comp<2, TX, TY, TZ>(rx, ry, rz, a.P.vz[p], a, x0, y0, z0, s_wn, s_wd);
+        window_flush<0, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, un, ud, s_un, s_ud);
+        window_flush<1, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, vn, vd, s_vn, s_vd);
+        window_flush<2, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, wn, wd, s_wn, s_wd);
     }
     __syncthreads();
     p2g_write_comp<0, TX, TY, TZ>(a, x0, y0, z0, s_un, s_ud);
@@ -717,8 +732,6 @@ void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, c
 
 void launch_p2g(const P2GArgs& a, hipStream_t st) {
     constexpr int TX = P2G_TX, TY = P2G_TY, TZ = P2G_TZ;
-    static_assert((TY + 2) * (TZ + 2) <= 256, "one halo row per thread");
-    static_assert(256 % 32 == 0, "lane spreading assumes 256-thread blocks");
     int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
     hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(nb), dim3(256), 0, st, a);
 }
