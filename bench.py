@@ -50,9 +50,9 @@ def build_scene(name):
     raise SystemExit(f"unknown config {name}")
 
 
-def make_solver(lib, N, lab, body, ppc, seed, tol=1e-6):
+def make_solver(lib, N, lab, body, ppc, seed, tol=1e-6, flags=0):
     s = lib.solver(dt=1.0 / 120, dx=1.0 / N, pic_weight=0.05, cfl=1.0, solver_tol=tol, solver_max_iter=400,
-                   particles_per_cell=ppc, seed=seed)
+                   particles_per_cell=ppc, seed=seed, solver_flags=flags)
     s.set_labels(lab)
     if body is not None:
         s.set_body_sdf(body["sdf"], body["origin"], body["cell_size"], body["pos"], body["mass"])
@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the inner PCG iterations eagerly")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the PIC/FLIP step is not sharded across GPUs in this build "
@@ -86,7 +87,7 @@ def main():
     hip = load_library()
 
     N, lab, body, ppc, seed, desc = build_scene(args.config)
-    s = make_solver(hip, N, lab, body, ppc, seed)
+    s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0)
     Np = s.num_particles
     s.step(args.warmup)
     torch.cuda.synchronize()
@@ -130,6 +131,7 @@ def main():
                    "fluid_cells": nf, "precision": "FP32 transfers/stencil, FP64 dots + refinement"},
         "pressure_solve_ms_plain": ms["pressure"],
         "pressure_iters_last": st.iterations,
+        "pressure_outer_last": st.outer,
         "pressure_rel_residual_last": st.rel_residual,
         "stage_ms": {k: round(v, 4) for k, v in ms.items()},
         "particles_per_s_transfers": Np / ((ms["p2g"] + ms["g2p_advect"]) * 1e-3),

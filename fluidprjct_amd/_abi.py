@@ -15,6 +15,7 @@ FS_OK = 0
 FS_SOLID, FS_FLUID, FS_AIR, FS_BODY = 0, 1, 2, 3
 FS_FMT_PLY, FS_FMT_OFF = 0, 1
 FS_PRECOND_MG, FS_PRECOND_LEARNED, FS_PRECOND_JACOBI = 0, 1, 2
+FS_SOLVER_NO_GRAPH = 1
 (FS_GRID_U, FS_GRID_V, FS_GRID_W, FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED, FS_GRID_P,
  FS_GRID_LABELS, FS_GRID_RHS, FS_GRID_P64, FS_GRID_CELLKEY, FS_GRID_SOLID) = range(12)
 STAGES = ("label", "p2g", "pressure", "project", "body", "g2p_advect", "sort")
@@ -31,7 +32,7 @@ class FsConfig(C.Structure):
         ("particles_per_cell", C.c_int), ("seed", C.c_uint64), ("gravity_y", C.c_float),
         ("density", C.c_float), ("advect_c", C.c_float), ("solver_tol", C.c_double),
         ("solver_max_iter", C.c_int), ("precond", C.c_int), ("stall_window", C.c_int),
-        ("cfl", C.c_float),
+        ("cfl", C.c_float), ("solver_flags", C.c_int),
     ]
 
 

@@ -327,6 +327,7 @@ struct Solver {
     int stage_pressure() {
         Preconditioner* pre = nullptr;
         if (cfg.precond == FS_PRECOND_JACOBI) pre = jacobi.get();
+        mg.use_graph = !(cfg.solver_flags & FS_SOLVER_NO_GRAPH);
         int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats);
         if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
         if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");

@@ -11,7 +11,16 @@
 // damped Jacobi (omega = 6/7) pre/post smoothing and a fixed number of Jacobi
 // sweeps on the coarsest level keep the V-cycle a symmetric positive definite
 // linear operator, as CG requires.
+//
+// Kernel count per iteration is what limits small levels on MI355X (~10 us of
+// launch-to-launch latency each), so the V-cycle fuses two pre-smoothing sweeps
+// from zero everywhere and, on levels of at most FUSE_MAX cells, prolongation +
+// first post-smoothing sweep (it recomputes prolongations at the stencil points
+// and is slower than separate passes on large levels); the CG scalars are derived
+// inside the consumer kernels from <= 2048 FP64 partial sums, summed in the same
+// fixed order by every block (deterministic, no extra launch).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -22,10 +31,15 @@ using namespace fsd;
 
 namespace fsk {
 
-constexpr int RED_BLOCKS = 4096;
+constexpr int RED_BLOCKS = 2048;      // partial-sum slots of every reducing kernel
+constexpr int COARSE_MAX = 4096;      // cells of the LDS-resident coarsest level
+constexpr long long FUSE_MAX = 1LL << 16;  // levels up to this size use the fused (recomputing) transfer kernels
 constexpr uint32_t SELF = 0x8000u;
 
-enum { FIN_BNORM = 0, FIN_OUTER, FIN_RZ, FIN_PQ, FIN_RR };
+// partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
+enum { PS_RZ0 = 0, PS_RZ1, PS_PQ, PS_RR, PS_B2, PS_B1, PS_RES, PS_COUNT };
+
+enum { FIN_BNORM = 0, FIN_OUTER };
 
 FS_DEV float stencil_Ax(const float* __restrict__ x, long long c, uint32_t code, int W, long long WH) {
     float s = (float)((code >> 8) & 7u) * x[c];
@@ -47,6 +61,26 @@ FS_DEV double stencil_Ax64(const double* __restrict__ x, long long c, uint32_t c
     if (code & 16u) s -= x[c + WH];
     if (code & 32u) s -= x[c - WH];
     return s;
+}
+
+// Sum of n (<= RED_BLOCKS) partials in a fixed order, result in every thread.
+FS_DEV double sum_partials(const double* __restrict__ p, int n, double* sm) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) a += p[i];
+    a = wave_sum(a);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sm[w] = a;
+    __syncthreads();
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sm[q];
+    __syncthreads();   // sm is reused by the caller's next reduction
+    return t;
+}
+
+FS_DEV void store_partial(double v, double* sm, double* __restrict__ partial) {
+    double t = block_sum(v, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 // ---------------------------------------------------------------- level setup
@@ -134,7 +168,11 @@ __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
     Cc.type[c] = any2 ? 2 : (any1 ? 1 : 0);
 }
 
+// per-cell stencil code; counters[0] += unknowns, counters[1] |= Dirichlet contact
 __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ counters) {
+    __shared__ int s_cnt[2];
+    if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
+    __syncthreads();
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     int unk = 0, dir = 0;
     if (c < L.n) {
@@ -157,9 +195,6 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
         }
         L.code[c] = (uint16_t)code;
     }
-    __shared__ int s_cnt[2];
-    if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
-    __syncthreads();
     unk = wave_sum(unk);
     dir = wave_sum(dir);
     if ((threadIdx.x & 63) == 0) {
@@ -174,7 +209,8 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
 }
 
 // ---------------------------------------------------------------- smoothing
-template <bool FIRST, bool DOT>
+// one damped-Jacobi sweep; DOT: grid-strided, partial sums of b.x_new
+template <bool DOT>
 __global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restrict__ b, const float* __restrict__ xin,
                                                 float* __restrict__ xout, float omega, double* __restrict__ partial,
                                                 const SolveScalars* __restrict__ sc) {
@@ -186,21 +222,65 @@ __global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restri
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
         float xn = 0.0f;
-        if (code & SELF) {
-            uint32_t d = (code >> 8) & 7u;
-            if (d) {
-                float bc = b[c];
-                if (FIRST) xn = omega * bc / (float)d;
-                else xn = xin[c] + omega * (bc - stencil_Ax(xin, c, code, L.W, WH)) / (float)d;
-                if (DOT) acc += (double)bc * (double)xn;
-            }
+        uint32_t d = (code >> 8) & 7u;
+        if ((code & SELF) && d) {
+            float bc = b[c];
+            xn = xin[c] + omega * (bc - stencil_Ax(xin, c, code, L.W, WH)) / (float)d;
+            if (DOT) acc += (double)bc * (double)xn;
         }
         xout[c] = xn;
     }
-    if (DOT) {
-        double t = block_sum(acc, sm);
-        if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (DOT) store_partial(acc, sm, partial);
+}
+
+// diagonal preconditioner z = r / d with partial sums of r.z
+__global__ __launch_bounds__(256) void k_diag(MGLevel L, const float* __restrict__ r, float* __restrict__ z,
+                                              double* __restrict__ partial, const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        uint32_t code = L.code[c];
+        uint32_t d = (code >> 8) & 7u;
+        float zn = ((code & SELF) && d) ? r[c] / (float)d : 0.0f;
+        z[c] = zn;
+        acc += (double)r[c] * (double)zn;
     }
+    store_partial(acc, sm, partial);
+}
+
+// Two damped-Jacobi sweeps from x = 0 in one pass: the first sweep's value at
+// a neighbour, omega b_n / d_n, is recomputed from b and the level code.
+// omega2 = 0 gives a single sweep.
+FS_DEV float jacobi_first(const float* __restrict__ b, const uint16_t* __restrict__ code, long long c, float omega) {
+    uint32_t cd = code[c];
+    uint32_t d = (cd >> 8) & 7u;
+    return ((cd & SELF) && d) ? omega * b[c] / (float)d : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restrict__ b, float* __restrict__ xout,
+                                                 float omega, float omega2, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= L.n) return;
+    uint32_t code = L.code[c];
+    float xn = 0.0f;
+    uint32_t d = (code >> 8) & 7u;
+    if ((code & SELF) && d) {
+        const long long WH = (long long)L.W * L.H;
+        float bc = b[c];
+        float x1 = omega * bc / (float)d;
+        float ax = (float)d * x1;
+        if (code & 1u) ax -= jacobi_first(b, L.code, c + 1, omega);
+        if (code & 2u) ax -= jacobi_first(b, L.code, c - 1, omega);
+        if (code & 4u) ax -= jacobi_first(b, L.code, c + L.W, omega);
+        if (code & 8u) ax -= jacobi_first(b, L.code, c - L.W, omega);
+        if (code & 16u) ax -= jacobi_first(b, L.code, c + WH, omega);
+        if (code & 32u) ax -= jacobi_first(b, L.code, c - WH, omega);
+        xn = x1 + omega2 * (bc - ax) / (float)d;
+    }
+    xout[c] = xn;
 }
 
 __global__ __launch_bounds__(256) void k_residual(MGLevel L, const float* __restrict__ b, const float* __restrict__ x,
@@ -209,11 +289,15 @@ __global__ __launch_bounds__(256) void k_residual(MGLevel L, const float* __rest
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
     uint32_t code = L.code[c];
-    r[c] = (code & SELF) ? b[c] - stencil_Ax(x, c, code, L.W, (long long)L.W * L.H) : 0.0f;
+    float bc = b ? b[c] : 0.0f;
+    r[c] = (code & SELF) ? bc - stencil_Ax(x, c, code, L.W, (long long)L.W * L.H) : 0.0f;
 }
 
-// coarse b = 1/2 * P^T r (trilinear weights 1/4,3/4,3/4,1/4 per axis)
+// coarse b = 1/2 * P^T r with trilinear weights 1/4,3/4,3/4,1/4 per axis.
+// FUSED: r = b - A x of the fine unknowns is recomputed here (smaller levels).
+template <bool FUSED>
 __global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const float* __restrict__ rf,
+                                                  const float* __restrict__ bf, const float* __restrict__ xf,
                                                   float* __restrict__ bc, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -221,6 +305,7 @@ __global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const f
     if (!(Cc.code[c] & SELF)) { bc[c] = 0.0f; return; }
     int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
     const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+    const long long WH = (long long)F.W * F.H;
     float s = 0.0f;
     for (int dk = 0; dk < 4; ++dk) {
         int k = 2 * K - 1 + dk;
@@ -229,28 +314,30 @@ __global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const f
             int j = 2 * J - 1 + dj;
             if (j < 0 || j >= F.H) continue;
             float wjk = wt[dj] * wt[dk];
-            const float* row = rf + (size_t)j * F.W + (size_t)k * F.W * F.H;
+            const long long row = (long long)j * F.W + (long long)k * WH;
             for (int di = 0; di < 4; ++di) {
                 int i = 2 * I - 1 + di;
                 if (i < 0 || i >= F.W) continue;
-                s += wt[di] * wjk * row[i];
+                float rv;
+                if (FUSED) {
+                    const long long f = row + i;
+                    uint32_t code = F.code[f];
+                    rv = (code & SELF) ? bf[f] - stencil_Ax(xf, f, code, F.W, WH) : 0.0f;
+                } else {
+                    rv = rf[row + i];
+                }
+                s += wt[di] * wjk * rv;
             }
         }
     }
     bc[c] = 0.5f * s;
 }
 
-// fine x += P xc
-__global__ __launch_bounds__(256) void k_prolong(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
-                                                 float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
-    if (sc && sc->done) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= F.n) return;
-    if (!(F.code[c] & SELF)) return;
-    int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / ((long long)F.W * F.H));
+// (P xc) at fine cell (i,j,k)
+FS_DEV float prolong_at(const MGLevel& Cc, const float* __restrict__ xc, int i, int j, int k) {
     int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
-    int Ia = (i & 1) ? I0 + 1 : I0 - 1, Ja = (j & 1) ? J0 + 1 : J0 - 1, Ka = (k & 1) ? K0 + 1 : K0 - 1;
-    int Is[2] = {I0, Ia}, Js[2] = {J0, Ja}, Ks[2] = {K0, Ka};
+    int Is[2] = {I0, (i & 1) ? I0 + 1 : I0 - 1}, Js[2] = {J0, (j & 1) ? J0 + 1 : J0 - 1},
+        Ks[2] = {K0, (k & 1) ? K0 + 1 : K0 - 1};
     const float w[2] = {0.75f, 0.25f};
     float s = 0.0f;
     for (int a = 0; a < 2; ++a) {
@@ -263,11 +350,57 @@ __global__ __launch_bounds__(256) void k_prolong(MGLevel F, MGLevel Cc, const fl
             }
         }
     }
-    xf[c] += s;
+    return s;
+}
+
+// fine x += P xc on unknowns
+__global__ __launch_bounds__(256) void k_prolong(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
+                                                 float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= F.n) return;
+    if (!(F.code[c] & SELF)) return;
+    int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / ((long long)F.W * F.H));
+    xf[c] += prolong_at(Cc, xc, i, j, k);
+}
+
+// x' = x + P xc (fine unknowns), then one damped-Jacobi sweep on x' -> xout.
+// DOT: grid-strided, partial sums of b.xout.
+template <bool DOT>
+__global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, const float* __restrict__ b,
+                                                        const float* __restrict__ x, const float* __restrict__ xc,
+                                                        float* __restrict__ xout, float omega,
+                                                        double* __restrict__ partial,
+                                                        const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[4];
+    const long long WH = (long long)F.W * F.H;
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < F.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        uint32_t code = F.code[c];
+        float xn = 0.0f;
+        uint32_t d = (code >> 8) & 7u;
+        if ((code & SELF) && d) {
+            int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / WH);
+            float xp = x[c] + prolong_at(Cc, xc, i, j, k);
+            float ax = (float)d * xp;
+            if (code & 1u) ax -= x[c + 1] + prolong_at(Cc, xc, i + 1, j, k);
+            if (code & 2u) ax -= x[c - 1] + prolong_at(Cc, xc, i - 1, j, k);
+            if (code & 4u) ax -= x[c + F.W] + prolong_at(Cc, xc, i, j + 1, k);
+            if (code & 8u) ax -= x[c - F.W] + prolong_at(Cc, xc, i, j - 1, k);
+            if (code & 16u) ax -= x[c + WH] + prolong_at(Cc, xc, i, j, k + 1);
+            if (code & 32u) ax -= x[c - WH] + prolong_at(Cc, xc, i, j, k - 1);
+            float bc = b[c];
+            xn = xp + omega * (bc - ax) / (float)d;
+            if (DOT) acc += (double)bc * (double)xn;
+        }
+        xout[c] = xn;
+    }
+    if (DOT) store_partial(acc, sm, partial);
 }
 
 // coarsest level: fixed number of damped-Jacobi sweeps from zero in LDS
-constexpr int COARSE_MAX = 4096;
 __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* __restrict__ b, float* __restrict__ x,
                                                        int sweeps, float omega, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
@@ -287,10 +420,8 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* _
         for (int c = threadIdx.x; c < n; c += blockDim.x) {
             uint32_t code = scode[c];
             float xn = 0.0f;
-            if (code & SELF) {
-                uint32_t d = (code >> 8) & 7u;
-                if (d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) / (float)d;
-            }
+            uint32_t d = (code >> 8) & 7u;
+            if ((code & SELF) && d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) / (float)d;
             sx[cur ^ 1][c] = xn;
         }
         __syncthreads();
@@ -300,14 +431,19 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* _
 }
 
 // ---------------------------------------------------------------- CG kernels
-// p_new = z + beta p_old ; q = A p_new ; partial p_new.q
+// beta = (r.z)_new / (r.z)_old from the partial sums (0 on the first iteration of
+// a refinement pass); p_new = z + beta p_old ; q = A p_new ; partials of p_new.q
 __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __restrict__ z,
                                                       const float* __restrict__ pold, float* __restrict__ pnew,
-                                                      float* __restrict__ q, double* __restrict__ partial,
-                                                      const SolveScalars* __restrict__ sc) {
+                                                      float* __restrict__ q, const double* __restrict__ rz_new,
+                                                      const double* __restrict__ rz_old, int np,
+                                                      double* __restrict__ ppq, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
     __shared__ double sm[4];
-    const float beta = (float)sc->beta;
+    const bool first = sc->first != 0;
+    double bn = sum_partials(rz_new, np, sm);
+    double bo = first ? 1.0 : sum_partials(rz_old, np, sm);
+    const float beta = first ? 0.0f : (float)(bn / bo);
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
@@ -328,17 +464,19 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
         pnew[c] = pn;
         q[c] = qc;
     }
-    double t = block_sum(acc, sm);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    store_partial(acc, sm, ppq);
 }
 
-// e += alpha p ; r -= alpha q ; partial r.r
+// alpha = (r.z) / (p.q) ; e += alpha p ; r -= alpha q ; partials of r.r
 __global__ __launch_bounds__(256) void k_update(long long n, const float* __restrict__ p, const float* __restrict__ q,
                                                 float* __restrict__ e, float* __restrict__ r,
-                                                double* __restrict__ partial, const SolveScalars* __restrict__ sc) {
+                                                const double* __restrict__ rz, const double* __restrict__ pq, int np,
+                                                double* __restrict__ prr, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
     __shared__ double sm[4];
-    const float alpha = (float)sc->alpha;
+    double srz = sum_partials(rz, np, sm);
+    double spq = sum_partials(pq, np, sm);
+    const float alpha = (spq > 0.0) ? (float)(srz / spq) : 0.0f;
     double acc = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         float rc = r[c] - alpha * q[c];
@@ -346,26 +484,54 @@ __global__ __launch_bounds__(256) void k_update(long long n, const float* __rest
         r[c] = rc;
         acc += (double)rc * (double)rc;
     }
-    double t = block_sum(acc, sm);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    store_partial(acc, sm, prr);
 }
 
-// sum of b^2 and b over unknowns
-__global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restrict__ b, double* __restrict__ p1,
-                                               double* __restrict__ p2) {
+// end of an iteration: ||r||^2, iteration count, stop flags
+__global__ __launch_bounds__(256) void k_fin_iter(const double* __restrict__ rz, const double* __restrict__ pq,
+                                                  const double* __restrict__ rr, int np, SolveScalars* sc) {
+    if (sc->done) return;
     __shared__ double sm[4];
-    double a1 = 0.0, a2 = 0.0;
+    double srz = sum_partials(rz, np, sm);
+    double spq = sum_partials(pq, np, sm);
+    double srr = sum_partials(rr, np, sm);
+    if (threadIdx.x != 0) return;
+    sc->rz = srz;
+    sc->rr = srr;
+    sc->iters += 1;
+    sc->first = 0;
+    if (!(srz > 0.0) || !(spq > 0.0) || isnan(srr)) sc->done = 3;   // breakdown: restart from the true residual
+    else if (srr <= sc->target) sc->done = 1;
+    else if (sc->iters >= sc->maxit) sc->done = 4;
+}
+
+// r.z over the solve box (for preconditioners whose last kernel does not
+// accumulate it)
+__global__ __launch_bounds__(256) void k_dot_rz(long long n, const float* __restrict__ r, const float* __restrict__ z,
+                                                double* __restrict__ partial, const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    double acc = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
+        acc += (double)r[c] * (double)z[c];
+    store_partial(acc, sm, partial);
+}
+
+// sums of b^2 and b over unknowns
+__global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restrict__ b, double* __restrict__ p2,
+                                               double* __restrict__ p1) {
+    __shared__ double sm[4];
+    double a2 = 0.0, a1 = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         if (L.code[c] & SELF) {
             double v = b[c];
-            a1 += v * v;
-            a2 += v;
+            a2 += v * v;
+            a1 += v;
         }
     }
-    double t1 = block_sum(a1, sm);
-    double t2 = block_sum(a2, sm);
-    if (threadIdx.x == 0) { p1[blockIdx.x] = t1; p2[blockIdx.x] = t2; }
+    store_partial(a2, sm, p2);
+    store_partial(a1, sm, p1);
 }
 
 // x64 += e ; e = 0
@@ -376,7 +542,7 @@ __global__ __launch_bounds__(256) void k_axpy64(long long n, double* __restrict_
     e[c] = 0.0f;
 }
 
-// r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partial r.r (FP64)
+// r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partials of r.r (FP64)
 __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, const double* __restrict__ x64,
                                                  double shift, float* __restrict__ r, double* __restrict__ partial) {
     __shared__ double sm[4];
@@ -393,8 +559,7 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
         }
         r[c] = rf;
     }
-    double t = block_sum(acc, sm);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    store_partial(acc, sm, partial);
 }
 
 __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64, float* __restrict__ e) {
@@ -404,60 +569,28 @@ __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__
     e[c] = 0.0f;
 }
 
-__global__ __launch_bounds__(256) void k_to_float(long long n, const uint8_t* __restrict__ labels,
-                                                  const double* __restrict__ x64, float* __restrict__ p) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    p[c] = labels[c] == FS_FLUID ? (float)x64[c] : 0.0f;
-}
-
 __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __restrict__ p1,
                                                   const double* __restrict__ p2, int np, SolveScalars* sc,
                                                   double tol2b) {
     __shared__ double sm[4];
-    if (mode != FIN_BNORM && mode != FIN_OUTER && sc->done) return;
-    double a1 = 0.0, a2 = 0.0;
-    for (int i = threadIdx.x; i < np; i += blockDim.x) {
-        a1 += p1[i];
-        if (p2) a2 += p2[i];
-    }
-    double s1 = block_sum(a1, sm);
-    double s2 = block_sum(a2, sm);
+    double s1 = sum_partials(p1, np, sm);
+    double s2 = p2 ? sum_partials(p2, np, sm) : 0.0;
     if (threadIdx.x != 0) return;
-    switch (mode) {
-        case FIN_BNORM:
-            sc->bnorm2 = s1;
-            sc->bsum = s2;
-            break;
-        case FIN_OUTER:
-            sc->rr_true = s1;
-            sc->tol2b = tol2b;
-            if (s1 <= tol2b) {
-                sc->done = 2;
-            } else {
-                sc->done = 0;
-                sc->first = 1;
-                sc->rr = s1;
-                sc->rr_start = s1;
-                sc->target = fmax(sc->inner_rtol2 * s1, 0.25 * tol2b);
-            }
-            break;
-        case FIN_RZ:
-            if (!(s1 > 0.0) || isnan(s1)) { sc->done = 3; break; }
-            sc->beta = sc->first ? 0.0 : s1 / sc->rz;
-            sc->rz = s1;
-            break;
-        case FIN_PQ:
-            if (!(s1 > 0.0) || isnan(s1)) { sc->done = 3; break; }
-            sc->alpha = sc->rz / s1;
-            break;
-        case FIN_RR:
+    if (mode == FIN_BNORM) {
+        sc->bnorm2 = s1;
+        sc->bsum = s2;
+    } else {
+        sc->rr_true = s1;
+        sc->tol2b = tol2b;
+        if (s1 <= tol2b) {
+            sc->done = 2;
+        } else {
+            sc->done = 0;
+            sc->first = 1;
             sc->rr = s1;
-            sc->iters += 1;
-            sc->first = 0;
-            if (s1 <= sc->target) sc->done = 1;
-            else if (sc->iters >= sc->maxit) sc->done = 4;
-            break;
+            sc->rr_start = s1;
+            sc->target = fmax(sc->inner_rtol2 * s1, 0.25 * tol2b);
+        }
     }
 }
 
@@ -467,6 +600,7 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 MGPCG::~MGPCG() { free_all(); }
 
 void MGPCG::free_all() {
+    clear_graphs();
     for (size_t l = 0; l < lv.size(); ++l) {
         MGLevel& L = lv[l];
         hipFree(L.code); hipFree(L.type);
@@ -476,12 +610,12 @@ void MGPCG::free_all() {
     lv.clear();
     nlev = 0;
     hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(e); hipFree(bb); hipFree(xb);
-    hipFree(partial); hipFree(partial2); hipFree(sc); hipFree(counters);
+    hipFree(partial); hipFree(sc); hipFree(counters);
     if (sc_host) hipHostFree(sc_host);
     if (host_counters) hipHostFree(host_counters);
     r = z = pa = pb = q = e = bb = nullptr;
     xb = nullptr;
-    partial = partial2 = nullptr;
+    partial = nullptr;
     sc = nullptr;
     sc_host = nullptr;
     counters = host_counters = nullptr;
@@ -502,6 +636,7 @@ static int level_dims(int W, int H, int D, long long coarse_max, int dims[][3], 
 
 int MGPCG::init(int W_, int H_, int D_) {
     free_all();
+    coarse_max = std::min<long long>(coarse_max, COARSE_MAX);
     W = W_; H = H_; D = D_;
     N = (long long)W * H * D;
     int dims[40][3];
@@ -525,18 +660,17 @@ int MGPCG::init(int W_, int H_, int D_) {
     }
     for (float** p : {&r, &z, &pa, &pb, &q, &e, &bb}) A((void**)p, sizeof(float) * N);
     A((void**)&xb, sizeof(double) * N);
-    A((void**)&partial, sizeof(double) * RED_BLOCKS);
-    A((void**)&partial2, sizeof(double) * RED_BLOCKS);
+    A((void**)&partial, sizeof(double) * RED_BLOCKS * PS_COUNT);
     A((void**)&sc, sizeof(SolveScalars));
     A((void**)&counters, sizeof(int) * 16);
     if (hipHostMalloc((void**)&sc_host, sizeof(SolveScalars)) != hipSuccess) ok = false;
     if (hipHostMalloc((void**)&host_counters, sizeof(int) * 16) != hipSuccess) ok = false;
     if (!ok) { free_all(); return -1; }
-    if (lv.back().n > COARSE_MAX) { free_all(); return -2; }
     // CG direction buffers start finite: the first update multiplies them by beta = 0
     hipMemset(pa, 0, sizeof(float) * N);
     hipMemset(pb, 0, sizeof(float) * N);
     hipMemset(e, 0, sizeof(float) * N);
+    hipMemset(partial, 0, sizeof(double) * RED_BLOCKS * PS_COUNT);
     configure(W, H, D);
     return 0;
 }
@@ -553,6 +687,8 @@ void MGPCG::configure(int Wb, int Hb, int Db) {
     nred = (int)std::min<long long>(RED_BLOCKS, (n0 + 255) / 256);
 }
 
+double* MGPCG::slot(int s) { return partial + (size_t)s * RED_BLOCKS; }
+
 void MGPCG::setup(const uint8_t* labels, hipStream_t st) {
     hipLaunchKernelGGL(k_type0, dim3(nblk(n0)), dim3(256), 0, st, labels, lv[0], W, H, box[0], box[1], box[2]);
     for (int l = 0; l + 1 < nlev; ++l)
@@ -567,38 +703,52 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     MGLevel& L = lv[l];
     if (l == nlev - 1) {
         hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st, L, L.b, L.x, coarse_sweeps, omega, s);
-        if (l == 0 && pdot) {
-            // single-level hierarchy: r.z partials from the coarse result (omega = 0 keeps x)
-            hipLaunchKernelGGL((k_jacobi<false, true>), dim3(nred), dim3(256), 0, st, L, L.b, L.x, L.xt, 0.0f,
-                               pdot, s);
-            return L.xt;
-        }
+        if (l == 0 && pdot)   // single-level hierarchy: r.z of the coarse result
+            hipLaunchKernelGGL(k_dot_rz, dim3(nred), dim3(256), 0, st, L.n, L.b, L.x, pdot, s);
         return L.x;
     }
     const unsigned nb = nblk(L.n);
-    // the buffer written first is chosen so that the last smoothing sweep lands in L.x
-    float* cur = ((nu_pre + nu_post - 1) & 1) ? L.xt : L.x;
+    const unsigned ng = (l == 0) ? (unsigned)nred : nb;   // grid of the r.z-accumulating sweep
+    // Buffer writes after the first: the extra pre sweeps plus the post sweeps;
+    // the first buffer is chosen so that the last sweep lands in L.x.
+    const int npre = std::max(nu_pre, 1), npost = std::max(nu_post, 1);
+    // the fused prolongation+sweep writes the other buffer; the plain
+    // prolongation adds in place, so either way npost sweeps follow the pre-smoothing
+    const int extra = (npre >= 2 ? npre - 2 : 0) + npost;
+    float* cur = (extra & 1) ? L.xt : L.x;
     float* oth = (cur == L.x) ? L.xt : L.x;
-    hipLaunchKernelGGL((k_jacobi<true, false>), dim3(nb), dim3(256), 0, st, L, L.b, (const float*)nullptr, cur, omega,
-                       (double*)nullptr, s);
-    for (int k = 1; k < nu_pre; ++k) {
-        hipLaunchKernelGGL((k_jacobi<false, false>), dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega,
-                           (double*)nullptr, s);
+    hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, omega, npre >= 2 ? omega : 0.0f, s);
+    for (int k = 2; k < npre; ++k) {
+        hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr, s);
         std::swap(cur, oth);
     }
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
     MGLevel& Cn = lv[l + 1];
-    hipLaunchKernelGGL(k_restrict, dim3(nblk(Cn.n)), dim3(256), 0, st, L, Cn, L.r, Cn.b, s);
+    const bool fuse = L.n <= FUSE_MAX;
+    // (a restriction that recomputes the fine residuals is slower at every level
+    // size: one long serial thread per coarse cell)
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
+    hipLaunchKernelGGL(k_restrict<false>, dim3(nblk(Cn.n)), dim3(256), 0, st, L, Cn, L.r, (const float*)nullptr,
+                       (const float*)nullptr, Cn.b, s);
     float* xc = vcycle_level(l + 1, st, nullptr, s);
-    hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, xc, cur, s);
-    for (int k = 0; k < nu_post; ++k) {
-        bool last = (k == nu_post - 1);
-        if (last && pdot) {
-            hipLaunchKernelGGL((k_jacobi<false, true>), dim3(nred), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
-        } else {
-            hipLaunchKernelGGL((k_jacobi<false, false>), dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega,
+    int k0;
+    if (fuse) {
+        if (pdot && npost == 1)
+            hipLaunchKernelGGL(k_prolong_smooth<true>, dim3(ng), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, omega,
+                               pdot, s);
+        else
+            hipLaunchKernelGGL(k_prolong_smooth<false>, dim3(nb), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, omega,
                                (double*)nullptr, s);
-        }
+        std::swap(cur, oth);
+        k0 = 1;
+    } else {
+        hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, xc, cur, s);
+        k0 = 0;
+    }
+    for (int k = k0; k < npost; ++k) {
+        if (k == npost - 1 && pdot)
+            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
+        else
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr, s);
         std::swap(cur, oth);
     }
     return cur;
@@ -618,13 +768,15 @@ void MGPCG::apply_A(const float* x, float* y, hipStream_t st) {
                        (const SolveScalars*)nullptr);
 }
 
+void MGPCG::dot_rz(const float* rr, const float* zz, double* pdot, SolveScalars* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_dot_rz, dim3(nred), dim3(256), 0, st, n0, rr, zz, pdot, s);
+}
+
 class JacobiPre : public Preconditioner {
 public:
     explicit JacobiPre(MGPCG* s) : s_(s) {}
     void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) override {
-        MGLevel L = s_->lv[0];
-        hipLaunchKernelGGL((k_jacobi<true, true>), dim3(s_->nred), dim3(256), 0, st, L, r, (const float*)nullptr, z,
-                           1.0f, partial, sc);
+        hipLaunchKernelGGL(k_diag, dim3(s_->nred), dim3(256), 0, st, s_->lv[0], r, z, partial, sc);
     }
 
 private:
@@ -633,13 +785,67 @@ private:
 
 Preconditioner* make_jacobi_preconditioner(MGPCG* s) { return new JacobiPre(s); }
 
+// One batch of inner PCG iterations (an even number, so the two direction
+// buffers and the two r.z partial slots are in the same roles at the start of
+// every batch).
+void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
+    MGLevel& L0 = lv[0];
+    float* pold = pa;
+    float* pnew = pb;
+    int cur = 0;
+    for (int it = 0; it < batch; ++it) {
+        double* rz_new = slot(PS_RZ0 + cur);
+        double* rz_old = slot(PS_RZ0 + (cur ^ 1));
+        pre->apply(r, z, rz_new, sc, st);   // z = M r and the partial sums of r.z
+        hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old, nred,
+                           slot(PS_PQ), sc);
+        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r, rz_new, slot(PS_PQ), nred,
+                           slot(PS_RR), sc);
+        hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, slot(PS_PQ), slot(PS_RR), nred, sc);
+        std::swap(pold, pnew);
+        cur ^= 1;
+    }
+}
+
+// The batch as a HIP graph, cached per solve-box shape and preconditioner
+// (kernel arguments depend only on those).  The graph holds kernel nodes only.
+hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, hipStream_t st) {
+    const long long key[5] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key()};
+    for (auto& g : graphs)
+        if (std::equal(key, key + 5, g.key)) return g.exec;
+    hipGraph_t graph;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    record_batch(pre, st);
+    if (hipStreamEndCapture(st, &graph) != hipSuccess) return nullptr;
+    hipGraphExec_t ex;
+    if (hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) != hipSuccess) {
+        hipGraphDestroy(graph);
+        return nullptr;
+    }
+    hipGraphDestroy(graph);
+    if (graphs.size() >= 8) {
+        hipGraphExecDestroy(graphs.front().exec);
+        graphs.erase(graphs.begin());
+    }
+    GraphEntry g;
+    std::copy(key, key + 5, g.key);
+    g.exec = ex;
+    graphs.push_back(g);
+    return ex;
+}
+
+void MGPCG::clear_graphs() {
+    for (auto& g : graphs) hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
 int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
     // fluid bounding box, expanded by one cell so that every neighbour of a
     // FLUID cell (AIR/SOLID/BODY or out of domain) is inside the solve box
     int init_box[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
     hipMemcpyAsync(counters, init_box, sizeof(init_box), hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_bbox, dim3((unsigned)std::min<long long>(RED_BLOCKS, (N + 255) / 256)), dim3(256), 0, st,
-                       labels, W, H, N, counters);
+    hipLaunchKernelGGL(k_bbox, dim3((unsigned)std::min<long long>(4096, (N + 255) / 256)), dim3(256), 0, st, labels, W,
+                       H, N, counters);
     hipMemcpyAsync(host_counters, counters, sizeof(int) * 6, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
     const int* hb = host_counters;
@@ -660,6 +866,7 @@ int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
 
 int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
                  Preconditioner* pre, hipStream_t st, fs_solve_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
     MGPreconditioner mgp(this);
     if (!pre) pre = &mgp;
     fs_solve_stats loc{};
@@ -678,8 +885,8 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     init.maxit = maxit;
     init.inner_rtol2 = inner_rtol * inner_rtol;
     hipMemcpyAsync(sc, &init, sizeof(SolveScalars), hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_bnorm, dim3(nred), dim3(256), 0, st, L0, bb, partial, partial2);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_BNORM, partial, partial2, nred, sc, 0.0);
+    hipLaunchKernelGGL(k_bnorm, dim3(nred), dim3(256), 0, st, L0, bb, slot(PS_B2), slot(PS_B1));
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_BNORM, slot(PS_B2), slot(PS_B1), nred, sc, 0.0);
     hipMemcpyAsync(host_counters + 8, counters + 8, sizeof(int) * 2, hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
@@ -688,37 +895,31 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     loc.fluid_cells = nunk;
     loc.singular = singular ? 1 : 0;
     hipLaunchKernelGGL(k_fill0, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
-    double shift = singular ? sc_host->bsum / nunk : 0.0;
-    double bn2 = singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
-    double tol2b = tol * tol * bn2;
+    const double shift = singular ? sc_host->bsum / nunk : 0.0;
+    const double bn2 =
+        singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
+    const double tol2b = tol * tol * bn2;
     int total_iters = 0, outer = 0;
     bool converged = false;
     double rr_true = 0.0;
     for (outer = 0;; ++outer) {
         if (outer > 0) hipLaunchKernelGGL(k_axpy64, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
-        hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, partial);
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, partial, (const double*)nullptr,
-                           nred, sc, tol2b);
+        hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, slot(PS_RES));
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
+                           (const double*)nullptr, nred, sc, tol2b);
         hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
         if (hipStreamSynchronize(st) != hipSuccess) return -5;
         rr_true = sc_host->rr_true;
         total_iters = sc_host->iters;
         if (sc_host->done == 2 || bn2 == 0.0) { converged = true; break; }
         if (outer >= max_outer || total_iters >= maxit) break;
-        float* pold = pa;
-        float* pnew = pb;
         while (true) {
-            for (int it = 0; it < batch; ++it) {
-                pre->apply(r, z, partial, sc, st);
-                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_RZ, partial,
-                                   (const double*)nullptr, nred, sc, 0.0);
-                hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, partial, sc);
-                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_PQ, partial,
-                                   (const double*)nullptr, nred, sc, 0.0);
-                hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r, partial, sc);
-                hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_RR, partial,
-                                   (const double*)nullptr, nred, sc, 0.0);
-                std::swap(pold, pnew);
+            if (use_graph) {
+                hipGraphExec_t ex = batch_graph(pre, st);
+                if (!ex) return -5;
+                if (hipGraphLaunch(ex, st) != hipSuccess) return -5;
+            } else {
+                record_batch(pre, st);
             }
             hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
             if (hipStreamSynchronize(st) != hipSuccess) return -5;
@@ -733,6 +934,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     loc.outer = outer + 1;
     loc.converged = converged ? 1 : 0;
     loc.rel_residual = bn2 > 0 ? std::sqrt(rr_true / bn2) : 0.0;
+    loc.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (stats) *stats = loc;
     return converged ? 0 : -4;
 }

@@ -39,6 +39,8 @@ struct MGLevel {
 struct Preconditioner {
     virtual ~Preconditioner() {}
     virtual void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) = 0;
+    // identifies the launch sequence of apply() for graph caching
+    virtual const void* graph_key() const { return this; }
 };
 
 class MGPCG {
@@ -71,10 +73,10 @@ public:
     long long n0 = 0;              // cells of the solve box (level 0)
     int box[3] = {0, 0, 0};        // solve-box origin in the full grid
     int nu_pre = 2, nu_post = 2, coarse_sweeps = 64;
-    long long coarse_max = 512;   // coarsen until a level has at most this many cells
+    long long coarse_max = 512;   // coarsen until a level has at most this many cells (LDS-resident)
     float omega = 6.0f / 7.0f;
     int batch = 4;              // inner iterations launched between host checks
-    double inner_rtol = 1e-5;   // inner relative reduction per refinement pass
+    double inner_rtol = 1e-6;   // inner relative reduction per refinement pass
     int max_outer = 12;
     // fine-grid CG vectors
     float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *e = nullptr;
@@ -82,14 +84,25 @@ public:
     double* xb = nullptr;          // FP64 solution on the solve box
     int* counters = nullptr;       // device: bbox [0..5], unknown count [8], Dirichlet flag [9]
     int* host_counters = nullptr;  // pinned mirror
-    double* partial = nullptr;
-    double* partial2 = nullptr;
+    double* partial = nullptr;     // partial-sum slots of the reducing kernels
+    double* slot(int s);
     SolveScalars* sc = nullptr;       // device
     SolveScalars* sc_host = nullptr;  // pinned
     int nred = 0;   // number of partial-sum slots used by reducing kernels
     int last_iters = 0, last_outer = 0;
+    bool use_graph = true;      // replay inner batches as HIP graphs
+    // partial sums of r.z over the solve box (for preconditioners that do not fuse it)
+    void dot_rz(const float* r, const float* z, double* partial, SolveScalars* sc, hipStream_t st);
 
 private:
+    struct GraphEntry {
+        long long key[5];
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    void record_batch(Preconditioner* pre, hipStream_t st);
+    hipGraphExec_t batch_graph(Preconditioner* pre, hipStream_t st);
+    void clear_graphs();
     void free_all();
     float* vcycle_level(int l, hipStream_t st, double* partial_dot, SolveScalars* sc);
 };
@@ -102,6 +115,7 @@ public:
     void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) override {
         s_->vcycle(r, z, partial, sc, st);
     }
+    const void* graph_key() const override { return s_; }
 
 private:
     MGPCG* s_;

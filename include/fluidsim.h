@@ -105,7 +105,11 @@ typedef struct fs_config {
     int      stall_window;       /* learned-FPCG fallback after this many non-decreasing iterations (5) */
     float    cfl;                /* > 0: dt = min(dt, cfl*dx/max|v_particle|) each step; 0: fixed dt
                                     (no reference counterpart: the reference's dt is fixed) */
+    int      solver_flags;       /* FS_SOLVER_* bits */
 } fs_config;
+
+/* solver_flags */
+#define FS_SOLVER_NO_GRAPH 1     /* launch the inner PCG iterations eagerly instead of as a HIP graph */
 
 typedef struct fs_body_state {
     float pos[3], vel[3], omega[3];
