@@ -100,6 +100,20 @@ _SIGS = {
 }
 ABI_SYMBOLS = tuple(_SIGS)
 
+# include/fluidsim_learned.h (HIP library only)
+_LEARNED_SIGS = {
+    "fs_learned_init": (C.c_int, [_vp, C.c_uint64]),
+    "fs_learned_get_params": (C.c_int, [_vp, _vp, C.c_int]),
+    "fs_learned_set_params": (C.c_int, [_vp, _vp, C.c_int]),
+    "fs_learned_apply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]),
+    "fs_learned_loss_grad": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), _vp]),
+    "fs_learned_train_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                        C.POINTER(C.c_double)]),
+    "fs_set_preconditioner": (C.c_int, [_vp, C.c_int, C.c_int]),
+}
+LEARNED_SYMBOLS = tuple(_LEARNED_SIGS)
+FSL_NUM_PARAMS = 27 * 3 * 16 + 16 + 3 * (27 * 16 * 16 + 16) + 27 * 16 + 1
+
 
 class FsLib:
     """A loaded library exporting the fs_* ABI."""
@@ -113,6 +127,12 @@ class FsLib:
             fn = getattr(self.lib, name)
             fn.restype = res
             fn.argtypes = args
+        self.has_learned = all(hasattr(self.lib, n) for n in _LEARNED_SIGS)
+        if self.has_learned:
+            for name, (res, args) in _LEARNED_SIGS.items():
+                fn = getattr(self.lib, name)
+                fn.restype = res
+                fn.argtypes = args
         self.backend = self.lib.fs_backend().decode()
 
     def check(self, rc):
@@ -288,3 +308,59 @@ class Solver:
         self.fl.check(self.lib.fs_solve_pressure_system(self.h, _ptr(lab), _ptr(bb), _ptr(p), W, H, D,
                                                         float(tol), C.byref(st)))
         return p, st
+
+    # ---- learned preconditioner (include/fluidsim_learned.h)
+    def _need_learned(self):
+        if not self.fl.has_learned:
+            raise RuntimeError(f"{self.fl.path} has no learned preconditioner")
+
+    def set_preconditioner(self, precond, stall_window=0):
+        self._need_learned()
+        self.fl.check(self.lib.fs_set_preconditioner(self.h, int(precond), int(stall_window)))
+
+    def learned_init(self, seed):
+        self._need_learned()
+        self.fl.check(self.lib.fs_learned_init(self.h, int(seed)))
+
+    def learned_params(self):
+        self._need_learned()
+        out = np.zeros(FSL_NUM_PARAMS, np.float32)
+        self.fl.check(self.lib.fs_learned_get_params(self.h, _ptr(out), out.size))
+        return out
+
+    def set_learned_params(self, params):
+        self._need_learned()
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        self.fl.check(self.lib.fs_learned_set_params(self.h, _ptr(a), a.size))
+
+    def learned_apply(self, labels, r):
+        self._need_learned()
+        lab = np.ascontiguousarray(labels, dtype=np.uint8)
+        rr = np.ascontiguousarray(r, dtype=np.float32)
+        D, H, W = lab.shape
+        z = np.zeros((D, H, W), np.float32)
+        self.fl.check(self.lib.fs_learned_apply(self.h, _ptr(lab), _ptr(rr), _ptr(z), W, H, D))
+        return z
+
+    def learned_loss_grad(self, labels, b, p):
+        self._need_learned()
+        lab = np.ascontiguousarray(labels, dtype=np.uint8)
+        bb = np.ascontiguousarray(b, dtype=np.float32)
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        D, H, W = lab.shape
+        g = np.zeros(FSL_NUM_PARAMS, np.float32)
+        loss = C.c_double()
+        self.fl.check(self.lib.fs_learned_loss_grad(self.h, _ptr(lab), _ptr(bb), _ptr(pp), W, H, D, C.byref(loss),
+                                                    _ptr(g)))
+        return loss.value, g
+
+    def learned_train_step(self, labels, b, p, lr):
+        self._need_learned()
+        lab = np.ascontiguousarray(labels, dtype=np.uint8)
+        bb = np.ascontiguousarray(b, dtype=np.float32)
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        D, H, W = lab.shape
+        loss = C.c_double()
+        self.fl.check(self.lib.fs_learned_train_step(self.h, _ptr(lab), _ptr(bb), _ptr(pp), W, H, D, float(lr),
+                                                     C.byref(loss)))
+        return loss.value

@@ -18,7 +18,9 @@
 #include <vector>
 
 #include "../../include/fluidsim.h"
+#include "../../include/fluidsim_learned.h"
 #include "fs_kernels.hpp"
+#include "fs_learned.hpp"
 #include "fs_pressure.hpp"
 
 using fsd::f3;
@@ -137,6 +139,8 @@ struct Solver {
     double* d_bacc = nullptr;
     MGPCG mg;
     std::unique_ptr<Preconditioner> jacobi;
+    std::unique_ptr<LNet> net;             // learned preconditioner (created on first use)
+    std::unique_ptr<LearnedPre> learned;
     fs_solve_stats stats{};
     double times[FS_NUM_STAGES] = {0};
     float dt_cur = 0.0f;        // dt of the current step
@@ -324,9 +328,29 @@ struct Solver {
         return FS_OK;
     }
 
+    // the network, with random parameters from cfg.seed until others are set
+    LNet* get_net() {
+        if (!net) {
+            std::unique_ptr<LNet> n(new LNet());
+            if (n->init_params(cfg.seed, st) != 0) return nullptr;
+            net = std::move(n);
+        }
+        return net.get();
+    }
+
     int stage_pressure() {
         Preconditioner* pre = nullptr;
         if (cfg.precond == FS_PRECOND_JACOBI) pre = jacobi.get();
+        if (cfg.precond == FS_PRECOND_LEARNED) {
+            if (!learned) {
+                LNet* n = get_net();
+                // activations for the whole grid, so that no solve box reallocates them
+                if (!n || n->ensure(W, H, D) != 0) return fail(FS_ERR_DEVICE, "learned preconditioner allocation failed");
+                learned.reset(new LearnedPre(&mg, n));
+            }
+            pre = learned.get();
+        }
+        mg.stall_window = cfg.stall_window > 0 ? cfg.stall_window : 5;
         mg.use_graph = !(cfg.solver_flags & FS_SOLVER_NO_GRAPH);
         int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats);
         if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
@@ -931,6 +955,130 @@ int fs_solve_pressure_system(void* h, const uint8_t* labels, const float* b, dou
     HIPCHK(hipMemcpy(p, s.d_p64, sizeof(double) * (size_t)W * H * D, hipMemcpyDeviceToHost));
     if (stats) *stats = s.stats;
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- learned preconditioner
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+    bool alloc(size_t b) { return hipMalloc(&p, std::max<size_t>(b, 1)) == hipSuccess; }
+};
+
+// rms of x over the FLUID cells (1 when there are none or x is zero there)
+double fluid_rms(const uint8_t* labels, const float* x, size_t n, int* nf) {
+    double s = 0.0;
+    int c = 0;
+    for (size_t i = 0; i < n; ++i)
+        if (labels[i] == FS_FLUID) { s += (double)x[i] * x[i]; ++c; }
+    *nf = c;
+    double r = c > 0 ? std::sqrt(s / c) : 0.0;
+    return r > 0.0 && std::isfinite(r) ? r : 1.0;
+}
+
+int learned_forward_host(Solver& s, const uint8_t* labels, const float* r, int W, int H, int D, DevBuf& dl,
+                         DevBuf& dr, DevBuf& dz, DevBuf& dzn, double* scale, int* nf) {
+    if (!labels || !r || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad grid");
+    LNet* n = s.get_net();
+    if (!n) return fail(FS_ERR_DEVICE, "learned preconditioner allocation failed");
+    const size_t C = (size_t)W * H * D;
+    *scale = fluid_rms(labels, r, C, nf);
+    if (!dl.alloc(C) || !dr.alloc(sizeof(float) * C) || !dz.alloc(sizeof(float) * C) ||
+        !dzn.alloc(sizeof(float) * C) || n->ensure(W, H, D) != 0)
+        return fail(FS_ERR_DEVICE, "allocation failed");
+    HIPCHK(hipMemcpyAsync(dl.p, labels, C, hipMemcpyHostToDevice, s.st));
+    HIPCHK(hipMemcpyAsync(dr.p, r, sizeof(float) * C, hipMemcpyHostToDevice, s.st));
+    // FLUID = 1 and AIR = 2 are also the network's unknown / Dirichlet types
+    n->forward((const uint8_t*)dl.p, (const float*)dr.p, (float*)dz.p, nullptr, (float)*scale, (float*)dzn.p, s.st);
+    return FS_OK;
+}
+
+int learned_loss_grad(Solver& s, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
+                      double* loss, float* grad, float lr, bool step) {
+    if (!p) return fail(FS_ERR_ARG, "null target");
+    DevBuf dl, dr, dz, dzn, dt;
+    double scale;
+    int nf;
+    int rc = learned_forward_host(s, labels, b, W, H, D, dl, dr, dz, dzn, &scale, &nf);
+    if (rc) return rc;
+    const size_t C = (size_t)W * H * D;
+    std::vector<float> t(C);
+    for (size_t i = 0; i < C; ++i) t[i] = labels[i] == FS_FLUID ? (float)(p[i] / scale) : 0.0f;
+    if (!dt.alloc(sizeof(float) * C)) return fail(FS_ERR_DEVICE, "allocation failed");
+    HIPCHK(hipMemcpyAsync(dt.p, t.data(), sizeof(float) * C, hipMemcpyHostToDevice, s.st));
+    LNet* n = s.net.get();
+    n->backward((const uint8_t*)dl.p, (const float*)dt.p, (const float*)dzn.p, nf, s.st);
+    double l = 0.0;
+    HIPCHK(hipMemcpyAsync(&l, n->d_loss, sizeof(double), hipMemcpyDeviceToHost, s.st));
+    if (grad) HIPCHK(hipMemcpyAsync(grad, n->d_grad, sizeof(float) * LN_NPARAM, hipMemcpyDeviceToHost, s.st));
+    if (step) n->adam(lr, s.st);
+    HIPCHK(hipStreamSynchronize(s.st));
+    if (loss) *loss = l;
+    return FS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_learned_init(void* h, uint64_t seed) {
+    Solver& s = *(Solver*)h;
+    LNet* n = s.get_net();
+    if (!n || n->init_params(seed, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
+    return FS_OK;
+}
+
+int fs_learned_get_params(void* h, float* params, int n) {
+    Solver& s = *(Solver*)h;
+    if (!params || n != LN_NPARAM) return fail(FS_ERR_ARG, "expected FSL_NUM_PARAMS parameters");
+    LNet* net = s.get_net();
+    if (!net || net->get_params(params, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter download failed");
+    return FS_OK;
+}
+
+int fs_learned_set_params(void* h, const float* params, int n) {
+    Solver& s = *(Solver*)h;
+    if (!params || n != LN_NPARAM) return fail(FS_ERR_ARG, "expected FSL_NUM_PARAMS parameters");
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return fail(FS_ERR_ARG, "non-finite parameter");
+    LNet* net = s.get_net();
+    if (!net || net->set_params(params, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
+    return FS_OK;
+}
+
+int fs_learned_apply(void* h, const uint8_t* labels, const float* r, float* z, int W, int H, int D) {
+    Solver& s = *(Solver*)h;
+    if (!z) return fail(FS_ERR_ARG, "null output");
+    DevBuf dl, dr, dz, dzn;
+    double scale;
+    int nf;
+    int rc = learned_forward_host(s, labels, r, W, H, D, dl, dr, dz, dzn, &scale, &nf);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(z, dz.p, sizeof(float) * (size_t)W * H * D, hipMemcpyDeviceToHost, s.st));
+    HIPCHK(hipStreamSynchronize(s.st));
+    return FS_OK;
+}
+
+int fs_learned_loss_grad(void* h, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
+                         double* loss, float* grad) {
+    return learned_loss_grad(*(Solver*)h, labels, b, p, W, H, D, loss, grad, 0.0f, false);
+}
+
+int fs_learned_train_step(void* h, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
+                          float lr, double* loss) {
+    if (!(lr > 0.0f)) return fail(FS_ERR_ARG, "learning rate must be positive");
+    return learned_loss_grad(*(Solver*)h, labels, b, p, W, H, D, loss, nullptr, lr, true);
+}
+
+int fs_set_preconditioner(void* h, int precond, int stall_window) {
+    Solver& s = *(Solver*)h;
+    if (precond < FS_PRECOND_MG || precond > FS_PRECOND_JACOBI) return fail(FS_ERR_ARG, "unknown preconditioner");
+    s.cfg.precond = precond;
+    if (stall_window > 0) s.cfg.stall_window = stall_window;
+    return FS_OK;
 }
 
 }  // extern "C"

@@ -37,7 +37,7 @@ constexpr long long FUSE_MAX = 1LL << 16;  // levels up to this size use the fus
 constexpr uint32_t SELF = 0x8000u;
 
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
-enum { PS_RZ0 = 0, PS_RZ1, PS_PQ, PS_RR, PS_B2, PS_B1, PS_RES, PS_COUNT };
+enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_COUNT };
 
 enum { FIN_BNORM = 0, FIN_OUTER };
 
@@ -432,18 +432,31 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* _
 
 // ---------------------------------------------------------------- CG kernels
 // beta = (r.z)_new / (r.z)_old from the partial sums (0 on the first iteration of
-// a refinement pass); p_new = z + beta p_old ; q = A p_new ; partials of p_new.q
+// a refinement pass); flexible variant (zq given): beta = -(z.A p_old) / (p_old.A p_old),
+// which makes p_new A-conjugate to p_old.  p_new = z + beta p_old ; q = A p_new ;
+// partials of p_new.q
 __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __restrict__ z,
                                                       const float* __restrict__ pold, float* __restrict__ pnew,
                                                       float* __restrict__ q, const double* __restrict__ rz_new,
-                                                      const double* __restrict__ rz_old, int np,
+                                                      const double* __restrict__ rz_old,
+                                                      const double* __restrict__ zq,
+                                                      const double* __restrict__ pq_old, int np,
                                                       double* __restrict__ ppq, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
     __shared__ double sm[4];
     const bool first = sc->first != 0;
-    double bn = sum_partials(rz_new, np, sm);
-    double bo = first ? 1.0 : sum_partials(rz_old, np, sm);
-    const float beta = first ? 0.0f : (float)(bn / bo);
+    float beta = 0.0f;
+    if (!first) {
+        if (zq) {
+            double a = sum_partials(zq, np, sm);
+            double b = sum_partials(pq_old, np, sm);
+            beta = b > 0.0 ? (float)(-a / b) : 0.0f;
+        } else {
+            double bn = sum_partials(rz_new, np, sm);
+            double bo = sum_partials(rz_old, np, sm);
+            beta = (float)(bn / bo);
+        }
+    }
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
@@ -487,9 +500,13 @@ __global__ __launch_bounds__(256) void k_update(long long n, const float* __rest
     store_partial(acc, sm, prr);
 }
 
-// end of an iteration: ||r||^2, iteration count, stop flags
+// end of an iteration: ||r||^2, iteration count, stop flags.  flex: a flexible
+// preconditioner (r.z may take either sign; only p.q <= 0 is a breakdown);
+// stall_window > 0: done = 5 after that many iterations without a new smallest
+// ||r||^2 in the refinement pass.
 __global__ __launch_bounds__(256) void k_fin_iter(const double* __restrict__ rz, const double* __restrict__ pq,
-                                                  const double* __restrict__ rr, int np, SolveScalars* sc) {
+                                                  const double* __restrict__ rr, int np, int flex, int stall_window,
+                                                  SolveScalars* sc) {
     if (sc->done) return;
     __shared__ double sm[4];
     double srz = sum_partials(rz, np, sm);
@@ -500,9 +517,18 @@ __global__ __launch_bounds__(256) void k_fin_iter(const double* __restrict__ rz,
     sc->rr = srr;
     sc->iters += 1;
     sc->first = 0;
-    if (!(srz > 0.0) || !(spq > 0.0) || isnan(srr)) sc->done = 3;   // breakdown: restart from the true residual
+    const bool broken = flex ? (!(spq > 0.0) || isnan(srz)) : (!(srz > 0.0) || !(spq > 0.0));
+    if (broken || isnan(srr)) sc->done = 3;   // breakdown: restart from the true residual
     else if (srr <= sc->target) sc->done = 1;
     else if (sc->iters >= sc->maxit) sc->done = 4;
+    else if (stall_window > 0) {
+        if (srr < sc->best_rr) {
+            sc->best_rr = srr;
+            sc->stall = 0;
+        } else if (++sc->stall >= stall_window) {
+            sc->done = 5;
+        }
+    }
 }
 
 // r.z over the solve box (for preconditioners whose last kernel does not
@@ -515,6 +541,23 @@ __global__ __launch_bounds__(256) void k_dot_rz(long long n, const float* __rest
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
         acc += (double)r[c] * (double)z[c];
     store_partial(acc, sm, partial);
+}
+
+// r.z and z.q over the solve box
+__global__ __launch_bounds__(256) void k_dot_rz_zq(long long n, const float* __restrict__ r,
+                                                   const float* __restrict__ z, const float* __restrict__ q,
+                                                   double* __restrict__ prz, double* __restrict__ pzq,
+                                                   const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    double a = 0.0, b = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        const double zc = z[c];
+        a += (double)r[c] * zc;
+        b += zc * (double)q[c];
+    }
+    store_partial(a, sm, prz);
+    store_partial(b, sm, pzq);
 }
 
 // sums of b^2 and b over unknowns
@@ -589,6 +632,8 @@ __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __rest
             sc->first = 1;
             sc->rr = s1;
             sc->rr_start = s1;
+            sc->best_rr = s1;
+            sc->stall = 0;
             sc->target = fmax(sc->inner_rtol2 * s1, 0.25 * tol2b);
         }
     }
@@ -772,6 +817,10 @@ void MGPCG::dot_rz(const float* rr, const float* zz, double* pdot, SolveScalars*
     hipLaunchKernelGGL(k_dot_rz, dim3(nred), dim3(256), 0, st, n0, rr, zz, pdot, s);
 }
 
+void MGPCG::dot_rz_zq(const float* rr, const float* zz, double* pdot, SolveScalars* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_dot_rz_zq, dim3(nred), dim3(256), 0, st, n0, rr, zz, (const float*)q, pdot, slot(PS_ZQ), s);
+}
+
 class JacobiPre : public Preconditioner {
 public:
     explicit JacobiPre(MGPCG* s) : s_(s) {}
@@ -790,18 +839,23 @@ Preconditioner* make_jacobi_preconditioner(MGPCG* s) { return new JacobiPre(s); 
 // every batch).
 void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
     MGLevel& L0 = lv[0];
+    const bool flex = pre->flexible();
     float* pold = pa;
     float* pnew = pb;
     int cur = 0;
     for (int it = 0; it < batch; ++it) {
         double* rz_new = slot(PS_RZ0 + cur);
         double* rz_old = slot(PS_RZ0 + (cur ^ 1));
-        pre->apply(r, z, rz_new, sc, st);   // z = M r and the partial sums of r.z
-        hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old, nred,
-                           slot(PS_PQ), sc);
-        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r, rz_new, slot(PS_PQ), nred,
+        double* pq_new = slot(PS_PQ0 + cur);
+        double* pq_old = slot(PS_PQ0 + (cur ^ 1));
+        pre->apply(r, z, rz_new, sc, st);   // z = M r and the partial sums of r.z (and z.q when flexible)
+        hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old,
+                           flex ? (const double*)slot(PS_ZQ) : (const double*)nullptr, (const double*)pq_old, nred,
+                           pq_new, sc);
+        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r, rz_new, pq_new, nred,
                            slot(PS_RR), sc);
-        hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, slot(PS_PQ), slot(PS_RR), nred, sc);
+        hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, pq_new, slot(PS_RR), nred, flex ? 1 : 0,
+                           flex ? stall_window : 0, sc);
         std::swap(pold, pnew);
         cur ^= 1;
     }
@@ -810,9 +864,10 @@ void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
 // The batch as a HIP graph, cached per solve-box shape and preconditioner
 // (kernel arguments depend only on those).  The graph holds kernel nodes only.
 hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, hipStream_t st) {
-    const long long key[5] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key()};
+    const long long key[7] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
+                              pre->graph_epoch()};
     for (auto& g : graphs)
-        if (std::equal(key, key + 5, g.key)) return g.exec;
+        if (std::equal(key, key + 7, g.key)) return g.exec;
     hipGraph_t graph;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     record_batch(pre, st);
@@ -828,7 +883,7 @@ hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, hipStream_t st) {
         graphs.erase(graphs.begin());
     }
     GraphEntry g;
-    std::copy(key, key + 5, g.key);
+    std::copy(key, key + 7, g.key);
     g.exec = ex;
     graphs.push_back(g);
     return ex;
@@ -880,6 +935,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         if (stats) *stats = loc;
         return -3;
     }
+    if (pre->prepare_box() != 0) return -5;
     MGLevel& L0 = lv[0];
     SolveScalars init{};
     init.maxit = maxit;
@@ -894,6 +950,8 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     const bool singular = (nunk > 0) && !host_counters[9];
     loc.fluid_cells = nunk;
     loc.singular = singular ? 1 : 0;
+    n_unk = nunk;
+    hipMemcpyAsync(&sc->n_unknown, counters + 8, sizeof(int), hipMemcpyDeviceToDevice, st);
     hipLaunchKernelGGL(k_fill0, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
     const double shift = singular ? sc_host->bsum / nunk : 0.0;
     const double bn2 =
@@ -924,6 +982,11 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
             if (hipStreamSynchronize(st) != hipSuccess) return -5;
             if (sc_host->done) break;
+        }
+        // a flexible preconditioner that stalls or breaks down hands over to the V-cycle
+        if (pre->flexible() && (sc_host->done == 5 || sc_host->done == 3)) {
+            pre = &mgp;
+            loc.fallback = 1;
         }
     }
     hipLaunchKernelGGL(k_scatter_out, dim3(nblk(N)), dim3(256), 0, st, labels, N, W, H, L0, box[0], box[1], box[2], xb,

@@ -19,7 +19,7 @@ namespace fsk {
 
 struct SolveScalars {
     double rz, alpha, beta, rr, target, rr_true, bnorm2, bsum, tol2b, inner_rtol2, rr_start;
-    double zsum_unused;
+    double best_rr;   // smallest ||r||^2 of the refinement pass (stall detection)
     int done, iters, first, maxit, n_unknown, dirichlet, stall, pad;
 };
 
@@ -41,6 +41,14 @@ struct Preconditioner {
     virtual void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) = 0;
     // identifies the launch sequence of apply() for graph caching
     virtual const void* graph_key() const { return this; }
+    // a preconditioner that is not a fixed SPD operator (flexible CG: beta makes
+    // the new direction A-conjugate to the previous one; stall detection on)
+    virtual bool flexible() const { return false; }
+    // called once the solve box is configured, before apply() is recorded
+    // (allocations belong here, never inside apply); 0 on success
+    virtual int prepare_box() { return 0; }
+    // changes whenever buffers that apply() records are reallocated
+    virtual long long graph_epoch() const { return 0; }
 };
 
 class MGPCG {
@@ -93,10 +101,14 @@ public:
     bool use_graph = true;      // replay inner batches as HIP graphs
     // partial sums of r.z over the solve box (for preconditioners that do not fuse it)
     void dot_rz(const float* r, const float* z, double* partial, SolveScalars* sc, hipStream_t st);
+    // r.z and z.q (q = A p of the previous iteration) for flexible preconditioners
+    void dot_rz_zq(const float* r, const float* z, double* partial, SolveScalars* sc, hipStream_t st);
+    int stall_window = 5;       // flexible path: iterations without a new smallest residual before falling back
+    long long n_unk = 0;        // unknowns of the current solve
 
 private:
     struct GraphEntry {
-        long long key[5];
+        long long key[7];
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs;

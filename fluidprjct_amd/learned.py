@@ -1,0 +1,129 @@
+"""Training and loading of the learned pressure preconditioner
+(include/fluidsim_learned.h).
+
+Training pairs (rhs, labels -> p) come from the plain MGPCG solver running
+seeded dam breaks (fluid block extents U[0.2, 0.6] per axis) and sphere drops
+(radius U[0.05, 0.12], density U[300, 900], dropped into a pool): every step
+of a scene leaves the labels, the right-hand side and the FP64 pressure of its
+solve on the device, and those are read back as one pair.  The network is
+trained with Adam on 0.5 * mean over fluid cells of (M(b)/rms(b) - p/rms(b))^2.
+
+    python -m fluidprjct_amd.learned --pairs 2000 --grid 64 --iters 3000 \
+        --out fluidprjct_amd/learned_params.npy
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import scenes
+from ._abi import FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS, FS_PRECOND_LEARNED, FSL_NUM_PARAMS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_PARAMS = os.path.join(HERE, "learned_params.npy")
+
+
+def load_params(path: str = DEFAULT_PARAMS) -> np.ndarray:
+    p = np.load(path, allow_pickle=False)
+    if p.dtype != np.float32 or p.shape != (FSL_NUM_PARAMS,):
+        raise ValueError(f"{path}: expected {FSL_NUM_PARAMS} float32 parameters, got {p.dtype} {p.shape}")
+    return p
+
+
+def use_learned(solver, path: str = DEFAULT_PARAMS, stall_window: int = 5):
+    """Load trained parameters into a solver and select the learned preconditioner."""
+    solver.set_learned_params(load_params(path))
+    solver.set_preconditioner(FS_PRECOND_LEARNED, stall_window)
+
+
+def _scene(rng, N):
+    """labels and an optional body of one random training scene"""
+    if rng.random() < 0.5:
+        ext = tuple(float(e) for e in rng.uniform(0.2, 0.6, 3))
+        return scenes.dam_break(N, ext), None
+    r = float(rng.uniform(0.05, 0.12))
+    lab = scenes.water_column(N, float(rng.uniform(0.25, 0.5)))
+    sdf, origin, cs = scenes.sphere_sdf(r, 1.0 / N)
+    pos = (float(rng.uniform(0.3, 0.7)), float(rng.uniform(0.6, 0.8)), float(rng.uniform(0.3, 0.7)))
+    body = dict(sdf=sdf, origin=origin, cell_size=cs, pos=pos, mass=scenes.sphere_mass(r, float(rng.uniform(300, 900))))
+    return lab, body
+
+
+def generate_pairs(lib, n_pairs: int, N: int = 64, seed: int = 4, steps_per_scene: int = 40, ppc: int = 8,
+                   log=print):
+    """(labels uint8, b float32, p float32) triples, each (N, N, N)."""
+    rng = np.random.default_rng(seed)
+    pairs = []
+    t0 = time.perf_counter()
+    scene_id = 0
+    while len(pairs) < n_pairs:
+        lab, body = _scene(rng, N)
+        s = lib.solver(dt=1.0 / 120, dx=1.0 / N, pic_weight=0.05, cfl=1.0, solver_tol=1e-6, solver_max_iter=400,
+                       particles_per_cell=ppc, seed=seed * 1000 + scene_id)
+        try:
+            s.set_labels(lab)
+            if body is not None:
+                s.set_body_sdf(body["sdf"], body["origin"], body["cell_size"], body["pos"], body["mass"])
+            s.seed_particles(ppc, seed * 1000 + scene_id)
+            for _ in range(steps_per_scene):
+                s.step(1)
+                labels = s.get_grid(FS_GRID_LABELS).astype(np.uint8)
+                b = s.get_grid(FS_GRID_RHS).astype(np.float32)
+                p = s.get_grid(FS_GRID_P64).astype(np.float32)
+                if (labels == 1).any() and np.abs(b).max() > 0:
+                    pairs.append((labels, b, p))
+                if len(pairs) >= n_pairs:
+                    break
+        finally:
+            s.close()
+        scene_id += 1
+        if log and scene_id % 10 == 0:
+            log(f"[learned] {len(pairs)} pairs from {scene_id} scenes, {time.perf_counter() - t0:.1f} s")
+    return pairs
+
+
+def train(solver, pairs, iters: int = 3000, lr: float = 2e-3, seed: int = 0, log=print, log_every: int = 250):
+    """Adam with cosine decay over random pairs; returns the loss history."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(iters):
+        lab, b, p = pairs[int(rng.integers(len(pairs)))]
+        a = lr * 0.5 * (1.0 + np.cos(np.pi * it / iters))
+        hist.append(solver.learned_train_step(lab, b, p.astype(np.float64), max(a, 1e-6)))
+        if log and (it % log_every == 0 or it == iters - 1):
+            m = float(np.mean(hist[-log_every:]))
+            log(f"[learned] iter {it} loss(avg) {m:.5f} lr {a:.2e} {time.perf_counter() - t0:.1f} s")
+    return np.array(hist)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=2000)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--out", default=DEFAULT_PARAMS)
+    ap.add_argument("--init", default="", help="start from these parameters instead of a random draw")
+    args = ap.parse_args()
+    from . import load_library
+    lib = load_library()
+    pairs = generate_pairs(lib, args.pairs, args.grid, args.seed)
+    N = args.grid
+    s = lib.solver(dx=1.0 / N, seed=args.seed)
+    if args.init:
+        s.set_learned_params(load_params(args.init))
+    else:
+        s.learned_init(args.seed)
+    hist = train(s, pairs, args.iters, args.lr)
+    np.save(args.out, s.learned_params())
+    print(f"[learned] saved {args.out}; loss first 100 {hist[:100].mean():.5f} last 100 {hist[-100:].mean():.5f}",
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()
