@@ -60,6 +60,21 @@ def make_solver(lib, N, lab, body, ppc, seed, tol=1e-6, flags=0):
     return s
 
 
+KERNEL_NAMES = {"g2p_advect": "k_g2p_advect_tile", "p2g": "k_p2g_tile", "project": "k_project"}
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per launch of kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{config}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def stage_delta(a, b):
     return {k: b[k] - a[k] for k in a}
 
@@ -73,6 +88,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the inner PCG iterations eagerly")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--learned-steps", type=int, default=5)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a float32 y = x + 1 kernel over MIB MiB (read MIB, write MIB): the "
+                         "reference for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the PIC/FLIP step is not sharded across GPUs in this build "
@@ -83,6 +103,15 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.zeros(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.add(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+        return
     from fluidprjct_amd import load_library
     hip = load_library()
 
@@ -136,11 +165,33 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in ms.items()},
         "particles_per_s_transfers": Np / ((ms["p2g"] + ms["g2p_advect"]) * 1e-3),
         "dt_last": dt_last,
-        "roofline": {"bound": "hbm", "kernel": {"g2p_advect": "k_g2p_advect", "p2g": "k_p2g_tile",
-                                                 "project": "k_project"}[dom],
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "alg_bytes": alg[dom], "avg_launch_ms": ms[dom]},
+                     "traffic": measured_traffic(KERNEL_NAMES[dom], args.config), "alg_bytes": alg[dom],
+                     "avg_launch_ms": ms[dom]},
     }
+    # learned vs plain: the same workload continued with the learned
+    # preconditioner (flexible PCG, multigrid fallback after 5 stalled iterations)
+    if not args.no_learned:
+        from fluidprjct_amd.learned import DEFAULT_PARAMS, use_learned
+        use_learned(s, DEFAULT_PARAMS, 5)
+        s.step(1)
+        torch.cuda.synchronize()
+        t0s = s.stage_times()
+        fb = 0
+        iters = []
+        for _ in range(args.learned_steps):
+            s.step(1)
+            st = s.solve_stats()
+            fb += st.fallback
+            iters.append(st.iterations)
+        torch.cuda.synchronize()
+        tl = stage_delta(t0s, s.stage_times())
+        result["pressure_solve_ms_learned"] = tl["pressure"] / args.learned_steps
+        result["learned"] = {"steps": args.learned_steps, "fallbacks": fb, "iterations": iters,
+                             "params": os.path.relpath(DEFAULT_PARAMS, ROOT),
+                             "note": "same workload continued after the timed steps; flexible PCG with the "
+                                     "BF16 conv net, multigrid fallback after 5 stalled iterations"}
     del s
     torch.cuda.empty_cache()
 

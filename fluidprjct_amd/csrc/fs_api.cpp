@@ -979,6 +979,30 @@ double fluid_rms(const uint8_t* labels, const float* x, size_t n, int* nf) {
     return r > 0.0 && std::isfinite(r) ? r : 1.0;
 }
 
+// p.(A p) of the pressure matrix of the labels (FluidSolver3D.cu:1598-1716):
+// on FLUID cells diag = #in-domain FLUID/AIR neighbours, -1 per FLUID neighbour
+double a_norm2(const uint8_t* labels, const double* p, int W, int H, int D) {
+    const long long WH = (long long)W * H;
+    double s = 0.0;
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const long long c = x + (long long)W * y + WH * z;
+                if (labels[c] != FS_FLUID) continue;
+                double ap = 0.0;
+                const int nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y},
+                          nz[6] = {z, z, z, z, z + 1, z - 1};
+                for (int q = 0; q < 6; ++q) {
+                    if (nx[q] < 0 || nx[q] >= W || ny[q] < 0 || ny[q] >= H || nz[q] < 0 || nz[q] >= D) continue;
+                    const long long n = nx[q] + (long long)W * ny[q] + WH * nz[q];
+                    if (labels[n] == FS_FLUID || labels[n] == FS_AIR) ap += p[c];
+                    if (labels[n] == FS_FLUID) ap -= p[n];
+                }
+                s += p[c] * ap;
+            }
+    return s;
+}
+
 int learned_forward_host(Solver& s, const uint8_t* labels, const float* r, int W, int H, int D, DevBuf& dl,
                          DevBuf& dr, DevBuf& dz, DevBuf& dzn, double* scale, int* nf) {
     if (!labels || !r || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad grid");
@@ -1005,8 +1029,12 @@ int learned_loss_grad(Solver& s, const uint8_t* labels, const float* b, const do
     int rc = learned_forward_host(s, labels, b, W, H, D, dl, dr, dz, dzn, &scale, &nf);
     if (rc) return rc;
     const size_t C = (size_t)W * H * D;
+    // target p / sqrt(p.Ap / n_fluid): flexible PCG is invariant to the scale
+    // of z, so only the shape of p is learned, with a loss of order one
+    double sp = nf > 0 ? std::sqrt(std::max(0.0, a_norm2(labels, p, W, H, D)) / nf) : 0.0;
+    if (!(sp > 0.0) || !std::isfinite(sp)) sp = 1.0;
     std::vector<float> t(C);
-    for (size_t i = 0; i < C; ++i) t[i] = labels[i] == FS_FLUID ? (float)(p[i] / scale) : 0.0f;
+    for (size_t i = 0; i < C; ++i) t[i] = labels[i] == FS_FLUID ? (float)(p[i] / sp) : 0.0f;
     if (!dt.alloc(sizeof(float) * C)) return fail(FS_ERR_DEVICE, "allocation failed");
     HIPCHK(hipMemcpyAsync(dt.p, t.data(), sizeof(float) * C, hipMemcpyHostToDevice, s.st));
     LNet* n = s.net.get();

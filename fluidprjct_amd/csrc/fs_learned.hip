@@ -271,20 +271,34 @@ __global__ __launch_bounds__(256) void k_upadd16(LGrid f, LGrid c, const uint16_
 }
 
 // ---------------------------------------------------------------- backward
-// dz = (zn - target) * mask / n_fluid ; loss partials 0.5 (zn - target)^2 / n_fluid
-__global__ __launch_bounds__(256) void k_loss_grad(long long n, const float* __restrict__ zn,
+// A-norm loss of the unscaled output: d = zn - target on the unknowns,
+// loss = 0.5 d.(A d) / n_fluid, dz = A d / n_fluid (A symmetric), with A the
+// pressure matrix of the types: diag = #in-grid unknown/Dirichlet neighbours,
+// -1 per unknown neighbour
+__global__ __launch_bounds__(256) void k_loss_grad(LGrid g, const float* __restrict__ zn,
                                                    const float* __restrict__ target, const uint8_t* __restrict__ type,
                                                    double inv_nf, float* __restrict__ dz, double* __restrict__ loss) {
     __shared__ double sm[4];
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0.0;
-    if (c < n) {
-        float d = 0.0f;
+    if (c < g.n) {
+        double ad = 0.0;
         if (type[c] == 1) {
-            d = zn[c] - target[c];
-            l = 0.5 * (double)d * (double)d * inv_nf;
+            const int x = (int)(c % g.W), y = (int)((c / g.W) % g.H), z = (int)(c / ((long long)g.W * g.H));
+            const double d = (double)zn[c] - (double)target[c];
+            const long long WH = (long long)g.W * g.H;
+            const int nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y},
+                      nz[6] = {z, z, z, z, z + 1, z - 1};
+            const long long off[6] = {1, -1, g.W, -(long long)g.W, WH, -WH};
+            for (int q = 0; q < 6; ++q) {
+                if (nx[q] < 0 || nx[q] >= g.W || ny[q] < 0 || ny[q] >= g.H || nz[q] < 0 || nz[q] >= g.D) continue;
+                const uint8_t t = type[c + off[q]];
+                if (t == 1 || t == 2) ad += d;
+                if (t == 1) ad -= (double)zn[c + off[q]] - (double)target[c + off[q]];
+            }
+            l = 0.5 * d * ad * inv_nf;
         }
-        dz[c] = (float)((double)d * inv_nf);
+        dz[c] = (float)(ad * inv_nf);
     }
     double t = block_sum(l, sm);
     if (threadIdx.x == 0 && t != 0.0) atomicAdd(loss, t);
@@ -578,7 +592,7 @@ void LNet::backward(const uint8_t* type, const float* target, const float* zn, i
     hipMemsetAsync(d_grad, 0, sizeof(float) * LN_NPARAM, st);
     hipMemsetAsync(d_loss, 0, sizeof(double), st);
     const double inv_nf = n_fluid > 0 ? 1.0 / n_fluid : 0.0;
-    hipLaunchKernelGGL(k_loss_grad, dim3(nblk(g0.n)), dim3(256), 0, st, g0.n, zn, target, type, inv_nf, dz, d_loss);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nblk(g0.n)), dim3(256), 0, st, g0, zn, target, type, inv_nf, dz, d_loss);
     // E
     hipLaunchKernelGGL((k_wgrad_valu<0>), dim3(ntiles(g0)), dim3(256), 0, st, g0, s0, (const float*)nullptr,
                        (const uint8_t*)nullptr, dz, (const uint16_t*)nullptr, d_grad + LN_EW, d_grad + LN_EB);

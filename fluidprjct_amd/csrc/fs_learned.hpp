@@ -37,8 +37,9 @@ public:
     // scale_host.  zn (optional): the unscaled output (training).
     void forward(const uint8_t* type, const float* r, float* z, const SolveScalars* sc, float scale_host, float* zn,
                  hipStream_t st);
-    // gradient of 0.5 mean_fluid (zn - target)^2 into d_grad (zeroed first); the
-    // forward pass must have run with zn.  loss partial sums -> d_loss[0].
+    // gradient of the A-norm loss 0.5 (zn - target).A(zn - target) / n_fluid into
+    // d_grad (zeroed first); the forward pass must have run with zn.  The loss
+    // -> d_loss[0].
     void backward(const uint8_t* type, const float* target, const float* zn, int n_fluid, hipStream_t st);
     void adam(float lr, hipStream_t st);
     void refresh_weights(hipStream_t st);   // FP32 params -> BF16 kernel layouts

@@ -37,7 +37,7 @@ constexpr long long FUSE_MAX = 1LL << 16;  // levels up to this size use the fus
 constexpr uint32_t SELF = 0x8000u;
 
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
-enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_COUNT };
+enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_RP, PS_COUNT };
 
 enum { FIN_BNORM = 0, FIN_OUTER };
 
@@ -434,14 +434,16 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* _
 // beta = (r.z)_new / (r.z)_old from the partial sums (0 on the first iteration of
 // a refinement pass); flexible variant (zq given): beta = -(z.A p_old) / (p_old.A p_old),
 // which makes p_new A-conjugate to p_old.  p_new = z + beta p_old ; q = A p_new ;
-// partials of p_new.q
+// partials of p_new.q, and of r.p_new when prp is given (the exact line search
+// of the flexible variant, alpha = r.p / p.q)
 __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __restrict__ z,
                                                       const float* __restrict__ pold, float* __restrict__ pnew,
                                                       float* __restrict__ q, const double* __restrict__ rz_new,
                                                       const double* __restrict__ rz_old,
                                                       const double* __restrict__ zq,
                                                       const double* __restrict__ pq_old, int np,
-                                                      double* __restrict__ ppq, const SolveScalars* __restrict__ sc) {
+                                                      double* __restrict__ ppq, const float* __restrict__ r,
+                                                      double* __restrict__ prp, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
     __shared__ double sm[4];
     const bool first = sc->first != 0;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
         }
     }
     const long long WH = (long long)L.W * L.H;
-    double acc = 0.0;
+    double acc = 0.0, acc_rp = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
@@ -473,14 +475,16 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
             if (code & 16u) qc -= z[c + WH] + beta * pold[c + WH];
             if (code & 32u) qc -= z[c - WH] + beta * pold[c - WH];
             acc += (double)pn * (double)qc;
+            if (prp) acc_rp += (double)r[c] * (double)pn;
         }
         pnew[c] = pn;
         q[c] = qc;
     }
     store_partial(acc, sm, ppq);
+    if (prp) store_partial(acc_rp, sm, prp);
 }
 
-// alpha = (r.z) / (p.q) ; e += alpha p ; r -= alpha q ; partials of r.r
+// alpha = (r.z) / (p.q) (flexible: rz holds r.p) ; e += alpha p ; r -= alpha q ; partials of r.r
 __global__ __launch_bounds__(256) void k_update(long long n, const float* __restrict__ p, const float* __restrict__ q,
                                                 float* __restrict__ e, float* __restrict__ r,
                                                 const double* __restrict__ rz, const double* __restrict__ pq, int np,
@@ -851,9 +855,9 @@ void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
         pre->apply(r, z, rz_new, sc, st);   // z = M r and the partial sums of r.z (and z.q when flexible)
         hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old,
                            flex ? (const double*)slot(PS_ZQ) : (const double*)nullptr, (const double*)pq_old, nred,
-                           pq_new, sc);
-        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r, rz_new, pq_new, nred,
-                           slot(PS_RR), sc);
+                           pq_new, (const float*)r, flex ? slot(PS_RP) : (double*)nullptr, sc);
+        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r,
+                           flex ? (const double*)slot(PS_RP) : (const double*)rz_new, pq_new, nred, slot(PS_RR), sc);
         hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, pq_new, slot(PS_RR), nred, flex ? 1 : 0,
                            flex ? stall_window : 0, sc);
         std::swap(pold, pnew);

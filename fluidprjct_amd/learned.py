@@ -6,7 +6,11 @@ seeded dam breaks (fluid block extents U[0.2, 0.6] per axis) and sphere drops
 (radius U[0.05, 0.12], density U[300, 900], dropped into a pool): every step
 of a scene leaves the labels, the right-hand side and the FP64 pressure of its
 solve on the device, and those are read back as one pair.  The network is
-trained with Adam on 0.5 * mean over fluid cells of (M(b)/rms(b) - p/rms(b))^2.
+trained with Adam on the A-norm loss of fs_learned_loss_grad (the error norm
+that CG minimises).  Rounds of residual pairs follow: for a pair (b, p) the
+residual r = b - c A M(b) left by one exact line search along the network's
+direction and its solution e = p - c M(b) form a new pair, so the network also
+learns the residuals it meets inside flexible PCG.
 
     python -m fluidprjct_amd.learned --pairs 2000 --grid 64 --iters 3000 \
         --out fluidprjct_amd/learned_params.npy
@@ -85,6 +89,43 @@ def generate_pairs(lib, n_pairs: int, N: int = 64, seed: int = 4, steps_per_scen
     return pairs
 
 
+def apply_A(lab, x):
+    """A x of the pressure matrix (FluidSolver3D.cu:1598-1716) in FP64 on the
+    host, for building residual pairs: on FLUID cells diag = #in-domain
+    FLUID/AIR neighbours, -1 per FLUID neighbour."""
+    f = lab == 1
+    fa = (lab == 1) | (lab == 2)
+    out = np.zeros(lab.shape, np.float64)
+    for ax in range(3):
+        for lo, hi in ((slice(0, -1), slice(1, None)), (slice(1, None), slice(0, -1))):
+            dst = [slice(None)] * 3
+            src = [slice(None)] * 3
+            dst[ax], src[ax] = lo, hi
+            dst, src = tuple(dst), tuple(src)
+            out[dst] += np.where(fa[src], x[dst], 0.0) - np.where(f[src], x[src], 0.0)
+    return np.where(f, out, 0.0)
+
+
+def residual_pairs(solver, pairs, n, seed=0):
+    """(labels, r, e) from one line search along M(b) for n random pairs"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in rng.choice(len(pairs), size=min(n, len(pairs)), replace=False):
+        lab, b, p = pairs[int(i)]
+        z = solver.learned_apply(lab, b).astype(np.float64)
+        az = apply_A(lab, z)
+        den = float((z * az).sum())
+        if not den > 0:
+            continue
+        c = float((b.astype(np.float64) * z).sum()) / den
+        f = lab == 1
+        r = np.where(f, b - c * az, 0.0)
+        e = np.where(f, p.astype(np.float64) - c * z, 0.0)
+        if np.abs(r).max() > 0:
+            out.append((lab, r.astype(np.float32), e.astype(np.float32)))
+    return out
+
+
 def train(solver, pairs, iters: int = 3000, lr: float = 2e-3, seed: int = 0, log=print, log_every: int = 250):
     """Adam with cosine decay over random pairs; returns the loss history."""
     rng = np.random.default_rng(seed)
@@ -109,6 +150,8 @@ def main():
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--out", default=DEFAULT_PARAMS)
     ap.add_argument("--init", default="", help="start from these parameters instead of a random draw")
+    ap.add_argument("--rounds", type=int, default=3, help="rounds of residual pairs after the first training")
+    ap.add_argument("--round-pairs", type=int, default=300)
     args = ap.parse_args()
     from . import load_library
     lib = load_library()
@@ -120,6 +163,10 @@ def main():
     else:
         s.learned_init(args.seed)
     hist = train(s, pairs, args.iters, args.lr)
+    train_set = list(pairs)
+    for rd in range(args.rounds):
+        train_set += residual_pairs(s, train_set, args.round_pairs, seed=rd)
+        hist = train(s, train_set, args.iters, args.lr * 0.5, seed=rd + 1)
     np.save(args.out, s.learned_params())
     print(f"[learned] saved {args.out}; loss first 100 {hist[:100].mean():.5f} last 100 {hist[-100:].mean():.5f}",
           flush=True)

@@ -42,9 +42,12 @@ int fs_learned_set_params(void* s, const float* params, int n);
 /* z = M(r) on a W x H x D grid (labels select the fluid cells; z = 0 elsewhere). */
 int fs_learned_apply(void* s, const uint8_t* labels, const float* r, float* z, int W, int H, int D);
 
-/* Loss 0.5 * mean over fluid cells of (M(b)/rms(b) - p/rms(b))^2 and its
- * gradient with respect to the parameters (grad: FSL_NUM_PARAMS floats, may be
- * null).  Nothing is updated. */
+/* Loss 0.5 d.(A d) / n_fluid with d = zn(b) - p / sqrt(p.Ap / n_fluid) on the
+ * fluid cells (A the pressure matrix of the labels; zn = M(b)/rms(b) the
+ * unscaled network output, rms over the fluid cells), and its gradient with
+ * respect to the parameters (grad: FSL_NUM_PARAMS floats, may be null).  The
+ * A-norm is the error norm that CG minimises; flexible PCG is invariant to the
+ * scale of z, so the target is the shape of p.  Nothing is updated. */
 int fs_learned_loss_grad(void* s, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
                          double* loss, float* grad);
 

@@ -147,6 +147,28 @@ std::vector<double> up_t(const Grid& f, const Grid& c, const std::vector<double>
     return out;
 }
 
+// A x of the pressure matrix (FluidSolver3D.cu:1598-1716) on FLUID cells: diag
+// = #in-grid FLUID/AIR neighbours, -1 per FLUID neighbour; 0 elsewhere
+std::vector<double> apply_A(const Grid& g, const uint8_t* labels, const std::vector<double>& x) {
+    std::vector<double> out(g.n(), 0.0);
+    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int z = 0; z < g.D; ++z)
+        for (int y = 0; y < g.H; ++y)
+            for (int xx = 0; xx < g.W; ++xx) {
+                const long long c = g.at(xx, y, z);
+                if (labels[c] != 1) continue;
+                double s = 0.0;
+                for (const auto& o : off) {
+                    if (!g.in(xx + o[0], y + o[1], z + o[2])) continue;
+                    const long long n = g.at(xx + o[0], y + o[1], z + o[2]);
+                    if (labels[n] == 1 || labels[n] == 2) s += x[c];
+                    if (labels[n] == 1) s -= x[n];
+                }
+                out[c] = s;
+            }
+    return out;
+}
+
 struct Net {
     Grid g0, g1, g2;
     bool bf;
@@ -224,20 +246,21 @@ struct Net {
         zn = conv(g0, s0, 16, we.data(), &prm[EB], 1);
     }
 
-    // gradient of 0.5 sum_fluid (zn - target)^2 / nf
+    // gradient of the A-norm loss 0.5 d.(A d) / nf, d = zn - target on FLUID cells
     double backward(const uint8_t* labels, const double* target, double* grad) {
         const long long n0 = g0.n();
         long long nf = 0;
         for (long long v = 0; v < n0; ++v) nf += labels[v] == 1;
         const double inv = nf > 0 ? 1.0 / nf : 0.0;
-        std::vector<double> dz(n0, 0.0);
-        double loss = 0.0;
+        std::vector<double> d(n0, 0.0), dz(n0, 0.0);
         for (long long v = 0; v < n0; ++v)
-            if (labels[v] == 1) {
-                double d = zn[v] - target[v];
-                loss += 0.5 * d * d * inv;
-                dz[v] = d * inv;
-            }
+            if (labels[v] == 1) d[v] = zn[v] - target[v];
+        std::vector<double> ad = apply_A(g0, labels, d);
+        double loss = 0.0;
+        for (long long v = 0; v < n0; ++v) {
+            loss += 0.5 * d[v] * ad[v] * inv;
+            dz[v] = ad[v] * inv;
+        }
         std::vector<double> gw(NPARAM, 0.0);
         // E (data gradient with the FP32 weights)
         conv_wgrad(g0, s0, 16, dz, 1, &gw[EW], &gw[EB]);
@@ -305,16 +328,27 @@ int oracle_lnet_forward(const float* params, const uint8_t* labels, const float*
     return 0;
 }
 
-/* loss 0.5 mean_fluid (zn - p / rms(b))^2 and its gradient (NPARAM doubles) */
+/* A-norm loss 0.5 d.(A d) / nf, d = zn - p / sqrt(p.Ap / nf) on the FLUID cells,
+   and its gradient (NPARAM doubles); zn is the unscaled output for the input b
+   (normalised by rms(b)) */
 int oracle_lnet_loss_grad(const float* params, const uint8_t* labels, const float* b, const double* p, int W, int H,
                           int D, int bf16, double* loss, double* grad) {
     if (!params || !labels || !b || !p || W <= 0 || H <= 0 || D <= 0) return -1;
     Net net(params, W, H, D, bf16);
     net.forward(labels, b);
     const long long n = net.g0.n();
+    long long nf = 0;
+    std::vector<double> pv(n, 0.0);
+    for (long long v = 0; v < n; ++v)
+        if (labels[v] == 1) { pv[v] = p[v]; ++nf; }
+    std::vector<double> ap = apply_A(net.g0, labels, pv);
+    double pap = 0.0;
+    for (long long v = 0; v < n; ++v) pap += pv[v] * ap[v];
+    double sp = nf > 0 ? std::sqrt(std::max(0.0, pap) / nf) : 0.0;
+    if (!(sp > 0.0) || !std::isfinite(sp)) sp = 1.0;
     std::vector<double> t(n, 0.0);
     for (long long v = 0; v < n; ++v)
-        if (labels[v] == 1) t[v] = bf16 ? (double)(float)(p[v] / net.scale) : p[v] / net.scale;
+        if (labels[v] == 1) t[v] = bf16 ? (double)(float)(p[v] / sp) : p[v] / sp;
     double l = net.backward(labels, t.data(), grad);
     if (loss) *loss = l;
     return 0;

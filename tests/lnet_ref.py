@@ -132,3 +132,25 @@ def fluid_rms(labels, r):
         return 1.0
     s = float(np.sqrt(np.mean(r[f].astype(np.float64) ** 2)))
     return s if s > 0 else 1.0
+
+
+def apply_A(lab, p):
+    """A p of the reference pressure matrix (FluidSolver3D.cu:1598-1716) in FP64:
+    on FLUID cells diag = #in-domain FLUID/AIR neighbours, -1 per FLUID neighbour."""
+    f = lab == 1
+    fa = (lab == 1) | (lab == 2)
+    out = np.zeros(lab.shape, np.float64)
+    for ax in range(3):
+        for sgn in (1, -1):
+            nb_fa = np.zeros(lab.shape, bool)
+            nb_p = np.zeros(lab.shape, np.float64)
+            src = [slice(None)] * 3
+            dst = [slice(None)] * 3
+            if sgn == 1:
+                dst[ax], src[ax] = slice(0, -1), slice(1, None)
+            else:
+                dst[ax], src[ax] = slice(1, None), slice(0, -1)
+            nb_fa[tuple(dst)] = fa[tuple(src)]
+            nb_p[tuple(dst)] = np.where(f[tuple(src)], p[tuple(src)], 0.0)
+            out += np.where(nb_fa, p, 0.0) - nb_p
+    return np.where(f, out, 0.0)

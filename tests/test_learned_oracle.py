@@ -100,15 +100,48 @@ def test_scale_equivariance_and_zero_input():
     assert (z0[lab != 1] == 0).all()
 
 
+def test_apply_A_matches_assembled_matrix(oracle):
+    """The numpy stencil used by these tests equals the oracle's assembled
+    reference matrix (upper CSR of constructA, FluidSolver3D.cu:1718-1777)."""
+    import ctypes as C
+    lab, _ = scene(16)
+    x = np.random.default_rng(17).normal(size=SHAPE) * (lab == 1)
+    D, H, W = SHAPE
+    f = lab.ravel() == 1
+    nf = int(f.sum())
+    asm = oracle.lib.oracle_assemble_upper_csr
+    asm.restype = C.c_int64
+    asm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lab_c = np.ascontiguousarray(lab)
+    nnz = asm(lab_c.ctypes.data, W, H, D, None, None, None)
+    off = np.zeros(nf + 1, np.int32)
+    col = np.zeros(nnz, np.int32)
+    val = np.zeros(nnz, np.float32)
+    asm(lab_c.ctypes.data, W, H, D, off.ctypes.data, col.ctypes.data, val.ctypes.data)
+    A = np.zeros((nf, nf))
+    for i in range(nf):
+        for k in range(off[i], off[i + 1]):
+            A[i, col[k]] = val[k]
+            A[col[k], i] = val[k]
+    xf = x.ravel()[f]
+    np.testing.assert_allclose(L.apply_A(lab, x).ravel()[f], A @ xf, rtol=1e-12, atol=1e-12)
+
+
 def test_loss_definition():
     lab, b = scene(6)
     p = L.random_params(8)
     target = np.random.default_rng(9).normal(size=SHAPE) * (lab == 1)
     loss, _ = L.loss_grad(p, lab, b, target)
     _, zn = L.forward(p, lab, b)
-    s = L.fluid_rms(lab, b)
     f = lab == 1
-    assert loss == pytest.approx(0.5 * np.mean((zn[f] - target[f] / s) ** 2), rel=1e-12)
+    nf = f.sum()
+    t = np.where(f, target, 0.0)
+    sp = np.sqrt(np.sum(t * L.apply_A(lab, t)) / nf)
+    d = np.where(f, zn - t / sp, 0.0)
+    assert loss == pytest.approx(0.5 * np.sum(d * L.apply_A(lab, d)) / nf, rel=1e-12)
+    # invariant to the scale of the target
+    loss2, _ = L.loss_grad(p, lab, b, 7.0 * target)
+    assert loss2 == pytest.approx(loss, rel=1e-12)
 
 
 def test_gradient_matches_central_differences():
