@@ -155,17 +155,19 @@ def test_training_lowers_loss_and_gains_over_no_network(hip):
     fallback in well under half the iterations of PCG without it."""
     from fluidprjct_amd.learned import residual_pairs, train
     N = 32
-    pairs = generate_pairs(hip, 100, N=N, seed=7, steps_per_scene=20, log=None)
+    pairs = generate_pairs(hip, 160, N=N, seed=7, steps_per_scene=20, log=None)
     s = hip.solver(dx=1.0 / N, seed=5)
     s.learned_init(5)
     held = pairs[-4:]
     pairs = pairs[:-4]
     before = np.mean([s.learned_loss_grad(l, b, p)[0] for l, b, p in held])
-    hist = train(s, pairs, iters=1500, lr=2e-3, log=None)
-    pairs = pairs + residual_pairs(s, pairs, 80, seed=1)
-    hist2 = train(s, pairs, iters=1500, lr=1e-3, log=None, seed=1)
+    hist = train(s, pairs, iters=2000, lr=2e-3, log=None)
+    assert np.isfinite(hist).all()
+    for rd in range(3):
+        pairs = pairs + residual_pairs(s, pairs, 120, seed=rd)
+        hist = train(s, pairs, iters=2000, lr=1e-3, log=None, seed=rd + 1)
+        assert np.isfinite(hist).all()
     after = np.mean([s.learned_loss_grad(l, b, p)[0] for l, b, p in held])
-    assert np.isfinite(hist).all() and np.isfinite(hist2).all()
     assert after < 0.1 * before, (before, after)
     gains = []
     for lab, b, _ in held:
@@ -183,6 +185,28 @@ def test_training_lowers_loss_and_gains_over_no_network(hip):
         pj, stj = s.solve_pressure_system(lab, b, 1e-6)
         assert st.converged == 1 and st.rel_residual <= 1e-6 and st.fallback == 0
         assert st.iterations < 0.5 * stj.iterations, (st.iterations, stj.iterations)
+
+
+def test_shipped_parameters_gain_at_64(hip):
+    """The committed parameters (fluidprjct_amd/learned_params.npy) on a 64^3
+    dam-break step: no fallback, under a third of the Jacobi-PCG iterations."""
+    from fluidprjct_amd import scenes
+    from fluidprjct_amd._abi import FS_GRID_LABELS, FS_GRID_RHS
+    from fluidprjct_amd.learned import use_learned
+    N = 64
+    s = hip.solver(dt=1 / 120, dx=1.0 / N, pic_weight=0.05, cfl=1.0, particles_per_cell=8, seed=2,
+                   solver_max_iter=1000)
+    s.set_labels(scenes.dam_break(N, (0.3, 0.5, 0.8)))
+    s.seed_particles(8, 2)
+    s.step(8)
+    lab, b = s.get_grid(FS_GRID_LABELS), s.get_grid(FS_GRID_RHS)
+    use_learned(s)
+    p, st = s.solve_pressure_system(lab, b, 1e-6)
+    s.set_preconditioner(FS_PRECOND_JACOBI)
+    pj, stj = s.solve_pressure_system(lab, b, 1e-6)
+    assert st.converged == 1 and st.fallback == 0
+    assert st.iterations < stj.iterations / 3, (st.iterations, stj.iterations)
+    assert rel(p, pj) <= 1e-4
 
 
 def identity_params():
