@@ -114,6 +114,17 @@ struct HostBody {
     }
 };
 
+// 1/x when x is a positive normal power of two whose inverse is also normal
+// (then v * (1/x) and v / x are the same bits for every v), else 0
+float pow2_inverse(float x) {
+    if (!(x > 0.0f) || !std::isnormal(x)) return 0.0f;
+    int e = 0;
+    const float m = std::frexp(x, &e);
+    if (m != 0.5f) return 0.0f;
+    const float inv = std::ldexp(1.0f, 1 - e);
+    return std::isnormal(inv) ? inv : 0.0f;
+}
+
 struct Solver {
     fs_config cfg{};
     int W = 0, H = 0, D = 0;
@@ -318,6 +329,7 @@ struct Solver {
         a.start = d_start;
         a.W = W; a.H = H; a.D = D;
         a.dx = cfg.dx; a.dt = dt_cur; a.gravity_y = cfg.gravity_y;
+        a.inv_dx = pow2_inverse(cfg.dx);
         a.u = d_u; a.v = d_v; a.w = d_w; a.us = d_us; a.vs = d_vs; a.ws = d_ws;
         launch_p2g(a, st);
         return FS_OK;
@@ -399,6 +411,7 @@ struct Solver {
         a.n = (int)np;
         a.W = W; a.H = H; a.D = D;
         a.dx = cfg.dx; a.dt = dt_cur; a.alpha = cfg.pic_weight; a.C = cfg.advect_c;
+        a.inv_dx = pow2_inverse(cfg.dx);
         a.u = d_u; a.v = d_v; a.w = d_w; a.us = d_us; a.vs = d_vs; a.ws = d_ws;
         a.nu = (int)NU(); a.nv = (int)NV(); a.nw = (int)NW();
         a.labels = d_labels;

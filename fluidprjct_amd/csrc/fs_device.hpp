@@ -43,6 +43,25 @@ FS_DEV int cell_key(float px, float py, float pz, float dx, int W, int H, int D)
     return i + j * W + k * (W * H);
 }
 
+// x / dx.  P2: dx is a power of two and inv = 1/dx, so the product is the
+// same bits as the IEEE quotient (both are exact scalings by 2^k) at a fraction
+// of the cost of a correctly rounded division.
+template <bool P2>
+FS_DEV float div_dx(float x, float dx, float inv) {
+    return P2 ? x * inv : x / dx;
+}
+
+template <bool P2>
+FS_DEV int cell_key_t(float px, float py, float pz, float dx, float inv, int W, int H, int D) {
+    int i = (int)floorf(div_dx<P2>(px, dx, inv));
+    int j = (int)floorf(div_dx<P2>(py, dx, inv));
+    int k = (int)floorf(div_dx<P2>(pz, dx, inv));
+    i = i < 0 ? 0 : (i >= W ? W - 1 : i);
+    j = j < 0 ? 0 : (j >= H ? H - 1 : j);
+    k = k < 0 ? 0 : (k >= D ? D - 1 : k);
+    return i + j * W + k * (W * H);
+}
+
 FS_DEV float clamp01(float f) { return fminf(fmaxf(f, 0.0f), 1.0f); }
 
 // Base corner and fractions of one MAC component (FluidSolver3D.cu:476-571, :833-939).

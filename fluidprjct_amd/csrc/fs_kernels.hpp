@@ -56,6 +56,7 @@ struct P2GArgs {
     const int* start;
     int W, H, D;
     float dx, dt, gravity_y;
+    float inv_dx;     // 1/dx when dx is a power of two, else 0
     float *u, *v, *w, *us, *vs, *ws;
 };
 
@@ -64,6 +65,7 @@ struct G2PArgs {
     int n;
     int W, H, D;
     float dx, dt, alpha, C;
+    float inv_dx;     // 1/dx when dx is a power of two (then x * inv_dx == x / dx exactly), else 0
     const float *u, *v, *w, *us, *vs, *ws;
     int nu, nv, nw;
     const uint8_t* labels;

@@ -342,7 +342,7 @@ FS_DEV void p2g_write_comp(const P2GArgs& a, int x0, int y0, int z0, const float
     }
 }
 
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, bool P2>
 __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     constexpr int NU = (TX + 1) * TY * TZ, NV = TX * (TY + 1) * TZ, NW = TX * TY * (TZ + 1);
     __shared__ float s_un[NU], s_ud[NU], s_vn[NV], s_vd[NV], s_wn[NW], s_wd[NW];
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     const int hz0 = max(z0 - 1, 0), hz1 = min(z0 + TZ, D - 1);
     const int nxh = hx1 - hx0 + 1, nyh = hy1 - hy0 + 1, nzh = hz1 - hz0 + 1;
     const int ncell = nxh * nyh * nzh;
-    const float dx = a.dx;
+    const float dx = a.dx, inv = a.inv_dx;
     for (int cl = threadIdx.x; cl < ncell; cl += blockDim.x) {
         const int ci = hx0 + cl % nxh, cj = hy0 + (cl / nxh) % nyh, ck = hz0 + cl / (nxh * nyh);
         const long long c = (long long)ci + (long long)cj * W + (long long)ck * W * H;
@@ -372,11 +372,18 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
         float un[18], ud[18], vn[18], vd[18], wn[18], wd[18];
 #pragma unroll
         for (int q = 0; q < 18; ++q) { un[q] = 0.f; ud[q] = 0.f; vn[q] = 0.f; vd[q] = 0.f; wn[q] = 0.f; wd[q] = 0.f; }
+        // the next particle's six values are loaded while the current one is accumulated
+        float px = a.P.px[beg], py = a.P.py[beg], pz = a.P.pz[beg];
+        float vx = a.P.vx[beg], vy = a.P.vy[beg], vz = a.P.vz[beg];
         for (int p = beg; p < end; ++p) {
-            const float rx = a.P.px[p] / dx, ry = a.P.py[p] / dx, rz = a.P.pz[p] / dx;
-            window_accumulate<0>(mac_base<0>(rx, ry, rz, W, H, D), ci, cj, ck, a.P.vx[p], un, ud);
-            window_accumulate<1>(mac_base<1>(rx, ry, rz, W, H, D), ci, cj, ck, a.P.vy[p], vn, vd);
-            window_accumulate<2>(mac_base<2>(rx, ry, rz, W, H, D), ci, cj, ck, a.P.vz[p], wn, wd);
+            const int q = p + 1 < end ? p + 1 : p;
+            const float npx = a.P.px[q], npy = a.P.py[q], npz = a.P.pz[q];
+            const float nvx = a.P.vx[q], nvy = a.P.vy[q], nvz = a.P.vz[q];
+            const float rx = div_dx<P2>(px, dx, inv), ry = div_dx<P2>(py, dx, inv), rz = div_dx<P2>(pz, dx, inv);
+            window_accumulate<0>(mac_base<0>(rx, ry, rz, W, H, D), ci, cj, ck, vx, un, ud);
+            window_accumulate<1>(mac_base<1>(rx, ry, rz, W, H, D), ci, cj, ck, vy, vn, vd);
+            window_accumulate<2>(mac_base<2>(rx, ry, rz, W, H, D), ci, cj, ck, vz, wn, wd);
+            px = npx; py = npy; pz = npz; vx = nvx; vy = nvy; vz = nvz;
         }
         window_flush<0, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, un, ud, s_un, s_ud);
         window_flush<1, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, vn, vd, s_vn, s_vd);
@@ -420,8 +427,10 @@ __constant__ signed char c_proj_off[34][3] = {
     {-2, -2, 0},  {0, -2, -2}, {-2, -2, -2}, {-2, 0, -2}, {2, -1, 0},  {0, -1, 2}};
 
 // projectParticleDevice3D :1161-1239 (FLUID pass only, as the reference's loop runs once)
-FS_DEV bool project_particle(f3& pos, const uint8_t* __restrict__ labels, int W, int H, int D, float dx) {
-    int cx = (int)floorf(pos.x / dx), cy = (int)floorf(pos.y / dx), cz = (int)floorf(pos.z / dx);
+template <bool P2>
+FS_DEV bool project_particle(f3& pos, const uint8_t* __restrict__ labels, int W, int H, int D, float dx, float inv) {
+    int cx = (int)floorf(div_dx<P2>(pos.x, dx, inv)), cy = (int)floorf(div_dx<P2>(pos.y, dx, inv)),
+        cz = (int)floorf(div_dx<P2>(pos.z, dx, inv));
     float bestD = 1e10f;
     int best = -1;
     for (int n = 0; n < 34; ++n) {
@@ -461,7 +470,7 @@ __global__ __launch_bounds__(256) void k_max_speed(ParticlesView P, int n, unsig
 // the RK3 substeps reads LDS.  A sample whose 8 corners are not all inside the
 // staged box (a particle that left the margin) reads global memory instead,
 // with the same indices, so both paths return the same values.
-template <int BX, int BY, int BZ>
+template <int BX, int BY, int BZ, bool P2>
 struct TileFields {
     int X0, Y0, Z0;
     const float* su;
@@ -482,15 +491,16 @@ struct TileFields {
     }
 
     template <int COMP>
-    FS_DEV float sample(const Base& b) const {
+    FS_DEV float lds_sample(const Base& b, int i0) const {
         constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1);
-        int i0;
-        if (in_box<COMP>(b, i0)) {
-            const float* s = COMP == 0 ? su : (COMP == 1 ? sv : sw);
-            constexpr int SXY = LX * LY;
-            return trilin(b.fx, b.fy, b.fz, s[i0], s[i0 + 1], s[i0 + SXY], s[i0 + 1 + SXY], s[i0 + LX],
-                          s[i0 + 1 + LX], s[i0 + LX + SXY], s[i0 + 1 + LX + SXY]);
-        }
+        const float* s = COMP == 0 ? su : (COMP == 1 ? sv : sw);
+        constexpr int SXY = LX * LY;
+        return trilin(b.fx, b.fy, b.fz, s[i0], s[i0 + 1], s[i0 + SXY], s[i0 + 1 + SXY], s[i0 + LX], s[i0 + 1 + LX],
+                      s[i0 + LX + SXY], s[i0 + 1 + LX + SXY]);
+    }
+
+    template <int COMP>
+    FS_DEV float global_sample(const Base& b) const {
         const int W = a->W, H = a->H;
         if (COMP == 0) return sample_comp(a->u, a->nu, b, W + 1, (W + 1) * H);
         if (COMP == 1) return sample_comp(a->v, a->nv, b, W, W * (H + 1));
@@ -528,12 +538,16 @@ struct TileFields {
     }
 
     FS_DEV f3 interp(f3 pos) const {
-        const float dx = a->dx;
-        float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
+        const float dx = a->dx, inv = a->inv_dx;
+        float rx = div_dx<P2>(pos.x, dx, inv), ry = div_dx<P2>(pos.y, dx, inv), rz = div_dx<P2>(pos.z, dx, inv);
         Base bu = mac_base<0>(rx, ry, rz, a->W, a->H, a->D);
         Base bv = mac_base<1>(rx, ry, rz, a->W, a->H, a->D);
         Base bw = mac_base<2>(rx, ry, rz, a->W, a->H, a->D);
-        return mk(sample<0>(bu), sample<1>(bv), sample<2>(bw));
+        // one branch for all 24 corners, so that their LDS reads issue together
+        int iu, iv, iw;
+        const bool in = in_box<0>(bu, iu) & in_box<1>(bv, iv) & in_box<2>(bw, iw);
+        if (in) return mk(lds_sample<0>(bu, iu), lds_sample<1>(bv, iv), lds_sample<2>(bw, iw));
+        return mk(global_sample<0>(bu), global_sample<1>(bv), global_sample<2>(bw));
     }
 
     // label of an in-domain cell
@@ -544,7 +558,7 @@ struct TileFields {
     }
 };
 
-template <int TX, int TY, int TZ, int M>
+template <int TX, int TY, int TZ, int M, bool P2>
 __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* __restrict__ start) {
     constexpr int BX = TX + 2 * M, BY = TY + 2 * M, BZ = TZ + 2 * M;
     constexpr int NU = (BX + 1) * BY * BZ, NV = BX * (BY + 1) * BZ, NW = BX * BY * (BZ + 1), NL = BX * BY * BZ;
@@ -601,8 +615,8 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
     }
     __syncthreads();
 
-    TileFields<BX, BY, BZ> F{X0, Y0, Z0, su, sv, sw, sl, &a};
-    const float dx = a.dx, dt = a.dt, C = a.C, al = a.alpha;
+    TileFields<BX, BY, BZ, P2> F{X0, Y0, Z0, su, sv, sw, sl, &a};
+    const float dx = a.dx, dt = a.dt, C = a.C, al = a.alpha, inv = a.inv_dx;
     const float vfloor = sqrtf(5.0f * dx * 9.8f);
     const float xmax = (W - 1) * dx, ymax = (H - 1) * dx, zmax = (D - 1) * dx;
 
@@ -622,7 +636,8 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
             f3 vel = mk(a.P.vx[p], a.P.vy[p], a.P.vz[p]);
             // ---- G2P (GridToParticleFunctor :822-979)
             {
-                float rx = pos.x / dx, ry = pos.y / dx, rz = pos.z / dx;
+                float rx = div_dx<P2>(pos.x, dx, inv), ry = div_dx<P2>(pos.y, dx, inv),
+                      rz = div_dx<P2>(pos.z, dx, inv);
                 Base bu = mac_base<0>(rx, ry, rz, W, H, D);
                 Base bv = mac_base<1>(rx, ry, rz, W, H, D);
                 Base bw = mac_base<2>(rx, ry, rz, W, H, D);
@@ -657,11 +672,12 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
                 pos.x = fmaxf(pos.x, 0.0f); pos.x = fminf(pos.x, xmax);
                 pos.y = fmaxf(pos.y, 0.0f); pos.y = fminf(pos.y, ymax);
                 pos.z = fmaxf(pos.z, 0.0f); pos.z = fminf(pos.z, zmax);
-                int cx = (int)floorf(pos.x / dx), cy = (int)floorf(pos.y / dx), cz = (int)floorf(pos.z / dx);
+                int cx = (int)floorf(div_dx<P2>(pos.x, dx, inv)), cy = (int)floorf(div_dx<P2>(pos.y, dx, inv)),
+                    cz = (int)floorf(div_dx<P2>(pos.z, dx, inv));
                 if (cx >= 0 && cx < W && cy >= 0 && cy < H && cz >= 0 && cz < D) {
                     uint8_t L = F.label(cx, cy, cz);
                     if (L == FS_SOLID || L == FS_BODY) {
-                        if (!project_particle(pos, a.labels, W, H, D, dx)) break;
+                        if (!project_particle<P2>(pos, a.labels, W, H, D, dx, inv)) break;
                     }
                 }
                 if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || isnan(pos.x) || isnan(pos.y) || isnan(pos.z))
@@ -669,7 +685,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
             }
             a.P.px[p] = pos.x; a.P.py[p] = pos.y; a.P.pz[p] = pos.z;
             a.P.vx[p] = vel.x; a.P.vy[p] = vel.y; a.P.vz[p] = vel.z;
-            if (a.next_count) key = cell_key(pos.x, pos.y, pos.z, dx, W, H, D);
+            if (a.next_count) key = cell_key_t<P2>(pos.x, pos.y, pos.z, dx, inv, W, H, D);
         }
         if (a.umax) wave_max_speed(speed_out, a.umax);
         if (a.next_count) {
@@ -733,7 +749,10 @@ void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, c
 void launch_p2g(const P2GArgs& a, hipStream_t st) {
     constexpr int TX = P2G_TX, TY = P2G_TY, TZ = P2G_TZ;
     int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
-    hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ>), dim3(nb), dim3(256), 0, st, a);
+    if (a.inv_dx > 0.0f)
+        hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ, true>), dim3(nb), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ, false>), dim3(nb), dim3(256), 0, st, a);
 }
 
 void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st) {
@@ -746,7 +765,10 @@ void launch_g2p_advect(const G2PArgs& a, const int* start, hipStream_t st) {
     constexpr int TX = ADV_TX, TY = ADV_TY, TZ = ADV_TZ;
     static_assert(TY * TZ <= 256, "one tile row per thread");
     int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
-    hipLaunchKernelGGL((k_g2p_advect_tile<TX, TY, TZ, ADV_MARGIN>), dim3(nb), dim3(256), 0, st, a, start);
+    if (a.inv_dx > 0.0f)
+        hipLaunchKernelGGL((k_g2p_advect_tile<TX, TY, TZ, ADV_MARGIN, true>), dim3(nb), dim3(256), 0, st, a, start);
+    else
+        hipLaunchKernelGGL((k_g2p_advect_tile<TX, TY, TZ, ADV_MARGIN, false>), dim3(nb), dim3(256), 0, st, a, start);
 }
 
 }  // namespace fsk

@@ -41,6 +41,11 @@ enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, P
 
 enum { FIN_BNORM = 0, FIN_OUTER };
 
+// 1/d for the smoothers (d = 1..6): the hardware reciprocal (1 ulp) instead of
+// a correctly rounded division.  The preconditioner only needs one fixed
+// diagonal scaling, the same in every sweep, which keeps it symmetric.
+FS_DEV float inv_diag(uint32_t d) { return __builtin_amdgcn_rcpf((float)d); }
+
 FS_DEV float stencil_Ax(const float* __restrict__ x, long long c, uint32_t code, int W, long long WH) {
     float s = (float)((code >> 8) & 7u) * x[c];
     if (code & 1u) s -= x[c + 1];
@@ -91,7 +96,7 @@ __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels
     if (threadIdx.x < 6) sb[threadIdx.x] = (threadIdx.x & 1) ? -1 : 0x7fffffff;
     __syncthreads();
     int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         if (labels[c] != FS_FLUID) continue;
         int v[3] = {(int)(c % W), (int)((c / W) % H), (int)(c / ((long long)W * H))};
         for (int q = 0; q < 3; ++q) { mn[q] = min(mn[q], v[q]); mx[q] = max(mx[q], v[q]); }
@@ -115,7 +120,7 @@ __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels
 // level-0 types of the solve box (offset ox, oy, oz in the full W x H grid)
 __global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ labels, MGLevel L, int W, int H, int ox,
                                                int oy, int oz) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
     int i = (int)(c % L.W) + ox, j = (int)((c / L.W) % L.H) + oy, k = (int)(c / ((long long)L.W * L.H)) + oz;
     uint8_t lab = labels[(size_t)i + (size_t)j * W + (size_t)k * W * H];
@@ -125,7 +130,7 @@ __global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ label
 // b of the solve box from the full-grid right-hand side
 __global__ __launch_bounds__(256) void k_gather_b(const float* __restrict__ bfull, MGLevel L, int W, int H, int ox,
                                                   int oy, int oz, float* __restrict__ bbox) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
     int i = (int)(c % L.W) + ox, j = (int)((c / L.W) % L.H) + oy, k = (int)(c / ((long long)L.W * L.H)) + oz;
     bbox[c] = bfull[(size_t)i + (size_t)j * W + (size_t)k * W * H];
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(256) void k_gather_b(const float* __restrict__ bful
 __global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__ labels, long long n, int W, int H,
                                                      MGLevel L, int ox, int oy, int oz, const double* __restrict__ xb,
                                                      double* __restrict__ p64, float* __restrict__ p) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
     int i = (int)(c % W) - ox, j = (int)((c / W) % H) - oy, k = (int)(c / ((long long)W * H)) - oz;
     double v = 0.0;
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__
 }
 
 __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= Cc.n) return;
     int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
     int any1 = 0, any2 = 0;
@@ -173,7 +178,7 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
     __shared__ int s_cnt[2];
     if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
     __syncthreads();
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int unk = 0, dir = 0;
     if (c < L.n) {
         int i = (int)(c % L.W), j = (int)((c / L.W) % L.H), k = (int)(c / ((long long)L.W * L.H));
@@ -218,14 +223,14 @@ __global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restri
     __shared__ double sm[4];
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
         float xn = 0.0f;
         uint32_t d = (code >> 8) & 7u;
         if ((code & SELF) && d) {
             float bc = b[c];
-            xn = xin[c] + omega * (bc - stencil_Ax(xin, c, code, L.W, WH)) / (float)d;
+            xn = xin[c] + omega * (bc - stencil_Ax(xin, c, code, L.W, WH)) * inv_diag(d);
             if (DOT) acc += (double)bc * (double)xn;
         }
         xout[c] = xn;
@@ -239,11 +244,11 @@ __global__ __launch_bounds__(256) void k_diag(MGLevel L, const float* __restrict
     if (sc->done) return;
     __shared__ double sm[4];
     double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
         uint32_t d = (code >> 8) & 7u;
-        float zn = ((code & SELF) && d) ? r[c] / (float)d : 0.0f;
+        float zn = ((code & SELF) && d) ? r[c] * inv_diag(d) : 0.0f;
         z[c] = zn;
         acc += (double)r[c] * (double)zn;
     }
@@ -256,13 +261,13 @@ __global__ __launch_bounds__(256) void k_diag(MGLevel L, const float* __restrict
 FS_DEV float jacobi_first(const float* __restrict__ b, const uint16_t* __restrict__ code, long long c, float omega) {
     uint32_t cd = code[c];
     uint32_t d = (cd >> 8) & 7u;
-    return ((cd & SELF) && d) ? omega * b[c] / (float)d : 0.0f;
+    return ((cd & SELF) && d) ? omega * b[c] * inv_diag(d) : 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restrict__ b, float* __restrict__ xout,
                                                  float omega, float omega2, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
     uint32_t code = L.code[c];
     float xn = 0.0f;
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restr
     if ((code & SELF) && d) {
         const long long WH = (long long)L.W * L.H;
         float bc = b[c];
-        float x1 = omega * bc / (float)d;
+        float x1 = omega * bc * inv_diag(d);
         float ax = (float)d * x1;
         if (code & 1u) ax -= jacobi_first(b, L.code, c + 1, omega);
         if (code & 2u) ax -= jacobi_first(b, L.code, c - 1, omega);
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restr
         if (code & 8u) ax -= jacobi_first(b, L.code, c - L.W, omega);
         if (code & 16u) ax -= jacobi_first(b, L.code, c + WH, omega);
         if (code & 32u) ax -= jacobi_first(b, L.code, c - WH, omega);
-        xn = x1 + omega2 * (bc - ax) / (float)d;
+        xn = x1 + omega2 * (bc - ax) * inv_diag(d);
     }
     xout[c] = xn;
 }
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restr
 __global__ __launch_bounds__(256) void k_residual(MGLevel L, const float* __restrict__ b, const float* __restrict__ x,
                                                   float* __restrict__ r, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
     uint32_t code = L.code[c];
     float bc = b ? b[c] : 0.0f;
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const f
                                                   const float* __restrict__ bf, const float* __restrict__ xf,
                                                   float* __restrict__ bc, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= Cc.n) return;
     if (!(Cc.code[c] & SELF)) { bc[c] = 0.0f; return; }
     int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
@@ -357,7 +362,7 @@ FS_DEV float prolong_at(const MGLevel& Cc, const float* __restrict__ xc, int i, 
 __global__ __launch_bounds__(256) void k_prolong(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
                                                  float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= F.n) return;
     if (!(F.code[c] & SELF)) return;
     int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / ((long long)F.W * F.H));
@@ -376,7 +381,7 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, c
     __shared__ double sm[4];
     const long long WH = (long long)F.W * F.H;
     double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < F.n;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < F.n;
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = F.code[c];
         float xn = 0.0f;
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, c
             if (code & 16u) ax -= x[c + WH] + prolong_at(Cc, xc, i, j, k + 1);
             if (code & 32u) ax -= x[c - WH] + prolong_at(Cc, xc, i, j, k - 1);
             float bc = b[c];
-            xn = xp + omega * (bc - ax) / (float)d;
+            xn = xp + omega * (bc - ax) * inv_diag(d);
             if (DOT) acc += (double)bc * (double)xn;
         }
         xout[c] = xn;
@@ -421,13 +426,160 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* _
             uint32_t code = scode[c];
             float xn = 0.0f;
             uint32_t d = (code >> 8) & 7u;
-            if ((code & SELF) && d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) / (float)d;
+            if ((code & SELF) && d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) * inv_diag(d);
             sx[cur ^ 1][c] = xn;
         }
         __syncthreads();
         cur ^= 1;
     }
     for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = sx[cur][c];
+}
+
+// ---------------------------------------------------------------- small levels
+// The V-cycle below a level of at most TAIL_MAX cells in one workgroup: on
+// such levels every kernel of a separate launch would be launch-latency bound
+// (each touches a few thousand cells).  The phases are the same operations as
+// the kernels above (two sweeps from zero, extra pre sweeps, residual,
+// restriction, coarsest solve in LDS, prolongation, post sweeps), separated by
+// workgroup barriers; the data stay in global memory (L2-resident).  The result
+// of the first tail level lands in its x buffer.
+constexpr int TAIL_LEVELS = 6;
+constexpr long long TAIL_MAX = 1LL << 11;
+struct TailLevels {
+    MGLevel lv[TAIL_LEVELS];
+    int n;   // lv[n - 1] is the coarsest level
+};
+
+__global__ __launch_bounds__(1024) void k_tail(TailLevels t, int npre, int npost, int coarse_sweeps, float omega,
+                                               const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ float sx[2][COARSE_MAX];
+    __shared__ float sb[COARSE_MAX];
+    __shared__ uint16_t scode[COARSE_MAX];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int extra = (npre >= 2 ? npre - 2 : 0) + npost;
+    float* fin[TAIL_LEVELS];   // buffer holding each level's current iterate
+    // ---- down
+    for (int l = 0; l < t.n - 1; ++l) {
+        const MGLevel& L = t.lv[l];
+        const MGLevel& Cc = t.lv[l + 1];
+        const long long WH = (long long)L.W * L.H;
+        float* cur = (extra & 1) ? L.xt : L.x;
+        float* oth = (cur == L.x) ? L.xt : L.x;
+        for (long long c = tid; c < L.n; c += nt) {
+            uint32_t code = L.code[c];
+            float xn = 0.0f;
+            uint32_t d = (code >> 8) & 7u;
+            if ((code & SELF) && d) {
+                float bc = L.b[c];
+                float x1 = omega * bc * inv_diag(d);
+                float ax = (float)d * x1;
+                if (code & 1u) ax -= jacobi_first(L.b, L.code, c + 1, omega);
+                if (code & 2u) ax -= jacobi_first(L.b, L.code, c - 1, omega);
+                if (code & 4u) ax -= jacobi_first(L.b, L.code, c + L.W, omega);
+                if (code & 8u) ax -= jacobi_first(L.b, L.code, c - L.W, omega);
+                if (code & 16u) ax -= jacobi_first(L.b, L.code, c + WH, omega);
+                if (code & 32u) ax -= jacobi_first(L.b, L.code, c - WH, omega);
+                xn = x1 + (npre >= 2 ? omega : 0.0f) * (L.b[c] - ax) * inv_diag(d);
+            }
+            cur[c] = xn;
+        }
+        __syncthreads();
+        for (int k = 2; k < npre; ++k) {
+            for (long long c = tid; c < L.n; c += nt) {
+                uint32_t code = L.code[c];
+                float xn = 0.0f;
+                uint32_t d = (code >> 8) & 7u;
+                if ((code & SELF) && d) xn = cur[c] + omega * (L.b[c] - stencil_Ax(cur, c, code, L.W, WH)) * inv_diag(d);
+                oth[c] = xn;
+            }
+            __syncthreads();
+            float* tmp = cur; cur = oth; oth = tmp;
+        }
+        for (long long c = tid; c < L.n; c += nt) {
+            uint32_t code = L.code[c];
+            L.r[c] = (code & SELF) ? L.b[c] - stencil_Ax(cur, c, code, L.W, WH) : 0.0f;
+        }
+        __syncthreads();
+        const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
+        for (long long c = tid; c < Cc.n; c += nt) {
+            if (!(Cc.code[c] & SELF)) { Cc.b[c] = 0.0f; continue; }
+            int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
+            float sum = 0.0f;
+            for (int dk = 0; dk < 4; ++dk) {
+                int k = 2 * K - 1 + dk;
+                if (k < 0 || k >= L.D) continue;
+                for (int dj = 0; dj < 4; ++dj) {
+                    int j = 2 * J - 1 + dj;
+                    if (j < 0 || j >= L.H) continue;
+                    float wjk = wt[dj] * wt[dk];
+                    const long long row = (long long)j * L.W + (long long)k * WH;
+                    for (int di = 0; di < 4; ++di) {
+                        int i = 2 * I - 1 + di;
+                        if (i < 0 || i >= L.W) continue;
+                        sum += wt[di] * wjk * L.r[row + i];
+                    }
+                }
+            }
+            Cc.b[c] = 0.5f * sum;
+        }
+        fin[l] = cur;
+        __syncthreads();
+    }
+    // ---- coarsest: damped-Jacobi sweeps from zero in LDS
+    {
+        const MGLevel& L = t.lv[t.n - 1];
+        const int n = (int)L.n;
+        const long long WH = (long long)L.W * L.H;
+        for (int c = tid; c < n; c += nt) {
+            sb[c] = L.b[c];
+            scode[c] = L.code[c];
+            sx[0][c] = 0.0f;
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int sw = 0; sw < coarse_sweeps; ++sw) {
+            for (int c = tid; c < n; c += nt) {
+                uint32_t code = scode[c];
+                float xn = 0.0f;
+                uint32_t d = (code >> 8) & 7u;
+                if ((code & SELF) && d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) * inv_diag(d);
+                sx[cur ^ 1][c] = xn;
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        for (int c = tid; c < n; c += nt) L.x[c] = sx[cur][c];
+        fin[t.n - 1] = L.x;
+        __syncthreads();
+    }
+    // ---- up
+    for (int l = t.n - 2; l >= 0; --l) {
+        const MGLevel& L = t.lv[l];
+        const MGLevel& Cc = t.lv[l + 1];
+        const long long WH = (long long)L.W * L.H;
+        const float* xc = fin[l + 1];
+        float* cur = fin[l];
+        float* oth = (cur == L.x) ? L.xt : L.x;
+        for (long long c = tid; c < L.n; c += nt) {
+            if (!(L.code[c] & SELF)) continue;
+            int i = (int)(c % L.W), j = (int)((c / L.W) % L.H), k = (int)(c / WH);
+            cur[c] += prolong_at(Cc, xc, i, j, k);
+        }
+        __syncthreads();
+        for (int k = 0; k < npost; ++k) {
+            for (long long c = tid; c < L.n; c += nt) {
+                uint32_t code = L.code[c];
+                float xn = 0.0f;
+                uint32_t d = (code >> 8) & 7u;
+                if ((code & SELF) && d) xn = cur[c] + omega * (L.b[c] - stencil_Ax(cur, c, code, L.W, WH)) * inv_diag(d);
+                oth[c] = xn;
+            }
+            __syncthreads();
+            float* tmp = cur; cur = oth; oth = tmp;
+        }
+        fin[l] = cur;
+    }
 }
 
 // ---------------------------------------------------------------- CG kernels
@@ -461,7 +613,7 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
     }
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0, acc_rp = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
         float pn = 0.0f, qc = 0.0f;
@@ -495,7 +647,7 @@ __global__ __launch_bounds__(256) void k_update(long long n, const float* __rest
     double spq = sum_partials(pq, np, sm);
     const float alpha = (spq > 0.0) ? (float)(srz / spq) : 0.0f;
     double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         float rc = r[c] - alpha * q[c];
         e[c] += alpha * p[c];
         r[c] = rc;
@@ -542,7 +694,7 @@ __global__ __launch_bounds__(256) void k_dot_rz(long long n, const float* __rest
     if (sc->done) return;
     __shared__ double sm[4];
     double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
         acc += (double)r[c] * (double)z[c];
     store_partial(acc, sm, partial);
 }
@@ -555,7 +707,7 @@ __global__ __launch_bounds__(256) void k_dot_rz_zq(long long n, const float* __r
     if (sc->done) return;
     __shared__ double sm[4];
     double a = 0.0, b = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         const double zc = z[c];
         a += (double)r[c] * zc;
         b += zc * (double)q[c];
@@ -569,7 +721,7 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
                                                double* __restrict__ p1) {
     __shared__ double sm[4];
     double a2 = 0.0, a1 = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         if (L.code[c] & SELF) {
             double v = b[c];
@@ -583,7 +735,7 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
 
 // x64 += e ; e = 0
 __global__ __launch_bounds__(256) void k_axpy64(long long n, double* __restrict__ x64, float* __restrict__ e) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
     x64[c] += (double)e[c];
     e[c] = 0.0f;
@@ -595,7 +747,7 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
     __shared__ double sm[4];
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
-    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < L.n;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
         float rf = 0.0f;
@@ -610,7 +762,7 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
 }
 
 __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64, float* __restrict__ e) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
     x64[c] = 0.0;
     e[c] = 0.0f;
@@ -750,6 +902,15 @@ MGLevel MGPCG::fine() const { return lv[0]; }
 
 float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s) {
     MGLevel& L = lv[l];
+    // small levels (not the fine one, whose sweep accumulates r.z): one workgroup
+    if (use_tail && l > 0 && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
+        TailLevels t{};
+        t.n = nlev - l;
+        for (int q = 0; q < t.n; ++q) t.lv[q] = lv[l + q];
+        hipLaunchKernelGGL(k_tail, dim3(1), dim3(1024), 0, st, t, std::max(nu_pre, 1), std::max(nu_post, 1),
+                           coarse_sweeps, omega, (const SolveScalars*)s);
+        return L.x;
+    }
     if (l == nlev - 1) {
         hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st, L, L.b, L.x, coarse_sweeps, omega, s);
         if (l == 0 && pdot)   // single-level hierarchy: r.z of the coarse result

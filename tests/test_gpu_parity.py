@@ -121,11 +121,13 @@ def test_rhs_and_project_bitexact(hip, oracle):
         assert np.array_equal(h.get_grid(g).view(np.uint32), o.get_grid(g).view(np.uint32)), g
 
 
-@pytest.mark.parametrize("speed", [0.7, 12.0])   # 12 m/s leaves the staged LDS margin (global path)
-def test_g2p_advect_bitexact(hip, oracle, speed):
+# 12 m/s leaves the staged LDS margin (global path); dx = 1/16 takes the
+# power-of-two path (x * (1/dx) for x / dx), dx = 1/20 the division path
+@pytest.mark.parametrize("speed", [0.7, 12.0])
+@pytest.mark.parametrize("dx", [1.0 / 20, 1.0 / 16])
+def test_g2p_advect_bitexact(hip, oracle, speed, dx):
     rng = np.random.default_rng(4)
     N = 20
-    dx = 1.0 / N
     lab = tank(N, block=(0.5, 0.5, 1.0))
     lab[8:11, 3:6, 8:12] = 0   # an interior obstacle: exercises the projection
     pos, vel = random_particles(rng, 30000, N - 2, N - 2, N - 2, dx)
