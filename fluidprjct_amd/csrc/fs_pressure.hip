@@ -618,15 +618,18 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
         uint32_t code = L.code[c];
         float pn = 0.0f, qc = 0.0f;
         if (code & SELF) {
+            // A p of the stored (rounded) p, summed in FP64: p is smooth and large
+            // next to A p, and an FP32 sum would lose A p to cancellation
             pn = z[c] + beta * pold[c];
-            qc = (float)((code >> 8) & 7u) * pn;
-            if (code & 1u) qc -= z[c + 1] + beta * pold[c + 1];
-            if (code & 2u) qc -= z[c - 1] + beta * pold[c - 1];
-            if (code & 4u) qc -= z[c + L.W] + beta * pold[c + L.W];
-            if (code & 8u) qc -= z[c - L.W] + beta * pold[c - L.W];
-            if (code & 16u) qc -= z[c + WH] + beta * pold[c + WH];
-            if (code & 32u) qc -= z[c - WH] + beta * pold[c - WH];
-            acc += (double)pn * (double)qc;
+            double qd = (double)((code >> 8) & 7u) * (double)pn;
+            if (code & 1u) qd -= (double)(z[c + 1] + beta * pold[c + 1]);
+            if (code & 2u) qd -= (double)(z[c - 1] + beta * pold[c - 1]);
+            if (code & 4u) qd -= (double)(z[c + L.W] + beta * pold[c + L.W]);
+            if (code & 8u) qd -= (double)(z[c - L.W] + beta * pold[c - L.W]);
+            if (code & 16u) qd -= (double)(z[c + WH] + beta * pold[c + WH]);
+            if (code & 32u) qd -= (double)(z[c - WH] + beta * pold[c - WH]);
+            qc = (float)qd;
+            acc += (double)pn * qd;
             if (prp) acc_rp += (double)r[c] * (double)pn;
         }
         pnew[c] = pn;
@@ -636,9 +639,12 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
     if (prp) store_partial(acc_rp, sm, prp);
 }
 
-// alpha = (r.z) / (p.q) (flexible: rz holds r.p) ; e += alpha p ; r -= alpha q ; partials of r.r
+// alpha = (r.z) / (p.q) (flexible: rz holds r.p) ; x64 += alpha p (FP64: the
+// product of two floats is exact in double, so the solution follows the
+// recurrence instead of losing the true residual at the FP32 resolution of a
+// large correction) ; r -= alpha q ; partials of r.r
 __global__ __launch_bounds__(256) void k_update(long long n, const float* __restrict__ p, const float* __restrict__ q,
-                                                float* __restrict__ e, float* __restrict__ r,
+                                                double* __restrict__ x64, float* __restrict__ r,
                                                 const double* __restrict__ rz, const double* __restrict__ pq, int np,
                                                 double* __restrict__ prr, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
@@ -647,9 +653,10 @@ __global__ __launch_bounds__(256) void k_update(long long n, const float* __rest
     double spq = sum_partials(pq, np, sm);
     const float alpha = (spq > 0.0) ? (float)(srz / spq) : 0.0f;
     double acc = 0.0;
-    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x) {
         float rc = r[c] - alpha * q[c];
-        e[c] += alpha * p[c];
+        x64[c] += (double)alpha * (double)p[c];
         r[c] = rc;
         acc += (double)rc * (double)rc;
     }
@@ -733,14 +740,6 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
     store_partial(a1, sm, p1);
 }
 
-// x64 += e ; e = 0
-__global__ __launch_bounds__(256) void k_axpy64(long long n, double* __restrict__ x64, float* __restrict__ e) {
-    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    x64[c] += (double)e[c];
-    e[c] = 0.0f;
-}
-
 // r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partials of r.r (FP64)
 __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, const double* __restrict__ x64,
                                                  double shift, float* __restrict__ r, double* __restrict__ partial) {
@@ -761,11 +760,10 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
     store_partial(acc, sm, partial);
 }
 
-__global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64, float* __restrict__ e) {
+__global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
     x64[c] = 0.0;
-    e[c] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __restrict__ p1,
@@ -810,11 +808,11 @@ void MGPCG::free_all() {
     }
     lv.clear();
     nlev = 0;
-    hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(e); hipFree(bb); hipFree(xb);
+    hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(bb); hipFree(xb);
     hipFree(partial); hipFree(sc); hipFree(counters);
     if (sc_host) hipHostFree(sc_host);
     if (host_counters) hipHostFree(host_counters);
-    r = z = pa = pb = q = e = bb = nullptr;
+    r = z = pa = pb = q = bb = nullptr;
     xb = nullptr;
     partial = nullptr;
     sc = nullptr;
@@ -859,7 +857,7 @@ int MGPCG::init(int W_, int H_, int D_) {
         A((void**)&L.xt, sizeof(float) * L.n);
         A((void**)&L.r, sizeof(float) * L.n);
     }
-    for (float** p : {&r, &z, &pa, &pb, &q, &e, &bb}) A((void**)p, sizeof(float) * N);
+    for (float** p : {&r, &z, &pa, &pb, &q, &bb}) A((void**)p, sizeof(float) * N);
     A((void**)&xb, sizeof(double) * N);
     A((void**)&partial, sizeof(double) * RED_BLOCKS * PS_COUNT);
     A((void**)&sc, sizeof(SolveScalars));
@@ -870,7 +868,6 @@ int MGPCG::init(int W_, int H_, int D_) {
     // CG direction buffers start finite: the first update multiplies them by beta = 0
     hipMemset(pa, 0, sizeof(float) * N);
     hipMemset(pb, 0, sizeof(float) * N);
-    hipMemset(e, 0, sizeof(float) * N);
     hipMemset(partial, 0, sizeof(double) * RED_BLOCKS * PS_COUNT);
     configure(W, H, D);
     return 0;
@@ -1017,7 +1014,7 @@ void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
         hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old,
                            flex ? (const double*)slot(PS_ZQ) : (const double*)nullptr, (const double*)pq_old, nred,
                            pq_new, (const float*)r, flex ? slot(PS_RP) : (double*)nullptr, sc);
-        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, e, r,
+        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, xb, r,
                            flex ? (const double*)slot(PS_RP) : (const double*)rz_new, pq_new, nred, slot(PS_RR), sc);
         hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, pq_new, slot(PS_RR), nred, flex ? 1 : 0,
                            flex ? stall_window : 0, sc);
@@ -1095,7 +1092,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     if (rc == -3) {
         box[0] = box[1] = box[2] = 0;
         configure(W, H, D);
-        hipLaunchKernelGGL(k_fill0, dim3(nblk(N)), dim3(256), 0, st, N, xb, e);
+        hipLaunchKernelGGL(k_fill0, dim3(nblk(N)), dim3(256), 0, st, N, xb);
         hipLaunchKernelGGL(k_scatter_out, dim3(nblk(N)), dim3(256), 0, st, labels, N, W, H, lv[0], 0, 0, 0, xb, x64, p);
         if (stats) *stats = loc;
         return -3;
@@ -1117,7 +1114,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     loc.singular = singular ? 1 : 0;
     n_unk = nunk;
     hipMemcpyAsync(&sc->n_unknown, counters + 8, sizeof(int), hipMemcpyDeviceToDevice, st);
-    hipLaunchKernelGGL(k_fill0, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
+    hipLaunchKernelGGL(k_fill0, dim3(nblk(n0)), dim3(256), 0, st, n0, xb);
     const double shift = singular ? sc_host->bsum / nunk : 0.0;
     const double bn2 =
         singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
@@ -1126,7 +1123,6 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     bool converged = false;
     double rr_true = 0.0;
     for (outer = 0;; ++outer) {
-        if (outer > 0) hipLaunchKernelGGL(k_axpy64, dim3(nblk(n0)), dim3(256), 0, st, n0, xb, e);
         hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, slot(PS_RES));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
                            (const double*)nullptr, nred, sc, tol2b);

@@ -84,11 +84,11 @@ public:
     long long coarse_max = 512;   // coarsen until a level has at most this many cells (LDS-resident)
     float omega = 6.0f / 7.0f;
     int batch = 8;              // inner iterations launched between host checks (even)
-    double inner_rtol = 1e-5;   // inner relative reduction per refinement pass (the FP32 recurrence
-                                // drifts from the true residual near 1e-4 at 256^3: refine early)
+    double inner_rtol = 0.0;    // inner relative reduction per refinement pass (0: straight to the
+                                // tolerance; FP64 A p sums and solution keep r with the true residual)
     int max_outer = 12;
     // fine-grid CG vectors
-    float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr, *e = nullptr;
+    float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr;
     float* bb = nullptr;           // b on the solve box
     double* xb = nullptr;          // FP64 solution on the solve box
     int* counters = nullptr;       // device: bbox [0..5], unknown count [8], Dirichlet flag [9]
