@@ -435,6 +435,279 @@ __global__ __launch_bounds__(1024) void k_coarse_solve(MGLevel L, const float* _
     for (int c = threadIdx.x; c < n; c += blockDim.x) x[c] = sx[cur][c];
 }
 
+// ---------------------------------------------------------------- fine level, 4 cells per thread
+// Used when the level's row length W is a multiple of 4 (the solve box is
+// widened to one when the grid allows): every row of 4 cells is 16-byte
+// aligned, so the centre and the +-y, +-z neighbours of 4 cells are single
+// float4 (ushort4 for the codes) loads; the +-x neighbours outside the group
+// are two scalar loads.  The arithmetic per cell is that of the scalar kernels.
+struct Codes4 {
+    uint32_t c[4];
+    uint32_t any;   // union of the neighbour bits
+};
+
+FS_DEV Codes4 load_codes4(const uint16_t* __restrict__ code, long long c0) {
+    const ushort4 v = *reinterpret_cast<const ushort4*>(code + c0);
+    Codes4 k;
+    k.c[0] = v.x; k.c[1] = v.y; k.c[2] = v.z; k.c[3] = v.w;
+    k.any = (k.c[0] | k.c[1] | k.c[2] | k.c[3]) & 63u;
+    return k;
+}
+
+FS_DEV float4 ld4(const float* __restrict__ p, long long c) { return *reinterpret_cast<const float4*>(p + c); }
+FS_DEV float f4(const float4& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// The 7-point neighbourhood of 4 cells of a field given by Fld: vec(c) -> values
+// at c..c+3 (c a multiple of 4), one(c) -> value at c.  Loads only what a code
+// bit asks for (a neighbour whose bit is set is inside the level).
+struct Nbhd4 {
+    float ctr[4], xm[4], xp[4], ym[4], yp[4], zm[4], zp[4];
+};
+
+template <class Fld>
+FS_DEV void gather4(const Fld& f, long long c0, const Codes4& k, int W, long long WH, Nbhd4& o) {
+    const float4 c = f.vec(c0);
+    o.ctr[0] = c.x; o.ctr[1] = c.y; o.ctr[2] = c.z; o.ctr[3] = c.w;
+    const float left = (k.c[0] & 2u) ? f.one(c0 - 1) : 0.0f;
+    const float right = (k.c[3] & 1u) ? f.one(c0 + 4) : 0.0f;
+    o.xm[0] = left; o.xm[1] = c.x; o.xm[2] = c.y; o.xm[3] = c.z;
+    o.xp[0] = c.y; o.xp[1] = c.z; o.xp[2] = c.w; o.xp[3] = right;
+    float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 ym = (k.any & 8u) ? f.vec(c0 - W) : z4;
+    const float4 yp = (k.any & 4u) ? f.vec(c0 + W) : z4;
+    const float4 zm = (k.any & 32u) ? f.vec(c0 - WH) : z4;
+    const float4 zp = (k.any & 16u) ? f.vec(c0 + WH) : z4;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        o.ym[q] = f4(ym, q); o.yp[q] = f4(yp, q); o.zm[q] = f4(zm, q); o.zp[q] = f4(zp, q);
+    }
+}
+
+// (A v)_q of cell q of the group, in the operation order of stencil_Ax
+FS_DEV float ax4(const Nbhd4& v, uint32_t code, int q) {
+    float s = (float)((code >> 8) & 7u) * v.ctr[q];
+    if (code & 1u) s -= v.xp[q];
+    if (code & 2u) s -= v.xm[q];
+    if (code & 4u) s -= v.yp[q];
+    if (code & 8u) s -= v.ym[q];
+    if (code & 16u) s -= v.zp[q];
+    if (code & 32u) s -= v.zm[q];
+    return s;
+}
+
+struct PlainField {
+    const float* __restrict__ p;
+    FS_DEV float4 vec(long long c) const { return ld4(p, c); }
+    FS_DEV float one(long long c) const { return p[c]; }
+};
+
+// omega b / d on unknowns (the first sweep from zero), 0 elsewhere
+struct FirstSweepField {
+    const float* __restrict__ b;
+    const uint16_t* __restrict__ code;
+    float omega;
+    FS_DEV float one(long long c) const { return jacobi_first(b, code, c, omega); }
+    FS_DEV float4 vec(long long c) const {
+        const float4 bv = ld4(b, c);
+        const ushort4 cv = *reinterpret_cast<const ushort4*>(code + c);
+        auto f = [&](float bb, uint32_t cd) {
+            uint32_t d = (cd >> 8) & 7u;
+            return ((cd & SELF) && d) ? omega * bb * inv_diag(d) : 0.0f;
+        };
+        return make_float4(f(bv.x, cv.x), f(bv.y, cv.y), f(bv.z, cv.z), f(bv.w, cv.w));
+    }
+};
+
+// p = z + beta pold, the new direction as the scalar kernel forms it
+struct DirField {
+    const float* __restrict__ z;
+    const float* __restrict__ pold;
+    float beta;
+    FS_DEV float one(long long c) const { return z[c] + beta * pold[c]; }
+    FS_DEV float4 vec(long long c) const {
+        const float4 a = ld4(z, c), b = ld4(pold, c);
+        return make_float4(a.x + beta * b.x, a.y + beta * b.y, a.z + beta * b.z, a.w + beta * b.w);
+    }
+};
+
+__global__ __launch_bounds__(256) void k_jacobi2_v4(MGLevel L, const float* __restrict__ b, float* __restrict__ xout,
+                                                    float omega, float omega2, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    const long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (c0 >= L.n) return;
+    const Codes4 k = load_codes4(L.code, c0);
+    const long long WH = (long long)L.W * L.H;
+    Nbhd4 v;
+    gather4(FirstSweepField{b, L.code, omega}, c0, k, L.W, WH, v);
+    const float4 bv = ld4(b, c0);
+    float out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t code = k.c[q], d = (code >> 8) & 7u;
+        float xn = 0.0f;
+        if ((code & SELF) && d) {
+            const float bc = f4(bv, q);
+            const float x1 = v.ctr[q];
+            const float ax = ax4(v, code, q);
+            xn = x1 + omega2 * (bc - ax) * inv_diag(d);
+        }
+        out[q] = xn;
+    }
+    *reinterpret_cast<float4*>(xout + c0) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+template <bool DOT>
+__global__ __launch_bounds__(256) void k_jacobi_v4(MGLevel L, const float* __restrict__ b, const float* __restrict__ xin,
+                                                   float* __restrict__ xout, float omega, double* __restrict__ partial,
+                                                   const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[4];
+    const long long WH = (long long)L.W * L.H;
+    double acc = 0.0;
+    for (long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x); c0 < L.n;
+         c0 += 4LL * gridDim.x * blockDim.x) {
+        const Codes4 k = load_codes4(L.code, c0);
+        Nbhd4 v;
+        gather4(PlainField{xin}, c0, k, L.W, WH, v);
+        const float4 bv = ld4(b, c0);
+        float out[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t code = k.c[q], d = (code >> 8) & 7u;
+            float xn = 0.0f;
+            if ((code & SELF) && d) {
+                const float bc = f4(bv, q);
+                xn = v.ctr[q] + omega * (bc - ax4(v, code, q)) * inv_diag(d);
+                if (DOT) acc += (double)bc * (double)xn;
+            }
+            out[q] = xn;
+        }
+        *reinterpret_cast<float4*>(xout + c0) = make_float4(out[0], out[1], out[2], out[3]);
+    }
+    if (DOT) store_partial(acc, sm, partial);
+}
+
+__global__ __launch_bounds__(256) void k_residual_v4(MGLevel L, const float* __restrict__ b, const float* __restrict__ x,
+                                                     float* __restrict__ r, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    const long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (c0 >= L.n) return;
+    const Codes4 k = load_codes4(L.code, c0);
+    Nbhd4 v;
+    gather4(PlainField{x}, c0, k, L.W, (long long)L.W * L.H, v);
+    const float4 bv = b ? ld4(b, c0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float out[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = (k.c[q] & SELF) ? f4(bv, q) - ax4(v, k.c[q], q) : 0.0f;
+    *reinterpret_cast<float4*>(r + c0) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// fine x += P xc for 4 fine cells along x (they share their coarse columns)
+__global__ __launch_bounds__(256) void k_prolong_v4(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
+                                                    float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    const long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (c0 >= F.n) return;
+    const Codes4 k = load_codes4(F.code, c0);
+    if (!((k.c[0] | k.c[1] | k.c[2] | k.c[3]) & SELF)) return;
+    const long long WH = (long long)F.W * F.H;
+    const int i0 = (int)(c0 % F.W), j = (int)((c0 / F.W) % F.H), kk = (int)(c0 / WH);
+    float4 xv = ld4(xf, c0);
+    float out[4] = {xv.x, xv.y, xv.z, xv.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (k.c[q] & SELF) out[q] += prolong_at(Cc, xc, i0 + q, j, kk);
+    *reinterpret_cast<float4*>(xf + c0) = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// k_pupdate_spmv for 4 cells per thread (same sums, FP64 A p)
+__global__ __launch_bounds__(256) void k_pupdate_spmv_v4(MGLevel L, const float* __restrict__ z,
+                                                         const float* __restrict__ pold, float* __restrict__ pnew,
+                                                         float* __restrict__ q, const double* __restrict__ rz_new,
+                                                         const double* __restrict__ rz_old,
+                                                         const double* __restrict__ zq,
+                                                         const double* __restrict__ pq_old, int np,
+                                                         double* __restrict__ ppq, const float* __restrict__ r,
+                                                         double* __restrict__ prp,
+                                                         const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    const bool first = sc->first != 0;
+    float beta = 0.0f;
+    if (!first) {
+        if (zq) {
+            double a = sum_partials(zq, np, sm);
+            double bsum = sum_partials(pq_old, np, sm);
+            beta = bsum > 0.0 ? (float)(-a / bsum) : 0.0f;
+        } else {
+            double bn = sum_partials(rz_new, np, sm);
+            double bo = sum_partials(rz_old, np, sm);
+            beta = (float)(bn / bo);
+        }
+    }
+    const long long WH = (long long)L.W * L.H;
+    double acc = 0.0, acc_rp = 0.0;
+    for (long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x); c0 < L.n;
+         c0 += 4LL * gridDim.x * blockDim.x) {
+        const Codes4 k = load_codes4(L.code, c0);
+        Nbhd4 v;
+        gather4(DirField{z, pold, beta}, c0, k, L.W, WH, v);
+        float pn[4], qc[4];
+        float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (prp) rv = ld4(r, c0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t code = k.c[t];
+            pn[t] = 0.0f;
+            qc[t] = 0.0f;
+            if (code & SELF) {
+                pn[t] = v.ctr[t];
+                double qd = (double)((code >> 8) & 7u) * (double)pn[t];
+                if (code & 1u) qd -= (double)v.xp[t];
+                if (code & 2u) qd -= (double)v.xm[t];
+                if (code & 4u) qd -= (double)v.yp[t];
+                if (code & 8u) qd -= (double)v.ym[t];
+                if (code & 16u) qd -= (double)v.zp[t];
+                if (code & 32u) qd -= (double)v.zm[t];
+                qc[t] = (float)qd;
+                acc += (double)pn[t] * qd;
+                if (prp) acc_rp += (double)f4(rv, t) * (double)pn[t];
+            }
+        }
+        *reinterpret_cast<float4*>(pnew + c0) = make_float4(pn[0], pn[1], pn[2], pn[3]);
+        *reinterpret_cast<float4*>(q + c0) = make_float4(qc[0], qc[1], qc[2], qc[3]);
+    }
+    store_partial(acc, sm, ppq);
+    if (prp) store_partial(acc_rp, sm, prp);
+}
+
+__global__ __launch_bounds__(256) void k_update_v4(long long n, const float* __restrict__ p, const float* __restrict__ q,
+                                                   double* __restrict__ x64, float* __restrict__ r,
+                                                   const double* __restrict__ rz, const double* __restrict__ pq, int np,
+                                                   double* __restrict__ prr, const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[4];
+    double srz = sum_partials(rz, np, sm);
+    double spq = sum_partials(pq, np, sm);
+    const float alpha = (spq > 0.0) ? (float)(srz / spq) : 0.0f;
+    double acc = 0.0;
+    for (long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x); c0 < n;
+         c0 += 4LL * gridDim.x * blockDim.x) {
+        const float4 pv = ld4(p, c0), qv = ld4(q, c0), rv0 = ld4(r, c0);
+        double2 xa = *reinterpret_cast<const double2*>(x64 + c0), xb2 = *reinterpret_cast<const double2*>(x64 + c0 + 2);
+        const float rc0 = rv0.x - alpha * qv.x, rc1 = rv0.y - alpha * qv.y, rc2 = rv0.z - alpha * qv.z,
+                    rc3 = rv0.w - alpha * qv.w;
+        xa.x += (double)alpha * (double)pv.x; xa.y += (double)alpha * (double)pv.y;
+        xb2.x += (double)alpha * (double)pv.z; xb2.y += (double)alpha * (double)pv.w;
+        *reinterpret_cast<double2*>(x64 + c0) = xa;
+        *reinterpret_cast<double2*>(x64 + c0 + 2) = xb2;
+        *reinterpret_cast<float4*>(r + c0) = make_float4(rc0, rc1, rc2, rc3);
+        acc += (double)rc0 * (double)rc0 + (double)rc1 * (double)rc1 + (double)rc2 * (double)rc2 +
+               (double)rc3 * (double)rc3;
+    }
+    store_partial(acc, sm, prr);
+}
+
 // ---------------------------------------------------------------- small levels
 // The V-cycle below a level of at most TAIL_MAX cells in one workgroup: on
 // such levels every kernel of a separate launch would be launch-latency bound
@@ -883,6 +1156,7 @@ void MGPCG::configure(int Wb, int Hb, int Db) {
     nlev = n;
     n0 = lv[0].n;
     nred = (int)std::min<long long>(RED_BLOCKS, (n0 + 255) / 256);
+    vec0 = use_vec && (lv[0].W % 4 == 0);
 }
 
 double* MGPCG::slot(int s) { return partial + (size_t)s * RED_BLOCKS; }
@@ -924,16 +1198,30 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     const int extra = (npre >= 2 ? npre - 2 : 0) + npost;
     float* cur = (extra & 1) ? L.xt : L.x;
     float* oth = (cur == L.x) ? L.xt : L.x;
-    hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, omega, npre >= 2 ? omega : 0.0f, s);
+    const bool v4 = (l == 0) && vec0;   // 4 cells per thread on the fine level
+    const unsigned nb4 = nblk(L.n / 4);
+    if (v4)
+        hipLaunchKernelGGL(k_jacobi2_v4, dim3(nb4), dim3(256), 0, st, L, L.b, cur, omega, npre >= 2 ? omega : 0.0f, s);
+    else
+        hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, omega, npre >= 2 ? omega : 0.0f, s);
     for (int k = 2; k < npre; ++k) {
-        hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr, s);
+        if (v4)
+            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, omega,
+                               (double*)nullptr, s);
+        else
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr,
+                               s);
         std::swap(cur, oth);
     }
     MGLevel& Cn = lv[l + 1];
     const bool fuse = L.n <= FUSE_MAX;
     // (a restriction that recomputes the fine residuals is slower at every level
     // size: one long serial thread per coarse cell)
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
+    if (v4)
+        hipLaunchKernelGGL(k_residual_v4, dim3(nb4), dim3(256), 0, st, L, (const float*)L.b, (const float*)cur, L.r,
+                           (const SolveScalars*)s);
+    else
+        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
     hipLaunchKernelGGL(k_restrict<false>, dim3(nblk(Cn.n)), dim3(256), 0, st, L, Cn, L.r, (const float*)nullptr,
                        (const float*)nullptr, Cn.b, s);
     float* xc = vcycle_level(l + 1, st, nullptr, s);
@@ -948,14 +1236,26 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         std::swap(cur, oth);
         k0 = 1;
     } else {
-        hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, xc, cur, s);
+        if (v4)
+            hipLaunchKernelGGL(k_prolong_v4, dim3(nb4), dim3(256), 0, st, L, Cn, (const float*)xc, cur,
+                               (const SolveScalars*)s);
+        else
+            hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, xc, cur, s);
         k0 = 0;
     }
     for (int k = k0; k < npost; ++k) {
-        if (k == npost - 1 && pdot)
+        const bool dot = k == npost - 1 && pdot;
+        if (v4) {
+            if (dot)
+                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
+            else
+                hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, omega,
+                                   (double*)nullptr, s);
+        } else if (dot) {
             hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
-        else
+        } else {
             hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr, s);
+        }
         std::swap(cur, oth);
     }
     return cur;
@@ -1011,11 +1311,21 @@ void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
         double* pq_new = slot(PS_PQ0 + cur);
         double* pq_old = slot(PS_PQ0 + (cur ^ 1));
         pre->apply(r, z, rz_new, sc, st);   // z = M r and the partial sums of r.z (and z.q when flexible)
-        hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old,
-                           flex ? (const double*)slot(PS_ZQ) : (const double*)nullptr, (const double*)pq_old, nred,
-                           pq_new, (const float*)r, flex ? slot(PS_RP) : (double*)nullptr, sc);
-        hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, xb, r,
-                           flex ? (const double*)slot(PS_RP) : (const double*)rz_new, pq_new, nred, slot(PS_RR), sc);
+        const double* zq = flex ? (const double*)slot(PS_ZQ) : (const double*)nullptr;
+        double* prp = flex ? slot(PS_RP) : (double*)nullptr;
+        const double* num = flex ? (const double*)slot(PS_RP) : (const double*)rz_new;
+        if (vec0) {
+            hipLaunchKernelGGL(k_pupdate_spmv_v4, dim3(nred), dim3(256), 0, st, L0, (const float*)z,
+                               (const float*)pold, pnew, q, (const double*)rz_new, (const double*)rz_old, zq,
+                               (const double*)pq_old, nred, pq_new, (const float*)r, prp, (const SolveScalars*)sc);
+            hipLaunchKernelGGL(k_update_v4, dim3(nred), dim3(256), 0, st, n0, (const float*)pnew, (const float*)q, xb,
+                               r, num, (const double*)pq_new, nred, slot(PS_RR), (const SolveScalars*)sc);
+        } else {
+            hipLaunchKernelGGL(k_pupdate_spmv, dim3(nred), dim3(256), 0, st, L0, z, pold, pnew, q, rz_new, rz_old, zq,
+                               (const double*)pq_old, nred, pq_new, (const float*)r, prp, sc);
+            hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, xb, r, num, pq_new, nred,
+                               slot(PS_RR), sc);
+        }
         hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, pq_new, slot(PS_RR), nred, flex ? 1 : 0,
                            flex ? stall_window : 0, sc);
         std::swap(pold, pnew);
@@ -1072,6 +1382,17 @@ int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
     for (int q = 0; q < 3; ++q) {
         lo[q] = std::max(0, hb[2 * q] - 1);
         hi[q] = std::min(dimsz[q] - 1, hb[2 * q + 1] + 1);
+    }
+    // widen the box along x to a multiple of 4 cells (aligned 4-cell groups on
+    // the fine level) when the grid is wide enough; the extra cells are not
+    // unknowns
+    if (use_vec) {
+        int w = hi[0] - lo[0] + 1;
+        const int w4 = (w + 3) & ~3;
+        if (w4 <= W) {
+            hi[0] = std::min(W - 1, lo[0] + w4 - 1);
+            lo[0] = hi[0] - w4 + 1;
+        }
     }
     box[0] = lo[0]; box[1] = lo[1]; box[2] = lo[2];
     configure(hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1);

@@ -100,7 +100,9 @@ public:
     int nred = 0;   // number of partial-sum slots used by reducing kernels
     int last_iters = 0, last_outer = 0;
     bool use_graph = true;      // replay inner batches as HIP graphs
-    bool use_tail = true;       // levels of at most 32K cells in one workgroup launch
+    bool use_tail = true;
+    bool use_vec = true;        // 4-cells-per-thread fine-level kernels when the box width allows
+    bool vec0 = false;          // the configured fine level uses them       // levels of at most 32K cells in one workgroup launch
     // partial sums of r.z over the solve box (for preconditioners that do not fuse it)
     void dot_rz(const float* r, const float* z, double* partial, SolveScalars* sc, hipStream_t st);
     // r.z and z.q (q = A p of the previous iteration) for flexible preconditioners
