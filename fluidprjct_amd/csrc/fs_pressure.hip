@@ -1299,13 +1299,13 @@ Preconditioner* make_jacobi_preconditioner(MGPCG* s) { return new JacobiPre(s); 
 // One batch of inner PCG iterations (an even number, so the two direction
 // buffers and the two r.z partial slots are in the same roles at the start of
 // every batch).
-void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
+void MGPCG::record_batch(Preconditioner* pre, int nb, hipStream_t st) {
     MGLevel& L0 = lv[0];
     const bool flex = pre->flexible();
     float* pold = pa;
     float* pnew = pb;
     int cur = 0;
-    for (int it = 0; it < batch; ++it) {
+    for (int it = 0; it < nb; ++it) {
         double* rz_new = slot(PS_RZ0 + cur);
         double* rz_old = slot(PS_RZ0 + (cur ^ 1));
         double* pq_new = slot(PS_PQ0 + cur);
@@ -1335,14 +1335,14 @@ void MGPCG::record_batch(Preconditioner* pre, hipStream_t st) {
 
 // The batch as a HIP graph, cached per solve-box shape and preconditioner
 // (kernel arguments depend only on those).  The graph holds kernel nodes only.
-hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, hipStream_t st) {
-    const long long key[7] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
-                              pre->graph_epoch()};
+hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, int nb, hipStream_t st) {
+    const long long key[8] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
+                              pre->graph_epoch(), nb};
     for (auto& g : graphs)
-        if (std::equal(key, key + 7, g.key)) return g.exec;
+        if (std::equal(key, key + 8, g.key)) return g.exec;
     hipGraph_t graph;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
-    record_batch(pre, st);
+    record_batch(pre, nb, st);
     if (hipStreamEndCapture(st, &graph) != hipSuccess) return nullptr;
     hipGraphExec_t ex;
     if (hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) != hipSuccess) {
@@ -1350,12 +1350,12 @@ hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, hipStream_t st) {
         return nullptr;
     }
     hipGraphDestroy(graph);
-    if (graphs.size() >= 8) {
+    if (graphs.size() >= 16) {
         hipGraphExecDestroy(graphs.front().exec);
         graphs.erase(graphs.begin());
     }
     GraphEntry g;
-    std::copy(key, key + 7, g.key);
+    std::copy(key, key + 8, g.key);
     g.exec = ex;
     graphs.push_back(g);
     return ex;
@@ -1453,17 +1453,33 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         total_iters = sc_host->iters;
         if (sc_host->done == 2 || bn2 == 0.0) { converged = true; break; }
         if (outer >= max_outer || total_iters >= maxit) break;
+        // Batches of `batch` iterations; once two checks give a convergence rate,
+        // the next batch is sized (8, 4 or 2) to the iterations still expected,
+        // since iterations launched after convergence still cost their launches.
+        double rr_prev = sc_host->rr;
+        int it_prev = sc_host->iters;
+        int nb = batch;
         while (true) {
             if (use_graph) {
-                hipGraphExec_t ex = batch_graph(pre, st);
+                hipGraphExec_t ex = batch_graph(pre, nb, st);
                 if (!ex) return -5;
                 if (hipGraphLaunch(ex, st) != hipSuccess) return -5;
             } else {
-                record_batch(pre, st);
+                record_batch(pre, nb, st);
             }
             hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
             if (hipStreamSynchronize(st) != hipSuccess) return -5;
             if (sc_host->done) break;
+            nb = batch;
+            const int dit = sc_host->iters - it_prev;
+            if (dit > 0 && sc_host->rr > 0.0 && rr_prev > sc_host->rr && sc_host->target > 0.0) {
+                const double per_it = std::log(sc_host->rr / rr_prev) / dit;   // < 0
+                const double need = std::log(sc_host->target / sc_host->rr) / per_it;
+                if (need <= 2.0) nb = 2;
+                else if (need <= 4.0) nb = std::min(batch, 4);
+            }
+            rr_prev = sc_host->rr;
+            it_prev = sc_host->iters;
         }
         // a flexible preconditioner that stalls or breaks down hands over to the V-cycle
         if (pre->flexible() && (sc_host->done == 5 || sc_host->done == 3)) {

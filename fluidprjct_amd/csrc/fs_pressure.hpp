@@ -80,7 +80,7 @@ public:
     int nlev = 0;
     long long n0 = 0;              // cells of the solve box (level 0)
     int box[3] = {0, 0, 0};        // solve-box origin in the full grid
-    int nu_pre = 2, nu_post = 2, coarse_sweeps = 64;
+    int nu_pre = 3, nu_post = 3, coarse_sweeps = 8;
     long long coarse_max = 512;   // coarsen until a level has at most this many cells (LDS-resident)
     float omega = 6.0f / 7.0f;
     int batch = 8;              // inner iterations launched between host checks (even)
@@ -112,12 +112,12 @@ public:
 
 private:
     struct GraphEntry {
-        long long key[7];
+        long long key[8];
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs;
-    void record_batch(Preconditioner* pre, hipStream_t st);
-    hipGraphExec_t batch_graph(Preconditioner* pre, hipStream_t st);
+    void record_batch(Preconditioner* pre, int nb, hipStream_t st);
+    hipGraphExec_t batch_graph(Preconditioner* pre, int nb, hipStream_t st);
     void clear_graphs();
     void free_all();
     float* vcycle_level(int l, hipStream_t st, double* partial_dot, SolveScalars* sc);
