@@ -9,6 +9,7 @@
 //
 // Particles are SoA (px,py,pz,vx,vy,vz,id), kept sorted by cell index so that
 // each P2G tile reads contiguous rows and G2P gathers hit neighbouring faces.
+#include <algorithm>
 #include "fs_kernels.hpp"
 
 using namespace fsd;
@@ -446,22 +447,31 @@ FS_DEV bool project_particle(f3& pos, const uint8_t* __restrict__ labels, int W,
     return true;
 }
 
-// max |v| over a wave, folded into a device scalar (float bits of a
-// non-negative value order like unsigned integers)
-FS_DEV void wave_max_speed(float sp, unsigned* umax) {
+// max |v| over a block, folded into a device scalar with one atomic per block
+// (float bits of a non-negative value order like unsigned integers).  Atomics
+// on one address serialise in L2, so one per wave would cost milliseconds at
+// tens of millions of particles.  Every thread of the block must call it.
+FS_DEV void block_max_speed(float sp, unsigned* umax) {
+    __shared__ float s_wmax[16];
     for (int off = 32; off > 0; off >>= 1) sp = fmaxf(sp, __shfl_xor(sp, off, 64));
-    if (__lane_id() == 0 && sp > 0.0f) atomicMax(umax, __float_as_uint(sp));
+    const int wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) s_wmax[wv] = sp;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int q = 0; q < (int)((blockDim.x + 63) >> 6); ++q) m = fmaxf(m, s_wmax[q]);
+        if (m > 0.0f) atomicMax(umax, __float_as_uint(m));
+    }
 }
 
 __global__ __launch_bounds__(256) void k_max_speed(ParticlesView P, int n, unsigned* umax) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     float sp = 0.0f;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
-        sp = sqrtf(vx * vx + vy * vy + vz * vz);
-        if (isnan(sp)) sp = 0.0f;
+        float s = sqrtf(vx * vx + vy * vy + vz * vz);
+        if (!isnan(s)) sp = fmaxf(sp, s);
     }
-    wave_max_speed(sp, umax);
+    block_max_speed(sp, umax);
 }
 
 // G2P + advection over a tile of TX x TY x TZ cells.  The particles of a tile
@@ -620,6 +630,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
     const float vfloor = sqrtf(5.0f * dx * 9.8f);
     const float xmax = (W - 1) * dx, ymax = (H - 1) * dx, zmax = (D - 1) * dx;
 
+    float speed_max = 0.0f;   // of this thread's particles, folded once per block below
     for (int base = 0; base < tot; base += blockDim.x) {
         const int t = base + threadIdx.x;
         const bool act = t < tot;
@@ -687,7 +698,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
             a.P.vx[p] = vel.x; a.P.vy[p] = vel.y; a.P.vz[p] = vel.z;
             if (a.next_count) key = cell_key_t<P2>(pos.x, pos.y, pos.z, dx, inv, W, H, D);
         }
-        if (a.umax) wave_max_speed(speed_out, a.umax);
+        speed_max = fmaxf(speed_max, speed_out);
         if (a.next_count) {
             int r = wave_agg_rank(key, act, a.next_count);
             if (act) {
@@ -696,6 +707,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
             }
         }
     }
+    if (a.umax) block_max_speed(speed_max, a.umax);
 }
 
 // ---------------------------------------------------------------------------
@@ -757,7 +769,7 @@ void launch_p2g(const P2GArgs& a, hipStream_t st) {
 
 void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_max_speed, dim3((n + 255) / 256), dim3(256), 0, st, P, n, umax);
+    hipLaunchKernelGGL(k_max_speed, dim3(std::min(2048, (n + 255) / 256)), dim3(256), 0, st, P, n, umax);
 }
 
 void launch_g2p_advect(const G2PArgs& a, const int* start, hipStream_t st) {
