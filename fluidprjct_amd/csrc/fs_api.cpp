@@ -189,6 +189,11 @@ struct Solver {
     }
 
     int alloc_grids(int W_, int H_, int D_) {
+        // the kernels index cells and faces with 32-bit arithmetic
+        const long long faces = std::max({(long long)(W_ + 1) * H_ * D_, (long long)W_ * (H_ + 1) * D_,
+                                          (long long)W_ * H_ * (D_ + 1)});
+        if (W_ <= 0 || H_ <= 0 || D_ <= 0 || faces >= (1LL << 31))
+            return fail(FS_ERR_ARG, "grid dimensions out of range (at most 2^31 - 1 faces per component)");
         free_grids();
         W = W_; H = H_; D = D_;
         C = (long long)W * H * D;
@@ -1018,7 +1023,8 @@ double a_norm2(const uint8_t* labels, const double* p, int W, int H, int D) {
 
 int learned_forward_host(Solver& s, const uint8_t* labels, const float* r, int W, int H, int D, DevBuf& dl,
                          DevBuf& dr, DevBuf& dz, DevBuf& dzn, double* scale, int* nf) {
-    if (!labels || !r || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad grid");
+    if (!labels || !r || W <= 0 || H <= 0 || D <= 0 || (long long)W * H * D >= (1LL << 31))
+        return fail(FS_ERR_ARG, "bad grid");
     LNet* n = s.get_net();
     if (!n) return fail(FS_ERR_DEVICE, "learned preconditioner allocation failed");
     const size_t C = (size_t)W * H * D;

@@ -168,6 +168,16 @@ FS_DEV double block_sum(double v, double* sm) {
     return t;
 }
 
+// (i, j, k) of the linear index c of a W x H x D grid, with 32-bit division
+// (every grid of the library has fewer than 2^31 cells or faces; 64-bit
+// division is a long instruction sequence on the GPU)
+FS_DEV void split3(long long c, int W, int H, int& i, int& j, int& k) {
+    const unsigned u = (unsigned)c, t = u / (unsigned)W;
+    i = (int)(u - t * (unsigned)W);
+    j = (int)(t % (unsigned)H);
+    k = (int)(t / (unsigned)H);
+}
+
 // Remap a linear block id so that consecutive tiles land on the same XCD
 // (block ids are dealt round-robin over the 8 XCDs of MI355X).
 FS_DEV int xcd_remap(int bid, int nblocks) {

@@ -30,7 +30,8 @@ __global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs
         return;
     }
     const int W = a.W, H = a.H;
-    int i = (int)(idx % W), j = (int)((idx / W) % H), k = (int)(idx / ((long long)W * H));
+    int i, j, k;
+    split3(idx, W, H, i, j, k);
     const float dx = a.dx;
     const float scale = (a.density * dx) / a.dt;
     size_t ui0 = (size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H;
@@ -98,25 +99,22 @@ __global__ __launch_bounds__(256) void k_project(GridArgs a, const float* __rest
     float* G;
     if (g < NU) {
         comp = 0; idx = g; G = a.u;
-        long long slice = (long long)(W + 1) * H;
-        int fk = (int)(idx / slice); long long rem = idx % slice;
-        int fj = (int)(rem / (W + 1)), fi = (int)(rem % (W + 1));
+        int fi, fj, fk;
+        split3(idx, W + 1, H, fi, fj, fk);
         left = (long long)(fi - 1) + (long long)fj * W + (long long)fk * W * H;
         right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
         fp = mk(fi * dx, (fj + 0.5f) * dx, (fk + 0.5f) * dx);
     } else if (g < NU + NV) {
         comp = 1; idx = g - NU; G = a.v;
-        long long slice = (long long)W * (H + 1);
-        int fk = (int)(idx / slice); long long rem = idx % slice;
-        int fj = (int)(rem / W), fi = (int)(rem % W);
+        int fi, fj, fk;
+        split3(idx, W, H + 1, fi, fj, fk);
         left = (long long)fi + (long long)(fj - 1) * W + (long long)fk * W * H;
         right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
         fp = mk((fi + 0.5f) * dx, fj * dx, (fk + 0.5f) * dx);
     } else {
         comp = 2; idx = g - NU - NV; G = a.w;
-        long long slice = (long long)W * H;
-        int fk = (int)(idx / slice); long long rem = idx % slice;
-        int fj = (int)(rem / W), fi = (int)(rem % W);
+        int fi, fj, fk;
+        split3(idx, W, H, fi, fj, fk);
         left = (long long)fi + (long long)fj * W + (long long)(fk - 1) * W * H;
         right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
         fp = mk((fi + 0.5f) * dx, (fj + 0.5f) * dx, fk * dx);
@@ -156,7 +154,8 @@ __global__ __launch_bounds__(256) void k_drag_lift(GridArgs a, double* __restric
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (idx < C && a.labels[idx] == FS_BODY) {
         const float dx = a.dx, rho = a.density;
-        int ci = (int)(idx % W), cj = (int)((idx / W) % H), ck = (int)(idx / ((long long)W * H));
+        int ci, cj, ck;
+        split3(idx, W, H, ci, cj, ck);
         f3 cc = mk((ci + 0.5f) * dx, (cj + 0.5f) * dx, (ck + 0.5f) * dx);
         const float area = dx * dx;
         const int NUc = (W + 1) * H * D, NVc = W * (H + 1) * D, NWc = W * H * (D + 1);
@@ -212,7 +211,8 @@ __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __res
     if (q < nvox) {
         float sv = sdf[q];
         if (sv <= 0.0f) {
-            int i = (int)(q % b.sw), j = (int)((q / b.sw) % b.sh), k = (int)(q / ((long long)b.sw * b.sh));
+            int i, j, k;
+            split3(q, b.sw, b.sh, i, j, k);
             const float cs = b.cs;
             f3 pl = b.file_origin + mk(i * cs, j * cs, k * cs);
             f3 pc = pl - b.local_com;
@@ -268,7 +268,8 @@ namespace fsk {
 __global__ __launch_bounds__(256) void k_mark_body(LabelArgs la, uint8_t* __restrict__ labels, long long C) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
-    int i = (int)(c % la.W), j = (int)((c / la.W) % la.H), k = (int)(c / ((long long)la.W * la.H));
+    int i, j, k;
+    split3(c, la.W, la.H, i, j, k);
     f3 cc = mk((i + 0.5f) * la.dx, (j + 0.5f) * la.dx, (k + 0.5f) * la.dx);
     if (labels[c] != FS_SOLID && body_contains(la.body, la.sdf, cc)) labels[c] = FS_BODY;
 }

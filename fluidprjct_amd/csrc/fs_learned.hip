@@ -244,7 +244,8 @@ __global__ __launch_bounds__(256) void k_pool16(LGrid f, LGrid c, const uint16_t
     if (q >= c.n * 16) return;
     long long V = q >> 4;
     int ch = (int)(q & 15);
-    int I = (int)(V % c.W), J = (int)((V / c.W) % c.H), K = (int)(V / ((long long)c.W * c.H));
+    int I, J, K;
+    split3(V, c.W, c.H, I, J, K);
     float s = 0.0f;
     for (int dk = 0; dk < 2; ++dk)
         for (int dj = 0; dj < 2; ++dj)
@@ -265,7 +266,8 @@ __global__ __launch_bounds__(256) void k_upadd16(LGrid f, LGrid c, const uint16_
     if (q >= f.n * 16) return;
     long long v = q >> 4;
     int ch = (int)(q & 15);
-    int i = (int)(v % f.W), j = (int)((v / f.W) % f.H), k = (int)(v / ((long long)f.W * f.H));
+    int i, j, k;
+    split3(v, f.W, f.H, i, j, k);
     long long V = (long long)(i >> 1) + (long long)(j >> 1) * c.W + (long long)(k >> 1) * c.W * c.H;
     out[q] = f2bf(bf2f(fine[q]) + bf2f(coarse[V * 16 + ch]));
 }
@@ -284,7 +286,8 @@ __global__ __launch_bounds__(256) void k_loss_grad(LGrid g, const float* __restr
     if (c < g.n) {
         double ad = 0.0;
         if (type[c] == 1) {
-            const int x = (int)(c % g.W), y = (int)((c / g.W) % g.H), z = (int)(c / ((long long)g.W * g.H));
+            int x, y, z;
+            split3(c, g.W, g.H, x, y, z);
             const double d = (double)zn[c] - (double)target[c];
             const long long WH = (long long)g.W * g.H;
             const int nx[6] = {x + 1, x - 1, x, x, x, x}, ny[6] = {y, y, y + 1, y - 1, y, y},
@@ -402,7 +405,8 @@ __global__ __launch_bounds__(256) void k_upT_relu16(LGrid f, LGrid c, const uint
     if (q >= c.n * 16) return;
     long long V = q >> 4;
     int ch = (int)(q & 15);
-    int I = (int)(V % c.W), J = (int)((V / c.W) % c.H), K = (int)(V / ((long long)c.W * c.H));
+    int I, J, K;
+    split3(V, c.W, c.H, I, J, K);
     float s = 0.0f;
     for (int dk = 0; dk < 2; ++dk)
         for (int dj = 0; dj < 2; ++dj)
@@ -423,7 +427,8 @@ __global__ __launch_bounds__(256) void k_poolT_add_relu16(LGrid f, LGrid c, cons
     if (q >= f.n * 16) return;
     long long v = q >> 4;
     int ch = (int)(q & 15);
-    int i = (int)(v % f.W), j = (int)((v / f.W) % f.H), k = (int)(v / ((long long)f.W * f.H));
+    int i, j, k;
+    split3(v, f.W, f.H, i, j, k);
     long long V = (long long)(i >> 1) + (long long)(j >> 1) * c.W + (long long)(k >> 1) * c.W * c.H;
     float s = bf2f(gs[q]) + 0.125f * bf2f(gc[V * 16 + ch]);
     out[q] = f2bf(bf2f(act[q]) > 0.0f ? s : 0.0f);

@@ -165,7 +165,8 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, co
             } else {
                 L = FS_AIR;
                 if (la.body.active) {
-                    int i = (int)(c % la.W), j = (int)((c / la.W) % la.H), k = (int)(c / ((long long)la.W * la.H));
+                    int i, j, k;
+                    split3(c, la.W, la.H, i, j, k);
                     f3 cc = mk((i + 0.5f) * la.dx, (j + 0.5f) * la.dx, (k + 0.5f) * la.dx);
                     if (body_contains(la.body, la.sdf, cc)) L = FS_BODY;
                 }
@@ -220,7 +221,8 @@ __global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels
                                               ParticlesView P) {
     long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C || labels[c] != FS_FLUID) return;
-    int i = (int)(c % W), j = (int)((c / W) % H), k = (int)(c / ((long long)W * H));
+    int i, j, k;
+    split3(c, W, H, i, j, k);
     const double lo = (double)(-0.24f), hi = (double)(0.24f);
     f3 cc = mk(((float)i + 0.5f) * dx, ((float)j + 0.5f) * dx, ((float)k + 0.5f) * dx);
     const float off = 0.25f * dx;

@@ -98,7 +98,8 @@ __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels
     int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         if (labels[c] != FS_FLUID) continue;
-        int v[3] = {(int)(c % W), (int)((c / W) % H), (int)(c / ((long long)W * H))};
+        int v[3];
+        split3(c, W, H, v[0], v[1], v[2]);
         for (int q = 0; q < 3; ++q) { mn[q] = min(mn[q], v[q]); mx[q] = max(mx[q], v[q]); }
     }
     for (int q = 0; q < 3; ++q) {
@@ -122,7 +123,11 @@ __global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ label
                                                int oy, int oz) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
-    int i = (int)(c % L.W) + ox, j = (int)((c / L.W) % L.H) + oy, k = (int)(c / ((long long)L.W * L.H)) + oz;
+    int i, j, k;
+    split3(c, L.W, L.H, i, j, k);
+    i = i + ox;
+    j = j + oy;
+    k = k + oz;
     uint8_t lab = labels[(size_t)i + (size_t)j * W + (size_t)k * W * H];
     L.type[c] = lab == FS_FLUID ? 1 : (lab == FS_AIR ? 2 : 0);
 }
@@ -132,7 +137,11 @@ __global__ __launch_bounds__(256) void k_gather_b(const float* __restrict__ bful
                                                   int oy, int oz, float* __restrict__ bbox) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
-    int i = (int)(c % L.W) + ox, j = (int)((c / L.W) % L.H) + oy, k = (int)(c / ((long long)L.W * L.H)) + oz;
+    int i, j, k;
+    split3(c, L.W, L.H, i, j, k);
+    i = i + ox;
+    j = j + oy;
+    k = k + oz;
     bbox[c] = bfull[(size_t)i + (size_t)j * W + (size_t)k * W * H];
 }
 
@@ -142,7 +151,11 @@ __global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__
                                                      double* __restrict__ p64, float* __restrict__ p) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    int i = (int)(c % W) - ox, j = (int)((c / W) % H) - oy, k = (int)(c / ((long long)W * H)) - oz;
+    int i, j, k;
+    split3(c, W, H, i, j, k);
+    i = i - ox;
+    j = j - oy;
+    k = k - oz;
     double v = 0.0;
     if (labels[c] == FS_FLUID && i >= 0 && i < L.W && j >= 0 && j < L.H && k >= 0 && k < L.D)
         v = xb[(size_t)i + (size_t)j * L.W + (size_t)k * L.W * L.H];
@@ -153,7 +166,8 @@ __global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__
 __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= Cc.n) return;
-    int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
+    int I, J, K;
+    split3(c, Cc.W, Cc.H, I, J, K);
     int any1 = 0, any2 = 0;
     for (int dk = 0; dk < 2; ++dk) {
         int k = 2 * K + dk;
@@ -174,17 +188,19 @@ __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
 }
 
 // per-cell stencil code; counters[0] += unknowns, counters[1] |= Dirichlet contact
+// (grid-strided, one pair of global atomics per block)
 __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ counters) {
     __shared__ int s_cnt[2];
     if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
     __syncthreads();
-    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int unk = 0, dir = 0;
-    if (c < L.n) {
-        int i = (int)(c % L.W), j = (int)((c / L.W) % L.H), k = (int)(c / ((long long)L.W * L.H));
+    const long long WH = (long long)L.W * L.H;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = 0;
         if (L.type[c] == 1) {
-            const long long WH = (long long)L.W * L.H;
+            int i, j, k;
+            split3(c, L.W, L.H, i, j, k);
             int nbr[6] = {i + 1 < L.W ? 1 : 0, i > 0 ? 1 : 0, j + 1 < L.H ? 1 : 0, j > 0 ? 1 : 0, k + 1 < L.D ? 1 : 0,
                           k > 0 ? 1 : 0};
             long long off[6] = {1, -1, L.W, -(long long)L.W, WH, -WH};
@@ -196,7 +212,7 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
                 else if (t == 2) { ++diag; dir = 1; }
             }
             code |= SELF | (diag << 8);
-            unk = 1;
+            ++unk;
         }
         L.code[c] = (uint16_t)code;
     }
@@ -308,7 +324,8 @@ __global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const f
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= Cc.n) return;
     if (!(Cc.code[c] & SELF)) { bc[c] = 0.0f; return; }
-    int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
+    int I, J, K;
+    split3(c, Cc.W, Cc.H, I, J, K);
     const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
     const long long WH = (long long)F.W * F.H;
     float s = 0.0f;
@@ -365,7 +382,8 @@ __global__ __launch_bounds__(256) void k_prolong(MGLevel F, MGLevel Cc, const fl
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= F.n) return;
     if (!(F.code[c] & SELF)) return;
-    int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / ((long long)F.W * F.H));
+    int i, j, k;
+    split3(c, F.W, F.H, i, j, k);
     xf[c] += prolong_at(Cc, xc, i, j, k);
 }
 
@@ -387,7 +405,8 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, c
         float xn = 0.0f;
         uint32_t d = (code >> 8) & 7u;
         if ((code & SELF) && d) {
-            int i = (int)(c % F.W), j = (int)((c / F.W) % F.H), k = (int)(c / WH);
+            int i, j, k;
+            split3(c, F.W, F.H, i, j, k);
             float xp = x[c] + prolong_at(Cc, xc, i, j, k);
             float ax = (float)d * xp;
             if (code & 1u) ax -= x[c + 1] + prolong_at(Cc, xc, i + 1, j, k);
@@ -611,7 +630,8 @@ __global__ __launch_bounds__(256) void k_prolong_v4(MGLevel F, MGLevel Cc, const
     const Codes4 k = load_codes4(F.code, c0);
     if (!((k.c[0] | k.c[1] | k.c[2] | k.c[3]) & SELF)) return;
     const long long WH = (long long)F.W * F.H;
-    const int i0 = (int)(c0 % F.W), j = (int)((c0 / F.W) % F.H), kk = (int)(c0 / WH);
+    int i0, j, kk;
+    split3(c0, F.W, F.H, i0, j, kk);
     float4 xv = ld4(xf, c0);
     float out[4] = {xv.x, xv.y, xv.z, xv.w};
 #pragma unroll
@@ -674,7 +694,8 @@ __global__ __launch_bounds__(256) void k_prolong_v4s(MGLevel F, MGLevel Cc, cons
     const Codes4 kc = load_codes4(F.code, c0);
     if (!((kc.c[0] | kc.c[1] | kc.c[2] | kc.c[3]) & SELF)) return;
     const long long WH = (long long)F.W * F.H;
-    const int i0 = (int)(c0 % F.W), j = (int)((c0 / F.W) % F.H), k = (int)(c0 / WH);
+    int i0, j, k;
+    split3(c0, F.W, F.H, i0, j, k);
     const int I0 = i0 >> 1;   // even
     const int J0 = j >> 1, K0 = k >> 1;
     const int Js[2] = {J0, (j & 1) ? J0 + 1 : J0 - 1}, Ks[2] = {K0, (k & 1) ? K0 + 1 : K0 - 1};
@@ -875,7 +896,8 @@ __global__ __launch_bounds__(1024) void k_tail(TailLevels t, int npre, int npost
         const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
         for (long long c = tid; c < Cc.n; c += nt) {
             if (!(Cc.code[c] & SELF)) { Cc.b[c] = 0.0f; continue; }
-            int I = (int)(c % Cc.W), J = (int)((c / Cc.W) % Cc.H), K = (int)(c / ((long long)Cc.W * Cc.H));
+            int I, J, K;
+            split3(c, Cc.W, Cc.H, I, J, K);
             float sum = 0.0f;
             for (int dk = 0; dk < 4; ++dk) {
                 int k = 2 * K - 1 + dk;
@@ -934,7 +956,8 @@ __global__ __launch_bounds__(1024) void k_tail(TailLevels t, int npre, int npost
         float* oth = (cur == L.x) ? L.xt : L.x;
         for (long long c = tid; c < L.n; c += nt) {
             if (!(L.code[c] & SELF)) continue;
-            int i = (int)(c % L.W), j = (int)((c / L.W) % L.H), k = (int)(c / WH);
+            int i, j, k;
+            split3(c, L.W, L.H, i, j, k);
             cur[c] += prolong_at(Cc, xc, i, j, k);
         }
         __syncthreads();
@@ -1264,7 +1287,8 @@ void MGPCG::setup(const uint8_t* labels, hipStream_t st) {
     for (int l = 0; l + 1 < nlev; ++l)
         hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
     for (int l = 0; l < nlev; ++l)
-        hipLaunchKernelGGL(k_code, dim3(nblk(lv[l].n)), dim3(256), 0, st, lv[l], counters + (l == 0 ? 8 : 10));
+        hipLaunchKernelGGL(k_code, dim3((unsigned)std::min<long long>(1024, nblk(lv[l].n))), dim3(256), 0, st, lv[l],
+                           counters + (l == 0 ? 8 : 10));
 }
 
 MGLevel MGPCG::fine() const { return lv[0]; }
@@ -1473,7 +1497,7 @@ int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
     // FLUID cell (AIR/SOLID/BODY or out of domain) is inside the solve box
     int init_box[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
     hipMemcpyAsync(counters, init_box, sizeof(init_box), hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_bbox, dim3((unsigned)std::min<long long>(4096, (N + 255) / 256)), dim3(256), 0, st, labels, W,
+    hipLaunchKernelGGL(k_bbox, dim3((unsigned)std::min<long long>(1024, (N + 255) / 256)), dim3(256), 0, st, labels, W,
                        H, N, counters);
     hipMemcpyAsync(host_counters, counters, sizeof(int) * 6, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
