@@ -321,6 +321,29 @@ FS_DEV void window_flush(const P2GArgs& a, int x0, int y0, int z0, int ci, int c
             }
 }
 
+// the window of one cell for one component: its particles' weights and
+// weighted velocities in registers, then one LDS atomic per touched face
+template <int COMP, int TX, int TY, int TZ, bool P2>
+FS_DEV void p2g_cell_comp(const P2GArgs& a, int x0, int y0, int z0, int ci, int cj, int ck, int beg, int end,
+                          float* __restrict__ snum, float* __restrict__ sden) {
+    const int W = a.W, H = a.H, D = a.D;
+    const float dx = a.dx, inv = a.inv_dx;
+    const float* __restrict__ vel = COMP == 0 ? a.P.vx : (COMP == 1 ? a.P.vy : a.P.vz);
+    float num[18], den[18];
+#pragma unroll
+    for (int q = 0; q < 18; ++q) { num[q] = 0.f; den[q] = 0.f; }
+    // the next particle's values are loaded while the current one is accumulated
+    float px = a.P.px[beg], py = a.P.py[beg], pz = a.P.pz[beg], vc = vel[beg];
+    for (int p = beg; p < end; ++p) {
+        const int q = p + 1 < end ? p + 1 : p;
+        const float npx = a.P.px[q], npy = a.P.py[q], npz = a.P.pz[q], nvc = vel[q];
+        const float rx = div_dx<P2>(px, dx, inv), ry = div_dx<P2>(py, dx, inv), rz = div_dx<P2>(pz, dx, inv);
+        window_accumulate<COMP>(mac_base<COMP>(rx, ry, rz, W, H, D), ci, cj, ck, vc, num, den);
+        px = npx; py = npy; pz = npz; vc = nvc;
+    }
+    window_flush<COMP, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, num, den, snum, sden);
+}
+
 template <int COMP, int TX, int TY, int TZ>
 FS_DEV void p2g_write_comp(const P2GArgs& a, int x0, int y0, int z0, const float* __restrict__ snum,
                            const float* __restrict__ sden) {
@@ -372,25 +395,10 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
         const long long c = (long long)ci + (long long)cj * W + (long long)ck * W * H;
         const int beg = a.start[c], end = a.start[c + 1];
         if (beg == end) continue;
-        float un[18], ud[18], vn[18], vd[18], wn[18], wd[18];
-#pragma unroll
-        for (int q = 0; q < 18; ++q) { un[q] = 0.f; ud[q] = 0.f; vn[q] = 0.f; vd[q] = 0.f; wn[q] = 0.f; wd[q] = 0.f; }
-        // the next particle's six values are loaded while the current one is accumulated
-        float px = a.P.px[beg], py = a.P.py[beg], pz = a.P.pz[beg];
-        float vx = a.P.vx[beg], vy = a.P.vy[beg], vz = a.P.vz[beg];
-        for (int p = beg; p < end; ++p) {
-            const int q = p + 1 < end ? p + 1 : p;
-            const float npx = a.P.px[q], npy = a.P.py[q], npz = a.P.pz[q];
-            const float nvx = a.P.vx[q], nvy = a.P.vy[q], nvz = a.P.vz[q];
-            const float rx = div_dx<P2>(px, dx, inv), ry = div_dx<P2>(py, dx, inv), rz = div_dx<P2>(pz, dx, inv);
-            window_accumulate<0>(mac_base<0>(rx, ry, rz, W, H, D), ci, cj, ck, vx, un, ud);
-            window_accumulate<1>(mac_base<1>(rx, ry, rz, W, H, D), ci, cj, ck, vy, vn, vd);
-            window_accumulate<2>(mac_base<2>(rx, ry, rz, W, H, D), ci, cj, ck, vz, wn, wd);
-            px = npx; py = npy; pz = npz; vx = nvx; vy = nvy; vz = nvz;
-        }
-        window_flush<0, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, un, ud, s_un, s_ud);
-        window_flush<1, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, vn, vd, s_vn, s_vd);
-        window_flush<2, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, wn, wd, s_wn, s_wd);
+        // one component at a time: 36 window accumulators live instead of 108
+        p2g_cell_comp<0, TX, TY, TZ, P2>(a, x0, y0, z0, ci, cj, ck, beg, end, s_un, s_ud);
+        p2g_cell_comp<1, TX, TY, TZ, P2>(a, x0, y0, z0, ci, cj, ck, beg, end, s_vn, s_vd);
+        p2g_cell_comp<2, TX, TY, TZ, P2>(a, x0, y0, z0, ci, cj, ck, beg, end, s_wn, s_wd);
     }
     __syncthreads();
     p2g_write_comp<0, TX, TY, TZ>(a, x0, y0, z0, s_un, s_ud);
