@@ -685,60 +685,6 @@ __global__ __launch_bounds__(256) void k_restrict_v2(MGLevel F, MGLevel Cc, cons
     bc[c + 1] = s1 ? 0.5f * sum1 : 0.0f;
 }
 
-// fine x += P xc for 4 fine cells along x sharing 4 coarse columns per row
-__global__ __launch_bounds__(256) void k_prolong_v4s(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
-                                                     float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
-    if (sc && sc->done) return;
-    const long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
-    if (c0 >= F.n) return;
-    const Codes4 kc = load_codes4(F.code, c0);
-    if (!((kc.c[0] | kc.c[1] | kc.c[2] | kc.c[3]) & SELF)) return;
-    const long long WH = (long long)F.W * F.H;
-    int i0, j, k;
-    split3(c0, F.W, F.H, i0, j, k);
-    const int I0 = i0 >> 1;   // even
-    const int J0 = j >> 1, K0 = k >> 1;
-    const int Js[2] = {J0, (j & 1) ? J0 + 1 : J0 - 1}, Ks[2] = {K0, (k & 1) ? K0 + 1 : K0 - 1};
-    // coarse columns I0-1 .. I0+2 of the 4 coarse rows
-    float col[2][2][4];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
-            const bool rowok = Ks[a] >= 0 && Ks[a] < Cc.D && Js[bb] >= 0 && Js[bb] < Cc.H;
-            const long long base = (long long)Js[bb] * Cc.W + (long long)Ks[a] * Cc.W * Cc.H;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int I = I0 - 1 + t;
-                col[a][bb][t] = (rowok && I >= 0 && I < Cc.W) ? xc[base + I] : 0.0f;
-            }
-        }
-    const float w[2] = {0.75f, 0.25f};
-    float4 xv = ld4(xf, c0);
-    float out[4] = {xv.x, xv.y, xv.z, xv.w};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (!(kc.c[q] & SELF)) continue;
-        const int i = i0 + q;
-        const int Ia = i >> 1, Ib = (i & 1) ? Ia + 1 : Ia - 1;
-        const int ta = Ia - (I0 - 1), tb = Ib - (I0 - 1);
-        const bool okb = Ib >= 0 && Ib < Cc.W;
-        float sum = 0.0f;
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            if (Ks[a] < 0 || Ks[a] >= Cc.D) continue;
-#pragma unroll
-            for (int bb = 0; bb < 2; ++bb) {
-                if (Js[bb] < 0 || Js[bb] >= Cc.H) continue;
-                sum += w[0] * w[bb] * w[a] * col[a][bb][ta];
-                if (okb) sum += w[1] * w[bb] * w[a] * col[a][bb][tb];
-            }
-        }
-        out[q] += sum;
-    }
-    *reinterpret_cast<float4*>(xf + c0) = make_float4(out[0], out[1], out[2], out[3]);
-}
-
 // k_pupdate_spmv for 4 cells per thread (same sums, FP64 A p)
 __global__ __launch_bounds__(256) void k_pupdate_spmv_v4(MGLevel L, const float* __restrict__ z,
                                                          const float* __restrict__ pold, float* __restrict__ pnew,
@@ -1363,7 +1309,7 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         k0 = 1;
     } else {
         if (v4)
-            hipLaunchKernelGGL(k_prolong_v4s, dim3(nb4), dim3(256), 0, st, L, Cn, (const float*)xc, cur,
+            hipLaunchKernelGGL(k_prolong_v4, dim3(nb4), dim3(256), 0, st, L, Cn, (const float*)xc, cur,
                                (const SolveScalars*)s);
         else
             hipLaunchKernelGGL(k_prolong, dim3(nb), dim3(256), 0, st, L, Cn, xc, cur, s);
