@@ -30,6 +30,11 @@
 #ifndef P2G_TZ
 #define P2G_TZ 8
 #endif
+// consecutive cells along x whose windows one P2G thread accumulates (1 is
+// fastest on gfx950: 2 and 3 cut the LDS atomics but cost more per thread)
+#ifndef P2G_GROUP
+#define P2G_GROUP 1
+#endif
 
 namespace fsk {
 

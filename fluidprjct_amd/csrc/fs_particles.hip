@@ -262,86 +262,71 @@ __global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels
 // owned faces with one LDS atomic per face.  Each owned face is finally
 // normalised (ComputeVelocityFunc :598-605), saved (saveVelocities) and given
 // gravity (ApplyScalarForce :687-698) in one write.
-template <int COMP>
-FS_DEV void window_weights(const Base& b, int ci, int cj, int ck, float* wx, float* wy, float* wz) {
-    // window origin: the 2-wide axis starts at the cell, the 3-wide axes one below
-    const int ox = (COMP == 0) ? ci : ci - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
-    const int rx = b.i - ox, ry = b.j - oy, rz = b.k - oz;   // each is 0 or 1
-    const float ax0 = 1.0f - b.fx, ax1 = b.fx, ay0 = 1.0f - b.fy, ay1 = b.fy, az0 = 1.0f - b.fz, az1 = b.fz;
-    // 2-wide axis: slot 1 receives the (1-f) weight when the base sits on slot 1;
-    // its f weight then targets a face beyond the domain, which the reference skips
-    if (COMP == 0) { wx[0] = rx ? 0.0f : ax0; wx[1] = rx ? ax0 : ax1; }
-    else { wx[0] = rx ? 0.0f : ax0; wx[1] = rx ? ax0 : ax1; wx[2] = rx ? ax1 : 0.0f; }
-    if (COMP == 1) { wy[0] = ry ? 0.0f : ay0; wy[1] = ry ? ay0 : ay1; }
-    else { wy[0] = ry ? 0.0f : ay0; wy[1] = ry ? ay0 : ay1; wy[2] = ry ? ay1 : 0.0f; }
-    if (COMP == 2) { wz[0] = rz ? 0.0f : az0; wz[1] = rz ? az0 : az1; }
-    else { wz[0] = rz ? 0.0f : az0; wz[1] = rz ? az0 : az1; wz[2] = rz ? az1 : 0.0f; }
-}
-
-template <int COMP>
-FS_DEV void window_accumulate(const Base& b, int ci, int cj, int ck, float vc, float* num, float* den) {
-    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
-    float wx[3], wy[3], wz[3];
-    window_weights<COMP>(b, ci, cj, ck, wx, wy, wz);
-#pragma unroll
-    for (int z = 0; z < NZ; ++z)
-#pragma unroll
-        for (int y = 0; y < NY; ++y)
-#pragma unroll
-            for (int x = 0; x < NX; ++x) {
-                const int s = x + NX * (y + NY * z);
-                float wgt = wx[x] * wy[y] * wz[z];
-                num[s] += vc * wgt;
-                den[s] += wgt;
-            }
-}
-
-template <int COMP, int TX, int TY, int TZ>
-FS_DEV void window_flush(const P2GArgs& a, int x0, int y0, int z0, int ci, int cj, int ck, const float* num,
-                         const float* den, float* __restrict__ snum, float* __restrict__ sden) {
-    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
-    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
-    const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
-    const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
-    const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
-    const int ox = (COMP == 0) ? ci : ci - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
-#pragma unroll
-    for (int z = 0; z < NZ; ++z)
-#pragma unroll
-        for (int y = 0; y < NY; ++y)
-#pragma unroll
-            for (int x = 0; x < NX; ++x) {
-                const int s = x + NX * (y + NY * z);
-                const int i = ox + x, j = oy + y, k = oz + z;
-                if (i < x0 || i > hix || j < y0 || j > hiy || k < z0 || k > hiz) continue;
-                if (den[s] == 0.0f) continue;
-                const int l = (i - x0) + LX * ((j - y0) + LY * (k - z0));
-                atomicAdd(&snum[l], num[s]);
-                atomicAdd(&sden[l], den[s]);
-            }
-}
-
-// the window of one cell for one component: its particles' weights and
-// weighted velocities in registers, then one LDS atomic per touched face
-template <int COMP, int TX, int TY, int TZ, bool P2>
-FS_DEV void p2g_cell_comp(const P2GArgs& a, int x0, int y0, int z0, int ci, int cj, int ck, int beg, int end,
-                          float* __restrict__ snum, float* __restrict__ sden) {
+// The window of G consecutive cells along x of one row (their particles are
+// one contiguous range of the sorted arrays) for one component: the windows
+// of neighbouring cells overlap along x, so a G-cell window has G+1 (u) or G+2
+// (v, w) x-slots instead of G times 2 or 3, and correspondingly fewer LDS
+// atomics.  Along each axis a particle's base slot s gets 1 - f and slot s+1
+// gets f; a slot beyond the window is a face beyond the domain, which the
+// reference skips (the same rule as the one-cell window above).
+template <int COMP, int G, int TX, int TY, int TZ, bool P2>
+FS_DEV void p2g_group_comp(const P2GArgs& a, int x0, int y0, int z0, int ci0, int cj, int ck, int beg, int end,
+                           float* __restrict__ snum, float* __restrict__ sden) {
+    constexpr int NX = (COMP == 0) ? G + 1 : G + 2, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
+    constexpr int NS = NX * NY * NZ;
     const int W = a.W, H = a.H, D = a.D;
     const float dx = a.dx, inv = a.inv_dx;
     const float* __restrict__ vel = COMP == 0 ? a.P.vx : (COMP == 1 ? a.P.vy : a.P.vz);
-    float num[18], den[18];
+    const int ox = (COMP == 0) ? ci0 : ci0 - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
+    float num[NS], den[NS];
 #pragma unroll
-    for (int q = 0; q < 18; ++q) { num[q] = 0.f; den[q] = 0.f; }
-    // the next particle's values are loaded while the current one is accumulated
+    for (int q = 0; q < NS; ++q) { num[q] = 0.f; den[q] = 0.f; }
     float px = a.P.px[beg], py = a.P.py[beg], pz = a.P.pz[beg], vc = vel[beg];
     for (int p = beg; p < end; ++p) {
         const int q = p + 1 < end ? p + 1 : p;
         const float npx = a.P.px[q], npy = a.P.py[q], npz = a.P.pz[q], nvc = vel[q];
         const float rx = div_dx<P2>(px, dx, inv), ry = div_dx<P2>(py, dx, inv), rz = div_dx<P2>(pz, dx, inv);
-        window_accumulate<COMP>(mac_base<COMP>(rx, ry, rz, W, H, D), ci, cj, ck, vc, num, den);
+        const Base b = mac_base<COMP>(rx, ry, rz, W, H, D);
+        const int sx = b.i - ox, sy = b.j - oy, sz = b.k - oz;
+        const float ax0 = 1.0f - b.fx, ax1 = b.fx, ay0 = 1.0f - b.fy, ay1 = b.fy, az0 = 1.0f - b.fz, az1 = b.fz;
+        float wx[NX], wy[NY], wz[NZ];
+#pragma unroll
+        for (int t = 0; t < NX; ++t) wx[t] = t == sx ? ax0 : (t == sx + 1 ? ax1 : 0.0f);
+#pragma unroll
+        for (int t = 0; t < NY; ++t) wy[t] = t == sy ? ay0 : (t == sy + 1 ? ay1 : 0.0f);
+#pragma unroll
+        for (int t = 0; t < NZ; ++t) wz[t] = t == sz ? az0 : (t == sz + 1 ? az1 : 0.0f);
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+#pragma unroll
+            for (int y = 0; y < NY; ++y)
+#pragma unroll
+                for (int x = 0; x < NX; ++x) {
+                    const int sl = x + NX * (y + NY * z);
+                    const float wgt = wx[x] * wy[y] * wz[z];
+                    num[sl] += vc * wgt;
+                    den[sl] += wgt;
+                }
         px = npx; py = npy; pz = npz; vc = nvc;
     }
-    window_flush<COMP, TX, TY, TZ>(a, x0, y0, z0, ci, cj, ck, num, den, snum, sden);
+    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
+    const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
+    const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
+    const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
+#pragma unroll
+    for (int z = 0; z < NZ; ++z)
+#pragma unroll
+        for (int y = 0; y < NY; ++y)
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const int sl = x + NX * (y + NY * z);
+                const int i = ox + x, j = oy + y, k = oz + z;
+                if (i < x0 || i > hix || j < y0 || j > hiy || k < z0 || k > hiz) continue;
+                if (den[sl] == 0.0f) continue;
+                const int l = (i - x0) + LX * ((j - y0) + LY * (k - z0));
+                atomicAdd(&snum[l], num[sl]);
+                atomicAdd(&sden[l], den[sl]);
+            }
 }
 
 template <int COMP, int TX, int TY, int TZ>
@@ -388,17 +373,20 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     const int hy0 = max(y0 - 1, 0), hy1 = min(y0 + TY, H - 1);
     const int hz0 = max(z0 - 1, 0), hz1 = min(z0 + TZ, D - 1);
     const int nxh = hx1 - hx0 + 1, nyh = hy1 - hy0 + 1, nzh = hz1 - hz0 + 1;
-    const int ncell = nxh * nyh * nzh;
-    const float dx = a.dx, inv = a.inv_dx;
-    for (int cl = threadIdx.x; cl < ncell; cl += blockDim.x) {
-        const int ci = hx0 + cl % nxh, cj = hy0 + (cl / nxh) % nyh, ck = hz0 + cl / (nxh * nyh);
-        const long long c = (long long)ci + (long long)cj * W + (long long)ck * W * H;
-        const int beg = a.start[c], end = a.start[c + 1];
+    // one thread per group of P2G_GROUP consecutive halo cells along x (the
+    // last group of a row may be shorter), one velocity component at a time
+    constexpr int G = P2G_GROUP;
+    const int ngx = (nxh + G - 1) / G;
+    const int ngroup = ngx * nyh * nzh;
+    for (int gl = threadIdx.x; gl < ngroup; gl += blockDim.x) {
+        const int gx = gl % ngx, cj = hy0 + (gl / ngx) % nyh, ck = hz0 + gl / (ngx * nyh);
+        const int ci0 = hx0 + gx * G, ci1 = min(ci0 + G, hx1 + 1);
+        const long long c = (long long)ci0 + (long long)cj * W + (long long)ck * W * H;
+        const int beg = a.start[c], end = a.start[c + (ci1 - ci0)];
         if (beg == end) continue;
-        // one component at a time: 36 window accumulators live instead of 108
-        p2g_cell_comp<0, TX, TY, TZ, P2>(a, x0, y0, z0, ci, cj, ck, beg, end, s_un, s_ud);
-        p2g_cell_comp<1, TX, TY, TZ, P2>(a, x0, y0, z0, ci, cj, ck, beg, end, s_vn, s_vd);
-        p2g_cell_comp<2, TX, TY, TZ, P2>(a, x0, y0, z0, ci, cj, ck, beg, end, s_wn, s_wd);
+        p2g_group_comp<0, G, TX, TY, TZ, P2>(a, x0, y0, z0, ci0, cj, ck, beg, end, s_un, s_ud);
+        p2g_group_comp<1, G, TX, TY, TZ, P2>(a, x0, y0, z0, ci0, cj, ck, beg, end, s_vn, s_vd);
+        p2g_group_comp<2, G, TX, TY, TZ, P2>(a, x0, y0, z0, ci0, cj, ck, beg, end, s_wn, s_wd);
     }
     __syncthreads();
     p2g_write_comp<0, TX, TY, TZ>(a, x0, y0, z0, s_un, s_ud);
