@@ -112,6 +112,13 @@ _LEARNED_SIGS = {
     "fs_set_preconditioner": (C.c_int, [_vp, C.c_int, C.c_int]),
 }
 LEARNED_SYMBOLS = tuple(_LEARNED_SIGS)
+
+# include/fluidsim_checkpoint.h (HIP library only)
+_CKPT_SIGS = {
+    "fs_save_checkpoint": (C.c_int, [_vp, C.c_char_p]),
+    "fs_load_checkpoint": (C.c_int, [_vp, C.c_char_p]),
+}
+CHECKPOINT_SYMBOLS = tuple(_CKPT_SIGS)
 FSL_NUM_PARAMS = 27 * 3 * 16 + 16 + 3 * (27 * 16 * 16 + 16) + 27 * 16 + 1
 
 
@@ -128,11 +135,13 @@ class FsLib:
             fn.restype = res
             fn.argtypes = args
         self.has_learned = all(hasattr(self.lib, n) for n in _LEARNED_SIGS)
-        if self.has_learned:
-            for name, (res, args) in _LEARNED_SIGS.items():
-                fn = getattr(self.lib, name)
-                fn.restype = res
-                fn.argtypes = args
+        self.has_checkpoint = all(hasattr(self.lib, n) for n in _CKPT_SIGS)
+        for present, sigs in ((self.has_learned, _LEARNED_SIGS), (self.has_checkpoint, _CKPT_SIGS)):
+            if present:
+                for name, (res, args) in sigs.items():
+                    fn = getattr(self.lib, name)
+                    fn.restype = res
+                    fn.argtypes = args
         self.backend = self.lib.fs_backend().decode()
 
     def check(self, rc):
@@ -364,3 +373,14 @@ class Solver:
         self.fl.check(self.lib.fs_learned_train_step(self.h, _ptr(lab), _ptr(bb), _ptr(pp), W, H, D, float(lr),
                                                      C.byref(loss)))
         return loss.value
+
+    # ---- checkpoint / resume (include/fluidsim_checkpoint.h)
+    def save_checkpoint(self, path):
+        if not self.fl.has_checkpoint:
+            raise RuntimeError(f"{self.fl.path} has no checkpoint support")
+        self.fl.check(self.lib.fs_save_checkpoint(self.h, str(path).encode()))
+
+    def load_checkpoint(self, path):
+        if not self.fl.has_checkpoint:
+            raise RuntimeError(f"{self.fl.path} has no checkpoint support")
+        self.fl.check(self.lib.fs_load_checkpoint(self.h, str(path).encode()))

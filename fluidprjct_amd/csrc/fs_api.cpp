@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/fluidsim.h"
+#include "../../include/fluidsim_checkpoint.h"
 #include "../../include/fluidsim_learned.h"
 #include "fs_kernels.hpp"
 #include "fs_learned.hpp"
@@ -1125,6 +1126,159 @@ int fs_set_preconditioner(void* h, int precond, int stall_window) {
     if (precond < FS_PRECOND_MG || precond > FS_PRECOND_JACOBI) return fail(FS_ERR_ARG, "unknown preconditioner");
     s.cfg.precond = precond;
     if (stall_window > 0) s.cfg.stall_window = stall_window;
+    return FS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- checkpoint
+namespace {
+
+constexpr char kCkptMagic[8] = {'F', 'S', 'C', 'K', 'P', 'T', '0', '1'};
+
+struct CkptWriter {
+    std::ofstream f;
+    bool ok = true;
+    void raw(const void* p, size_t n) {
+        if (n) f.write((const char*)p, (std::streamsize)n);
+        ok = ok && (bool)f;
+    }
+    template <class T>
+    void put(const T& v) { raw(&v, sizeof(T)); }
+};
+
+struct CkptReader {
+    std::ifstream f;
+    bool ok = true;
+    void raw(void* p, size_t n) {
+        if (n) f.read((char*)p, (std::streamsize)n);
+        ok = ok && (bool)f;
+    }
+    template <class T>
+    void get(T& v) { raw(&v, sizeof(T)); }
+};
+
+// the body fields a step reads (everything but the SDF values), each passed
+// to io(pointer, bytes)
+template <class IO>
+void body_fields(HostBody& b, IO io) {
+    auto f = [&](auto& v) { io((void*)&v, sizeof(v)); };
+    f(b.sw); f(b.sh); f(b.sd);
+    f(b.file_origin); f(b.sdf_origin); f(b.local_com); f(b.size); f(b.cs);
+    f(b.pos); f(b.vel); f(b.force); f(b.omega); f(b.torque);
+    f(b.mass); f(b.inertia); f(b.inv_inertia); f(b.theta); f(b.psi); f(b.phi);
+    f(b.R); f(b.last_force); f(b.last_torque); f(b.last_sub);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_save_checkpoint(void* h, const char* path) {
+    Solver& s = *(Solver*)h;
+    if (!path) return fail(FS_ERR_ARG, "null path");
+    if (s.C == 0) return fail(FS_ERR_STATE, "nothing to save: no scene");
+    HIPCHK(hipStreamSynchronize(s.st));
+    CkptWriter w;
+    w.f.open(path, std::ios::binary);
+    if (!w.f.is_open()) return fail(FS_ERR_IO, std::string("cannot write ") + path);
+    w.raw(kCkptMagic, 8);
+    const int32_t cfg_size = (int32_t)sizeof(fs_config);
+    w.put(cfg_size);
+    w.put(s.cfg);
+    w.put(s.W); w.put(s.H); w.put(s.D);
+    w.put(s.dt_cur); w.put(s.sim_time); w.put(s.nsteps);
+    const int64_t np = s.np;
+    w.put(np);
+    std::vector<uint8_t> g((size_t)s.C);
+    HIPCHK(hipMemcpy(g.data(), s.d_solid, (size_t)s.C, hipMemcpyDeviceToHost));
+    w.raw(g.data(), g.size());
+    HIPCHK(hipMemcpy(g.data(), s.d_labels, (size_t)s.C, hipMemcpyDeviceToHost));
+    w.raw(g.data(), g.size());
+    std::vector<float> a((size_t)np);
+    for (float* src : {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz}) {
+        if (np) HIPCHK(hipMemcpy(a.data(), src, sizeof(float) * np, hipMemcpyDeviceToHost));
+        w.raw(a.data(), sizeof(float) * np);
+    }
+    std::vector<int> ids((size_t)np);
+    if (np) HIPCHK(hipMemcpy(ids.data(), s.PA.id, sizeof(int) * np, hipMemcpyDeviceToHost));
+    w.raw(ids.data(), sizeof(int) * np);
+    const int32_t active = s.body.active ? 1 : 0;
+    w.put(active);
+    if (active) {
+        body_fields(s.body, [&](void* p, size_t n) { w.raw(p, n); });
+        w.raw(s.body.sdf.data(), sizeof(float) * s.body.sdf.size());
+    }
+    w.f.close();
+    if (!w.ok || w.f.fail()) return fail(FS_ERR_IO, std::string("write failed: ") + path);
+    return FS_OK;
+}
+
+int fs_load_checkpoint(void* h, const char* path) {
+    Solver& s = *(Solver*)h;
+    if (!path) return fail(FS_ERR_ARG, "null path");
+    CkptReader r;
+    r.f.open(path, std::ios::binary);
+    if (!r.f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
+    char magic[8];
+    r.raw(magic, 8);
+    int32_t cfg_size = 0;
+    r.get(cfg_size);
+    if (!r.ok || std::memcmp(magic, kCkptMagic, 8) != 0 || cfg_size != (int32_t)sizeof(fs_config))
+        return fail(FS_ERR_IO, std::string("not a checkpoint of this version: ") + path);
+    fs_config cfg;
+    r.get(cfg);
+    int W = 0, H = 0, D = 0;
+    r.get(W); r.get(H); r.get(D);
+    float dt_cur = 0.0f;
+    double sim_time = 0.0;
+    int64_t nsteps = 0, np = 0;
+    r.get(dt_cur); r.get(sim_time); r.get(nsteps); r.get(np);
+    if (!r.ok || W <= 0 || H <= 0 || D <= 0 || np < 0 || np > 0x7fffffffLL)
+        return fail(FS_ERR_IO, std::string("corrupt checkpoint header: ") + path);
+    const size_t C = (size_t)W * H * D;
+    std::vector<uint8_t> solid(C), labels(C);
+    r.raw(solid.data(), C);
+    r.raw(labels.data(), C);
+    std::vector<float> arr[6];
+    for (auto& a : arr) {
+        a.resize((size_t)np);
+        r.raw(a.data(), sizeof(float) * np);
+    }
+    std::vector<int> ids((size_t)np);
+    r.raw(ids.data(), sizeof(int) * np);
+    int32_t active = 0;
+    r.get(active);
+    HostBody body;
+    if (r.ok && active) {
+        body.active = true;
+        body_fields(body, [&](void* p, size_t n) { r.raw(p, n); });
+        if (!(body.sw > 0 && body.sh > 0 && body.sd > 0)) return fail(FS_ERR_IO, "corrupt checkpoint body");
+        body.sdf.resize((size_t)body.sw * body.sh * body.sd);
+        r.raw(body.sdf.data(), sizeof(float) * body.sdf.size());
+    }
+    if (!r.ok) return fail(FS_ERR_IO, std::string("truncated checkpoint: ") + path);
+    // the state of the handle is replaced only once the whole file has been read
+    s.cfg = cfg;
+    int rc = s.alloc_grids(W, H, D);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(s.d_solid, solid.data(), C, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s.d_labels, labels.data(), C, hipMemcpyHostToDevice));
+    rc = s.alloc_particles(np);
+    if (rc) return rc;
+    float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
+    for (int c = 0; c < 6; ++c)
+        if (np) HIPCHK(hipMemcpy(dst[c], arr[c].data(), sizeof(float) * np, hipMemcpyHostToDevice));
+    if (np) HIPCHK(hipMemcpy(s.PA.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    s.body = std::move(body);
+    dfree(s.d_sdf);
+    if (s.body.active) {
+        HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * s.body.sdf.size()));
+        HIPCHK(hipMemcpy(s.d_sdf, s.body.sdf.data(), sizeof(float) * s.body.sdf.size(), hipMemcpyHostToDevice));
+    }
+    s.dt_cur = dt_cur;
+    s.sim_time = sim_time;
+    s.nsteps = nsteps;
     return FS_OK;
 }
 

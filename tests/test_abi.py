@@ -12,6 +12,7 @@ from fluidprjct_amd import _abi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fluidsim.h")
 LEARNED_HEADER = os.path.join(ROOT, "include", "fluidsim_learned.h")
+CKPT_HEADER = os.path.join(ROOT, "include", "fluidsim_checkpoint.h")
 
 
 def header_functions(path=HEADER):
@@ -23,6 +24,7 @@ def header_functions(path=HEADER):
 def test_header_and_ctypes_agree():
     assert header_functions() == sorted(_abi.ABI_SYMBOLS)
     assert header_functions(LEARNED_HEADER) == sorted(_abi.LEARNED_SYMBOLS)
+    assert header_functions(CKPT_HEADER) == sorted(_abi.CHECKPOINT_SYMBOLS)
 
 
 def test_learned_parameter_count_matches_header():
@@ -34,9 +36,10 @@ def test_learned_parameter_count_matches_header():
 def test_hip_library_exports_learned_symbols():
     path = os.path.join(ROOT, "fluidprjct_amd", "libfluidsim_hip.so")
     dll = C.CDLL(path)
-    missing = [f for f in header_functions(LEARNED_HEADER) if not hasattr(dll, f)]
+    missing = [f for f in header_functions(LEARNED_HEADER) + header_functions(CKPT_HEADER) if not hasattr(dll, f)]
     assert not missing, missing
-    assert _abi.FsLib(path).has_learned
+    fl = _abi.FsLib(path)
+    assert fl.has_learned and fl.has_checkpoint
 
 
 @pytest.mark.parametrize("lib", ["hip", "oracle"])
