@@ -13,7 +13,7 @@ import numpy as np
 
 FS_OK = 0
 FS_SOLID, FS_FLUID, FS_AIR, FS_BODY = 0, 1, 2, 3
-FS_FMT_PLY, FS_FMT_OFF = 0, 1
+FS_FMT_PLY, FS_FMT_OFF, FS_FMT_PLY_BINARY = 0, 1, 2
 FS_PRECOND_MG, FS_PRECOND_LEARNED, FS_PRECOND_JACOBI = 0, 1, 2
 FS_SOLVER_NO_GRAPH = 1
 (FS_GRID_U, FS_GRID_V, FS_GRID_W, FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED, FS_GRID_P,

@@ -15,6 +15,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fluidsim.h"
@@ -534,32 +535,59 @@ int upload_scene(Solver& s, int W, int H, int D, const std::vector<uint8_t>& L) 
     return FS_OK;
 }
 
-int write_particle_file(Solver& s, const char* path, int fmt) {
-    std::vector<float> pos((size_t)s.np * 3), vel;
-    {
-        std::vector<float> px(s.np), py(s.np), pz(s.np);
-        std::vector<int> id(s.np);
-        if (s.np) {
-            HIPCHK(hipMemcpy(px.data(), s.PA.px, sizeof(float) * s.np, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(py.data(), s.PA.py, sizeof(float) * s.np, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(pz.data(), s.PA.pz, sizeof(float) * s.np, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(id.data(), s.PA.id, sizeof(int) * s.np, hipMemcpyDeviceToHost));
-        }
-        for (long long i = 0; i < s.np; ++i) {
-            size_t d = (size_t)id[i];
-            pos[3 * d] = px[i]; pos[3 * d + 1] = py[i]; pos[3 * d + 2] = pz[i];
-        }
+// positions in original particle order (D2H of the current order and its ids)
+int gather_positions(Solver& s, std::vector<float>& pos) {
+    pos.assign((size_t)s.np * 3, 0.0f);
+    std::vector<float> px(s.np), py(s.np), pz(s.np);
+    std::vector<int> id(s.np);
+    if (s.np) {
+        HIPCHK(hipStreamSynchronize(s.st));
+        HIPCHK(hipMemcpy(px.data(), s.PA.px, sizeof(float) * s.np, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(py.data(), s.PA.py, sizeof(float) * s.np, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(pz.data(), s.PA.pz, sizeof(float) * s.np, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(id.data(), s.PA.id, sizeof(int) * s.np, hipMemcpyDeviceToHost));
     }
-    std::ofstream f(path);
-    if (!f.is_open()) return fail(FS_ERR_IO, std::string("Error opening file: ") + path);
-    if (fmt == FS_FMT_OFF) {
-        f << "OFF\n" << s.np << " 0 0\n";
-    } else {
-        f << "ply\nformat ascii 1.0\nelement vertex " << s.np
-          << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+    for (long long i = 0; i < s.np; ++i) {
+        size_t d = (size_t)id[i];
+        pos[3 * d] = px[i]; pos[3 * d + 1] = py[i]; pos[3 * d + 2] = pz[i];
     }
-    for (long long q = 0; q < s.np; ++q) f << pos[3 * q] << " " << pos[3 * q + 1] << " " << pos[3 * q + 2] << "\n";
     return FS_OK;
+}
+
+// save3dParticlesToPLY / ...ToOFF utility.cu:91-127 (ASCII, default ostream
+// precision), or binary little-endian PLY; host only
+int format_particle_file(const std::vector<float>& pos, const std::string& path, int fmt, std::string* err) {
+    const long long n = (long long)(pos.size() / 3);
+    std::ofstream f(path, fmt == FS_FMT_PLY_BINARY ? std::ios::binary : std::ios::out);
+    if (!f.is_open()) {
+        *err = "Error opening file: " + path;
+        return FS_ERR_IO;
+    }
+    if (fmt == FS_FMT_OFF) {
+        f << "OFF\n" << n << " 0 0\n";
+    } else {
+        f << "ply\nformat " << (fmt == FS_FMT_PLY_BINARY ? "binary_little_endian" : "ascii") << " 1.0\nelement vertex "
+          << n << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+    }
+    if (fmt == FS_FMT_PLY_BINARY) {
+        f.write((const char*)pos.data(), (std::streamsize)(sizeof(float) * pos.size()));
+    } else {
+        for (long long q = 0; q < n; ++q) f << pos[3 * q] << " " << pos[3 * q + 1] << " " << pos[3 * q + 2] << "\n";
+    }
+    if (!f) {
+        *err = "write failed: " + path;
+        return FS_ERR_IO;
+    }
+    return FS_OK;
+}
+
+int write_particle_file(Solver& s, const char* path, int fmt) {
+    std::vector<float> pos;
+    int rc = gather_positions(s, pos);
+    if (rc) return rc;
+    std::string err;
+    rc = format_particle_file(pos, path, fmt, &err);
+    return rc ? fail(rc, err) : FS_OK;
 }
 
 // RigidBody::generateSurfacePoints + exportToPLY utility.cuh:341-400
@@ -833,24 +861,45 @@ int fs_step(void* h, int nsteps) {
     return FS_OK;
 }
 
+// run() FluidSolver3D.cu:3100-3163: max_steps * iterPerFrame + 1 steps, a
+// particle file every iterPerFrame steps.  The positions of a frame are copied
+// to the host after its step; formatting and writing the file runs on a host
+// thread while the following steps run on the GPU (one frame in flight).
 int fs_run(void* h, int frames) {
     Solver& s = *(Solver*)h;
     const int ipf = std::max(1, s.cfg.iter_per_frame);
     std::ofstream csv("OutputData/csvRes.csv");
     if (csv.is_open()) csv << "frame,centerX1,centerX2,centerX3,angleX1,angleX2,angleX3\n";
-    for (int i = 0; i <= frames * ipf; ++i) {
-        int rc = s.step_once();
-        if (rc) return rc;
+    std::thread writer;
+    std::vector<float> frame_pos;
+    int writer_rc = FS_OK;
+    std::string writer_err;
+    auto join = [&]() {
+        if (writer.joinable()) writer.join();
+        return writer_rc;
+    };
+    int rc = FS_OK;
+    for (int i = 0; i <= frames * ipf && rc == FS_OK; ++i) {
+        rc = s.step_once();
+        if (rc) break;
         if (i % ipf == 0) {
-            std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
-                             (s.cfg.output_format == FS_FMT_OFF ? ".off" : ".ply");
-            if ((rc = write_particle_file(s, fn.c_str(), s.cfg.output_format))) return rc;
+            if ((rc = join())) break;
+            if ((rc = gather_positions(s, frame_pos))) break;
+            const std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
+                                   (s.cfg.output_format == FS_FMT_OFF ? ".off" : ".ply");
+            const int fmt = s.cfg.output_format;
+            writer = std::thread([&frame_pos, fn, fmt, &writer_rc, &writer_err]() {
+                writer_rc = format_particle_file(frame_pos, fn, fmt, &writer_err);
+            });
             if (s.body.active) write_body_ply(s, "OutputData/body_" + std::to_string(i / ipf) + ".ply");
             if (csv.is_open())
                 csv << i / ipf << "," << s.body.pos.x << "," << s.body.pos.y << "," << s.body.pos.z << ","
                     << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";
         }
     }
+    const int wrc = join();
+    if (rc) return rc;
+    if (wrc) return fail(wrc, writer_err);
     return FS_OK;
 }
 

@@ -46,6 +46,7 @@ extern "C" {
 /* particle output formats (utility.cuh:58-61) */
 #define FS_FMT_PLY 0
 #define FS_FMT_OFF 1
+#define FS_FMT_PLY_BINARY 2   /* binary_little_endian PLY, same vertex order (no reference counterpart) */
 
 /* pressure preconditioners (replace the cuDSS direct solve, FluidSolver3D.cu:1930-2008) */
 #define FS_PRECOND_MG       0   /* multigrid V-cycle inside PCG (default)            */

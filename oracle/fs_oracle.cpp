@@ -1041,16 +1041,19 @@ void init_body(Solver& s, const float* sdf, int w, int h, int d, f3 origin, floa
 }
 
 int write_particles(const Solver& s, const char* path, int fmt) {
-    std::ofstream f(path);
+    std::ofstream f(path, fmt == FS_FMT_PLY_BINARY ? std::ios::binary : std::ios::out);
     if (!f.is_open()) return fail(FS_ERR_IO, std::string("Error opening file: ") + path);
     int64_t n = s.np();
     if (fmt == FS_FMT_OFF) {
         f << "OFF\n" << n << " 0 0\n";
     } else {
-        f << "ply\nformat ascii 1.0\nelement vertex " << n
-          << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+        f << "ply\nformat " << (fmt == FS_FMT_PLY_BINARY ? "binary_little_endian" : "ascii") << " 1.0\nelement vertex "
+          << n << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
     }
-    for (int64_t q = 0; q < n; ++q) f << s.pos[3 * q] << " " << s.pos[3 * q + 1] << " " << s.pos[3 * q + 2] << "\n";
+    if (fmt == FS_FMT_PLY_BINARY)
+        f.write((const char*)s.pos.data(), (std::streamsize)(sizeof(float) * 3 * n));
+    else
+        for (int64_t q = 0; q < n; ++q) f << s.pos[3 * q] << " " << s.pos[3 * q + 1] << " " << s.pos[3 * q + 2] << "\n";
     return FS_OK;
 }
 

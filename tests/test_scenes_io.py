@@ -56,3 +56,38 @@ def test_ply_and_off_output(oracle, tmp_path):
                          "property float z", "end_header"]
     assert lines[7] == "0.15 0.25 0.35" and lines[8] == "0.123457 0.3 0.45"
     assert off.read_text().splitlines()[:2] == ["OFF", "2 0 0"]
+
+
+def read_ply(path):
+    """(format, vertices (n, 3) float32) of an ASCII or binary little-endian PLY"""
+    data = open(path, "rb").read()
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    header = data[:end].decode().split("\n")
+    fmt = header[1].split()[1]
+    n = int(header[2].split()[2])
+    body = data[end:]
+    if fmt == "binary_little_endian":
+        v = np.frombuffer(body, dtype="<f4", count=3 * n).reshape(n, 3)
+    else:
+        v = np.array([list(map(float, ln.split())) for ln in body.decode().splitlines()], np.float32).reshape(n, 3)
+    return fmt, v
+
+
+def test_binary_and_ascii_particle_files(oracle, tmp_path):
+    """utility.cu:91-109 ASCII PLY (6 significant digits) and the binary PLY
+    option hold the same particles in the same order."""
+    from fluidprjct_amd._abi import FS_FMT_PLY, FS_FMT_PLY_BINARY
+    s = oracle.solver(dx=0.05)
+    lab = np.full((6, 7, 8), 2, np.uint8)
+    lab[0], lab[:, 0], lab[:, :, 0] = 0, 0, 0
+    lab[1:4, 1:4, 1:5] = 1
+    s.set_labels(lab)
+    s.seed_particles(4, 9)
+    pos, _ = s.particles()
+    s.write_particles(tmp_path / "a.ply", FS_FMT_PLY)
+    s.write_particles(tmp_path / "b.ply", FS_FMT_PLY_BINARY)
+    fa, va = read_ply(tmp_path / "a.ply")
+    fb, vb = read_ply(tmp_path / "b.ply")
+    assert (fa, fb) == ("ascii", "binary_little_endian")
+    assert np.array_equal(vb, pos)
+    np.testing.assert_allclose(va, pos, rtol=1e-5, atol=1e-6)
