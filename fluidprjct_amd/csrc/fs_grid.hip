@@ -152,7 +152,10 @@ __global__ __launch_bounds__(256) void k_drag_lift(GridArgs a, double* __restric
     const long long C = (long long)W * H * D;
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (idx < C && a.labels[idx] == FS_BODY) {
+    const bool body = idx < C && a.labels[idx] == FS_BODY;
+    // most blocks hold no body cell: skip their six block reductions
+    if (!__syncthreads_or(body)) return;
+    if (body) {
         const float dx = a.dx, rho = a.density;
         int ci, cj, ck;
         split3(idx, W, H, ci, cj, ck);
