@@ -315,44 +315,50 @@ __global__ __launch_bounds__(256) void k_residual(MGLevel L, const float* __rest
 }
 
 // coarse b = 1/2 * P^T r with trilinear weights 1/4,3/4,3/4,1/4 per axis.
-// FUSED: r = b - A x of the fine unknowns is recomputed here (smaller levels).
-template <bool FUSED>
-__global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const float* __restrict__ rf,
-                                                  const float* __restrict__ bf, const float* __restrict__ xf,
-                                                  float* __restrict__ bc, const SolveScalars* __restrict__ sc) {
-    if (sc && sc->done) return;
-    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (c >= Cc.n) return;
-    if (!(Cc.code[c] & SELF)) { bc[c] = 0.0f; return; }
-    int I, J, K;
-    split3(c, Cc.W, Cc.H, I, J, K);
+// Summation order (shared by every kernel that restricts, so that they agree
+// bit for bit): one partial sum per fine z-slice dk = 0..3, each over dj then
+// di, and the four partials added in dk order.
+FS_DEV float restrict_slice(int FW, int FH, int FD, const float* __restrict__ rf, int I, int J, int K, int dk) {
     const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
-    const long long WH = (long long)F.W * F.H;
+    const int k = 2 * K - 1 + dk;
     float s = 0.0f;
-    for (int dk = 0; dk < 4; ++dk) {
-        int k = 2 * K - 1 + dk;
-        if (k < 0 || k >= F.D) continue;
-        for (int dj = 0; dj < 4; ++dj) {
-            int j = 2 * J - 1 + dj;
-            if (j < 0 || j >= F.H) continue;
-            float wjk = wt[dj] * wt[dk];
-            const long long row = (long long)j * F.W + (long long)k * WH;
-            for (int di = 0; di < 4; ++di) {
-                int i = 2 * I - 1 + di;
-                if (i < 0 || i >= F.W) continue;
-                float rv;
-                if (FUSED) {
-                    const long long f = row + i;
-                    uint32_t code = F.code[f];
-                    rv = (code & SELF) ? bf[f] - stencil_Ax(xf, f, code, F.W, WH) : 0.0f;
-                } else {
-                    rv = rf[row + i];
-                }
-                s += wt[di] * wjk * rv;
-            }
+    if (k < 0 || k >= FD) return s;
+    const long long WH = (long long)FW * FH;
+    for (int dj = 0; dj < 4; ++dj) {
+        int j = 2 * J - 1 + dj;
+        if (j < 0 || j >= FH) continue;
+        float wjk = wt[dj] * wt[dk];
+        const long long row = (long long)j * FW + (long long)k * WH;
+        for (int di = 0; di < 4; ++di) {
+            int i = 2 * I - 1 + di;
+            if (i < 0 || i >= FW) continue;
+            s += wt[di] * wjk * rf[row + i];
         }
     }
-    bc[c] = 0.5f * s;
+    return s;
+}
+
+FS_DEV float restrict_sum(float p0, float p1, float p2, float p3) { return 0.5f * ((((0.0f + p0) + p1) + p2) + p3); }
+
+// Four lanes per coarse cell (one fine z-slice each): the 64 fine reads of a
+// coarse cell are spread over 4 lanes instead of serialised in one.
+__global__ __launch_bounds__(256) void k_restrict(MGLevel F, MGLevel Cc, const float* __restrict__ rf,
+                                                  float* __restrict__ bc, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    const long long t = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long long c = t >> 2;
+    const int dk = (int)(t & 3);
+    const bool in = c < Cc.n;
+    const bool unk = in && (Cc.code[c] & SELF);
+    float p = 0.0f;
+    if (unk) {
+        int I, J, K;
+        split3(c, Cc.W, Cc.H, I, J, K);
+        p = restrict_slice(F.W, F.H, F.D, rf, I, J, K, dk);
+    }
+    // the four lanes of a cell are consecutive lanes of one wave
+    const float p1 = __shfl_down(p, 1, 64), p2 = __shfl_down(p, 2, 64), p3 = __shfl_down(p, 3, 64);
+    if (in && dk == 0) bc[c] = unk ? restrict_sum(p, p1, p2, p3) : 0.0f;
 }
 
 // (P xc) at fine cell (i,j,k)
@@ -622,6 +628,9 @@ __global__ __launch_bounds__(256) void k_residual_v4(MGLevel L, const float* __r
 }
 
 // fine x += P xc for 4 fine cells along x (they share their coarse columns)
+// The four fine cells 4m..4m+3 of a row share their coarse columns 2m-1 ..
+// 2m+2 (16 coarse values for the group instead of 8 per cell); each cell's
+// sum is formed in prolong_at's order.  Coarse indices fit in 32 bits.
 __global__ __launch_bounds__(256) void k_prolong_v4(MGLevel F, MGLevel Cc, const float* __restrict__ xc,
                                                     float* __restrict__ xf, const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
@@ -629,14 +638,50 @@ __global__ __launch_bounds__(256) void k_prolong_v4(MGLevel F, MGLevel Cc, const
     if (c0 >= F.n) return;
     const Codes4 k = load_codes4(F.code, c0);
     if (!((k.c[0] | k.c[1] | k.c[2] | k.c[3]) & SELF)) return;
-    const long long WH = (long long)F.W * F.H;
     int i0, j, kk;
     split3(c0, F.W, F.H, i0, j, kk);
+    const int m2 = i0 >> 1;   // even; columns m2 - 1 .. m2 + 2
+    const int J0 = j >> 1, K0 = kk >> 1;
+    const int Js[2] = {J0, (j & 1) ? J0 + 1 : J0 - 1}, Ks[2] = {K0, (kk & 1) ? K0 + 1 : K0 - 1};
+    const bool jok[2] = {true, Js[1] >= 0 && Js[1] < Cc.H}, kok[2] = {true, Ks[1] >= 0 && Ks[1] < Cc.D};
+    const bool lok = m2 >= 1, rok = m2 + 2 < Cc.W;
+    float col[2][2][4];   // [K][J][column m2 - 1 + t]
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+            col[a][bb][0] = col[a][bb][1] = col[a][bb][2] = col[a][bb][3] = 0.0f;
+            if (!(kok[a] && jok[bb])) continue;
+            const int row = Js[bb] * Cc.W + Ks[a] * Cc.W * Cc.H;
+            const float2 mid = *reinterpret_cast<const float2*>(xc + row + m2);
+            col[a][bb][1] = mid.x;
+            col[a][bb][2] = mid.y;
+            if (lok) col[a][bb][0] = xc[row + m2 - 1];
+            if (rok) col[a][bb][3] = xc[row + m2 + 2];
+        }
     float4 xv = ld4(xf, c0);
     float out[4] = {xv.x, xv.y, xv.z, xv.w};
+    const float w[2] = {0.75f, 0.25f};
+    // cell q: its own column and its second column (t indices into col), and
+    // whether the second one exists
+    const int t0[4] = {1, 1, 2, 2}, t1[4] = {0, 2, 1, 3};
+    const bool ok1[4] = {lok, true, true, rok};
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (k.c[q] & SELF) out[q] += prolong_at(Cc, xc, i0 + q, j, kk);
+    for (int q = 0; q < 4; ++q) {
+        if (!(k.c[q] & SELF)) continue;
+        float s = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            if (!kok[a]) continue;
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) {
+                if (!jok[bb]) continue;
+                s += w[0] * w[bb] * w[a] * col[a][bb][t0[q]];
+                if (ok1[q]) s += w[1] * w[bb] * w[a] * col[a][bb][t1[q]];
+            }
+        }
+        out[q] += s;
+    }
     *reinterpret_cast<float4*>(xf + c0) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
@@ -653,14 +698,16 @@ __global__ __launch_bounds__(256) void k_restrict_v2(MGLevel F, MGLevel Cc, cons
     const int J = (int)((pidx / half) % Cc.H), K = (int)(pidx / ((long long)half * Cc.H));
     const long long c = (long long)(2 * m) + (long long)J * Cc.W + (long long)K * Cc.W * Cc.H;
     const bool s0 = (Cc.code[c] & SELF) != 0, s1 = (Cc.code[c + 1] & SELF) != 0;
-    float sum0 = 0.0f, sum1 = 0.0f;
+    float q0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, q1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     if (s0 || s1) {
         const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
         const long long WH = (long long)F.W * F.H;
         const int x0 = 4 * m;
+#pragma unroll
         for (int dk = 0; dk < 4; ++dk) {
             int k = 2 * K - 1 + dk;
             if (k < 0 || k >= F.D) continue;
+            float sum0 = 0.0f, sum1 = 0.0f;
             for (int dj = 0; dj < 4; ++dj) {
                 int j = 2 * J - 1 + dj;
                 if (j < 0 || j >= F.H) continue;
@@ -679,10 +726,12 @@ __global__ __launch_bounds__(256) void k_restrict_v2(MGLevel F, MGLevel Cc, cons
                 sum1 += wt[2] * wjk * v.w;
                 if (x0 + 4 < F.W) sum1 += wt[3] * wjk * r;
             }
+            q0[dk] = sum0;
+            q1[dk] = sum1;
         }
     }
-    bc[c] = s0 ? 0.5f * sum0 : 0.0f;
-    bc[c + 1] = s1 ? 0.5f * sum1 : 0.0f;
+    bc[c] = s0 ? restrict_sum(q0[0], q0[1], q0[2], q0[3]) : 0.0f;
+    bc[c + 1] = s1 ? restrict_sum(q1[0], q1[1], q1[2], q1[3]) : 0.0f;
 }
 
 // k_pupdate_spmv for 4 cells per thread (same sums, FP64 A p)
@@ -774,151 +823,303 @@ __global__ __launch_bounds__(256) void k_update_v4(long long n, const float* __r
 }
 
 // ---------------------------------------------------------------- small levels
-// The V-cycle below a level of at most TAIL_MAX cells in one workgroup: on
-// such levels every kernel of a separate launch would be launch-latency bound
-// (each touches a few thousand cells).  The phases are the same operations as
-// the kernels above (two sweeps from zero, extra pre sweeps, residual,
-// restriction, coarsest solve in LDS, prolongation, post sweeps), separated by
-// workgroup barriers; the data stay in global memory (L2-resident).  The result
-// of the first tail level lands in its x buffer.
+// The V-cycle from the first level of at most TAIL_MAX cells down to the
+// coarsest, in one 1024-thread workgroup with every iterate in LDS.  Each of
+// these levels is a few thousand cells, so as separate launches every phase
+// would cost a kernel's fixed latency; here a phase is a workgroup barrier.
+// The operations and their floating-point order are those of the per-level
+// kernels (k_jacobi2, k_jacobi, k_residual, k_restrict, prolong_at,
+// k_coarse_solve), so the tail gives the same bits as the launch sequence.
+// LDS layout (floats): top level x, xt (its b and codes stay in registers,
+// TAIL_CPT cells per thread); each lower level x, xt, b and its codes
+// (uint16) at off[q].  Only the top level's result is written back (to x).
 constexpr int TAIL_LEVELS = 6;
-constexpr long long TAIL_MAX = 1LL << 11;
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_CPT = 4;
+constexpr long long TAIL_MAX = (long long)TAIL_THREADS * TAIL_CPT;
+constexpr int TAIL_LDS_FLOATS = 38912;   // 152 KiB of the 160 KiB per workgroup
 struct TailLevels {
     MGLevel lv[TAIL_LEVELS];
+    int off[TAIL_LEVELS];
     int n;   // lv[n - 1] is the coarsest level
 };
 
-__global__ __launch_bounds__(1024) void k_tail(TailLevels t, int npre, int npost, int coarse_sweeps, float omega,
-                                               const SolveScalars* __restrict__ sc) {
-    if (sc && sc->done) return;
-    __shared__ float sx[2][COARSE_MAX];
-    __shared__ float sb[COARSE_MAX];
-    __shared__ uint16_t scode[COARSE_MAX];
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int extra = (npre >= 2 ? npre - 2 : 0) + npost;
-    float* fin[TAIL_LEVELS];   // buffer holding each level's current iterate
-    // ---- down
-    for (int l = 0; l < t.n - 1; ++l) {
-        const MGLevel& L = t.lv[l];
-        const MGLevel& Cc = t.lv[l + 1];
-        const long long WH = (long long)L.W * L.H;
-        float* cur = (extra & 1) ? L.xt : L.x;
-        float* oth = (cur == L.x) ? L.xt : L.x;
-        for (long long c = tid; c < L.n; c += nt) {
-            uint32_t code = L.code[c];
-            float xn = 0.0f;
-            uint32_t d = (code >> 8) & 7u;
-            if ((code & SELF) && d) {
-                float bc = L.b[c];
-                float x1 = omega * bc * inv_diag(d);
-                float ax = (float)d * x1;
-                if (code & 1u) ax -= jacobi_first(L.b, L.code, c + 1, omega);
-                if (code & 2u) ax -= jacobi_first(L.b, L.code, c - 1, omega);
-                if (code & 4u) ax -= jacobi_first(L.b, L.code, c + L.W, omega);
-                if (code & 8u) ax -= jacobi_first(L.b, L.code, c - L.W, omega);
-                if (code & 16u) ax -= jacobi_first(L.b, L.code, c + WH, omega);
-                if (code & 32u) ax -= jacobi_first(L.b, L.code, c - WH, omega);
-                xn = x1 + (npre >= 2 ? omega : 0.0f) * (L.b[c] - ax) * inv_diag(d);
-            }
-            cur[c] = xn;
-        }
-        __syncthreads();
-        for (int k = 2; k < npre; ++k) {
-            for (long long c = tid; c < L.n; c += nt) {
-                uint32_t code = L.code[c];
-                float xn = 0.0f;
-                uint32_t d = (code >> 8) & 7u;
-                if ((code & SELF) && d) xn = cur[c] + omega * (L.b[c] - stencil_Ax(cur, c, code, L.W, WH)) * inv_diag(d);
-                oth[c] = xn;
-            }
-            __syncthreads();
-            float* tmp = cur; cur = oth; oth = tmp;
-        }
-        for (long long c = tid; c < L.n; c += nt) {
-            uint32_t code = L.code[c];
-            L.r[c] = (code & SELF) ? L.b[c] - stencil_Ax(cur, c, code, L.W, WH) : 0.0f;
-        }
-        __syncthreads();
-        const float wt[4] = {0.25f, 0.75f, 0.75f, 0.25f};
-        for (long long c = tid; c < Cc.n; c += nt) {
-            if (!(Cc.code[c] & SELF)) { Cc.b[c] = 0.0f; continue; }
+// keeps the scheduler from interleaving the unrolled per-cell bodies of the
+// top level (which would hold every cell's operands at once and spill)
+#define CELL_FENCE() __builtin_amdgcn_sched_barrier(0)
+// makes the row and plane strides opaque at the start of a top-level pass, so
+// that the neighbour addresses of the TAIL_CPT cells are formed inside the
+// pass instead of being hoisted out of the sweep loop and held in registers
+#define OPAQUE_STRIDES(W, WH) asm volatile("" : "+s"(W), "+s"(WH))
+// the same for the register-held b and codes: values derived from them (1/d,
+// the first sweep's x) are recomputed in each pass rather than kept live
+#define OPAQUE_TOP(b, c)                                        \
+    _Pragma("unroll") for (int q_ = 0; q_ < TAIL_CPT; ++q_)     \
+        asm volatile("" : "+v"(b[q_]), "+v"(c[q_]))
+
+// floats of LDS for a level below the top one: x, xt, b and the codes
+FS_HD inline int tail_level_floats(long long n) { return (int)((3 * n + (n + 1) / 2 + 3) & ~3LL); }
+
+// Jacobi sweep on an LDS level: xout = xin + omega (b - A xin) / d
+FS_DEV void lds_sweep(const MGLevel& L, const float* xin, float* xout, const float* b, const uint16_t* code,
+                      float omega) {
+    const int W = L.W, WH = L.W * L.H, n = (int)L.n;
+    for (int c = threadIdx.x; c < n; c += TAIL_THREADS) {
+        uint32_t cd = code[c];
+        float xn = 0.0f;
+        uint32_t d = (cd >> 8) & 7u;
+        if ((cd & SELF) && d) xn = xin[c] + omega * (b[c] - stencil_Ax(xin, c, cd, W, WH)) * inv_diag(d);
+        xout[c] = xn;
+    }
+}
+
+// b of the coarse level Cc from the fine residual rf (both in LDS)
+FS_DEV void lds_restrict(const MGLevel& F, const float* rf, const MGLevel& Cc, float* bc, const uint16_t* ccode) {
+    const int n = (int)Cc.n;
+    for (int c = threadIdx.x; c < n; c += TAIL_THREADS) {
+        float v = 0.0f;
+        if (ccode[c] & SELF) {
             int I, J, K;
             split3(c, Cc.W, Cc.H, I, J, K);
-            float sum = 0.0f;
-            for (int dk = 0; dk < 4; ++dk) {
-                int k = 2 * K - 1 + dk;
-                if (k < 0 || k >= L.D) continue;
-                for (int dj = 0; dj < 4; ++dj) {
-                    int j = 2 * J - 1 + dj;
-                    if (j < 0 || j >= L.H) continue;
-                    float wjk = wt[dj] * wt[dk];
-                    const long long row = (long long)j * L.W + (long long)k * WH;
-                    for (int di = 0; di < 4; ++di) {
-                        int i = 2 * I - 1 + di;
-                        if (i < 0 || i >= L.W) continue;
-                        sum += wt[di] * wjk * L.r[row + i];
-                    }
-                }
-            }
-            Cc.b[c] = 0.5f * sum;
+            v = restrict_sum(restrict_slice(F.W, F.H, F.D, rf, I, J, K, 0),
+                             restrict_slice(F.W, F.H, F.D, rf, I, J, K, 1),
+                             restrict_slice(F.W, F.H, F.D, rf, I, J, K, 2),
+                             restrict_slice(F.W, F.H, F.D, rf, I, J, K, 3));
         }
-        fin[l] = cur;
-        __syncthreads();
+        bc[c] = v;
     }
-    // ---- coarsest: damped-Jacobi sweeps from zero in LDS
-    {
-        const MGLevel& L = t.lv[t.n - 1];
-        const int n = (int)L.n;
-        const long long WH = (long long)L.W * L.H;
-        for (int c = tid; c < n; c += nt) {
-            sb[c] = L.b[c];
-            scode[c] = L.code[c];
-            sx[0][c] = 0.0f;
-        }
-        __syncthreads();
-        int cur = 0;
-        for (int sw = 0; sw < coarse_sweeps; ++sw) {
-            for (int c = tid; c < n; c += nt) {
-                uint32_t code = scode[c];
-                float xn = 0.0f;
-                uint32_t d = (code >> 8) & 7u;
-                if ((code & SELF) && d) xn = sx[cur][c] + omega * (sb[c] - stencil_Ax(sx[cur], c, code, L.W, WH)) * inv_diag(d);
-                sx[cur ^ 1][c] = xn;
-            }
-            __syncthreads();
-            cur ^= 1;
-        }
-        for (int c = tid; c < n; c += nt) L.x[c] = sx[cur][c];
-        fin[t.n - 1] = L.x;
-        __syncthreads();
+}
+
+// x += P xc on the unknowns of an LDS level
+FS_DEV void lds_prolong(const MGLevel& L, float* x, const uint16_t* code, const MGLevel& Cc, const float* xc) {
+    const int n = (int)L.n;
+    for (int c = threadIdx.x; c < n; c += TAIL_THREADS) {
+        if (!(code[c] & SELF)) continue;
+        int i, j, k;
+        split3(c, L.W, L.H, i, j, k);
+        x[c] += prolong_at(Cc, xc, i, j, k);
     }
-    // ---- up
-    for (int l = t.n - 2; l >= 0; --l) {
+}
+
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, int npost, int coarse_sweeps,
+                                                       float omega, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ float lds[TAIL_LDS_FLOATS];
+    const int tid = threadIdx.x;
+    const float omega2 = npre >= 2 ? omega : 0.0f;
+    const MGLevel T = t.lv[0];
+    const int nT = (int)T.n;
+    int WT = T.W, WHT = T.W * T.H;
+    // top level: b and codes in registers, cells tid + q * TAIL_THREADS
+    float bT[TAIL_CPT];
+    uint32_t cT[TAIL_CPT];
+#pragma unroll
+    for (int q = 0; q < TAIL_CPT; ++q) {
+        const int c = tid + q * TAIL_THREADS;
+        bT[q] = 0.0f;
+        cT[q] = 0u;
+        if (c < nT) {
+            bT[q] = T.b[c];
+            cT[q] = T.code[c];
+        }
+    }
+    // codes of the lower levels
+    for (int l = 1; l < t.n; ++l) {
         const MGLevel& L = t.lv[l];
-        const MGLevel& Cc = t.lv[l + 1];
-        const long long WH = (long long)L.W * L.H;
-        const float* xc = fin[l + 1];
-        float* cur = fin[l];
-        float* oth = (cur == L.x) ? L.xt : L.x;
-        for (long long c = tid; c < L.n; c += nt) {
-            if (!(L.code[c] & SELF)) continue;
+        uint16_t* code = reinterpret_cast<uint16_t*>(lds + t.off[l] + 3 * L.n);
+        for (int c = tid; c < (int)L.n; c += TAIL_THREADS) code[c] = L.code[c];
+    }
+    float* tx = lds;
+    float* txt = lds + nT;
+    // ---- top level down: two sweeps from zero (k_jacobi2), extra pre sweeps, residual
+#pragma unroll
+    for (int q = 0; q < TAIL_CPT; ++q) {
+        const int c = tid + q * TAIL_THREADS;
+        const uint32_t d = (cT[q] >> 8) & 7u;
+        if (c < nT) tx[c] = ((cT[q] & SELF) && d) ? omega * bT[q] * inv_diag(d) : 0.0f;
+    }
+    __syncthreads();
+    OPAQUE_TOP(bT, cT);
+#pragma unroll
+    for (int q = 0; q < TAIL_CPT; ++q) {
+        const int c = tid + q * TAIL_THREADS;
+        if (c >= nT) continue;
+        const uint32_t code = cT[q], d = (code >> 8) & 7u;
+        float xn = 0.0f;
+        if ((code & SELF) && d) {
+            const float x1 = omega * bT[q] * inv_diag(d);
+            float ax = (float)d * x1;
+            if (code & 1u) ax -= tx[c + 1];
+            if (code & 2u) ax -= tx[c - 1];
+            if (code & 4u) ax -= tx[c + WT];
+            if (code & 8u) ax -= tx[c - WT];
+            if (code & 16u) ax -= tx[c + WHT];
+            if (code & 32u) ax -= tx[c - WHT];
+            xn = x1 + omega2 * (bT[q] - ax) * inv_diag(d);
+        }
+        txt[c] = xn;
+        CELL_FENCE();
+    }
+    __syncthreads();
+    float* cur = txt;
+    float* oth = tx;
+    for (int k = 2; k < npre; ++k) {
+        OPAQUE_STRIDES(WT, WHT);
+        OPAQUE_TOP(bT, cT);
+#pragma unroll
+        for (int q = 0; q < TAIL_CPT; ++q) {
+            const int c = tid + q * TAIL_THREADS;
+            if (c >= nT) continue;
+            const uint32_t code = cT[q], d = (code >> 8) & 7u;
+            float xn = 0.0f;
+            if ((code & SELF) && d) xn = cur[c] + omega * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
+            oth[c] = xn;
+            CELL_FENCE();
+        }
+        __syncthreads();
+        float* tmp = cur; cur = oth; oth = tmp;
+    }
+    OPAQUE_STRIDES(WT, WHT);
+    OPAQUE_TOP(bT, cT);
+#pragma unroll
+    for (int q = 0; q < TAIL_CPT; ++q) {
+        const int c = tid + q * TAIL_THREADS;
+        if (c < nT) oth[c] = (cT[q] & SELF) ? bT[q] - stencil_Ax(cur, c, cT[q], WT, WHT) : 0.0f;
+        CELL_FENCE();
+    }
+    __syncthreads();
+    {
+        const MGLevel& C1 = t.lv[1];
+        lds_restrict(T, oth, C1, lds + t.off[1] + 2 * C1.n,
+                     reinterpret_cast<const uint16_t*>(lds + t.off[1] + 3 * C1.n));
+    }
+    __syncthreads();
+    float* top = cur;   // the top level's iterate
+    // ---- lower levels down; bit l of in_xt: level l's iterate is in its xt
+    unsigned in_xt = 0;
+    for (int l = 1; l < t.n - 1; ++l) {
+        const MGLevel& L = t.lv[l];
+        float* x = lds + t.off[l];
+        float* xt = x + L.n;
+        const float* b = x + 2 * L.n;
+        const uint16_t* code = reinterpret_cast<const uint16_t*>(x + 3 * L.n);
+        const int W = L.W, WH = L.W * L.H, n = (int)L.n;
+        for (int c = tid; c < n; c += TAIL_THREADS) {
+            const uint32_t cd = code[c], d = (cd >> 8) & 7u;
+            x[c] = ((cd & SELF) && d) ? omega * b[c] * inv_diag(d) : 0.0f;
+        }
+        __syncthreads();
+        for (int c = tid; c < n; c += TAIL_THREADS) {
+            const uint32_t cd = code[c], d = (cd >> 8) & 7u;
+            float xn = 0.0f;
+            if ((cd & SELF) && d) {
+                const float x1 = omega * b[c] * inv_diag(d);
+                float ax = (float)d * x1;
+                if (cd & 1u) ax -= x[c + 1];
+                if (cd & 2u) ax -= x[c - 1];
+                if (cd & 4u) ax -= x[c + W];
+                if (cd & 8u) ax -= x[c - W];
+                if (cd & 16u) ax -= x[c + WH];
+                if (cd & 32u) ax -= x[c - WH];
+                xn = x1 + omega2 * (b[c] - ax) * inv_diag(d);
+            }
+            xt[c] = xn;
+        }
+        __syncthreads();
+        float* lc = xt;
+        float* lo = x;
+        for (int k = 2; k < npre; ++k) {
+            lds_sweep(L, lc, lo, b, code, omega);
+            __syncthreads();
+            float* tmp = lc; lc = lo; lo = tmp;
+        }
+        for (int c = tid; c < n; c += TAIL_THREADS) {
+            const uint32_t cd = code[c];
+            lo[c] = (cd & SELF) ? b[c] - stencil_Ax(lc, c, cd, W, WH) : 0.0f;
+        }
+        __syncthreads();
+        const MGLevel& Cn = t.lv[l + 1];
+        lds_restrict(L, lo, Cn, lds + t.off[l + 1] + 2 * Cn.n,
+                     reinterpret_cast<const uint16_t*>(lds + t.off[l + 1] + 3 * Cn.n));
+        __syncthreads();
+        if (lc == xt) in_xt |= 1u << l;
+    }
+    // ---- coarsest: damped-Jacobi sweeps from zero (k_coarse_solve)
+    {
+        const int l = t.n - 1;
+        const MGLevel& L = t.lv[l];
+        float* x = lds + t.off[l];
+        float* xt = x + L.n;
+        const float* b = x + 2 * L.n;
+        const uint16_t* code = reinterpret_cast<const uint16_t*>(x + 3 * L.n);
+        for (int c = tid; c < (int)L.n; c += TAIL_THREADS) x[c] = 0.0f;
+        __syncthreads();
+        float* lc = x;
+        float* lo = xt;
+        for (int sw = 0; sw < coarse_sweeps; ++sw) {
+            lds_sweep(L, lc, lo, b, code, omega);
+            __syncthreads();
+            float* tmp = lc; lc = lo; lo = tmp;
+        }
+        if (lc == xt) in_xt |= 1u << l;
+    }
+    // ---- lower levels up: prolongation, post sweeps
+    for (int l = t.n - 2; l >= 1; --l) {
+        const MGLevel& L = t.lv[l];
+        const MGLevel& Cn = t.lv[l + 1];
+        float* x = lds + t.off[l];
+        float* xt = x + L.n;
+        const float* b = x + 2 * L.n;
+        const uint16_t* code = reinterpret_cast<const uint16_t*>(x + 3 * L.n);
+        const float* xc = lds + t.off[l + 1] + (((in_xt >> (l + 1)) & 1u) ? Cn.n : 0);
+        float* lc = ((in_xt >> l) & 1u) ? xt : x;
+        float* lo = (lc == x) ? xt : x;
+        lds_prolong(L, lc, code, Cn, xc);
+        __syncthreads();
+        for (int k = 0; k < npost; ++k) {
+            lds_sweep(L, lc, lo, b, code, omega);
+            __syncthreads();
+            float* tmp = lc; lc = lo; lo = tmp;
+        }
+        in_xt = (in_xt & ~(1u << l)) | ((lc == xt ? 1u : 0u) << l);
+    }
+    // ---- top level up
+    {
+        const MGLevel& C1 = t.lv[1];
+        const float* xc = lds + t.off[1] + (((in_xt >> 1) & 1u) ? C1.n : 0);
+        cur = top;
+        oth = (cur == tx) ? txt : tx;
+        OPAQUE_TOP(bT, cT);
+#pragma unroll
+        for (int q = 0; q < TAIL_CPT; ++q) {
+            const int c = tid + q * TAIL_THREADS;
+            if (c >= nT || !(cT[q] & SELF)) continue;
             int i, j, k;
-            split3(c, L.W, L.H, i, j, k);
-            cur[c] += prolong_at(Cc, xc, i, j, k);
+            split3(c, T.W, T.H, i, j, k);
+            cur[c] += prolong_at(C1, xc, i, j, k);
+            CELL_FENCE();
         }
         __syncthreads();
         for (int k = 0; k < npost; ++k) {
-            for (long long c = tid; c < L.n; c += nt) {
-                uint32_t code = L.code[c];
+            OPAQUE_STRIDES(WT, WHT);
+        OPAQUE_TOP(bT, cT);
+#pragma unroll
+            for (int q = 0; q < TAIL_CPT; ++q) {
+                const int c = tid + q * TAIL_THREADS;
+                if (c >= nT) continue;
+                const uint32_t code = cT[q], d = (code >> 8) & 7u;
                 float xn = 0.0f;
-                uint32_t d = (code >> 8) & 7u;
-                if ((code & SELF) && d) xn = cur[c] + omega * (L.b[c] - stencil_Ax(cur, c, code, L.W, WH)) * inv_diag(d);
+                if ((code & SELF) && d) xn = cur[c] + omega * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
                 oth[c] = xn;
             }
             __syncthreads();
             float* tmp = cur; cur = oth; oth = tmp;
         }
-        fin[l] = cur;
+#pragma unroll
+        for (int q = 0; q < TAIL_CPT; ++q) {
+            const int c = tid + q * TAIL_THREADS;
+            if (c < nT) T.x[c] = cur[c];
+        }
     }
 }
 
@@ -1245,10 +1446,18 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     if (use_tail && l > 0 && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
         TailLevels t{};
         t.n = nlev - l;
-        for (int q = 0; q < t.n; ++q) t.lv[q] = lv[l + q];
-        hipLaunchKernelGGL(k_tail, dim3(1), dim3(1024), 0, st, t, std::max(nu_pre, 1), std::max(nu_post, 1),
-                           coarse_sweeps, omega, (const SolveScalars*)s);
-        return L.x;
+        long long off = (2 * L.n + 3) & ~3LL;
+        for (int q = 0; q < t.n; ++q) {
+            t.lv[q] = lv[l + q];
+            if (q == 0) continue;
+            t.off[q] = (int)off;
+            off += tail_level_floats(lv[l + q].n);
+        }
+        if (off <= TAIL_LDS_FLOATS) {
+            hipLaunchKernelGGL(k_tail, dim3(1), dim3(TAIL_THREADS), 0, st, t, std::max(nu_pre, 1),
+                               std::max(nu_post, 1), coarse_sweeps, omega, (const SolveScalars*)s);
+            return L.x;
+        }
     }
     if (l == nlev - 1) {
         hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(1024), 0, st, L, L.b, L.x, coarse_sweeps, omega, s);
@@ -1294,8 +1503,8 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         hipLaunchKernelGGL(k_restrict_v2, dim3(nblk(Cn.n / 2)), dim3(256), 0, st, L, Cn, (const float*)L.r, Cn.b,
                            (const SolveScalars*)s);
     else
-        hipLaunchKernelGGL(k_restrict<false>, dim3(nblk(Cn.n)), dim3(256), 0, st, L, Cn, L.r, (const float*)nullptr,
-                           (const float*)nullptr, Cn.b, s);
+        hipLaunchKernelGGL(k_restrict, dim3(nblk(4 * Cn.n)), dim3(256), 0, st, L, Cn, (const float*)L.r, Cn.b,
+                           (const SolveScalars*)s);
     float* xc = vcycle_level(l + 1, st, nullptr, s);
     int k0;
     if (fuse) {

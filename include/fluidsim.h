@@ -111,6 +111,7 @@ typedef struct fs_config {
 
 /* solver_flags */
 #define FS_SOLVER_NO_GRAPH 1     /* launch the inner PCG iterations eagerly instead of as a HIP graph */
+#define FS_SOLVER_NO_TAIL  2     /* run every multigrid level as separate launches (no single-workgroup tail) */
 
 typedef struct fs_body_state {
     float pos[3], vel[3], omega[3];

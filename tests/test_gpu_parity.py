@@ -211,3 +211,19 @@ def test_graph_replays_match_eager(hip):
         pe, se = e.solve_pressure_system(lab, b, 1e-7)
         assert sg.iterations == se.iterations and sg.converged == 1
         assert np.array_equal(pg, pe), rep
+
+
+@pytest.mark.parametrize("N,block", [(64, (0.6, 0.5, 1.0)), (96, (1.0, 0.4, 1.0)), (128, (1.0, 0.4, 1.0))])
+def test_tail_matches_level_launches(hip, N, block):
+    """The V-cycle's small levels run in one workgroup with the iterates in LDS;
+    the result is bit-identical to running each of those levels as separate
+    kernel launches (same operations in the same floating-point order)."""
+    from fluidprjct_amd._abi import FS_SOLVER_NO_TAIL
+    rng = np.random.default_rng(N)
+    lab = tank(N, block=block)
+    b = (rng.standard_normal(lab.shape) * 1e3).astype(np.float32) * (lab == 1)
+    pt, st = hip.solver().solve_pressure_system(lab, b, 1e-7)
+    pn, sn = hip.solver(solver_flags=FS_SOLVER_NO_TAIL).solve_pressure_system(lab, b, 1e-7)
+    assert st.converged == 1 and sn.converged == 1
+    assert st.iterations == sn.iterations
+    assert np.array_equal(pt.view(np.uint32), pn.view(np.uint32))
