@@ -89,16 +89,18 @@ FS_DEV Base mac_base(float rx, float ry, float rz, int W, int H, int D) {
     return b;
 }
 
-// trilinearInterpolation FluidSolver3D.cu:784-801
+// trilinearInterpolation FluidSolver3D.cu:784-801; each lerp a*(1-t) + b*t is
+// fma(a, 1-t, b*t) (the contraction a CUDA build makes by default; the oracle
+// uses the same expression, so G2P and advection stay bit-exact against it)
 FS_DEV float trilin(float xd, float yd, float zd, float c000, float c100, float c001, float c101,
                     float c010, float c110, float c011, float c111) {
-    float c00 = c000 * (1.0f - xd) + c100 * xd;
-    float c01 = c001 * (1.0f - xd) + c101 * xd;
-    float c10 = c010 * (1.0f - xd) + c110 * xd;
-    float c11 = c011 * (1.0f - xd) + c111 * xd;
-    float c0 = c00 * (1.0f - yd) + c10 * yd;
-    float c1 = c01 * (1.0f - yd) + c11 * yd;
-    return c0 * (1.0f - zd) + c1 * zd;
+    float c00 = fmaf(c000, 1.0f - xd, c100 * xd);
+    float c01 = fmaf(c001, 1.0f - xd, c101 * xd);
+    float c10 = fmaf(c010, 1.0f - xd, c110 * xd);
+    float c11 = fmaf(c011, 1.0f - xd, c111 * xd);
+    float c0 = fmaf(c00, 1.0f - yd, c10 * yd);
+    float c1 = fmaf(c01, 1.0f - yd, c11 * yd);
+    return fmaf(c0, 1.0f - zd, c1 * zd);
 }
 
 // Rigid-body pose used by the label pass and the face boundary conditions.

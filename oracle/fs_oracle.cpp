@@ -730,16 +730,18 @@ void stage_project(Solver& s) {
     }
 }
 
-/* trilinearInterpolation FluidSolver3D.cu:784-801 */
+/* trilinearInterpolation FluidSolver3D.cu:784-801.  Each lerp a*(1-t) + b*t is
+   one fused multiply-add over the rounded product b*t, the form a CUDA build
+   contracts it to by default; the HIP kernels use the same expression. */
 inline float trilin(float xd, float yd, float zd, float c000, float c100, float c001, float c101,
                     float c010, float c110, float c011, float c111) {
-    float c00 = c000 * (1.0f - xd) + c100 * xd;
-    float c01 = c001 * (1.0f - xd) + c101 * xd;
-    float c10 = c010 * (1.0f - xd) + c110 * xd;
-    float c11 = c011 * (1.0f - xd) + c111 * xd;
-    float c0 = c00 * (1.0f - yd) + c10 * yd;
-    float c1 = c01 * (1.0f - yd) + c11 * yd;
-    return c0 * (1.0f - zd) + c1 * zd;
+    float c00 = fmaf(c000, 1.0f - xd, c100 * xd);
+    float c01 = fmaf(c001, 1.0f - xd, c101 * xd);
+    float c10 = fmaf(c010, 1.0f - xd, c110 * xd);
+    float c11 = fmaf(c011, 1.0f - xd, c111 * xd);
+    float c0 = fmaf(c00, 1.0f - yd, c10 * yd);
+    float c1 = fmaf(c01, 1.0f - yd, c11 * yd);
+    return fmaf(c0, 1.0f - zd, c1 * zd);
 }
 
 /* 8-corner sample of one MAC component at base b (reads clamped into the array) */

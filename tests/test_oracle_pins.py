@@ -297,6 +297,40 @@ def test_g2p_uniform_fields_and_advection(oracle):
     assert np.allclose(p2, pos + np.array(U) * dt, atol=2e-6)
 
 
+@pytest.mark.parametrize("axis", [None, 0, 1, 2])
+def test_g2p_samples_faces_exactly(oracle, axis):
+    """A particle on a u-face centre (i dx, (j + 1/2) dx, (k + 1/2) dx) has all
+    three u fractions 0, so with PIC weight 1 its new v_x is that face's value
+    bit for bit; moved by dx/2 along one axis, that fraction is 1/2 and v_x is
+    the mean of the face and its neighbour along that axis, rounded once.
+    Pins the base index and the orientation of each axis of the trilinear
+    interpolation (a swapped axis or stride reads a different face)."""
+    N, dx = 12, 1.0 / 16
+    s = oracle.solver(dx=dx, dt=1e-6, pic_weight=1.0)
+    s.set_labels(np.full((N, N, N), 2, np.uint8))
+    rng = np.random.default_rng(9)
+    f = {}
+    for g, gs in ((FS_GRID_U, FS_GRID_USAVED), (FS_GRID_V, FS_GRID_VSAVED), (FS_GRID_W, FS_GRID_WSAVED)):
+        shape, _ = s.grid_shape(g)
+        f[g] = rng.standard_normal(shape).astype(np.float32)
+        s.set_grid(g, f[g])
+        s.set_grid(gs, f[g])
+    ijk = rng.integers(1, N - 2, (300, 3))
+    pos = np.stack([ijk[:, 0] * dx, (ijk[:, 1] + 0.5) * dx, (ijk[:, 2] + 0.5) * dx], 1)
+    u = f[FS_GRID_U]
+    want = u[ijk[:, 2], ijk[:, 1], ijk[:, 0]]
+    if axis is not None:
+        pos[:, axis] += 0.5 * dx
+        nb = ijk.copy()
+        nb[:, axis] += 1
+        half = np.float32(0.5)
+        want = want * half + u[nb[:, 2], nb[:, 1], nb[:, 0]] * half
+    s.set_particles(pos.astype(np.float32), np.zeros(pos.shape, np.float32))
+    s.stage("g2p_advect")
+    _, v = s.particles()
+    assert np.array_equal(v[:, 0], want.astype(np.float32))
+
+
 def test_advection_clamps_to_domain(oracle):
     N, dx, dt = 8, 0.125, 0.1
     s = oracle.solver(dx=dx, dt=dt, pic_weight=1.0)
