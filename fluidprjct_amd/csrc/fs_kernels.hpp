@@ -20,20 +20,16 @@
 #define ADV_MARGIN 2
 #endif
 
-// P2G LDS tile (cells); one 256-thread block per tile.
+// P2G LDS tile (cells); one 256-thread block per tile.  (TX + 2) x (TY + 2)
+// halo cells per z-plane, one per thread of the window phase.
 #ifndef P2G_TX
 #define P2G_TX 16
 #endif
 #ifndef P2G_TY
-#define P2G_TY 8
+#define P2G_TY 12
 #endif
 #ifndef P2G_TZ
 #define P2G_TZ 8
-#endif
-// consecutive cells along x whose windows one P2G thread accumulates (1 is
-// fastest on gfx950: 2 and 3 cut the LDS atomics but cost more per thread)
-#ifndef P2G_GROUP
-#define P2G_GROUP 1
 #endif
 
 namespace fsk {

@@ -251,36 +251,37 @@ __global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels
 }
 
 // ---------------------------------------------------------------------------
-// P2G over an LDS tile.  A block owns the faces of a TX x TY x TZ cell tile and
-// visits the cells of the tile plus a one-cell halo (the only cells whose
-// particles reach the owned faces), one thread per cell.  A particle of cell
-// (ci,cj,ck) reaches, per component, a 2x3x3 window of faces around its cell
-// (u: i in {ci,ci+1}, j in {cj-1..cj+1}, k in {ck-1..ck+1}; v and w alike with
-// the 2-wide axis along y or z), so the thread sums the weights and weighted
-// velocities of all its particles into that window in registers (weights and
-// products as AccumulateVelocities :468-594) and then adds the window to the
-// owned faces with one LDS atomic per face.  Each owned face is finally
-// normalised (ComputeVelocityFunc :598-605), saved (saveVelocities) and given
-// gravity (ApplyScalarForce :687-698) in one write.
-// The window of G consecutive cells along x of one row (their particles are
-// one contiguous range of the sorted arrays) for one component: the windows
-// of neighbouring cells overlap along x, so a G-cell window has G+1 (u) or G+2
-// (v, w) x-slots instead of G times 2 or 3, and correspondingly fewer LDS
-// atomics.  Along each axis a particle's base slot s gets 1 - f and slot s+1
-// gets f; a slot beyond the window is a face beyond the domain, which the
-// reference skips (the same rule as the one-cell window above).
-template <int COMP, int G, int TX, int TY, int TZ, bool P2>
-FS_DEV void p2g_group_comp(const P2GArgs& a, int x0, int y0, int z0, int ci0, int cj, int ck, int beg, int end,
-                           float* __restrict__ snum, float* __restrict__ sden) {
-    constexpr int NX = (COMP == 0) ? G + 1 : G + 2, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
-    constexpr int NS = NX * NY * NZ;
+// P2G over an LDS tile.  A block owns the faces of a TX x TY x TZ cell tile;
+// the particles that reach them are those of the tile's cells plus a one-cell
+// halo.  A particle of cell (ci,cj,ck) reaches, per component, a 2x3x3 window
+// of faces around its cell (u: i in {ci,ci+1}, j in {cj-1..cj+1}, k in
+// {ck-1..ck+1}; v and w alike with the 2-wide axis along y or z).  Along each
+// axis a particle's base slot s gets 1 - f and slot s+1 gets f; a slot beyond
+// the window is a face beyond the domain, which the reference skips.
+// One component at a time, the halo is walked plane by plane along z:
+//   A. one thread per halo cell of the plane sums the weights and weighted
+//      velocities of its particles into its window in registers (weights and
+//      products as AccumulateVelocities :468-594) and stores the window in LDS;
+//   B. one thread per owned face in the planes those windows reach adds the
+//      window slots that land on it (at most 3 x 3 cells of the plane).
+// No LDS atomics: every face sum has one writer per phase, so the result is
+// also the same from run to run.  Each owned face is finally normalised
+// (ComputeVelocityFunc :598-605), saved (saveVelocities) and given gravity
+// (ApplyScalarForce :687-698) in one write.
+constexpr int P2G_SLOTS = 18;
+
+// window of one cell for one component: slot x + NX (y + NY z)
+template <int COMP, bool P2>
+FS_DEV void p2g_cell_window(const P2GArgs& a, int ci, int cj, int ck, int beg, int end, float (&num)[P2G_SLOTS],
+                            float (&den)[P2G_SLOTS]) {
+    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
     const int W = a.W, H = a.H, D = a.D;
     const float dx = a.dx, inv = a.inv_dx;
     const float* __restrict__ vel = COMP == 0 ? a.P.vx : (COMP == 1 ? a.P.vy : a.P.vz);
-    const int ox = (COMP == 0) ? ci0 : ci0 - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
-    float num[NS], den[NS];
+    const int ox = (COMP == 0) ? ci : ci - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
 #pragma unroll
-    for (int q = 0; q < NS; ++q) { num[q] = 0.f; den[q] = 0.f; }
+    for (int q = 0; q < P2G_SLOTS; ++q) { num[q] = 0.f; den[q] = 0.f; }
+    if (beg >= end) return;
     float px = a.P.px[beg], py = a.P.py[beg], pz = a.P.pz[beg], vc = vel[beg];
     for (int p = beg; p < end; ++p) {
         const int q = p + 1 < end ? p + 1 : p;
@@ -309,24 +310,80 @@ FS_DEV void p2g_group_comp(const P2GArgs& a, int x0, int y0, int z0, int ci0, in
                 }
         px = npx; py = npy; pz = npz; vc = nvc;
     }
-    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
-    const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
-    const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
-    const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
-#pragma unroll
-    for (int z = 0; z < NZ; ++z)
-#pragma unroll
-        for (int y = 0; y < NY; ++y)
-#pragma unroll
-            for (int x = 0; x < NX; ++x) {
-                const int sl = x + NX * (y + NY * z);
-                const int i = ox + x, j = oy + y, k = oz + z;
-                if (i < x0 || i > hix || j < y0 || j > hiy || k < z0 || k > hiz) continue;
-                if (den[sl] == 0.0f) continue;
-                const int l = (i - x0) + LX * ((j - y0) + LY * (k - z0));
-                atomicAdd(&snum[l], num[sl]);
-                atomicAdd(&sden[l], den[sl]);
+}
+
+// one component: phases A and B over the halo planes, then the write of the
+// owned faces.  swn/swd: windows of one halo plane, slot-major [slot][cell];
+// sfn/sfd: sums of the owned faces of this component.
+template <int COMP, int TX, int TY, int TZ, bool P2>
+FS_DEV void p2g_component(const P2GArgs& a, int x0, int y0, int z0, bool empty, float* __restrict__ swn,
+                          float* __restrict__ swd, float* __restrict__ sfn, float* __restrict__ sfd) {
+    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
+    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1), LZ = TZ + (COMP == 2), NF = LX * LY * LZ;
+    const int W = a.W, H = a.H, D = a.D;
+    // owned faces: [x0, hix] x [y0, hiy] x [z0, hiz]
+    const int hix = (COMP == 0) ? ((x0 + TX >= W) ? W : x0 + TX - 1) : min(x0 + TX, W) - 1;
+    const int hiy = (COMP == 1) ? ((y0 + TY >= H) ? H : y0 + TY - 1) : min(y0 + TY, H) - 1;
+    const int hiz = (COMP == 2) ? ((z0 + TZ >= D) ? D : z0 + TZ - 1) : min(z0 + TZ, D) - 1;
+    for (int l = threadIdx.x; l < NF; l += blockDim.x) { sfn[l] = 0.f; sfd[l] = 0.f; }
+    // no particle in the tile and its halo: every owned face is empty
+    const int kc0 = max(z0 - 1, 0), kc1 = empty ? kc0 - 1 : min(z0 + TZ, D - 1);
+    if (empty) __syncthreads();
+    for (int kc = kc0; kc <= kc1; ++kc) {
+        // ---- A: windows of the halo cells of plane kc
+        for (int t = threadIdx.x; t < HP; t += blockDim.x) {
+            const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
+            int beg = 0, end = 0;
+            if (ci >= 0 && ci < W && cj >= 0 && cj < H) {
+                const long long c = (long long)ci + (long long)cj * W + (long long)kc * W * H;
+                beg = a.start[c];
+                end = a.start[c + 1];
             }
+            float num[P2G_SLOTS], den[P2G_SLOTS];
+            p2g_cell_window<COMP, P2>(a, ci, cj, kc, beg, end, num, den);
+#pragma unroll
+            for (int q = 0; q < NX * NY * NZ; ++q) {
+                swn[q * HP + t] = num[q];
+                swd[q * HP + t] = den[q];
+            }
+        }
+        __syncthreads();
+        // ---- B: owned faces in the planes k = kc - (NZ == 3) .. kc + 1
+        const int k0 = max(kc - (NZ == 3 ? 1 : 0), z0), k1 = min(kc + 1, hiz);
+        const int nk = k1 - k0 + 1;
+        const int nfp = LX * LY;
+        for (int f = threadIdx.x; f < nk * nfp; f += blockDim.x) {
+            const int li = f % LX, lj = (f / LX) % LY, k = k0 + f / nfp;
+            const int i = x0 + li, j = y0 + lj;
+            if (i > hix || j > hiy) continue;
+            const int z = k - (kc - (NZ == 3 ? 1 : 0));
+            float sn = 0.0f, sd = 0.0f;
+            // cells ci with window x-slot x = i - ox(ci), ox = ci - (NX == 3)
+#pragma unroll
+            for (int y = 0; y < NY; ++y) {
+                const int cj = j - y + (NY == 3 ? 1 : 0);
+                const int hy = cj - (y0 - 1);
+                if (hy < 0 || hy >= HY) continue;
+#pragma unroll
+                for (int x = 0; x < NX; ++x) {
+                    const int ci = i - x + (NX == 3 ? 1 : 0);
+                    const int hx = ci - (x0 - 1);
+                    if (hx < 0 || hx >= HX) continue;
+                    const int q = x + NX * (y + NY * z);
+                    const int cell = hx + HX * hy;
+                    sn += swn[q * HP + cell];
+                    sd += swd[q * HP + cell];
+                }
+            }
+            const int l = li + LX * (lj + LY * (k - z0));
+            sfn[l] += sn;
+            sfd[l] += sd;
+        }
+        __syncthreads();
+    }
+    p2g_write_comp<COMP, TX, TY, TZ>(a, x0, y0, z0, sfn, sfd);
+    __syncthreads();
 }
 
 template <int COMP, int TX, int TY, int TZ>
@@ -355,43 +412,33 @@ FS_DEV void p2g_write_comp(const P2GArgs& a, int x0, int y0, int z0, const float
 
 template <int TX, int TY, int TZ, bool P2>
 __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
-    constexpr int NU = (TX + 1) * TY * TZ, NV = TX * (TY + 1) * TZ, NW = TX * TY * (TZ + 1);
-    __shared__ float s_un[NU], s_ud[NU], s_vn[NV], s_vd[NV], s_wn[NW], s_wd[NW];
+    constexpr int HP = (TX + 2) * (TY + 2);
+    constexpr int NFU = (TX + 1) * TY * TZ, NFV = TX * (TY + 1) * TZ, NFW = TX * TY * (TZ + 1);
+    constexpr int NF = NFU > NFV ? (NFU > NFW ? NFU : NFW) : (NFV > NFW ? NFV : NFW);
+    __shared__ float swn[P2G_SLOTS * HP], swd[P2G_SLOTS * HP], sfn[NF], sfd[NF];
 
-    const int W = a.W, H = a.H, D = a.D;
+    const int W = a.W, H = a.H;
     const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-
-    for (int l = threadIdx.x; l < NU; l += blockDim.x) { s_un[l] = 0.f; s_ud[l] = 0.f; }
-    for (int l = threadIdx.x; l < NV; l += blockDim.x) { s_vn[l] = 0.f; s_vd[l] = 0.f; }
-    for (int l = threadIdx.x; l < NW; l += blockDim.x) { s_wn[l] = 0.f; s_wd[l] = 0.f; }
-    __syncthreads();
-
-    const int hx0 = max(x0 - 1, 0), hx1 = min(x0 + TX, W - 1);
-    const int hy0 = max(y0 - 1, 0), hy1 = min(y0 + TY, H - 1);
-    const int hz0 = max(z0 - 1, 0), hz1 = min(z0 + TZ, D - 1);
-    const int nxh = hx1 - hx0 + 1, nyh = hy1 - hy0 + 1, nzh = hz1 - hz0 + 1;
-    // one thread per group of P2G_GROUP consecutive halo cells along x (the
-    // last group of a row may be shorter), one velocity component at a time
-    constexpr int G = P2G_GROUP;
-    const int ngx = (nxh + G - 1) / G;
-    const int ngroup = ngx * nyh * nzh;
-    for (int gl = threadIdx.x; gl < ngroup; gl += blockDim.x) {
-        const int gx = gl % ngx, cj = hy0 + (gl / ngx) % nyh, ck = hz0 + gl / (ngx * nyh);
-        const int ci0 = hx0 + gx * G, ci1 = min(ci0 + G, hx1 + 1);
-        const long long c = (long long)ci0 + (long long)cj * W + (long long)ck * W * H;
-        const int beg = a.start[c], end = a.start[c + (ci1 - ci0)];
-        if (beg == end) continue;
-        p2g_group_comp<0, G, TX, TY, TZ, P2>(a, x0, y0, z0, ci0, cj, ck, beg, end, s_un, s_ud);
-        p2g_group_comp<1, G, TX, TY, TZ, P2>(a, x0, y0, z0, ci0, cj, ck, beg, end, s_vn, s_vd);
-        p2g_group_comp<2, G, TX, TY, TZ, P2>(a, x0, y0, z0, ci0, cj, ck, beg, end, s_wn, s_wd);
+    // particles in the halo box: one thread per halo row (cj, ck)
+    bool any = false;
+    {
+        constexpr int HY = TY + 2, HZ = TZ + 2;
+        static_assert(HY * HZ <= 256, "one thread per halo row");
+        const int t = threadIdx.x;
+        const int cj = y0 - 1 + t % HY, ck = z0 - 1 + t / HY;
+        if (t < HY * HZ && cj >= 0 && cj < H && ck >= 0 && ck < a.D) {
+            const int lo = max(x0 - 1, 0), hi = min(x0 + TX, W - 1);
+            const long long row = (long long)cj * W + (long long)ck * W * H;
+            any = a.start[row + hi + 1] > a.start[row + lo];
+        }
     }
-    __syncthreads();
-    p2g_write_comp<0, TX, TY, TZ>(a, x0, y0, z0, s_un, s_ud);
-    p2g_write_comp<1, TX, TY, TZ>(a, x0, y0, z0, s_vn, s_vd);
-    p2g_write_comp<2, TX, TY, TZ>(a, x0, y0, z0, s_wn, s_wd);
+    const bool empty = !__syncthreads_or(any);
+    p2g_component<0, TX, TY, TZ, P2>(a, x0, y0, z0, empty, swn, swd, sfn, sfd);
+    p2g_component<1, TX, TY, TZ, P2>(a, x0, y0, z0, empty, swn, swd, sfn, sfd);
+    p2g_component<2, TX, TY, TZ, P2>(a, x0, y0, z0, empty, swn, swd, sfn, sfd);
 }
 
 // ---------------------------------------------------------------------------
