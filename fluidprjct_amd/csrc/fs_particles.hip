@@ -313,13 +313,16 @@ FS_DEV void p2g_cell_window(const P2GArgs& a, int ci, int cj, int ck, int beg, i
 }
 
 // one component: phases A and B over the halo planes, then the write of the
-// owned faces.  swn/swd: windows of one halo plane, slot-major [slot][cell];
-// sfn/sfd: sums of the owned faces of this component.
+// owned faces.  A window is handed over one z-slice at a time (the slots that
+// land on one face plane), so swn/swd hold [slot in slice][cell] for one
+// slice of one halo plane; sfn/sfd: sums of the owned faces of this component.
 template <int COMP, int TX, int TY, int TZ, bool P2>
 FS_DEV void p2g_component(const P2GArgs& a, int x0, int y0, int z0, bool empty, float* __restrict__ swn,
                           float* __restrict__ swd, float* __restrict__ sfn, float* __restrict__ sfd) {
     constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
+    constexpr int NXY = NX * NY;
     constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    static_assert(HP <= 256, "one thread per halo cell of a plane");
     constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1), LZ = TZ + (COMP == 2), NF = LX * LY * LZ;
     const int W = a.W, H = a.H, D = a.D;
     // owned faces: [x0, hix] x [y0, hiy] x [z0, hiz]
@@ -330,57 +333,58 @@ FS_DEV void p2g_component(const P2GArgs& a, int x0, int y0, int z0, bool empty, 
     // no particle in the tile and its halo: every owned face is empty
     const int kc0 = max(z0 - 1, 0), kc1 = empty ? kc0 - 1 : min(z0 + TZ, D - 1);
     if (empty) __syncthreads();
+    const int t = threadIdx.x;
+    const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
+    const bool cell_in = t < HP && ci >= 0 && ci < W && cj >= 0 && cj < H;
     for (int kc = kc0; kc <= kc1; ++kc) {
-        // ---- A: windows of the halo cells of plane kc
-        for (int t = threadIdx.x; t < HP; t += blockDim.x) {
-            const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
-            int beg = 0, end = 0;
-            if (ci >= 0 && ci < W && cj >= 0 && cj < H) {
-                const long long c = (long long)ci + (long long)cj * W + (long long)kc * W * H;
-                beg = a.start[c];
-                end = a.start[c + 1];
-            }
-            float num[P2G_SLOTS], den[P2G_SLOTS];
-            p2g_cell_window<COMP, P2>(a, ci, cj, kc, beg, end, num, den);
-#pragma unroll
-            for (int q = 0; q < NX * NY * NZ; ++q) {
-                swn[q * HP + t] = num[q];
-                swd[q * HP + t] = den[q];
-            }
+        // ---- A: the window of this thread's halo cell of plane kc
+        int beg = 0, end = 0;
+        if (cell_in) {
+            const long long c = (long long)ci + (long long)cj * W + (long long)kc * W * H;
+            beg = a.start[c];
+            end = a.start[c + 1];
         }
-        __syncthreads();
-        // ---- B: owned faces in the planes k = kc - (NZ == 3) .. kc + 1
-        const int k0 = max(kc - (NZ == 3 ? 1 : 0), z0), k1 = min(kc + 1, hiz);
-        const int nk = k1 - k0 + 1;
-        const int nfp = LX * LY;
-        for (int f = threadIdx.x; f < nk * nfp; f += blockDim.x) {
-            const int li = f % LX, lj = (f / LX) % LY, k = k0 + f / nfp;
-            const int i = x0 + li, j = y0 + lj;
-            if (i > hix || j > hiy) continue;
-            const int z = k - (kc - (NZ == 3 ? 1 : 0));
-            float sn = 0.0f, sd = 0.0f;
-            // cells ci with window x-slot x = i - ox(ci), ox = ci - (NX == 3)
+        float num[P2G_SLOTS], den[P2G_SLOTS];
+        p2g_cell_window<COMP, P2>(a, ci, cj, kc, beg, end, num, den);
+        // ---- B, one z-slice z of the windows at a time: face plane k = kc - (NZ == 3) + z
 #pragma unroll
-            for (int y = 0; y < NY; ++y) {
-                const int cj = j - y + (NY == 3 ? 1 : 0);
-                const int hy = cj - (y0 - 1);
-                if (hy < 0 || hy >= HY) continue;
+        for (int z = 0; z < NZ; ++z) {
+            if (t < HP) {
 #pragma unroll
-                for (int x = 0; x < NX; ++x) {
-                    const int ci = i - x + (NX == 3 ? 1 : 0);
-                    const int hx = ci - (x0 - 1);
-                    if (hx < 0 || hx >= HX) continue;
-                    const int q = x + NX * (y + NY * z);
-                    const int cell = hx + HX * hy;
-                    sn += swn[q * HP + cell];
-                    sd += swd[q * HP + cell];
+                for (int q = 0; q < NXY; ++q) {
+                    swn[q * HP + t] = num[q + NXY * z];
+                    swd[q * HP + t] = den[q + NXY * z];
                 }
             }
-            const int l = li + LX * (lj + LY * (k - z0));
-            sfn[l] += sn;
-            sfd[l] += sd;
+            __syncthreads();
+            const int k = kc - (NZ == 3 ? 1 : 0) + z;
+            if (k >= z0 && k <= hiz) {
+                for (int f = t; f < LX * LY; f += blockDim.x) {
+                    const int li = f % LX, lj = f / LX;
+                    const int i = x0 + li, j = y0 + lj;
+                    if (i > hix || j > hiy) continue;
+                    float sn = 0.0f, sd = 0.0f;
+                    // cells with window slot x = i - ox(ci), ox = ci - (NX == 3); y alike
+#pragma unroll
+                    for (int y = 0; y < NY; ++y) {
+                        const int hy = j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
+                        if (hy < 0 || hy >= HY) continue;
+#pragma unroll
+                        for (int x = 0; x < NX; ++x) {
+                            const int hx = i - x + (NX == 3 ? 1 : 0) - (x0 - 1);
+                            if (hx < 0 || hx >= HX) continue;
+                            const int q = x + NX * y, cell = hx + HX * hy;
+                            sn += swn[q * HP + cell];
+                            sd += swd[q * HP + cell];
+                        }
+                    }
+                    const int l = li + LX * (lj + LY * (k - z0));
+                    sfn[l] += sn;
+                    sfd[l] += sd;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     p2g_write_comp<COMP, TX, TY, TZ>(a, x0, y0, z0, sfn, sfd);
     __syncthreads();
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     constexpr int HP = (TX + 2) * (TY + 2);
     constexpr int NFU = (TX + 1) * TY * TZ, NFV = TX * (TY + 1) * TZ, NFW = TX * TY * (TZ + 1);
     constexpr int NF = NFU > NFV ? (NFU > NFW ? NFU : NFW) : (NFV > NFW ? NFV : NFW);
-    __shared__ float swn[P2G_SLOTS * HP], swd[P2G_SLOTS * HP], sfn[NF], sfd[NF];
+    __shared__ float swn[9 * HP], swd[9 * HP], sfn[NF], sfd[NF];   // 9: largest z-slice of a window
 
     const int W = a.W, H = a.H;
     const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
