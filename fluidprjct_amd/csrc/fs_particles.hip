@@ -537,6 +537,14 @@ struct TileFields {
     const float* sw;
     const uint8_t* sl;
     const G2PArgs* a;
+    // fast-path window of r = x / dx per axis, [lo, hi): see interp()
+    float lox, hix, loy, hiy, loz, hiz;
+
+    FS_DEV void set_window() {
+        lox = (float)(max(X0, 0) + 1); hix = (float)(min(X0 + BX, a->W) - 1);
+        loy = (float)(max(Y0, 0) + 1); hiy = (float)(min(Y0 + BY, a->H) - 1);
+        loz = (float)(max(Z0, 0) + 1); hiz = (float)(min(Z0 + BZ, a->D) - 1);
+    }
 
     template <int COMP>
     FS_DEV bool in_box(const Base& b, int& i0) const {
@@ -599,26 +607,29 @@ struct TileFields {
     FS_DEV f3 interp(f3 pos) const {
         const float dx = a->dx, inv = a->inv_dx;
         float rx = div_dx<P2>(pos.x, dx, inv), ry = div_dx<P2>(pos.y, dx, inv), rz = div_dx<P2>(pos.z, dx, inv);
-        // Fast path: no base index of any component needs clamping and all 24
-        // corners lie in the staged box.  Then mac_base's clamps are identities
-        // (r - floor(r) is exact and in [0, 1)), and the bases follow from the
-        // two floors per axis, floor(r) and floor(r - 1/2).
-        if (isfinite(rx) && isfinite(ry) && isfinite(rz)) {
+        // Fast path: floor(r) in [lo0 + 1, hi - 2] on every axis, lo0 = max(X0, 0)
+        // and hi = min(X0 + BX, W).  Then both bases per axis, floor(r) and
+        // floor(r - 1/2) (which is floor(r) or floor(r) - 1), lie in the staged
+        // box and the domain with their +1 corners, mac_base's clamps are
+        // identities, r >= 1 makes truncation equal floor, and the fractions
+        // r - floor(r) are exact (so the hardware fract gives the same bits).
+        // NaN or infinite r fails the comparisons.  Other cases take the
+        // general path below, which returns the same values.
+        if (rx >= lox && rx < hix && ry >= loy && ry < hiy && rz >= loz && rz < hiz) {
             const float hx = rx - 0.5f, hy = ry - 0.5f, hz = rz - 0.5f;
-            const int fx = (int)floorf(rx), fy = (int)floorf(ry), fz = (int)floorf(rz);
-            const int gx = (int)floorf(hx), gy = (int)floorf(hy), gz = (int)floorf(hz);
-            const int hix = min(X0 + BX, a->W), hiy = min(Y0 + BY, a->H), hiz = min(Z0 + BZ, a->D);
-            const bool ok = gx >= max(X0, 0) && gy >= max(Y0, 0) && gz >= max(Z0, 0) && fx + 1 <= hix &&
-                            gx + 2 <= hix && fy + 1 <= hiy && gy + 2 <= hiy && fz + 1 <= hiz && gz + 2 <= hiz;
-            if (ok) {
-                Base bu{fx, gy, gz, rx - fx, hy - gy, hz - gz};
-                Base bv{gx, fy, gz, hx - gx, ry - fy, hz - gz};
-                Base bw{gx, gy, fz, hx - gx, hy - gy, rz - fz};
-                const int iu = (fx - X0) + (BX + 1) * ((gy - Y0) + BY * (gz - Z0));
-                const int iv = (gx - X0) + BX * ((fy - Y0) + (BY + 1) * (gz - Z0));
-                const int iw = (gx - X0) + BX * ((gy - Y0) + BY * (fz - Z0));
-                return mk(lds_sample<0>(bu, iu), lds_sample<1>(bv, iv), lds_sample<2>(bw, iw));
-            }
+            const int fx = (int)rx, fy = (int)ry, fz = (int)rz;
+            const int gx = (int)hx, gy = (int)hy, gz = (int)hz;
+            const float frx = __builtin_amdgcn_fractf(rx), fry = __builtin_amdgcn_fractf(ry),
+                        frz = __builtin_amdgcn_fractf(rz);
+            const float fhx = __builtin_amdgcn_fractf(hx), fhy = __builtin_amdgcn_fractf(hy),
+                        fhz = __builtin_amdgcn_fractf(hz);
+            Base bu{fx, gy, gz, frx, fhy, fhz};
+            Base bv{gx, fy, gz, fhx, fry, fhz};
+            Base bw{gx, gy, fz, fhx, fhy, frz};
+            const int iu = (fx - X0) + (BX + 1) * ((gy - Y0) + BY * (gz - Z0));
+            const int iv = (gx - X0) + BX * ((fy - Y0) + (BY + 1) * (gz - Z0));
+            const int iw = (gx - X0) + BX * ((gy - Y0) + BY * (fz - Z0));
+            return mk(lds_sample<0>(bu, iu), lds_sample<1>(bv, iv), lds_sample<2>(bw, iw));
         }
         Base bu = mac_base<0>(rx, ry, rz, a->W, a->H, a->D);
         Base bv = mac_base<1>(rx, ry, rz, a->W, a->H, a->D);
@@ -696,6 +707,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
     __syncthreads();
 
     TileFields<BX, BY, BZ, P2> F{X0, Y0, Z0, su, sv, sw, sl, &a};
+    F.set_window();
     const float dx = a.dx, dt = a.dt, C = a.C, al = a.alpha, inv = a.inv_dx;
     const float vfloor = sqrtf(5.0f * dx * 9.8f);
     const float xmax = (W - 1) * dx, ymax = (H - 1) * dx, zmax = (D - 1) * dx;

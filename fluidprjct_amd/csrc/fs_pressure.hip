@@ -83,6 +83,39 @@ FS_DEV double sum_partials(const double* __restrict__ p, int n, double* sm) {
     return t;
 }
 
+// Sums of two or three partial arrays, each in sum_partials' order (so the
+// results are the same bits), with the loads of all of them in flight at
+// once.  sm holds at least 3 * 4 doubles.
+template <int K>
+FS_DEV void sum_partials_k(const double* const (&p)[K], int n, double* sm, double (&out)[K]) {
+    double a[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) a[q] = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double v[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = p[q][i];
+#pragma unroll
+        for (int q = 0; q < K; ++q) a[q] += v[q];
+    }
+    const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+#pragma unroll
+    for (int q = 0; q < K; ++q) a[q] = wave_sum(a[q]);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) sm[q * 4 + w] = a[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        double t = 0.0;
+        for (int r = 0; r < nw; ++r) t += sm[q * 4 + r];
+        out[q] = t;
+    }
+    __syncthreads();
+}
+
 FS_DEV void store_partial(double v, double* sm, double* __restrict__ partial) {
     double t = block_sum(v, sm);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
@@ -236,7 +269,7 @@ __global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restri
                                                 float* __restrict__ xout, float omega, double* __restrict__ partial,
                                                 const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
@@ -258,7 +291,7 @@ __global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restri
 __global__ __launch_bounds__(256) void k_diag(MGLevel L, const float* __restrict__ r, float* __restrict__ z,
                                               double* __restrict__ partial, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     double acc = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
@@ -402,7 +435,7 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, c
                                                         double* __restrict__ partial,
                                                         const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     const long long WH = (long long)F.W * F.H;
     double acc = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < F.n;
@@ -586,7 +619,7 @@ __global__ __launch_bounds__(256) void k_jacobi_v4(MGLevel L, const float* __res
                                                    float* __restrict__ xout, float omega, double* __restrict__ partial,
                                                    const SolveScalars* __restrict__ sc) {
     if (sc && sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
     for (long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x); c0 < L.n;
@@ -745,17 +778,19 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv_v4(MGLevel L, const float*
                                                          double* __restrict__ prp,
                                                          const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     const bool first = sc->first != 0;
     float beta = 0.0f;
     if (!first) {
         if (zq) {
-            double a = sum_partials(zq, np, sm);
-            double bsum = sum_partials(pq_old, np, sm);
+            double r2[2];
+            sum_partials_k<2>({zq, pq_old}, np, sm, r2);
+            const double a = r2[0], bsum = r2[1];
             beta = bsum > 0.0 ? (float)(-a / bsum) : 0.0f;
         } else {
-            double bn = sum_partials(rz_new, np, sm);
-            double bo = sum_partials(rz_old, np, sm);
+            double r2[2];
+            sum_partials_k<2>({rz_new, rz_old}, np, sm, r2);
+            const double bn = r2[0], bo = r2[1];
             beta = (float)(bn / bo);
         }
     }
@@ -800,9 +835,10 @@ __global__ __launch_bounds__(256) void k_update_v4(long long n, const float* __r
                                                    const double* __restrict__ rz, const double* __restrict__ pq, int np,
                                                    double* __restrict__ prr, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
-    double srz = sum_partials(rz, np, sm);
-    double spq = sum_partials(pq, np, sm);
+    __shared__ double sm[12];
+    double r2[2];
+    sum_partials_k<2>({rz, pq}, np, sm, r2);
+    const double srz = r2[0], spq = r2[1];
     const float alpha = (spq > 0.0) ? (float)(srz / spq) : 0.0f;
     double acc = 0.0;
     for (long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x); c0 < n;
@@ -1138,17 +1174,19 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
                                                       double* __restrict__ ppq, const float* __restrict__ r,
                                                       double* __restrict__ prp, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     const bool first = sc->first != 0;
     float beta = 0.0f;
     if (!first) {
         if (zq) {
-            double a = sum_partials(zq, np, sm);
-            double b = sum_partials(pq_old, np, sm);
+            double r2[2];
+            sum_partials_k<2>({zq, pq_old}, np, sm, r2);
+            const double a = r2[0], b = r2[1];
             beta = b > 0.0 ? (float)(-a / b) : 0.0f;
         } else {
-            double bn = sum_partials(rz_new, np, sm);
-            double bo = sum_partials(rz_old, np, sm);
+            double r2[2];
+            sum_partials_k<2>({rz_new, rz_old}, np, sm, r2);
+            const double bn = r2[0], bo = r2[1];
             beta = (float)(bn / bo);
         }
     }
@@ -1189,9 +1227,10 @@ __global__ __launch_bounds__(256) void k_update(long long n, const float* __rest
                                                 const double* __restrict__ rz, const double* __restrict__ pq, int np,
                                                 double* __restrict__ prr, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
-    double srz = sum_partials(rz, np, sm);
-    double spq = sum_partials(pq, np, sm);
+    __shared__ double sm[12];
+    double r2[2];
+    sum_partials_k<2>({rz, pq}, np, sm, r2);
+    const double srz = r2[0], spq = r2[1];
     const float alpha = (spq > 0.0) ? (float)(srz / spq) : 0.0f;
     double acc = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n;
@@ -1212,10 +1251,10 @@ __global__ __launch_bounds__(256) void k_fin_iter(const double* __restrict__ rz,
                                                   const double* __restrict__ rr, int np, int flex, int stall_window,
                                                   SolveScalars* sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
-    double srz = sum_partials(rz, np, sm);
-    double spq = sum_partials(pq, np, sm);
-    double srr = sum_partials(rr, np, sm);
+    __shared__ double sm[12];
+    double r3[3];
+    sum_partials_k<3>({rz, pq, rr}, np, sm, r3);
+    const double srz = r3[0], spq = r3[1], srr = r3[2];
     if (threadIdx.x != 0) return;
     sc->rz = srz;
     sc->rr = srr;
@@ -1240,7 +1279,7 @@ __global__ __launch_bounds__(256) void k_fin_iter(const double* __restrict__ rz,
 __global__ __launch_bounds__(256) void k_dot_rz(long long n, const float* __restrict__ r, const float* __restrict__ z,
                                                 double* __restrict__ partial, const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     double acc = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
         acc += (double)r[c] * (double)z[c];
@@ -1253,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_dot_rz_zq(long long n, const float* __r
                                                    double* __restrict__ prz, double* __restrict__ pzq,
                                                    const SolveScalars* __restrict__ sc) {
     if (sc->done) return;
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     double a = 0.0, b = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         const double zc = z[c];
@@ -1267,7 +1306,7 @@ __global__ __launch_bounds__(256) void k_dot_rz_zq(long long n, const float* __r
 // sums of b^2 and b over unknowns
 __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restrict__ b, double* __restrict__ p2,
                                                double* __restrict__ p1) {
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     double a2 = 0.0, a1 = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
          c += (long long)gridDim.x * blockDim.x) {
@@ -1284,7 +1323,7 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
 // r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partials of r.r (FP64)
 __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, const double* __restrict__ x64,
                                                  double shift, float* __restrict__ r, double* __restrict__ partial) {
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
     for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
@@ -1310,7 +1349,7 @@ __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__
 __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __restrict__ p1,
                                                   const double* __restrict__ p2, int np, SolveScalars* sc,
                                                   double tol2b) {
-    __shared__ double sm[4];
+    __shared__ double sm[12];
     double s1 = sum_partials(p1, np, sm);
     double s2 = p2 ? sum_partials(p2, np, sm) : 0.0;
     if (threadIdx.x != 0) return;
