@@ -449,7 +449,12 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
 // G2P + advection, one thread per particle.
 
 // PIC and FLIP samples of one component: du = u - uSaved per corner (:990-1012)
-template <int COMP>
+// GLOBAL_F: read f through a global-address-space pointer, so that the
+// compiler cannot merge these reads with the LDS reads of the same corners on
+// the staged path into generic (flat) loads that serve both address spaces
+typedef __attribute__((address_space(1))) const float gfloat_t;
+typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
+template <int COMP, bool GLOBAL_F = false>
 FS_DEV void sample_pic_flip(const float* __restrict__ f, const float* __restrict__ fs, int n, const Base& b, int sx,
                             int sxy, float& pic, float& flip) {
     int i0 = b.i + b.j * sx + b.k * sxy;
@@ -458,8 +463,8 @@ FS_DEV void sample_pic_flip(const float* __restrict__ f, const float* __restrict
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         int id = ix[q] < 0 ? 0 : (ix[q] >= n ? n - 1 : ix[q]);
-        c[q] = f[id];
-        d[q] = f[id] - fs[id];
+        c[q] = GLOBAL_F ? *(gfloat_t*)(f + id) : f[id];
+        d[q] = c[q] - fs[id];
     }
     pic = trilin(b.fx, b.fy, b.fz, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7]);
     flip = trilin(b.fx, b.fy, b.fz, d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7]);
@@ -601,7 +606,7 @@ struct TileFields {
             return;
         }
         const float* f = COMP == 0 ? a->u : (COMP == 1 ? a->v : a->w);
-        sample_pic_flip<COMP>(f, fs, n, b, sx, sxy, pic, flip);
+        sample_pic_flip<COMP, true>(f, fs, n, b, sx, sxy, pic, flip);
     }
 
     FS_DEV f3 interp(f3 pos) const {
@@ -641,11 +646,15 @@ struct TileFields {
         return mk(global_sample<0>(bu), global_sample<1>(bv), global_sample<2>(bw));
     }
 
-    // label of an in-domain cell
+    // label of an in-domain cell: an LDS read (index clamped to the box), and a
+    // global read only for a cell outside the staged box, so that the common
+    // case is a plain LDS load rather than one generic load for both spaces
     FS_DEV uint8_t label(int cx, int cy, int cz) const {
-        int li = cx - X0, lj = cy - Y0, lk = cz - Z0;
-        if (li >= 0 && li < BX && lj >= 0 && lj < BY && lk >= 0 && lk < BZ) return sl[li + BX * (lj + BY * lk)];
-        return a->labels[(size_t)cx + (size_t)cy * a->W + (size_t)cz * a->W * a->H];
+        const int li = cx - X0, lj = cy - Y0, lk = cz - Z0;
+        const bool in = (unsigned)li < (unsigned)BX && (unsigned)lj < (unsigned)BY && (unsigned)lk < (unsigned)BZ;
+        uint8_t L = sl[in ? li + BX * (lj + BY * lk) : 0];
+        if (!in) L = *(gbyte_t*)(a->labels + (size_t)cx + (size_t)cy * a->W + (size_t)cz * a->W * a->H);
+        return L;
     }
 };
 
