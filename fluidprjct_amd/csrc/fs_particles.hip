@@ -312,120 +312,116 @@ FS_DEV void p2g_cell_window(const P2GArgs& a, int ci, int cj, int ck, int beg, i
     }
 }
 
-// one component: phases A and B over the halo planes, then the write of the
-// owned faces.  A window is handed over one z-slice at a time (the slots that
-// land on one face plane), so swn/swd hold [slot in slice][cell] for one
-// slice of one halo plane; sfn/sfd: sums of the owned faces of this component.
+// One component on one halo plane kc: phases A and B.  A window is handed
+// over one z-slice at a time (the slots that land on one face plane), so
+// swn/swd hold [slot in slice][cell] for one slice of the plane; sfn/sfd are
+// the sums of this component's owned faces.
 template <int COMP, int TX, int TY, int TZ, bool P2>
-FS_DEV void p2g_component(const P2GArgs& a, int x0, int y0, int z0, bool empty, float* __restrict__ swn,
-                          float* __restrict__ swd, float* __restrict__ sfn, float* __restrict__ sfd) {
+FS_DEV void p2g_plane(const P2GArgs& a, int x0, int y0, int z0, int kc, int beg, int end, float* __restrict__ swn,
+                      float* __restrict__ swd, float* __restrict__ sfn, float* __restrict__ sfd) {
     constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
     constexpr int NXY = NX * NY;
     constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
     static_assert(HP <= 256, "one thread per halo cell of a plane");
-    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1), LZ = TZ + (COMP == 2), NF = LX * LY * LZ;
+    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
     const int W = a.W, H = a.H, D = a.D;
     // owned faces: [x0, hix] x [y0, hiy] x [z0, hiz]
     const int hix = (COMP == 0) ? ((x0 + TX >= W) ? W : x0 + TX - 1) : min(x0 + TX, W) - 1;
     const int hiy = (COMP == 1) ? ((y0 + TY >= H) ? H : y0 + TY - 1) : min(y0 + TY, H) - 1;
     const int hiz = (COMP == 2) ? ((z0 + TZ >= D) ? D : z0 + TZ - 1) : min(z0 + TZ, D) - 1;
-    for (int l = threadIdx.x; l < NF; l += blockDim.x) { sfn[l] = 0.f; sfd[l] = 0.f; }
-    // no particle in the tile and its halo: every owned face is empty
-    const int kc0 = max(z0 - 1, 0), kc1 = empty ? kc0 - 1 : min(z0 + TZ, D - 1);
-    if (empty) __syncthreads();
     const int t = threadIdx.x;
     const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
-    const bool cell_in = t < HP && ci >= 0 && ci < W && cj >= 0 && cj < H;
-    for (int kc = kc0; kc <= kc1; ++kc) {
-        // ---- A: the window of this thread's halo cell of plane kc
-        int beg = 0, end = 0;
-        if (cell_in) {
-            const long long c = (long long)ci + (long long)cj * W + (long long)kc * W * H;
-            beg = a.start[c];
-            end = a.start[c + 1];
-        }
-        float num[P2G_SLOTS], den[P2G_SLOTS];
-        p2g_cell_window<COMP, P2>(a, ci, cj, kc, beg, end, num, den);
-        // ---- B, one z-slice z of the windows at a time: face plane k = kc - (NZ == 3) + z
+    // ---- A: the window of this thread's halo cell of plane kc
+    float num[P2G_SLOTS], den[P2G_SLOTS];
+    p2g_cell_window<COMP, P2>(a, ci, cj, kc, beg, end, num, den);
+    // ---- B, one z-slice z of the windows at a time: face plane k = kc - (NZ == 3) + z
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            if (t < HP) {
+    for (int z = 0; z < NZ; ++z) {
+        if (t < HP) {
 #pragma unroll
-                for (int q = 0; q < NXY; ++q) {
-                    swn[q * HP + t] = num[q + NXY * z];
-                    swd[q * HP + t] = den[q + NXY * z];
-                }
+            for (int q = 0; q < NXY; ++q) {
+                swn[q * HP + t] = num[q + NXY * z];
+                swd[q * HP + t] = den[q + NXY * z];
             }
-            __syncthreads();
-            const int k = kc - (NZ == 3 ? 1 : 0) + z;
-            if (k >= z0 && k <= hiz) {
-                for (int f = t; f < LX * LY; f += blockDim.x) {
-                    const int li = f % LX, lj = f / LX;
-                    const int i = x0 + li, j = y0 + lj;
-                    if (i > hix || j > hiy) continue;
-                    float sn = 0.0f, sd = 0.0f;
-                    // cells with window slot x = i - ox(ci), ox = ci - (NX == 3); y alike
+        }
+        __syncthreads();
+        const int k = kc - (NZ == 3 ? 1 : 0) + z;
+        if (k >= z0 && k <= hiz) {
+            for (int f = t; f < LX * LY; f += blockDim.x) {
+                const int li = f % LX, lj = f / LX;
+                const int i = x0 + li, j = y0 + lj;
+                if (i > hix || j > hiy) continue;
+                float sn = 0.0f, sd = 0.0f;
+                // cells with window slot x = i - ox(ci), ox = ci - (NX == 3); y alike
 #pragma unroll
-                    for (int y = 0; y < NY; ++y) {
-                        const int hy = j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
-                        if (hy < 0 || hy >= HY) continue;
+                for (int y = 0; y < NY; ++y) {
+                    const int hy = j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
+                    if (hy < 0 || hy >= HY) continue;
 #pragma unroll
-                        for (int x = 0; x < NX; ++x) {
-                            const int hx = i - x + (NX == 3 ? 1 : 0) - (x0 - 1);
-                            if (hx < 0 || hx >= HX) continue;
-                            const int q = x + NX * y, cell = hx + HX * hy;
-                            sn += swn[q * HP + cell];
-                            sd += swd[q * HP + cell];
-                        }
+                    for (int x = 0; x < NX; ++x) {
+                        const int hx = i - x + (NX == 3 ? 1 : 0) - (x0 - 1);
+                        if (hx < 0 || hx >= HX) continue;
+                        const int q = x + NX * y, cell = hx + HX * hy;
+                        sn += swn[q * HP + cell];
+                        sd += swd[q * HP + cell];
                     }
-                    const int l = li + LX * (lj + LY * (k - z0));
-                    sfn[l] += sn;
-                    sfd[l] += sd;
                 }
+                const int l = li + LX * (lj + LY * (k % 3));
+                sfn[l] += sn;
+                sfd[l] += sd;
             }
-            __syncthreads();
         }
+        __syncthreads();
     }
-    p2g_write_comp<COMP, TX, TY, TZ>(a, x0, y0, z0, sfn, sfd);
-    __syncthreads();
 }
 
+// Normalise, save and add gravity to the owned faces of face plane k of one
+// component (sums in ring slot k % 3), then clear the slot for plane k + 3.
 template <int COMP, int TX, int TY, int TZ>
-FS_DEV void p2g_write_comp(const P2GArgs& a, int x0, int y0, int z0, const float* __restrict__ snum,
-                           const float* __restrict__ sden) {
-    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1), LZ = TZ + (COMP == 2);
+FS_DEV void p2g_write_plane(const P2GArgs& a, int x0, int y0, int k, float* __restrict__ sfn,
+                            float* __restrict__ sfd) {
+    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
     const int nx = a.W + (COMP == 0), ny = a.H + (COMP == 1);
     const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
     const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
-    const int hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
     float* __restrict__ g = COMP == 0 ? a.u : COMP == 1 ? a.v : a.w;
     float* __restrict__ gs = COMP == 0 ? a.us : COMP == 1 ? a.vs : a.ws;
     const float acc = COMP == 1 ? a.gravity_y : 0.0f;
     const float vel_unknown = -2147483648.0f;  // (float)INT_MIN, utility.cuh:56
-    for (int l = threadIdx.x; l < LX * LY * LZ; l += blockDim.x) {
-        int li = l % LX, lj = (l / LX) % LY, lk = l / (LX * LY);
-        int i = x0 + li, j = y0 + lj, k = z0 + lk;
-        if (i > hix || j > hiy || k > hiz) continue;
-        float num = snum[l], den = sden[l];
-        float val = (den > 1e-8f) ? num / den : 0.0f;
-        size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
-        gs[idx] = val;
-        g[idx] = (val > vel_unknown) ? val + a.dt * acc : val;
+    for (int f = threadIdx.x; f < LX * LY; f += blockDim.x) {
+        const int li = f % LX, lj = f / LX;
+        const int l = li + LX * (lj + LY * (k % 3));
+        const int i = x0 + li, j = y0 + lj;
+        if (i <= hix && j <= hiy) {
+            const float num = sfn[l], den = sfd[l];
+            const float val = (den > 1e-8f) ? num / den : 0.0f;
+            const size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
+            gs[idx] = val;
+            g[idx] = (val > vel_unknown) ? val + a.dt * acc : val;
+        }
+        sfn[l] = 0.0f;
+        sfd[l] = 0.0f;
     }
 }
 
 template <int TX, int TY, int TZ, bool P2>
 __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
-    constexpr int HP = (TX + 2) * (TY + 2);
-    constexpr int NFU = (TX + 1) * TY * TZ, NFV = TX * (TY + 1) * TZ, NFW = TX * TY * (TZ + 1);
-    constexpr int NF = NFU > NFV ? (NFU > NFW ? NFU : NFW) : (NFV > NFW ? NFV : NFW);
-    __shared__ float swn[9 * HP], swd[9 * HP], sfn[NF], sfd[NF];   // 9: largest z-slice of a window
+    constexpr int HX = TX + 2, HP = HX * (TY + 2);
+    // face sums: a ring of three face planes per component (slot k % 3)
+    constexpr int RU = 3 * (TX + 1) * TY, RV = 3 * TX * (TY + 1), RW = 3 * TX * TY;
+    __shared__ float swn[9 * HP], swd[9 * HP];   // 9: largest z-slice of a window
+    __shared__ float sun[RU], sud[RU], svn[RV], svd[RV], swfn[RW], swfd[RW];
 
-    const int W = a.W, H = a.H;
+    const int W = a.W, H = a.H, D = a.D;
     const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+    for (int l = threadIdx.x; l < RU; l += blockDim.x) { sun[l] = 0.f; sud[l] = 0.f; }
+    for (int l = threadIdx.x; l < RV; l += blockDim.x) { svn[l] = 0.f; svd[l] = 0.f; }
+    for (int l = threadIdx.x; l < RW; l += blockDim.x) { swfn[l] = 0.f; swfd[l] = 0.f; }
+    // owned face planes: u, v k in [z0, hiz]; w k in [z0, hizw]
+    const int hiz = min(z0 + TZ, D) - 1, hizw = (z0 + TZ >= D) ? D : z0 + TZ - 1;
     // particles in the halo box: one thread per halo row (cj, ck)
     bool any = false;
     {
@@ -433,16 +429,51 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
         static_assert(HY * HZ <= 256, "one thread per halo row");
         const int t = threadIdx.x;
         const int cj = y0 - 1 + t % HY, ck = z0 - 1 + t / HY;
-        if (t < HY * HZ && cj >= 0 && cj < H && ck >= 0 && ck < a.D) {
+        if (t < HY * HZ && cj >= 0 && cj < H && ck >= 0 && ck < D) {
             const int lo = max(x0 - 1, 0), hi = min(x0 + TX, W - 1);
             const long long row = (long long)cj * W + (long long)ck * W * H;
             any = a.start[row + hi + 1] > a.start[row + lo];
         }
     }
-    const bool empty = !__syncthreads_or(any);
-    p2g_component<0, TX, TY, TZ, P2>(a, x0, y0, z0, empty, swn, swd, sfn, sfd);
-    p2g_component<1, TX, TY, TZ, P2>(a, x0, y0, z0, empty, swn, swd, sfn, sfd);
-    p2g_component<2, TX, TY, TZ, P2>(a, x0, y0, z0, empty, swn, swd, sfn, sfd);
+    // (the barrier also orders the zero fill before the plane passes)
+    if (!__syncthreads_or(any)) {
+        // no particle reaches the tile: every owned face is empty
+        for (int k = z0; k <= hiz; ++k) {
+            p2g_write_plane<0, TX, TY, TZ>(a, x0, y0, k, sun, sud);
+            p2g_write_plane<1, TX, TY, TZ>(a, x0, y0, k, svn, svd);
+        }
+        for (int k = z0; k <= hizw; ++k) p2g_write_plane<2, TX, TY, TZ>(a, x0, y0, k, swfn, swfd);
+        return;
+    }
+    // halo planes in z; the three components of a plane back to back, so that
+    // the second and third reads of its particles hit the caches.  After cell
+    // plane kc, u and v face plane kc - 1 and w face plane kc are complete.
+    const int t = threadIdx.x;
+    const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
+    const bool cell_in = t < HP && ci >= 0 && ci < W && cj >= 0 && cj < H;
+    const int kc0 = max(z0 - 1, 0), kc1 = min(z0 + TZ, D - 1);
+    for (int kc = kc0; kc <= kc1; ++kc) {
+        int beg = 0, end = 0;
+        if (cell_in) {
+            const long long c = (long long)ci + (long long)cj * W + (long long)kc * W * H;
+            beg = a.start[c];
+            end = a.start[c + 1];
+        }
+        p2g_plane<0, TX, TY, TZ, P2>(a, x0, y0, z0, kc, beg, end, swn, swd, sun, sud);
+        p2g_plane<1, TX, TY, TZ, P2>(a, x0, y0, z0, kc, beg, end, swn, swd, svn, svd);
+        p2g_plane<2, TX, TY, TZ, P2>(a, x0, y0, z0, kc, beg, end, swn, swd, swfn, swfd);
+        if (kc - 1 >= z0 && kc - 1 <= hiz) {
+            p2g_write_plane<0, TX, TY, TZ>(a, x0, y0, kc - 1, sun, sud);
+            p2g_write_plane<1, TX, TY, TZ>(a, x0, y0, kc - 1, svn, svd);
+        }
+        if (kc >= z0 && kc <= hizw) p2g_write_plane<2, TX, TY, TZ>(a, x0, y0, kc, swfn, swfd);
+        // (the next plane's first barrier orders these clears before its sums)
+    }
+    if (kc1 >= z0 && kc1 <= hiz) {
+        p2g_write_plane<0, TX, TY, TZ>(a, x0, y0, kc1, sun, sud);
+        p2g_write_plane<1, TX, TY, TZ>(a, x0, y0, kc1, svn, svd);
+    }
+    if (kc1 + 1 >= z0 && kc1 + 1 <= hizw) p2g_write_plane<2, TX, TY, TZ>(a, x0, y0, kc1 + 1, swfn, swfd);
 }
 
 // ---------------------------------------------------------------------------
