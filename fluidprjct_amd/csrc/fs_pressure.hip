@@ -8,9 +8,10 @@
 // operator is re-discretised on every level with unit coefficients; transfers
 // are cell-centred trilinear prolongation P and restriction R = P^T / 2 (the
 // factor 4 of the unit-coefficient coarse operator times the 1/8 average);
-// damped Jacobi (omega = 6/7) pre/post smoothing and a fixed number of Jacobi
-// sweeps on the coarsest level keep the V-cycle a symmetric positive definite
-// linear operator, as CG requires.
+// a degree-3 Chebyshev polynomial smoother in D^-1 A (three Jacobi-form sweeps
+// with the steps of fs_pressure.hpp) before and after the coarse correction and
+// a fixed number of damped-Jacobi sweeps on the coarsest level keep the V-cycle
+// a symmetric positive definite linear operator, as CG requires.
 //
 // Kernel count per iteration is what limits small levels on MI355X (~10 us of
 // launch-to-launch latency each), so the V-cycle fuses two pre-smoothing sweeps
@@ -877,7 +878,8 @@ constexpr int TAIL_LDS_FLOATS = 38912;   // 152 KiB of the 160 KiB per workgroup
 struct TailLevels {
     MGLevel lv[TAIL_LEVELS];
     int off[TAIL_LEVELS];
-    int n;   // lv[n - 1] is the coarsest level
+    int n;        // lv[n - 1] is the coarsest level
+    float w[3];   // smoothing step of pre/post sweep k (k mod 3)
 };
 
 // keeps the scheduler from interleaving the unrolled per-cell bodies of the
@@ -942,7 +944,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
     if (sc && sc->done) return;
     __shared__ float lds[TAIL_LDS_FLOATS];
     const int tid = threadIdx.x;
-    const float omega2 = npre >= 2 ? omega : 0.0f;
+    // omega: the coarsest level's sweeps; t.w[k % 3]: pre/post sweep k
+    const float w0 = t.w[0], omega2 = npre >= 2 ? t.w[1] : 0.0f;
     const MGLevel T = t.lv[0];
     const int nT = (int)T.n;
     int WT = T.W, WHT = T.W * T.H;
@@ -972,7 +975,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
     for (int q = 0; q < TAIL_CPT; ++q) {
         const int c = tid + q * TAIL_THREADS;
         const uint32_t d = (cT[q] >> 8) & 7u;
-        if (c < nT) tx[c] = ((cT[q] & SELF) && d) ? omega * bT[q] * inv_diag(d) : 0.0f;
+        if (c < nT) tx[c] = ((cT[q] & SELF) && d) ? w0 * bT[q] * inv_diag(d) : 0.0f;
     }
     __syncthreads();
     OPAQUE_TOP(bT, cT);
@@ -983,7 +986,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
         const uint32_t code = cT[q], d = (code >> 8) & 7u;
         float xn = 0.0f;
         if ((code & SELF) && d) {
-            const float x1 = omega * bT[q] * inv_diag(d);
+            const float x1 = w0 * bT[q] * inv_diag(d);
             float ax = (float)d * x1;
             if (code & 1u) ax -= tx[c + 1];
             if (code & 2u) ax -= tx[c - 1];
@@ -1008,7 +1011,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
             if (c >= nT) continue;
             const uint32_t code = cT[q], d = (code >> 8) & 7u;
             float xn = 0.0f;
-            if ((code & SELF) && d) xn = cur[c] + omega * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
+            if ((code & SELF) && d)
+                xn = cur[c] + t.w[k % 3] * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
             oth[c] = xn;
             CELL_FENCE();
         }
@@ -1042,14 +1046,14 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
         const int W = L.W, WH = L.W * L.H, n = (int)L.n;
         for (int c = tid; c < n; c += TAIL_THREADS) {
             const uint32_t cd = code[c], d = (cd >> 8) & 7u;
-            x[c] = ((cd & SELF) && d) ? omega * b[c] * inv_diag(d) : 0.0f;
+            x[c] = ((cd & SELF) && d) ? w0 * b[c] * inv_diag(d) : 0.0f;
         }
         __syncthreads();
         for (int c = tid; c < n; c += TAIL_THREADS) {
             const uint32_t cd = code[c], d = (cd >> 8) & 7u;
             float xn = 0.0f;
             if ((cd & SELF) && d) {
-                const float x1 = omega * b[c] * inv_diag(d);
+                const float x1 = w0 * b[c] * inv_diag(d);
                 float ax = (float)d * x1;
                 if (cd & 1u) ax -= x[c + 1];
                 if (cd & 2u) ax -= x[c - 1];
@@ -1065,7 +1069,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
         float* lc = xt;
         float* lo = x;
         for (int k = 2; k < npre; ++k) {
-            lds_sweep(L, lc, lo, b, code, omega);
+            lds_sweep(L, lc, lo, b, code, t.w[k % 3]);
             __syncthreads();
             float* tmp = lc; lc = lo; lo = tmp;
         }
@@ -1113,7 +1117,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
         lds_prolong(L, lc, code, Cn, xc);
         __syncthreads();
         for (int k = 0; k < npost; ++k) {
-            lds_sweep(L, lc, lo, b, code, omega);
+            lds_sweep(L, lc, lo, b, code, t.w[k % 3]);
             __syncthreads();
             float* tmp = lc; lc = lo; lo = tmp;
         }
@@ -1145,7 +1149,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
                 if (c >= nT) continue;
                 const uint32_t code = cT[q], d = (code >> 8) & 7u;
                 float xn = 0.0f;
-                if ((code & SELF) && d) xn = cur[c] + omega * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
+                if ((code & SELF) && d)
+                    xn = cur[c] + t.w[k % 3] * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
                 oth[c] = xn;
             }
             __syncthreads();
@@ -1485,6 +1490,7 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     if (use_tail && l > 0 && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
         TailLevels t{};
         t.n = nlev - l;
+        for (int q = 0; q < 3; ++q) t.w[q] = sweep_omega[q];
         long long off = (2 * L.n + 3) & ~3LL;
         for (int q = 0; q < t.n; ++q) {
             t.lv[q] = lv[l + q];
@@ -1517,16 +1523,18 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     const bool v4 = (l == 0) && vec0;   // 4 cells per thread on the fine level
     const unsigned nb4 = nblk(L.n / 4);
     if (v4)
-        hipLaunchKernelGGL(k_jacobi2_v4, dim3(nb4), dim3(256), 0, st, L, L.b, cur, omega, npre >= 2 ? omega : 0.0f, s);
+        hipLaunchKernelGGL(k_jacobi2_v4, dim3(nb4), dim3(256), 0, st, L, L.b, cur, sweep_omega[0],
+                           npre >= 2 ? sweep_omega[1] : 0.0f, s);
     else
-        hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, omega, npre >= 2 ? omega : 0.0f, s);
+        hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, sweep_omega[0],
+                           npre >= 2 ? sweep_omega[1] : 0.0f, s);
     for (int k = 2; k < npre; ++k) {
         if (v4)
-            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, omega,
+            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
                                (double*)nullptr, s);
         else
-            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr,
-                               s);
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+                               (double*)nullptr, s);
         std::swap(cur, oth);
     }
     MGLevel& Cn = lv[l + 1];
@@ -1548,10 +1556,10 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     int k0;
     if (fuse) {
         if (pdot && npost == 1)
-            hipLaunchKernelGGL(k_prolong_smooth<true>, dim3(ng), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, omega,
+            hipLaunchKernelGGL(k_prolong_smooth<true>, dim3(ng), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, sweep_omega[0],
                                pdot, s);
         else
-            hipLaunchKernelGGL(k_prolong_smooth<false>, dim3(nb), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, omega,
+            hipLaunchKernelGGL(k_prolong_smooth<false>, dim3(nb), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, sweep_omega[0],
                                (double*)nullptr, s);
         std::swap(cur, oth);
         k0 = 1;
@@ -1567,14 +1575,17 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         const bool dot = k == npost - 1 && pdot;
         if (v4) {
             if (dot)
-                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
+                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+                                   pdot, s);
             else
-                hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, omega,
-                                   (double*)nullptr, s);
+                hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth,
+                                   sweep_omega[k % 3], (double*)nullptr, s);
         } else if (dot) {
-            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, omega, pdot, s);
+            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3], pdot,
+                               s);
         } else {
-            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, omega, (double*)nullptr, s);
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+                               (double*)nullptr, s);
         }
         std::swap(cur, oth);
     }
