@@ -8,7 +8,7 @@
 // operator is re-discretised on every level with unit coefficients; transfers
 // are cell-centred trilinear prolongation P and restriction R = P^T / 2 (the
 // factor 4 of the unit-coefficient coarse operator times the 1/8 average);
-// a degree-3 Chebyshev polynomial smoother in D^-1 A (three Jacobi-form sweeps
+// a degree-4 Chebyshev polynomial smoother in D^-1 A (four Jacobi-form sweeps
 // with the steps of fs_pressure.hpp) before and after the coarse correction and
 // a fixed number of damped-Jacobi sweeps on the coarsest level keep the V-cycle
 // a symmetric positive definite linear operator, as CG requires.
@@ -879,7 +879,7 @@ struct TailLevels {
     MGLevel lv[TAIL_LEVELS];
     int off[TAIL_LEVELS];
     int n;        // lv[n - 1] is the coarsest level
-    float w[3];   // smoothing step of pre/post sweep k (k mod 3)
+    float w[SMOOTH_STEPS];   // smoothing step of pre/post sweep k
 };
 
 // keeps the scheduler from interleaving the unrolled per-cell bodies of the
@@ -944,7 +944,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
     if (sc && sc->done) return;
     __shared__ float lds[TAIL_LDS_FLOATS];
     const int tid = threadIdx.x;
-    // omega: the coarsest level's sweeps; t.w[k % 3]: pre/post sweep k
+    // omega: the coarsest level's sweeps; t.w[k % SMOOTH_STEPS]: pre/post sweep k
     const float w0 = t.w[0], omega2 = npre >= 2 ? t.w[1] : 0.0f;
     const MGLevel T = t.lv[0];
     const int nT = (int)T.n;
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
             const uint32_t code = cT[q], d = (code >> 8) & 7u;
             float xn = 0.0f;
             if ((code & SELF) && d)
-                xn = cur[c] + t.w[k % 3] * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
+                xn = cur[c] + t.w[k % SMOOTH_STEPS] * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
             oth[c] = xn;
             CELL_FENCE();
         }
@@ -1069,7 +1069,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
         float* lc = xt;
         float* lo = x;
         for (int k = 2; k < npre; ++k) {
-            lds_sweep(L, lc, lo, b, code, t.w[k % 3]);
+            lds_sweep(L, lc, lo, b, code, t.w[k % SMOOTH_STEPS]);
             __syncthreads();
             float* tmp = lc; lc = lo; lo = tmp;
         }
@@ -1117,7 +1117,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
         lds_prolong(L, lc, code, Cn, xc);
         __syncthreads();
         for (int k = 0; k < npost; ++k) {
-            lds_sweep(L, lc, lo, b, code, t.w[k % 3]);
+            lds_sweep(L, lc, lo, b, code, t.w[k % SMOOTH_STEPS]);
             __syncthreads();
             float* tmp = lc; lc = lo; lo = tmp;
         }
@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailLevels t, int npre, i
                 const uint32_t code = cT[q], d = (code >> 8) & 7u;
                 float xn = 0.0f;
                 if ((code & SELF) && d)
-                    xn = cur[c] + t.w[k % 3] * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
+                    xn = cur[c] + t.w[k % SMOOTH_STEPS] * (bT[q] - stencil_Ax(cur, c, code, WT, WHT)) * inv_diag(d);
                 oth[c] = xn;
             }
             __syncthreads();
@@ -1490,7 +1490,7 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     if (use_tail && l > 0 && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
         TailLevels t{};
         t.n = nlev - l;
-        for (int q = 0; q < 3; ++q) t.w[q] = sweep_omega[q];
+        for (int q = 0; q < SMOOTH_STEPS; ++q) t.w[q] = sweep_omega[q];
         long long off = (2 * L.n + 3) & ~3LL;
         for (int q = 0; q < t.n; ++q) {
             t.lv[q] = lv[l + q];
@@ -1530,10 +1530,10 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
                            npre >= 2 ? sweep_omega[1] : 0.0f, s);
     for (int k = 2; k < npre; ++k) {
         if (v4)
-            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
                                (double*)nullptr, s);
         else
-            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
                                (double*)nullptr, s);
         std::swap(cur, oth);
     }
@@ -1575,16 +1575,16 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         const bool dot = k == npost - 1 && pdot;
         if (v4) {
             if (dot)
-                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
                                    pdot, s);
             else
                 hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth,
-                                   sweep_omega[k % 3], (double*)nullptr, s);
+                                   sweep_omega[k % SMOOTH_STEPS], (double*)nullptr, s);
         } else if (dot) {
-            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3], pdot,
+            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS], pdot,
                                s);
         } else {
-            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % 3],
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
                                (double*)nullptr, s);
         }
         std::swap(cur, oth);

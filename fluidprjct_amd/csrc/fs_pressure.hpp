@@ -51,6 +51,9 @@ struct Preconditioner {
     virtual long long graph_epoch() const { return 0; }
 };
 
+// pre and post smoothing sweeps per level (and the degree of their polynomial)
+constexpr int SMOOTH_STEPS = 4;
+
 class MGPCG {
 public:
     MGPCG() {}
@@ -80,20 +83,21 @@ public:
     int nlev = 0;
     long long n0 = 0;              // cells of the solve box (level 0)
     int box[3] = {0, 0, 0};        // solve-box origin in the full grid
-    int nu_pre = 3, nu_post = 3, coarse_sweeps = 8;
+    int nu_pre = SMOOTH_STEPS, nu_post = SMOOTH_STEPS, coarse_sweeps = 8;
     long long coarse_max = 512;   // coarsen until a level has at most this many cells (LDS-resident)
     float omega = 6.0f / 7.0f;          // coarsest-level sweeps (damped Jacobi)
-    // Step of pre/post sweep k (k mod 3): x += w_k D^-1 (b - A x).  The three
-    // steps are the reciprocal roots of the degree-3 Chebyshev polynomial on
-    // [b/20, b], b = 2 (Gershgorin bound of the spectrum of D^-1 A for this
-    // operator), theta_k = (a + b)/2 + (b - a)/2 cos((2k - 1) pi / 6): they
-    // damp the upper 95% of the spectrum to |p| <= 0.1 where three damped
-    // Jacobi sweeps (w = 6/7) leave 0.36.  The polynomial is in D^-1 A, so
-    // pre and post smoothing commute with their A-adjoints and the V-cycle
-    // stays symmetric (a valid CG preconditioner).  Measured: 16 -> 12 CG
-    // iterations at 256^3, 26 -> 19 at 512^3 (the interval b/20 measured
-    // best among b/3 .. b/50).  Equal steps give damped Jacobi.
-    float sweep_omega[3] = {0.53398f, 0.95238f, 4.39994f};
+    // Step of pre/post sweep k: x += w_k D^-1 (b - A x).  The SMOOTH_STEPS
+    // steps are the reciprocal roots of the Chebyshev polynomial of that
+    // degree on [b/40, b], b = 2 (Gershgorin bound of the spectrum of D^-1 A
+    // for this operator), theta_k = (a + b)/2 + (b - a)/2 cos((2k - 1) pi / 8).
+    // The polynomial is in D^-1 A, so pre and post smoothing commute with
+    // their A-adjoints and the V-cycle stays symmetric (a valid CG
+    // preconditioner).  Measured at 256^3 (CG iterations / solve time):
+    // damped Jacobi w = 6/7 V(3,3) 16 / 6.7 ms; Chebyshev V(3,3) on [b/20, b]
+    // 12 / 5.4 ms; V(4,4) on [b/40, b] 10 / 5.1 ms (at 512^3 19 and 17
+    // iterations, within 3% in time); V(2,2), V(5,5) and other intervals
+    // were slower.  Equal steps give damped Jacobi.
+    float sweep_omega[SMOOTH_STEPS] = {0.5192694f, 0.7152481f, 1.5340161f, 8.0503983f};
     int batch = 8;              // inner iterations launched between host checks (even)
     double inner_rtol = 0.0;    // inner relative reduction per refinement pass (0: straight to the
                                 // tolerance; FP64 A p sums and solution keep r with the true residual)
