@@ -229,26 +229,46 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
     __syncthreads();
     int unk = 0, dir = 0;
     const long long WH = (long long)L.W * L.H;
-    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
-         c += (long long)gridDim.x * blockDim.x) {
-        uint32_t code = 0;
-        if (L.type[c] == 1) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    // four cells per thread and pass, their type and neighbour-type loads
+    // issued together (the grid is capped, so each thread makes many passes)
+    for (long long c0 = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c0 < L.n;
+         c0 += 4 * stride) {
+        uint8_t t[4], nt[4][6];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long c = c0 + u * stride;
+            t[u] = 0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) nt[u][q] = 0;
+            if (c >= L.n) continue;
+            t[u] = L.type[c];
             int i, j, k;
             split3(c, L.W, L.H, i, j, k);
-            int nbr[6] = {i + 1 < L.W ? 1 : 0, i > 0 ? 1 : 0, j + 1 < L.H ? 1 : 0, j > 0 ? 1 : 0, k + 1 < L.D ? 1 : 0,
-                          k > 0 ? 1 : 0};
-            long long off[6] = {1, -1, L.W, -(long long)L.W, WH, -WH};
-            uint32_t diag = 0;
-            for (int q = 0; q < 6; ++q) {
-                if (!nbr[q]) continue;
-                uint8_t t = L.type[c + off[q]];
-                if (t == 1) { code |= (1u << q); ++diag; }
-                else if (t == 2) { ++diag; dir = 1; }
-            }
-            code |= SELF | (diag << 8);
-            ++unk;
+            if (i + 1 < L.W) nt[u][0] = L.type[c + 1];
+            if (i > 0) nt[u][1] = L.type[c - 1];
+            if (j + 1 < L.H) nt[u][2] = L.type[c + L.W];
+            if (j > 0) nt[u][3] = L.type[c - L.W];
+            if (k + 1 < L.D) nt[u][4] = L.type[c + WH];
+            if (k > 0) nt[u][5] = L.type[c - WH];
         }
-        L.code[c] = (uint16_t)code;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long c = c0 + u * stride;
+            if (c >= L.n) continue;
+            uint32_t code = 0;
+            if (t[u] == 1) {
+                uint32_t diag = 0;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    if (nt[u][q] == 1) { code |= (1u << q); ++diag; }
+                    else if (nt[u][q] == 2) { ++diag; dir = 1; }
+                }
+                code |= SELF | (diag << 8);
+                ++unk;
+            }
+            L.code[c] = (uint16_t)code;
+        }
     }
     unk = wave_sum(unk);
     dir = wave_sum(dir);
@@ -1326,6 +1346,8 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
 }
 
 // r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partials of r.r (FP64)
+// ZERO_X: x64 is known to be zero (the first pass), so r = b - shift
+template <bool ZERO_X>
 __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, const double* __restrict__ x64,
                                                  double shift, float* __restrict__ r, double* __restrict__ partial) {
     __shared__ double sm[12];
@@ -1336,7 +1358,8 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
         uint32_t code = L.code[c];
         float rf = 0.0f;
         if (code & SELF) {
-            double rv = ((double)b[c] - shift) - stencil_Ax64(x64, c, code, L.W, WH);
+            double rv = (double)b[c] - shift;
+            if (!ZERO_X) rv -= stencil_Ax64(x64, c, code, L.W, WH);
             acc += rv * rv;
             rf = (float)rv;
         }
@@ -1775,7 +1798,12 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     bool converged = false;
     double rr_true = 0.0;
     for (outer = 0;; ++outer) {
-        hipLaunchKernelGGL(k_resid64, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, slot(PS_RES));
+        if (outer == 0)
+            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, (const double*)xb, shift, r,
+                               slot(PS_RES));
+        else
+            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, (const double*)xb, shift, r,
+                               slot(PS_RES));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
                            (const double*)nullptr, nred, sc, tol2b);
         hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
