@@ -39,6 +39,10 @@ def build_scene(name):
         desc = ("256^3 water column y<0.4, 8 ppc, rigid sphere r=0.08 rho=500 at (0.5,0.8,0.5), "
                 "FLIP 0.95, dt=min(CFL 1,1/120), MGPCG 1e-6")
         return N, lab, body, 8, 3, desc
+    if name == "c1":
+        N = 128
+        lab = scenes.dam_break_2d(N, (0.25, 0.5))
+        return N, lab, None, 4, 1, "128^2 2D dam break x<=0.25 y<=0.5 (one-cell-deep slab), 4 ppc, FLIP 0.95, CFL 1"
     if name == "c2":
         N = 128
         lab = scenes.dam_break(N, (0.25, 0.5, 1.0))
@@ -130,8 +134,9 @@ def main():
     st = s.solve_stats()
     simt, dt_last, nsteps = s.clock()
 
-    C = N ** 3
-    F = (N + 1) * N * N
+    D, H, W = lab.shape
+    C = W * H * D
+    F = ((W + 1) * H * D + W * (H + 1) * D + W * H * (D + 1)) // 3   # faces per component (mean)
     labs = s.get_grid(7)
     nf = int((labs == 1).sum())
     ms = {k: v / K for k, v in t_stage.items()}
@@ -156,7 +161,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": args.config + ": " + desc, "grid": [N, N, N], "particles": Np,
+        "config": {"workload": args.config + ": " + desc, "grid": [W, H, D], "particles": Np,
                    "fluid_cells": nf, "precision": "FP32 transfers/stencil, FP64 dots + refinement"},
         "pressure_solve_ms_plain": ms["pressure"],
         "pressure_iters_last": st.iterations,

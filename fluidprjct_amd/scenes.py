@@ -42,6 +42,20 @@ def dam_break(N, extent=(0.25, 0.5, 1.0), dx=None):
     return lab
 
 
+def dam_break_2d(N, extent=(0.25, 0.5), dx=None):
+    """The 2D dam break (block x<=ex, y<=ey) as a slab one cell deep: solid walls
+    in x and y, D = 1 (no z neighbours, so w stays 0 and advection clamps z to
+    0 - the 3D step restricted to a plane)."""
+    dx = dx or 1.0 / N
+    lab = np.full((1, N, N), AIR, np.uint8)
+    lab[:, 0], lab[:, -1] = SOLID, SOLID
+    lab[:, :, 0], lab[:, :, -1] = SOLID, SOLID
+    x, y = _centres(N, dx), _centres(N, dx)
+    m = (y[None, :, None] <= extent[1]) & (x[None, None, :] <= extent[0])
+    lab[m & (lab == AIR)] = FLUID
+    return lab
+
+
 def water_column(N, level=0.4, dx=None):
     """Fluid layer y < level in a unit box with solid walls."""
     dx = dx or 1.0 / N

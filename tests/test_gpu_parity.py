@@ -227,3 +227,27 @@ def test_tail_matches_level_launches(hip, N, block):
     assert st.converged == 1 and sn.converged == 1
     assert st.iterations == sn.iterations
     assert np.array_equal(pt.view(np.uint32), pn.view(np.uint32))
+
+
+def test_c1_slab_steps_close(hip, oracle):
+    """The 2D dam break (128^2, 4 ppc) as a one-cell-deep slab: the HIP step
+    and the oracle stay within the whole-step bars for 5 steps, and the slab
+    stays planar (z = 0, w = 0)."""
+    from fluidprjct_amd import scenes
+    N = 128
+    lab = scenes.dam_break_2d(N, (0.25, 0.5))
+    h, o = pair(hip, oracle, dx=1.0 / N, dt=1.0 / 120, solver_tol=1e-8)
+    for s in (h, o):
+        s.set_labels(lab)
+        s.seed_particles(4, 1)
+    for step in range(5):
+        for s in (h, o):
+            s.step(1)
+        ph, vh = h.particles()
+        po, vo = o.particles()
+        assert ph.shape == po.shape
+        assert np.abs(ph - po).max() <= 2e-5, (step, np.abs(ph - po).max())
+        assert np.abs(vh - vo).max() <= 2e-3 * max(1.0, np.abs(vo).max()), (step, np.abs(vh - vo).max())
+    assert not ph[:, 2].any() and not vh[:, 2].any()
+    assert not h.get_grid(FS_GRID_W).any()
+    assert (h.get_grid(FS_GRID_LABELS) != o.get_grid(FS_GRID_LABELS)).sum() <= 2

@@ -91,3 +91,22 @@ def test_binary_and_ascii_particle_files(oracle, tmp_path):
     assert (fa, fb) == ("ascii", "binary_little_endian")
     assert np.array_equal(vb, pos)
     np.testing.assert_allclose(va, pos, rtol=1e-5, atol=1e-6)
+
+
+def test_dam_break_2d_slab(oracle):
+    """The 2D case (c1) is a slab one cell deep: 31 x 63 FLUID cells inside the
+    walls, 4 particles each; stepping keeps it planar (z = 0, no w)."""
+    from fluidprjct_amd._abi import FS_GRID_W
+    lab = scenes.dam_break_2d(128, (0.25, 0.5))
+    assert lab.shape == (1, 128, 128)
+    assert int((lab == 1).sum()) == 31 * 63
+    assert (lab[0, 0] == 0).all() and (lab[0, :, 0] == 0).all()
+    s = oracle.solver(dt=1.0 / 120, dx=1.0 / 128, pic_weight=0.05, cfl=1.0, particles_per_cell=4, seed=1)
+    s.set_labels(lab)
+    s.seed_particles(4, 1)
+    s.step(3)
+    pos, vel = s.particles()
+    assert pos.shape == (4 * 31 * 63, 3)
+    assert not pos[:, 2].any() and not vel[:, 2].any() and not s.get_grid(FS_GRID_W).any()
+    assert vel[:, 1].min() < 0.0   # falling under gravity
+    assert (pos[:, :2] >= 0).all() and (pos[:, :2] <= 127 / 128).all()
