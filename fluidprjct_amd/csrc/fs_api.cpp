@@ -372,6 +372,7 @@ struct Solver {
         mg.stall_window = cfg.stall_window > 0 ? cfg.stall_window : 5;
         mg.use_graph = !(cfg.solver_flags & FS_SOLVER_NO_GRAPH);
         mg.use_tail = !(cfg.solver_flags & FS_SOLVER_NO_TAIL);
+        mg.use_tb = !(cfg.solver_flags & FS_SOLVER_NO_FUSE);
         int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats);
         if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
         if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");

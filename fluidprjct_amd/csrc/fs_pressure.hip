@@ -35,6 +35,7 @@ namespace fsk {
 constexpr int RED_BLOCKS = 2048;      // partial-sum slots of every reducing kernel
 constexpr int COARSE_MAX = 4096;      // cells of the LDS-resident coarsest level
 constexpr long long FUSE_MAX = 1LL << 16;  // levels up to this size use the fused (recomputing) transfer kernels
+constexpr long long TB_MAX = 1LL << 18;    // coarse levels up to this size use the temporally blocked smoothers
 constexpr uint32_t SELF = 0x8000u;
 
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
@@ -482,6 +483,129 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, c
         xout[c] = xn;
     }
     if (DOT) store_partial(acc, sm, partial);
+}
+
+// ---------------------------------------------------------------- coarse levels, temporally blocked
+// On the small levels every kernel is bound by its launch latency, so all the
+// pre-smoothing sweeps and the residual run as one kernel, and the
+// prolongation and all the post-smoothing sweeps as another: a block owns an
+// 8x8x4-cell tile and recomputes what its tile needs from a halo of NS cells
+// in LDS (the sweep before the last on the tile plus one cell, and so on).
+// Every cell value is computed with the operations of k_jacobi2, k_jacobi,
+// k_residual and k_prolong_smooth in the same order, so the results are the
+// same bits as those kernels'.
+constexpr int TB_TX = 8, TB_TY = 8, TB_TZ = 4;
+struct SweepSteps {
+    float w[SMOOTH_STEPS];
+};
+
+// PRE:  x_1 = w_0 b / d pointwise on the tile + NS, then NS - 1 sweeps (the
+//       valid region shrinking by one cell each), x -> xout and the residual
+//       b - A x -> rout on the tile.
+// !PRE: x + P xc on the tile + NS (xin is the level's iterate, xc the coarse
+//       correction), NS sweeps, x -> xout on the tile.
+constexpr int TB_THREADS = 1024;
+template <int NS, bool PRE>
+__global__ __launch_bounds__(TB_THREADS) void k_smooth_tb(MGLevel L, const float* __restrict__ b,
+                                                   const float* __restrict__ xin, float* __restrict__ xout,
+                                                   float* __restrict__ rout, MGLevel Cc, const float* __restrict__ xc,
+                                                   SweepSteps st, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    constexpr int G = NS, RX = TB_TX + 2 * G, RY = TB_TY + 2 * G, RZ = TB_TZ + 2 * G, RN = RX * RY * RZ;
+    constexpr int RXY = RX * RY;
+    // coarse values the region's prolongation reads: coarse cells
+    // (o >> 1) - 1 .. ((o + R - 1) >> 1) + 1 per axis
+    constexpr int CX = RX / 2 + 3, CY = RY / 2 + 3, CZ = RZ / 2 + 3, CN = CX * CY * CZ;
+    __shared__ float sx0[RN], sx1[RN], sbv[RN];
+    __shared__ float sxc[PRE ? 1 : CN];
+    __shared__ uint16_t scd[RN];
+    const int W = L.W, H = L.H, D = L.D;
+    const long long WH = (long long)W * H;
+    const int ntx = (W + TB_TX - 1) / TB_TX, nty = (H + TB_TY - 1) / TB_TY;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int ox = (bid % ntx) * TB_TX - G, oy = ((bid / ntx) % nty) * TB_TY - G, oz = (bid / (ntx * nty)) * TB_TZ - G;
+    // (ox, oy, oz are even: tile sizes and G are even)
+    const int cx0 = (ox >> 1) - 1, cy0 = (oy >> 1) - 1, cz0 = (oz >> 1) - 1;
+    if (!PRE) {
+        for (int l = threadIdx.x; l < CN; l += blockDim.x) {
+            const int I = cx0 + l % CX, J = cy0 + (l / CX) % CY, K = cz0 + l / (CX * CY);
+            const bool in = I >= 0 && I < Cc.W && J >= 0 && J < Cc.H && K >= 0 && K < Cc.D;
+            sxc[l] = in ? xc[(size_t)I + (size_t)J * Cc.W + (size_t)K * Cc.W * Cc.H] : 0.0f;
+        }
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < RN; l += blockDim.x) {
+        const int i = ox + l % RX, j = oy + (l / RX) % RY, k = oz + l / RXY;
+        const bool in = i >= 0 && i < W && j >= 0 && j < H && k >= 0 && k < D;
+        const long long g = in ? (long long)i + (long long)j * W + (long long)k * WH : 0;
+        const uint32_t cd = in ? L.code[g] : 0u;
+        scd[l] = (uint16_t)cd;
+        sbv[l] = in ? b[g] : 0.0f;
+        if (!PRE) {
+            float xv = 0.0f;
+            if (cd & SELF) {
+                // prolong_at with the coarse values from LDS (same terms, same order)
+                const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+                const int Is[2] = {I0, (i & 1) ? I0 + 1 : I0 - 1}, Js[2] = {J0, (j & 1) ? J0 + 1 : J0 - 1},
+                          Ks[2] = {K0, (k & 1) ? K0 + 1 : K0 - 1};
+                const float w[2] = {0.75f, 0.25f};
+                float sp = 0.0f;
+                for (int a = 0; a < 2; ++a) {
+                    if (Ks[a] < 0 || Ks[a] >= Cc.D) continue;
+                    for (int bb = 0; bb < 2; ++bb) {
+                        if (Js[bb] < 0 || Js[bb] >= Cc.H) continue;
+                        for (int cc = 0; cc < 2; ++cc) {
+                            if (Is[cc] < 0 || Is[cc] >= Cc.W) continue;
+                            sp += w[cc] * w[bb] * w[a] *
+                                  sxc[(Is[cc] - cx0) + CX * ((Js[bb] - cy0) + CY * (Ks[a] - cz0))];
+                        }
+                    }
+                }
+                xv = xin[g] + sp;
+            }
+            sx0[l] = xv;
+        }
+    }
+    __syncthreads();
+    float* cur = sx0;
+    float* nxt = sx1;
+    int m = G;   // margin (cells around the tile) on which cur is valid
+    int k0 = 0;
+    if (PRE) {
+        for (int l = threadIdx.x; l < RN; l += blockDim.x) {
+            const uint32_t cd = scd[l], d = (cd >> 8) & 7u;
+            cur[l] = ((cd & SELF) && d) ? st.w[0] * sbv[l] * inv_diag(d) : 0.0f;
+        }
+        __syncthreads();
+        k0 = 1;
+    }
+    for (int k = k0; k < NS; ++k) {
+        --m;
+        const int ex = TB_TX + 2 * m, ey = TB_TY + 2 * m, ez = TB_TZ + 2 * m, en = ex * ey * ez;
+        const float w = st.w[k];
+        for (int q = threadIdx.x; q < en; q += blockDim.x) {
+            const int l = (G - m + q % ex) + RX * (G - m + (q / ex) % ey) + RXY * (G - m + q / (ex * ey));
+            const uint32_t cd = scd[l], d = (cd >> 8) & 7u;
+            float xn = 0.0f;
+            if ((cd & SELF) && d) xn = cur[l] + w * (sbv[l] - stencil_Ax(cur, l, cd, RX, RXY)) * inv_diag(d);
+            nxt[l] = xn;
+        }
+        __syncthreads();
+        float* t = cur; cur = nxt; nxt = t;
+    }
+    // tile cells inside the level: x (and the residual)
+    for (int q = threadIdx.x; q < TB_TX * TB_TY * TB_TZ; q += blockDim.x) {
+        const int li = q % TB_TX, lj = (q / TB_TX) % TB_TY, lk = q / (TB_TX * TB_TY);
+        const int i = ox + G + li, j = oy + G + lj, k = oz + G + lk;
+        if (i >= W || j >= H || k >= D) continue;
+        const int l = (G + li) + RX * (G + lj) + RXY * (G + lk);
+        const long long g = (long long)i + (long long)j * W + (long long)k * WH;
+        xout[g] = cur[l];
+        if (PRE) {
+            const uint32_t cd = scd[l];
+            rout[g] = (cd & SELF) ? sbv[l] - stencil_Ax(cur, l, cd, RX, RXY) : 0.0f;
+        }
+    }
 }
 
 // coarsest level: fixed number of damped-Jacobi sweeps from zero in LDS
@@ -1532,6 +1656,23 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         if (l == 0 && pdot)   // single-level hierarchy: r.z of the coarse result
             hipLaunchKernelGGL(k_dot_rz, dim3(nred), dim3(256), 0, st, L.n, L.b, L.x, pdot, s);
         return L.x;
+    }
+    // small coarse levels: pre-smoothing + residual and prolongation +
+    // post-smoothing as one temporally blocked kernel each
+    if (use_tb && l > 0 && L.n <= TB_MAX && nu_pre == SMOOTH_STEPS && nu_post == SMOOTH_STEPS) {
+        MGLevel& Cn = lv[l + 1];
+        const unsigned ntb = (unsigned)(((L.W + TB_TX - 1) / TB_TX) * ((L.H + TB_TY - 1) / TB_TY) *
+                                        ((L.D + TB_TZ - 1) / TB_TZ));
+        SweepSteps ss;
+        for (int q = 0; q < SMOOTH_STEPS; ++q) ss.w[q] = sweep_omega[q];
+        hipLaunchKernelGGL((k_smooth_tb<SMOOTH_STEPS, true>), dim3(ntb), dim3(TB_THREADS), 0, st, L, (const float*)L.b,
+                           (const float*)nullptr, L.x, L.r, Cn, (const float*)nullptr, ss, (const SolveScalars*)s);
+        hipLaunchKernelGGL(k_restrict, dim3(nblk(4 * Cn.n)), dim3(256), 0, st, L, Cn, (const float*)L.r, Cn.b,
+                           (const SolveScalars*)s);
+        const float* xc = vcycle_level(l + 1, st, nullptr, s);
+        hipLaunchKernelGGL((k_smooth_tb<SMOOTH_STEPS, false>), dim3(ntb), dim3(TB_THREADS), 0, st, L, (const float*)L.b,
+                           (const float*)L.x, L.xt, (float*)nullptr, Cn, xc, ss, (const SolveScalars*)s);
+        return L.xt;
     }
     const unsigned nb = nblk(L.n);
     const unsigned ng = (l == 0) ? (unsigned)nred : nb;   // grid of the r.z-accumulating sweep

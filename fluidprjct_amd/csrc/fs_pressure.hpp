@@ -116,6 +116,7 @@ public:
     int last_iters = 0, last_outer = 0;
     bool use_graph = true;      // replay inner batches as HIP graphs
     bool use_tail = true;
+    bool use_tb = true;         // temporally blocked smoothing kernels on small coarse levels
     bool use_vec = true;        // 4-cells-per-thread fine-level kernels when the box width allows
     bool vec0 = false;          // the configured fine level uses them       // levels of at most 32K cells in one workgroup launch
     // partial sums of r.z over the solve box (for preconditioners that do not fuse it)
