@@ -253,7 +253,8 @@ def test_c1_slab_steps_close(hip, oracle):
     assert (h.get_grid(FS_GRID_LABELS) != o.get_grid(FS_GRID_LABELS)).sum() <= 2
 
 
-@pytest.mark.parametrize("N,block", [(96, (1.0, 0.4, 1.0)), (128, (0.6, 0.5, 1.0))])
+# 192: a level of more than 256K cells below the fine one (per-sweep launches there)
+@pytest.mark.parametrize("N,block", [(96, (1.0, 0.4, 1.0)), (128, (0.6, 0.5, 1.0)), (192, (1.0, 0.4, 1.0))])
 def test_blocked_smoothing_matches_per_sweep_launches(hip, N, block):
     """On the small coarse levels all pre-smoothing sweeps + the residual, and
     the prolongation + all post-smoothing sweeps, run as one temporally blocked
