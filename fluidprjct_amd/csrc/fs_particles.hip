@@ -816,8 +816,10 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
                 pos.x = fmaxf(pos.x, 0.0f); pos.x = fminf(pos.x, xmax);
                 pos.y = fmaxf(pos.y, 0.0f); pos.y = fminf(pos.y, ymax);
                 pos.z = fmaxf(pos.z, 0.0f); pos.z = fminf(pos.z, zmax);
-                int cx = (int)floorf(div_dx<P2>(pos.x, dx, inv)), cy = (int)floorf(div_dx<P2>(pos.y, dx, inv)),
-                    cz = (int)floorf(div_dx<P2>(pos.z, dx, inv));
+                // the clamps leave pos finite and >= 0 (fmaxf returns the non-NaN
+                // operand), so truncation is the floor
+                int cx = (int)div_dx<P2>(pos.x, dx, inv), cy = (int)div_dx<P2>(pos.y, dx, inv),
+                    cz = (int)div_dx<P2>(pos.z, dx, inv);
                 if (cx >= 0 && cx < W && cy >= 0 && cy < H && cz >= 0 && cz < D) {
                     uint8_t L = F.label(cx, cy, cz);
                     if (L == FS_SOLID || L == FS_BODY) {
