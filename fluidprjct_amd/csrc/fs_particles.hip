@@ -565,6 +565,26 @@ __global__ __launch_bounds__(256) void k_max_speed(ParticlesView P, int n, unsig
 // the RK3 substeps reads LDS.  A sample whose 8 corners are not all inside the
 // staged box (a particle that left the margin) reads global memory instead,
 // with the same indices, so both paths return the same values.
+// LDS layout of one staged face component: row and plane pitches padded so
+// that the corners a half-wave reads (lanes spread along x over a few cells,
+// differing by one row or plane) fall in different bank ranges.
+constexpr int lds_row_pitch(int lx) {
+    int p = lx;
+    while (p % 32 < 9 || p % 32 > 23) ++p;
+    return p;
+}
+constexpr int lds_plane_pitch(int lx, int ly) {
+    const int pr = lds_row_pitch(lx), want = (2 * (pr % 32)) % 32;
+    int p = pr * ly;
+    while (p % 32 != want) ++p;
+    return p;
+}
+template <int BX, int BY, int BZ, int COMP>
+struct LdsFace {
+    static constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1), LZ = BZ + (COMP == 2);
+    static constexpr int PR = lds_row_pitch(LX), PP = lds_plane_pitch(LX, LY), SIZE = PP * LZ;
+};
+
 template <int BX, int BY, int BZ, bool P2>
 struct TileFields {
     int X0, Y0, Z0;
@@ -584,22 +604,22 @@ struct TileFields {
 
     template <int COMP>
     FS_DEV bool in_box(const Base& b, int& i0) const {
-        constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1), LZ = BZ + (COMP == 2);
+        using F = LdsFace<BX, BY, BZ, COMP>;
         const int nx = a->W + (COMP == 0), ny = a->H + (COMP == 1), nz = a->D + (COMP == 2);
         int li = b.i - X0, lj = b.j - Y0, lk = b.k - Z0;
-        bool ok = li >= 0 && li + 1 < LX && lj >= 0 && lj + 1 < LY && lk >= 0 && lk + 1 < LZ && b.i + 1 < nx &&
-                  b.j + 1 < ny && b.k + 1 < nz;
-        i0 = li + LX * (lj + LY * lk);
+        bool ok = li >= 0 && li + 1 < F::LX && lj >= 0 && lj + 1 < F::LY && lk >= 0 && lk + 1 < F::LZ &&
+                  b.i + 1 < nx && b.j + 1 < ny && b.k + 1 < nz;
+        i0 = li + F::PR * lj + F::PP * lk;
         return ok;
     }
 
     template <int COMP>
     FS_DEV float lds_sample(const Base& b, int i0) const {
-        constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1);
+        using F = LdsFace<BX, BY, BZ, COMP>;
         const float* s = COMP == 0 ? su : (COMP == 1 ? sv : sw);
-        constexpr int SXY = LX * LY;
-        return trilin(b.fx, b.fy, b.fz, s[i0], s[i0 + 1], s[i0 + SXY], s[i0 + 1 + SXY], s[i0 + LX], s[i0 + 1 + LX],
-                      s[i0 + LX + SXY], s[i0 + 1 + LX + SXY]);
+        constexpr int R = F::PR, P = F::PP;
+        return trilin(b.fx, b.fy, b.fz, s[i0], s[i0 + 1], s[i0 + P], s[i0 + 1 + P], s[i0 + R], s[i0 + 1 + R],
+                      s[i0 + R + P], s[i0 + 1 + R + P]);
     }
 
     template <int COMP>
@@ -613,7 +633,7 @@ struct TileFields {
     // PIC value and FLIP delta (u - uSaved per corner, FluidSolver3D.cu:990-1012)
     template <int COMP>
     FS_DEV void pic_flip(const Base& b, float& pic, float& flip) const {
-        constexpr int LX = BX + (COMP == 0), LY = BY + (COMP == 1);
+        constexpr int LX = LdsFace<BX, BY, BZ, COMP>::PR, SXY = LdsFace<BX, BY, BZ, COMP>::PP;
         const int W = a->W, H = a->H;
         const int sx = COMP == 0 ? W + 1 : W;
         const int sxy = COMP == 0 ? (W + 1) * H : (COMP == 1 ? W * (H + 1) : W * H);
@@ -622,7 +642,6 @@ struct TileFields {
         int i0;
         if (in_box<COMP>(b, i0)) {
             const float* s = COMP == 0 ? su : (COMP == 1 ? sv : sw);
-            constexpr int SXY = LX * LY;
             const int lo[8] = {0, 1, SXY, 1 + SXY, LX, 1 + LX, LX + SXY, 1 + LX + SXY};
             const int g0 = b.i + b.j * sx + b.k * sxy;
             const int go[8] = {0, 1, sxy, 1 + sxy, sx, 1 + sx, sx + sxy, 1 + sx + sxy};
@@ -662,9 +681,12 @@ struct TileFields {
             Base bu{fx, gy, gz, frx, fhy, fhz};
             Base bv{gx, fy, gz, fhx, fry, fhz};
             Base bw{gx, gy, fz, fhx, fhy, frz};
-            const int iu = (fx - X0) + (BX + 1) * ((gy - Y0) + BY * (gz - Z0));
-            const int iv = (gx - X0) + BX * ((fy - Y0) + (BY + 1) * (gz - Z0));
-            const int iw = (gx - X0) + BX * ((gy - Y0) + BY * (fz - Z0));
+            using FU = LdsFace<BX, BY, BZ, 0>;
+            using FV = LdsFace<BX, BY, BZ, 1>;
+            using FW = LdsFace<BX, BY, BZ, 2>;
+            const int iu = (fx - X0) + FU::PR * (gy - Y0) + FU::PP * (gz - Z0);
+            const int iv = (gx - X0) + FV::PR * (fy - Y0) + FV::PP * (gz - Z0);
+            const int iw = (gx - X0) + FW::PR * (gy - Y0) + FW::PP * (fz - Z0);
             return mk(lds_sample<0>(bu, iu), lds_sample<1>(bv, iv), lds_sample<2>(bw, iw));
         }
         Base bu = mac_base<0>(rx, ry, rz, a->W, a->H, a->D);
@@ -692,9 +714,12 @@ struct TileFields {
 template <int TX, int TY, int TZ, int M, bool P2>
 __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* __restrict__ start) {
     constexpr int BX = TX + 2 * M, BY = TY + 2 * M, BZ = TZ + 2 * M;
+    using FU = LdsFace<BX, BY, BZ, 0>;
+    using FV = LdsFace<BX, BY, BZ, 1>;
+    using FW = LdsFace<BX, BY, BZ, 2>;
     constexpr int NU = (BX + 1) * BY * BZ, NV = BX * (BY + 1) * BZ, NW = BX * BY * (BZ + 1), NL = BX * BY * BZ;
     constexpr int ROWS = TY * TZ;
-    __shared__ float su[NU], sv[NV], sw[NW];
+    __shared__ float su[FU::SIZE], sv[FV::SIZE], sw[FW::SIZE];
     __shared__ uint8_t sl[NL];
     __shared__ int s_beg[ROWS], s_pref[ROWS + 1], s_scan[4];
 
@@ -725,18 +750,21 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
     // stage the velocity faces and labels of the tile + margin
     const int X0 = x0 - M, Y0 = y0 - M, Z0 = z0 - M;
     for (int l = threadIdx.x; l < NU; l += blockDim.x) {
-        int i = X0 + l % (BX + 1), j = Y0 + (l / (BX + 1)) % BY, k = Z0 + l / ((BX + 1) * BY);
-        su[l] = (i >= 0 && i <= W && j >= 0 && j < H && k >= 0 && k < D)
+        int li = l % (BX + 1), lj = (l / (BX + 1)) % BY, lk = l / ((BX + 1) * BY);
+        int i = X0 + li, j = Y0 + lj, k = Z0 + lk;
+        su[li + FU::PR * lj + FU::PP * lk] = (i >= 0 && i <= W && j >= 0 && j < H && k >= 0 && k < D)
                     ? a.u[(size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H] : 0.0f;
     }
     for (int l = threadIdx.x; l < NV; l += blockDim.x) {
-        int i = X0 + l % BX, j = Y0 + (l / BX) % (BY + 1), k = Z0 + l / (BX * (BY + 1));
-        sv[l] = (i >= 0 && i < W && j >= 0 && j <= H && k >= 0 && k < D)
+        int li = l % BX, lj = (l / BX) % (BY + 1), lk = l / (BX * (BY + 1));
+        int i = X0 + li, j = Y0 + lj, k = Z0 + lk;
+        sv[li + FV::PR * lj + FV::PP * lk] = (i >= 0 && i < W && j >= 0 && j <= H && k >= 0 && k < D)
                     ? a.v[(size_t)i + (size_t)j * W + (size_t)k * W * (H + 1)] : 0.0f;
     }
     for (int l = threadIdx.x; l < NW; l += blockDim.x) {
-        int i = X0 + l % BX, j = Y0 + (l / BX) % BY, k = Z0 + l / (BX * BY);
-        sw[l] = (i >= 0 && i < W && j >= 0 && j < H && k >= 0 && k <= D)
+        int li = l % BX, lj = (l / BX) % BY, lk = l / (BX * BY);
+        int i = X0 + li, j = Y0 + lj, k = Z0 + lk;
+        sw[li + FW::PR * lj + FW::PP * lk] = (i >= 0 && i < W && j >= 0 && j < H && k >= 0 && k <= D)
                     ? a.w[(size_t)i + (size_t)j * W + (size_t)k * W * H] : 0.0f;
     }
     for (int l = threadIdx.x; l < NL; l += blockDim.x) {
