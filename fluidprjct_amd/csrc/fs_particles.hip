@@ -485,6 +485,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
 // the staged path into generic (flat) loads that serve both address spaces
 typedef __attribute__((address_space(1))) const float gfloat_t;
 typedef __attribute__((address_space(1))) const uint8_t gbyte_t;
+typedef __attribute__((address_space(3))) const uint8_t lbyte_t;
 template <int COMP, bool GLOBAL_F = false>
 FS_DEV void sample_pic_flip(const float* __restrict__ f, const float* __restrict__ fs, int n, const Base& b, int sx,
                             int sxy, float& pic, float& flip) {
@@ -684,9 +685,11 @@ struct TileFields {
             using FU = LdsFace<BX, BY, BZ, 0>;
             using FV = LdsFace<BX, BY, BZ, 1>;
             using FW = LdsFace<BX, BY, BZ, 2>;
-            const int iu = (fx - X0) + FU::PR * (gy - Y0) + FU::PP * (gz - Z0);
-            const int iv = (gx - X0) + FV::PR * (fy - Y0) + FV::PP * (gz - Z0);
-            const int iw = (gx - X0) + FW::PR * (gy - Y0) + FW::PP * (fz - Z0);
+            // box-local offsets are small and non-negative here: 24-bit multiplies
+            // (full rate) instead of 32-bit ones (quarter rate)
+            const int iu = (fx - X0) + __mul24(FU::PR, gy - Y0) + __mul24(FU::PP, gz - Z0);
+            const int iv = (gx - X0) + __mul24(FV::PR, fy - Y0) + __mul24(FV::PP, gz - Z0);
+            const int iw = (gx - X0) + __mul24(FW::PR, gy - Y0) + __mul24(FW::PP, fz - Z0);
             return mk(lds_sample<0>(bu, iu), lds_sample<1>(bv, iv), lds_sample<2>(bw, iw));
         }
         Base bu = mac_base<0>(rx, ry, rz, a->W, a->H, a->D);
@@ -703,9 +706,11 @@ struct TileFields {
     // global read only for a cell outside the staged box, so that the common
     // case is a plain LDS load rather than one generic load for both spaces
     FS_DEV uint8_t label(int cx, int cy, int cz) const {
-        const int li = cx - X0, lj = cy - Y0, lk = cz - Z0;
-        const bool in = (unsigned)li < (unsigned)BX && (unsigned)lj < (unsigned)BY && (unsigned)lk < (unsigned)BZ;
-        uint8_t L = sl[in ? li + BX * (lj + BY * lk) : 0];
+        const unsigned li = cx - X0, lj = cy - Y0, lk = cz - Z0;
+        const bool in = (li < (unsigned)BX) & (lj < (unsigned)BY) & (lk < (unsigned)BZ);
+        // 32-bit LDS address arithmetic
+        const unsigned off = li + __umul24(BX, lj) + __umul24(BX * BY, lk);
+        uint8_t L = ((lbyte_t*)sl)[in ? off : 0u];
         if (!in) L = *(gbyte_t*)(a->labels + (size_t)cx + (size_t)cy * a->W + (size_t)cz * a->W * a->H);
         return L;
     }
@@ -848,9 +853,13 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
                 // operand), so truncation is the floor
                 int cx = (int)div_dx<P2>(pos.x, dx, inv), cy = (int)div_dx<P2>(pos.y, dx, inv),
                     cz = (int)div_dx<P2>(pos.z, dx, inv);
-                if (cx >= 0 && cx < W && cy >= 0 && cy < H && cz >= 0 && cz < D) {
-                    uint8_t L = F.label(cx, cy, cz);
-                    if (L == FS_SOLID || L == FS_BODY) {
+                // one predicate and one branch (the rare projection) per substep
+                const bool inside = ((unsigned)cx < (unsigned)W) & ((unsigned)cy < (unsigned)H) &
+                                    ((unsigned)cz < (unsigned)D);
+                if (inside) {
+                    const uint8_t L = F.label(cx, cy, cz);
+                    const bool blocked = (((1u << (L & 3u)) & ((1u << FS_SOLID) | (1u << FS_BODY))) != 0) & (L < 4);
+                    if (blocked) {
                         if (!project_particle<P2>(pos, a.labels, W, H, D, dx, inv)) break;
                     }
                 }
