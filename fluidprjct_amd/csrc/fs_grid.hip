@@ -21,128 +21,154 @@ FS_DEV float p_at(const float* __restrict__ p, long long idx, long long C) {
     return (idx < 0 || idx >= C) ? 0.0f : p[idx];
 }
 
-__global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs) {
-    const long long C = (long long)a.W * a.H * a.D;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= C) return;
-    if (a.labels[idx] != FS_FLUID) {
-        rhs[idx] = 0.0f;
-        return;
-    }
-    const int W = a.W, H = a.H;
-    int i, j, k;
-    split3(idx, W, H, i, j, k);
+// The RHS and projection kernels walk columns: thread (i, j) of a 64 x 4
+// block visits COL_KC cells along z and carries the label (and pressure) of
+// the cell below from the previous step.  No index division, and the loads
+// of one step are independent of each other.  Per cell and face the
+// operations and their order are those of one thread per element.
+constexpr int COL_TX = 64, COL_TY = 4;
+
+FS_DEV bool is_wall(int L) { return L == FS_SOLID || L == FS_BODY; }
+
+__global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs, int kc) {
+    const int W = a.W, H = a.H, D = a.D;
+    const int i = blockIdx.x * COL_TX + (int)(threadIdx.x % COL_TX);
+    const int j = blockIdx.y * COL_TY + (int)(threadIdx.x / COL_TX);
+    if (i >= W || j >= H) return;
+    const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, D);
+    const long long WH = (long long)W * H, C = WH * D;
     const float dx = a.dx;
     const float scale = (a.density * dx) / a.dt;
-    size_t ui0 = (size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H;
-    size_t vi0 = (size_t)i + (size_t)j * W + (size_t)k * W * (H + 1);
-    size_t wi0 = (size_t)i + (size_t)j * W + (size_t)k * W * H;
     const float *u = a.u, *v = a.v, *w = a.w;
-    float div = u[ui0 + 1] - u[ui0] + v[vi0 + W] - v[vi0] + w[wi0 + (size_t)W * H] - w[wi0];
-    float r = -scale * div;
-    if (i - 1 >= 0) {
-        int L = label_at(a.labels, idx - 1, C);
-        if (L == FS_SOLID || L == FS_BODY) {
-            float s = (L == FS_BODY) ? body_velocity(a.body, mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
-            r -= scale * (u[ui0] - s);
+    long long idx = i + (long long)j * W + (long long)k0 * WH;
+    int Lm = label_at(a.labels, idx - WH, C);   // cell (i, j, k - 1)
+    int Lc = label_at(a.labels, idx, C);
+    for (int k = k0; k < k1; ++k, idx += WH) {
+        const int Lp = label_at(a.labels, idx + WH, C);
+        if (Lc != FS_FLUID) {
+            rhs[idx] = 0.0f;
+        } else {
+            size_t ui0 = (size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H;
+            size_t vi0 = (size_t)i + (size_t)j * W + (size_t)k * W * (H + 1);
+            size_t wi0 = (size_t)idx;
+            float div = u[ui0 + 1] - u[ui0] + v[vi0 + W] - v[vi0] + w[wi0 + (size_t)WH] - w[wi0];
+            float r = -scale * div;
+            if (i - 1 >= 0) {
+                int L = label_at(a.labels, idx - 1, C);
+                if (is_wall(L)) {
+                    float s = (L == FS_BODY) ? body_velocity(a.body, mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
+                    r -= scale * (u[ui0] - s);
+                }
+            }
+            {
+                int L = label_at(a.labels, idx + 1, C);
+                if (is_wall(L)) {
+                    float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
+                    r += scale * (u[ui0 + 1] - s);
+                }
+            }
+            if (j - 1 >= 0) {
+                int L = label_at(a.labels, idx - W, C);
+                if (is_wall(L)) {
+                    float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx)).y : 0.0f;
+                    r -= scale * (v[vi0] - s);
+                }
+            }
+            {
+                int L = label_at(a.labels, idx + W, C);
+                if (is_wall(L)) {
+                    float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx)).y : 0.0f;
+                    r += scale * (v[vi0 + W] - s);
+                }
+            }
+            if (k - 1 >= 0 && is_wall(Lm)) {
+                float s = (Lm == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx)).z : 0.0f;
+                r -= scale * (w[wi0] - s);
+            }
+            if (is_wall(Lp)) {
+                float s = (Lp == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx)).z : 0.0f;
+                r += scale * (w[wi0 + (size_t)WH] - s);
+            }
+            rhs[idx] = r;
         }
+        Lm = Lc;
+        Lc = Lp;
     }
-    {
-        int L = label_at(a.labels, idx + 1, C);
-        if (L == FS_SOLID || L == FS_BODY) {
-            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
-            r += scale * (u[ui0 + 1] - s);
-        }
-    }
-    if (j - 1 >= 0) {
-        int L = label_at(a.labels, idx - W, C);
-        if (L == FS_SOLID || L == FS_BODY) {
-            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx)).y : 0.0f;
-            r -= scale * (v[vi0] - s);
-        }
-    }
-    {
-        int L = label_at(a.labels, idx + W, C);
-        if (L == FS_SOLID || L == FS_BODY) {
-            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx)).y : 0.0f;
-            r += scale * (v[vi0 + W] - s);
-        }
-    }
-    if (k - 1 >= 0) {
-        int L = label_at(a.labels, idx - (long long)W * H, C);
-        if (L == FS_SOLID || L == FS_BODY) {
-            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx)).z : 0.0f;
-            r -= scale * (w[wi0] - s);
-        }
-    }
-    {
-        int L = label_at(a.labels, idx + (long long)W * H, C);
-        if (L == FS_SOLID || L == FS_BODY) {
-            float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx)).z : 0.0f;
-            r += scale * (w[wi0 + (size_t)W * H] - s);
-        }
-    }
-    rhs[idx] = r;
 }
 
-// One launch over the concatenated u, v, w faces.
-__global__ __launch_bounds__(256) void k_project(GridArgs a, const float* __restrict__ p) {
-    const int W = a.W, H = a.H, D = a.D;
-    const long long C = (long long)W * H * D;
-    const long long NU = (long long)(W + 1) * H * D, NV = (long long)W * (H + 1) * D, NW = (long long)W * H * (D + 1);
-    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NU + NV + NW) return;
+// One face of component COMP at face index (fi, fj, fk) between the cells
+// with labels Ll, Lr and pressures pl, pr.
+template <int COMP>
+FS_DEV void project_face(const GridArgs& a, float* __restrict__ G, size_t idx, int Ll, float pl, int Lr, float pr,
+                         int fi, int fj, int fk, float scale, float invScale) {
     const float dx = a.dx;
-    int comp;
-    long long idx, left, right;
-    f3 fp;
-    float* G;
-    if (g < NU) {
-        comp = 0; idx = g; G = a.u;
-        int fi, fj, fk;
-        split3(idx, W + 1, H, fi, fj, fk);
-        left = (long long)(fi - 1) + (long long)fj * W + (long long)fk * W * H;
-        right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
-        fp = mk(fi * dx, (fj + 0.5f) * dx, (fk + 0.5f) * dx);
-    } else if (g < NU + NV) {
-        comp = 1; idx = g - NU; G = a.v;
-        int fi, fj, fk;
-        split3(idx, W, H + 1, fi, fj, fk);
-        left = (long long)fi + (long long)(fj - 1) * W + (long long)fk * W * H;
-        right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
-        fp = mk((fi + 0.5f) * dx, fj * dx, (fk + 0.5f) * dx);
-    } else {
-        comp = 2; idx = g - NU - NV; G = a.w;
-        int fi, fj, fk;
-        split3(idx, W, H, fi, fj, fk);
-        left = (long long)fi + (long long)fj * W + (long long)(fk - 1) * W * H;
-        right = (long long)fi + (long long)fj * W + (long long)fk * W * H;
-        fp = mk((fi + 0.5f) * dx, (fj + 0.5f) * dx, fk * dx);
-    }
-    const float scale = a.dt / (a.density * dx);
-    const float invScale = 1.0f / scale;
-    int Ll = label_at(a.labels, left, C), Lr = label_at(a.labels, right, C);
-    float gv = G[idx];
+    const float gv = G[idx];
     float p0, p1;
     if ((Ll == FS_FLUID || Ll == FS_AIR) && (Lr == FS_FLUID || Lr == FS_AIR)) {
-        p0 = p_at(p, left, C);
-        p1 = p_at(p, right, C);
-    } else if (Ll == FS_SOLID || Ll == FS_BODY) {
+        p0 = pl;
+        p1 = pr;
+    } else if (is_wall(Ll) || is_wall(Lr)) {
+        const int Lw = is_wall(Ll) ? Ll : Lr;
         float sol = 0.0f;
-        if (Ll == FS_BODY) { f3 vb = body_velocity(a.body, fp); sol = comp == 0 ? vb.x : comp == 1 ? vb.y : vb.z; }
-        p0 = p_at(p, right, C) - invScale * (gv - sol);
-        p1 = p_at(p, right, C);
-    } else if (Lr == FS_SOLID || Lr == FS_BODY) {
-        float sol = 0.0f;
-        if (Lr == FS_BODY) { f3 vb = body_velocity(a.body, fp); sol = comp == 0 ? vb.x : comp == 1 ? vb.y : vb.z; }
-        p0 = p_at(p, left, C);
-        p1 = p_at(p, left, C) + invScale * (gv - sol);
+        if (Lw == FS_BODY) {
+            const f3 fp = COMP == 0 ? mk(fi * dx, (fj + 0.5f) * dx, (fk + 0.5f) * dx)
+                        : COMP == 1 ? mk((fi + 0.5f) * dx, fj * dx, (fk + 0.5f) * dx)
+                                    : mk((fi + 0.5f) * dx, (fj + 0.5f) * dx, fk * dx);
+            const f3 vb = body_velocity(a.body, fp);
+            sol = COMP == 0 ? vb.x : COMP == 1 ? vb.y : vb.z;
+        }
+        if (is_wall(Ll)) {
+            p0 = pr - invScale * (gv - sol);
+            p1 = pr;
+        } else {
+            p0 = pl;
+            p1 = pl + invScale * (gv - sol);
+        }
     } else {
-        p0 = p_at(p, left, C);
-        p1 = p_at(p, right, C);
+        p0 = pl;
+        p1 = pr;
     }
     G[idx] = gv - scale * (p1 - p0);
 }
+
+// Faces (i, j, k) of u, v and w for every cell, plus the faces i = W, j = H
+// and k = D by the threads of the last column, row and plane.  The cells on
+// either side of a face are at linear indices c - 1 | c, c - W | c and
+// c - WH | c (wrapped past a row or slab end as the reference indexes them).
+__global__ __launch_bounds__(256) void k_project(GridArgs a, const float* __restrict__ p, int kc) {
+    const int W = a.W, H = a.H, D = a.D;
+    const int i = blockIdx.x * COL_TX + (int)(threadIdx.x % COL_TX);
+    const int j = blockIdx.y * COL_TY + (int)(threadIdx.x / COL_TX);
+    if (i >= W || j >= H) return;
+    const int k0 = blockIdx.z * kc, k1 = min(k0 + kc, D);
+    const long long WH = (long long)W * H, C = WH * D;
+    const float scale = a.dt / (a.density * a.dx);
+    const float invScale = 1.0f / scale;
+    long long c = i + (long long)j * W + (long long)k0 * WH;
+    int Lb = label_at(a.labels, c - WH, C);
+    float pb = p_at(p, c - WH, C);
+    for (int k = k0; k < k1; ++k, c += WH) {
+        const int Lc = label_at(a.labels, c, C), Lx = label_at(a.labels, c - 1, C), Ly = label_at(a.labels, c - W, C);
+        const float pc = p_at(p, c, C), px = p_at(p, c - 1, C), py = p_at(p, c - W, C);
+        const size_t ui = (size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H;
+        const size_t vi = (size_t)i + (size_t)j * W + (size_t)k * W * (H + 1);
+        project_face<0>(a, a.u, ui, Lx, px, Lc, pc, i, j, k, scale, invScale);
+        project_face<1>(a, a.v, vi, Ly, py, Lc, pc, i, j, k, scale, invScale);
+        project_face<2>(a, a.w, (size_t)c, Lb, pb, Lc, pc, i, j, k, scale, invScale);
+        if (i == W - 1)
+            project_face<0>(a, a.u, ui + 1, Lc, pc, label_at(a.labels, c + 1, C), p_at(p, c + 1, C), i + 1, j, k,
+                            scale, invScale);
+        if (j == H - 1)
+            project_face<1>(a, a.v, vi + W, Lc, pc, label_at(a.labels, c + W, C), p_at(p, c + W, C), i, j + 1, k,
+                            scale, invScale);
+        if (k == D - 1)
+            project_face<2>(a, a.w, (size_t)(c + WH), Lc, pc, label_at(a.labels, c + WH, C), p_at(p, c + WH, C), i,
+                            j, k + 1, scale, invScale);
+        Lb = Lc;
+        pb = pc;
+    }
+}
+
 
 // Drag and lift on BODY-FLUID faces.  The lift term is skipped when
 // |n x u_rel| == 0, where the reference divides by zero (:2774-2775).
@@ -244,14 +270,22 @@ __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __res
     if (threadIdx.x == 0 && t != 0.0) atomicAdd(&out[6], t);
 }
 
+// z cells per column thread: 16, or 4 where that leaves fewer than 2048 blocks
+static int column_chunk(int W, int H, int D) {
+    const long long cols = (long long)((W + COL_TX - 1) / COL_TX) * ((H + COL_TY - 1) / COL_TY);
+    return cols * ((D + 15) / 16) >= 2048 ? 16 : 4;
+}
+
 void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st) {
-    long long C = (long long)a.W * a.H * a.D;
-    hipLaunchKernelGGL(k_rhs, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, a, rhs);
+    const int kc = column_chunk(a.W, a.H, a.D);
+    dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + kc - 1) / kc);
+    hipLaunchKernelGGL(k_rhs, grid, dim3(256), 0, st, a, rhs, kc);
 }
 
 void launch_project(const GridArgs& a, const float* p, hipStream_t st) {
-    long long n = (long long)(a.W + 1) * a.H * a.D + (long long)a.W * (a.H + 1) * a.D + (long long)a.W * a.H * (a.D + 1);
-    hipLaunchKernelGGL(k_project, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, p);
+    const int kc = column_chunk(a.W, a.H, a.D);
+    dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + kc - 1) / kc);
+    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, st, a, p, kc);
 }
 
 void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P, double* out,

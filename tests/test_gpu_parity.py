@@ -100,16 +100,24 @@ def test_p2g_matches_oracle(hip, oracle):
         assert np.abs(a - b).max() <= 2e-6 * scale, (g, np.abs(a - b).max())
 
 
-def test_rhs_and_project_bitexact(hip, oracle):
+# one column block, and a grid that is ragged against the 64 x 4 column blocks
+# with more than one z chunk per column and BODY cells next to the faces
+@pytest.mark.parametrize("W,H,D,labels", [(16, 14, 12, (0, 1, 1, 1, 2)), (70, 9, 37, (0, 1, 1, 1, 2, 3))])
+def test_rhs_and_project_bitexact(hip, oracle, W, H, D, labels):
     rng = np.random.default_rng(3)
-    W, H, D = 16, 14, 12
-    lab = rng.choice(np.array([0, 1, 1, 1, 2], np.uint8), size=(D, H, W))
+    lab = rng.choice(np.array(labels, np.uint8), size=(D, H, W))
     h, o = pair(hip, oracle, dx=0.1, dt=0.02)
     shapes = {FS_GRID_U: (D, H, W + 1), FS_GRID_V: (D, H + 1, W), FS_GRID_W: (D + 1, H, W)}
     fields = {g: rng.standard_normal(s).astype(np.float32) for g, s in shapes.items()}
     p = (rng.standard_normal((D, H, W)) * 50).astype(np.float32) * (lab == 1)
     for s in (h, o):
         s.set_labels(lab)
+        if 3 in labels:   # a moving rigid body: BODY faces take V + Omega x r
+            st = s.body_state()
+            st.pos[:] = (0.5 * W * 0.1, 0.4 * H * 0.1, 0.6 * D * 0.1)
+            st.vel[:] = (0.3, -0.2, 0.1)
+            st.omega[:] = (0.5, 0.7, -0.4)
+            s.set_body_state(st)
         for g, a in fields.items():
             s.set_grid(g, a)
         s.stage("rhs")
