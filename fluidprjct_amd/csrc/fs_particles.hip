@@ -697,7 +697,7 @@ struct TileFields {
         Base bw = mac_base<2>(rx, ry, rz, a->W, a->H, a->D);
         // one branch for all 24 corners, so that their LDS reads issue together
         int iu, iv, iw;
-        const bool in = in_box<0>(bu, iu) & in_box<1>(bv, iv) & in_box<2>(bw, iw);
+        const bool in = (int)in_box<0>(bu, iu) + (int)in_box<1>(bv, iv) + (int)in_box<2>(bw, iw) == 3;
         if (in) return mk(lds_sample<0>(bu, iu), lds_sample<1>(bv, iv), lds_sample<2>(bw, iw));
         return mk(global_sample<0>(bu), global_sample<1>(bv), global_sample<2>(bw));
     }

@@ -125,17 +125,45 @@ FS_DEV void store_partial(double v, double* sm, double* __restrict__ partial) {
 
 // ---------------------------------------------------------------- level setup
 // fluid bounding box (min/max i, j, k of FLUID cells) -> box[0..5]
+// Fluid bounding box: per block [min i, max i, min j, max j, min k, max k]
+// into part[6 * block] (no global atomics: the host folds the blocks).  A
+// thread scans chunks of 16 consecutive cells (one 16-byte load when rows are
+// a multiple of 16 cells), splitting only a chunk's first index into (i, j, k)
+// and stepping along the rows from there.
+constexpr int BBOX_BLOCKS = 256;
+template <bool VEC16>
 __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels, int W, int H, long long n,
-                                              int* __restrict__ box) {
+                                              int* __restrict__ part) {
     __shared__ int sb[6];
     if (threadIdx.x < 6) sb[threadIdx.x] = (threadIdx.x & 1) ? -1 : 0x7fffffff;
     __syncthreads();
     int mn[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, mx[3] = {-1, -1, -1};
-    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
-        if (labels[c] != FS_FLUID) continue;
-        int v[3];
-        split3(c, W, H, v[0], v[1], v[2]);
-        for (int q = 0; q < 3; ++q) { mn[q] = min(mn[q], v[q]); mx[q] = max(mx[q], v[q]); }
+    for (long long c0 = 16 * ((long long)blockIdx.x * blockDim.x + threadIdx.x); c0 < n;
+         c0 += 16LL * gridDim.x * blockDim.x) {
+        uint8_t v[16];
+        if (VEC16) {
+            const uint4 q = *reinterpret_cast<const uint4*>(labels + c0);
+            const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = (uint8_t)(w4[t >> 2] >> (8 * (t & 3)));
+        } else {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) v[t] = c0 + t < n ? labels[c0 + t] : (uint8_t)FS_SOLID;
+        }
+        int i, j, k;
+        split3(c0, W, H, i, j, k);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            if (v[t] == FS_FLUID) {
+                mn[0] = min(mn[0], i); mx[0] = max(mx[0], i);
+                mn[1] = min(mn[1], j); mx[1] = max(mx[1], j);
+                mn[2] = min(mn[2], k); mx[2] = max(mx[2], k);
+            }
+            if (++i == W) {
+                i = 0;
+                if (++j == H) { j = 0; ++k; }
+            }
+        }
     }
     for (int q = 0; q < 3; ++q) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -147,10 +175,7 @@ __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels
         for (int q = 0; q < 3; ++q) { atomicMin(&sb[2 * q], mn[q]); atomicMax(&sb[2 * q + 1], mx[q]); }
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
-        if (threadIdx.x & 1) atomicMax(&box[threadIdx.x], sb[threadIdx.x]);
-        else atomicMin(&box[threadIdx.x], sb[threadIdx.x]);
-    }
+    if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sb[threadIdx.x];
 }
 
 // level-0 types of the solve box (offset ox, oy, oz in the full W x H grid)
@@ -1040,7 +1065,7 @@ struct TailLevels {
         asm volatile("" : "+v"(b[q_]), "+v"(c[q_]))
 
 // floats of LDS for a level below the top one: x, xt, b and the codes
-FS_HD inline int tail_level_floats(long long n) { return (int)((3 * n + (n + 1) / 2 + 3) & ~3LL); }
+FS_HD int tail_level_floats(long long n) { return (int)((3 * n + (n + 1) / 2 + 3) & ~3LL); }
 
 // Jacobi sweep on an LDS level: xout = xin + omega (b - A xin) / d
 FS_DEV void lds_sweep(const MGLevel& L, const float* xin, float* xout, const float* b, const uint16_t* code,
@@ -1593,9 +1618,9 @@ int MGPCG::init(int W_, int H_, int D_) {
     A((void**)&xb, sizeof(double) * N);
     A((void**)&partial, sizeof(double) * RED_BLOCKS * PS_COUNT);
     A((void**)&sc, sizeof(SolveScalars));
-    A((void**)&counters, sizeof(int) * 16);
+    A((void**)&counters, sizeof(int) * (COUNTERS_BBOX + 6 * BBOX_BLOCKS));
     if (hipHostMalloc((void**)&sc_host, sizeof(SolveScalars)) != hipSuccess) ok = false;
-    if (hipHostMalloc((void**)&host_counters, sizeof(int) * 16) != hipSuccess) ok = false;
+    if (hipHostMalloc((void**)&host_counters, sizeof(int) * (COUNTERS_BBOX + 6 * BBOX_BLOCKS)) != hipSuccess) ok = false;
     if (!ok) { free_all(); return -1; }
     // CG direction buffers start finite: the first update multiplies them by beta = 0
     hipMemset(pa, 0, sizeof(float) * N);
@@ -1864,13 +1889,20 @@ void MGPCG::clear_graphs() {
 int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
     // fluid bounding box, expanded by one cell so that every neighbour of a
     // FLUID cell (AIR/SOLID/BODY or out of domain) is inside the solve box
-    int init_box[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
-    hipMemcpyAsync(counters, init_box, sizeof(init_box), hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_bbox, dim3((unsigned)std::min<long long>(1024, (N + 255) / 256)), dim3(256), 0, st, labels, W,
-                       H, N, counters);
-    hipMemcpyAsync(host_counters, counters, sizeof(int) * 6, hipMemcpyDeviceToHost, st);
+    const int nbb = (int)std::min<long long>(BBOX_BLOCKS, (N + 16 * 256 - 1) / (16 * 256));
+    int* part = counters + COUNTERS_BBOX;
+    if (W % 16 == 0)
+        hipLaunchKernelGGL(k_bbox<true>, dim3(nbb), dim3(256), 0, st, labels, W, H, N, part);
+    else
+        hipLaunchKernelGGL(k_bbox<false>, dim3(nbb), dim3(256), 0, st, labels, W, H, N, part);
+    hipMemcpyAsync(host_counters + COUNTERS_BBOX, part, sizeof(int) * 6 * nbb, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
-    const int* hb = host_counters;
+    int hb[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
+    for (int blk = 0; blk < nbb; ++blk)
+        for (int q = 0; q < 3; ++q) {
+            hb[2 * q] = std::min(hb[2 * q], host_counters[COUNTERS_BBOX + 6 * blk + 2 * q]);
+            hb[2 * q + 1] = std::max(hb[2 * q + 1], host_counters[COUNTERS_BBOX + 6 * blk + 2 * q + 1]);
+        }
     if (hb[1] < 0) return -3;   // no fluid
     int lo[3], hi[3];
     const int dimsz[3] = {W, H, D};

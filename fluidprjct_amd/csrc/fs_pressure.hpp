@@ -106,7 +106,8 @@ public:
     float *r = nullptr, *z = nullptr, *pa = nullptr, *pb = nullptr, *q = nullptr;
     float* bb = nullptr;           // b on the solve box
     double* xb = nullptr;          // FP64 solution on the solve box
-    int* counters = nullptr;       // device: bbox [0..5], unknown count [8], Dirichlet flag [9]
+    static constexpr int COUNTERS_BBOX = 16;   // per-block bounding boxes from here on
+    int* counters = nullptr;       // device: unknown count [8], Dirichlet flag [9], per-block bbox [16..]
     int* host_counters = nullptr;  // pinned mirror
     double* partial = nullptr;     // partial-sum slots of the reducing kernels
     double* slot(int s);
