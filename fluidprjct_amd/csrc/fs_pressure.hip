@@ -71,9 +71,21 @@ FS_DEV double stencil_Ax64(const double* __restrict__ x, long long c, uint32_t c
 }
 
 // Sum of n (<= RED_BLOCKS) partials in a fixed order, result in every thread.
+// A thread's slots (threadIdx.x + u * blockDim.x) are all loaded before they
+// are added, in the order of u.
+constexpr int PART_UNROLL = RED_BLOCKS / 256;
 FS_DEV double sum_partials(const double* __restrict__ p, int n, double* sm) {
+    double v[PART_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u) {
+        const int i = threadIdx.x + u * blockDim.x;
+        v[u] = i < n ? p[i] : 0.0;
+    }
     double a = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) a += p[i];
+#pragma unroll
+    for (int u = 0; u < PART_UNROLL; ++u)
+        if ((int)threadIdx.x + u * (int)blockDim.x < n) a += v[u];
+    for (int i = threadIdx.x + PART_UNROLL * blockDim.x; i < n; i += blockDim.x) a += p[i];
     a = wave_sum(a);
     const int w = threadIdx.x >> 6;
     __syncthreads();
@@ -93,12 +105,24 @@ FS_DEV void sum_partials_k(const double* const (&p)[K], int n, double* sm, doubl
     double a[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) a[q] = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double v[K];
+    {
+        double v[K][PART_UNROLL];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = p[q][i];
+        for (int u = 0; u < PART_UNROLL; ++u) {
+            const int i = threadIdx.x + u * blockDim.x;
 #pragma unroll
-        for (int q = 0; q < K; ++q) a[q] += v[q];
+            for (int q = 0; q < K; ++q) v[q][u] = i < n ? p[q][i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PART_UNROLL; ++u) {
+            if ((int)threadIdx.x + u * (int)blockDim.x >= n) continue;
+#pragma unroll
+            for (int q = 0; q < K; ++q) a[q] += v[q][u];
+        }
+    }
+    for (int i = threadIdx.x + PART_UNROLL * blockDim.x; i < n; i += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) a[q] += p[q][i];
     }
     const int w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
 #pragma unroll
@@ -124,7 +148,6 @@ FS_DEV void store_partial(double v, double* sm, double* __restrict__ partial) {
 }
 
 // ---------------------------------------------------------------- level setup
-// fluid bounding box (min/max i, j, k of FLUID cells) -> box[0..5]
 // Fluid bounding box: per block [min i, max i, min j, max j, min k, max k]
 // into part[6 * block] (no global atomics: the host folds the blocks).  A
 // thread scans chunks of 16 consecutive cells (one 16-byte load when rows are
