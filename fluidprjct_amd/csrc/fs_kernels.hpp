@@ -7,18 +7,19 @@
 #include "fs_device.hpp"
 
 // G2P/advection LDS tile (cells) and staged margin; one 256-thread block per
-// tile.  32x2x2 measured faster than 32x4x4 (more, smaller blocks balance the
-// uneven particle counts of tiles; 5.4 -> 5.1 ms at 256^3, 0.62 -> 0.44 ms in
-// a developed 128^3 dam break); a 1-cell margin is faster at rest but slower
-// once particles cross tiles.
+// tile.  Stage ms at 256^3 / in a developed 128^3 dam break: 8x4x8 4.64 /
+// 0.28, 16x4x4 4.68 / 0.31, 8x8x4 4.69 / 0.28, 8x4x4 4.77 / 0.25, 32x2x2
+// 5.02 / 0.34, 32x4x4 5.08, 64x2x1 5.54.  An 8x4x8 tile stages 4.9 faces per
+// cell and component (tile + margin over tile), 32x2x2 10.4.  A 1-cell margin
+// is faster at rest but slower once particles cross tiles.
 #ifndef ADV_TX
-#define ADV_TX 32
+#define ADV_TX 8
 #endif
 #ifndef ADV_TY
-#define ADV_TY 2
+#define ADV_TY 4
 #endif
 #ifndef ADV_TZ
-#define ADV_TZ 2
+#define ADV_TZ 8
 #endif
 #ifndef ADV_MARGIN
 #define ADV_MARGIN 2
