@@ -112,6 +112,17 @@ struct HostBody {
         b.file_origin = file_origin;
         b.local_com = local_com;
         for (int i = 0; i < 9; ++i) b.R[i] = R[i];
+        // body_contains maps a world point w to local Rt (w - pos) + local_com
+        // and accepts it inside the SDF box; R is a rotation, so the box's
+        // circumsphere maps to a world sphere of the same radius
+        const double hx = 0.5 * cs * sw, hy = 0.5 * cs * sh, hz = 0.5 * cs * sd;
+        const double lx = file_origin.x + hx - local_com.x, ly = file_origin.y + hy - local_com.y,
+                     lz = file_origin.z + hz - local_com.z;
+        b.bcen = mk((float)(pos.x + R[0] * lx + R[1] * ly + R[2] * lz),
+                    (float)(pos.y + R[3] * lx + R[4] * ly + R[5] * lz),
+                    (float)(pos.z + R[6] * lx + R[7] * ly + R[8] * lz));
+        const double r = std::sqrt(hx * hx + hy * hy + hz * hz) + 2.0 * cs;
+        b.brad2 = (float)(r * r * 1.0001);
         return b;
     }
 };
@@ -138,7 +149,7 @@ struct Solver {
     float *d_u = nullptr, *d_v = nullptr, *d_w = nullptr, *d_us = nullptr, *d_vs = nullptr, *d_ws = nullptr;
     float *d_p = nullptr, *d_rhs = nullptr;
     double* d_p64 = nullptr;
-    int *d_count = nullptr, *d_start = nullptr, *d_bsum = nullptr, *d_total = nullptr, *d_nfluid = nullptr;
+    int *d_count = nullptr, *d_start = nullptr, *d_bsum = nullptr, *d_total = nullptr;
     // particles
     long long np = 0;
     ParticlesView PA{}, PB{};
@@ -178,7 +189,6 @@ struct Solver {
     void free_grids() {
         dfree(d_solid); dfree(d_labels); dfree(d_u); dfree(d_v); dfree(d_w); dfree(d_us); dfree(d_vs); dfree(d_ws);
         dfree(d_p); dfree(d_rhs); dfree(d_p64); dfree(d_count); dfree(d_start); dfree(d_bsum); dfree(d_total);
-        dfree(d_nfluid);
     }
     static void free_view(ParticlesView& P) {
         dfree(P.px); dfree(P.py); dfree(P.pz); dfree(P.vx); dfree(P.vy); dfree(P.vz); dfree(P.id);
@@ -211,7 +221,6 @@ struct Solver {
         HIPCHK(hipMalloc(&d_start, sizeof(int) * (C + 1)));
         HIPCHK(hipMalloc(&d_bsum, sizeof(int) * (scan_blocks(C) + 1)));
         HIPCHK(hipMalloc(&d_total, sizeof(int) * 2));
-        HIPCHK(hipMalloc(&d_nfluid, sizeof(int) * 2));
         HIPCHK(hipMemset(d_count, 0, sizeof(int) * (C + 4)));
         count_clean = true;
         HIPCHK(hipMemset(d_start, 0, sizeof(int) * (C + 1)));
@@ -278,7 +287,6 @@ struct Solver {
         la.dx = cfg.dx;
         la.body = body.params();
         la.sdf = d_sdf;
-        la.n_fluid = d_nfluid;
         return la;
     }
 
@@ -296,7 +304,6 @@ struct Solver {
     int stage_sort_label() {
         int rc = ensure_keys();
         if (rc) return rc;
-        HIPCHK(hipMemsetAsync(d_nfluid, 0, sizeof(int), st));
         LabelArgs la = label_args();
         launch_scan_label(d_count, d_solid, d_labels, C, d_bsum, d_total, d_start, la, st);
         return finish_sort();

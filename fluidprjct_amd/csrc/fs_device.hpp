@@ -111,7 +111,16 @@ struct BodyParams {
     f3 pos, vel, omega;
     f3 file_origin, local_com;
     float R[9];
+    // a world sphere that holds every point body_contains can accept (the
+    // SDF box's circumsphere plus two SDF cells), for a cheap early reject
+    f3 bcen;
+    float brad2;
 };
+
+FS_DEV bool body_sphere_has(const BodyParams& b, f3 wp) {
+    const float dx = wp.x - b.bcen.x, dy = wp.y - b.bcen.y, dz = wp.z - b.bcen.z;
+    return dx * dx + dy * dy + dz * dz <= b.brad2;
+}
 
 // Utility::contains utility.cu:163-206 (the reference passes fileOrigin as
 // sdf_origin, FluidSolver3D.cu:183,359).

@@ -57,7 +57,6 @@ struct LabelArgs {
     float dx;
     fsd::BodyParams body;
     const float* sdf;
-    int* n_fluid;
 };
 
 struct P2GArgs {

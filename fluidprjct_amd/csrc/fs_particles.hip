@@ -136,8 +136,8 @@ __global__ __launch_bounds__(1024) void k_scan_top(int* __restrict__ bsum, int n
 }
 
 // Final pass.  MODE 0 (cell counts): writes start[], labels (labelGrid :340-397
-// order: SOLID kept, BODY marked, then FLUID where particles are), clears the
-// count for the next step and accumulates the fluid-cell count.
+// order: SOLID kept, BODY marked, then FLUID where particles are) and clears
+// the count for the next step.
 // MODE 1 (fluid flags for seeding): writes start[] only.
 // MODE 2 (cell counts, re-sort without relabelling): start[] and cleared counts.
 template <int MODE>
@@ -153,7 +153,6 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, co
     int pre = block_excl_scan(s, sm, &tot) + bsum[blockIdx.x];
     int e[4] = {pre, pre + v.x, pre + v.x + v.y, pre + v.x + v.y + v.z};
     int vv[4] = {v.x, v.y, v.z, v.w};
-    int nfl = 0;
     for (int q = 0; q < 4; ++q) {
         long long c = c4 + q;
         if (c >= C) break;
@@ -168,12 +167,11 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, co
                     int i, j, k;
                     split3(c, la.W, la.H, i, j, k);
                     f3 cc = mk((i + 0.5f) * la.dx, (j + 0.5f) * la.dx, (k + 0.5f) * la.dx);
-                    if (body_contains(la.body, la.sdf, cc)) L = FS_BODY;
+                    if (body_sphere_has(la.body, cc) && body_contains(la.body, la.sdf, cc)) L = FS_BODY;
                 }
                 if (L == FS_AIR && vv[q] > 0) L = FS_FLUID;
             }
             labels[c] = L;
-            nfl += (L == FS_FLUID);
         }
     }
     if (MODE != 1) {
@@ -183,16 +181,6 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, co
             for (int q = 0; q < 4; ++q)
                 if (c4 + q < C) cnt[c4 + q] = 0;
         }
-    }
-    if (MODE == 0) {
-        // one atomic per block
-        __shared__ int s_nfl;
-        if (threadIdx.x == 0) s_nfl = 0;
-        __syncthreads();
-        nfl = wave_sum(nfl);
-        if ((threadIdx.x & 63) == 0 && nfl) atomicAdd(&s_nfl, nfl);
-        __syncthreads();
-        if (threadIdx.x == 0 && s_nfl) atomicAdd(la.n_fluid, s_nfl);
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[C] = bsum[blockIdx.x] + tot;
 }

@@ -229,21 +229,48 @@ __global__ __launch_bounds__(256) void k_gather_b(const float* __restrict__ bful
 }
 
 // full-grid pressure (FP64 and FP32, 0 off fluid) from the solve box
+// p = the box solution on FLUID cells, 0 elsewhere, FP64 and FP32, over the
+// full grid; 4 consecutive cells per thread (vector loads and stores when
+// the group is whole), one index split per group.
 __global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__ labels, long long n, int W, int H,
                                                      MGLevel L, int ox, int oy, int oz, const double* __restrict__ xb,
                                                      double* __restrict__ p64, float* __restrict__ p) {
-    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    const long long c0 = 4 * ((long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x);
+    if (c0 >= n) return;
     int i, j, k;
-    split3(c, W, H, i, j, k);
-    i = i - ox;
-    j = j - oy;
-    k = k - oz;
-    double v = 0.0;
-    if (labels[c] == FS_FLUID && i >= 0 && i < L.W && j >= 0 && j < L.H && k >= 0 && k < L.D)
-        v = xb[(size_t)i + (size_t)j * L.W + (size_t)k * L.W * L.H];
-    p64[c] = v;
-    p[c] = (float)v;
+    split3(c0, W, H, i, j, k);
+    const bool whole = c0 + 3 < n;
+    uint8_t lab[4];
+    if (whole) {
+        const uint32_t lw = *reinterpret_cast<const uint32_t*>(labels + c0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lab[q] = (uint8_t)(lw >> (8 * q));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lab[q] = c0 + q < n ? labels[c0 + q] : (uint8_t)FS_SOLID;
+    }
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int bi = i - ox, bj = j - oy, bk = k - oz;
+        v[q] = 0.0;
+        if (lab[q] == FS_FLUID && bi >= 0 && bi < L.W && bj >= 0 && bj < L.H && bk >= 0 && bk < L.D)
+            v[q] = xb[(size_t)bi + (size_t)bj * L.W + (size_t)bk * L.W * L.H];
+        if (++i == W) {
+            i = 0;
+            if (++j == H) { j = 0; ++k; }
+        }
+    }
+    if (whole) {
+        *reinterpret_cast<double2*>(p64 + c0) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2*>(p64 + c0 + 2) = make_double2(v[2], v[3]);
+        *reinterpret_cast<float4*>(p + c0) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    } else {
+        for (int q = 0; q < 4 && c0 + q < n; ++q) {
+            p64[c0 + q] = v[q];
+            p[c0 + q] = (float)v[q];
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
@@ -1964,7 +1991,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         box[0] = box[1] = box[2] = 0;
         configure(W, H, D);
         hipLaunchKernelGGL(k_fill0, dim3(nblk(N)), dim3(256), 0, st, N, xb);
-        hipLaunchKernelGGL(k_scatter_out, dim3(nblk(N)), dim3(256), 0, st, labels, N, W, H, lv[0], 0, 0, 0, xb, x64, p);
+        hipLaunchKernelGGL(k_scatter_out, dim3(nblk((N + 3) / 4)), dim3(256), 0, st, labels, N, W, H, lv[0], 0, 0, 0, xb, x64, p);
         if (stats) *stats = loc;
         return -3;
     }
@@ -2042,7 +2069,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             loc.fallback = 1;
         }
     }
-    hipLaunchKernelGGL(k_scatter_out, dim3(nblk(N)), dim3(256), 0, st, labels, N, W, H, L0, box[0], box[1], box[2], xb,
+    hipLaunchKernelGGL(k_scatter_out, dim3(nblk((N + 3) / 4)), dim3(256), 0, st, labels, N, W, H, L0, box[0], box[1], box[2], xb,
                        x64, p);
     last_iters = total_iters;
     last_outer = outer + 1;

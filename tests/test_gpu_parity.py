@@ -82,6 +82,36 @@ def test_keys_and_labels_bitexact(hip, oracle):
     assert np.array_equal(ph, pos) and np.array_equal(vh, vel)
 
 
+def test_body_labels_bitexact(hip, oracle):
+    """BODY cells of a rotated, off-centre box-shaped body (the SDF is not
+    symmetric, so the rotation and the local centre of mass both matter)
+    among FLUID and AIR cells: same labels as the oracle."""
+    rng = np.random.default_rng(8)
+    N = 40
+    dx = 1.0 / N
+    lab = np.full((N, N, N), 2, np.uint8)
+    lab[0], lab[:, 0], lab[:, :, 0] = 0, 0, 0
+    pos, vel = random_particles(rng, 60000, N, N, N, dx)
+    cs = 0.5 * dx
+    n = (36, 20, 28)                       # SDF voxels along z, y, x
+    z, y, x = np.meshgrid(*[(np.arange(m) + 0.5) * cs - 0.5 * m * cs for m in n], indexing="ij")
+    sdf = np.maximum(np.maximum(np.abs(x) - 0.28, np.abs(y) - 0.15), np.abs(z) - 0.37).astype(np.float32)
+    sdf -= np.float32(0.02) * (x > 0.1)    # off-centre mass
+    origin = [-0.5 * n[2] * cs, -0.5 * n[1] * cs, -0.5 * n[0] * cs]
+    h, o = pair(hip, oracle, dx=dx)
+    for s in (h, o):
+        s.set_labels(lab)
+        s.set_body_sdf(sdf, origin, cs, (0.47, 0.52, 0.55), 1.0)
+        st = s.body_state()
+        st.theta, st.psi, st.phi = 0.3, -0.5, 0.7
+        s.set_body_state(st)
+        s.set_particles(pos, vel)
+        s.stage("label")
+    lh, lo = h.get_grid(FS_GRID_LABELS), o.get_grid(FS_GRID_LABELS)
+    assert (lo == 3).sum() > 200 and (lo == 1).sum() > 1000
+    assert np.array_equal(lh, lo)
+
+
 def test_p2g_matches_oracle(hip, oracle):
     rng = np.random.default_rng(2)
     W, H, D = 37, 21, 18   # not multiples of the LDS tile
