@@ -161,6 +161,8 @@ struct Solver {
     HostBody body;
     float* d_sdf = nullptr;
     double* d_bacc = nullptr;
+    double* h_bacc = nullptr;   // pinned copy of d_bacc, read after body_done
+    hipEvent_t body_done = nullptr;
     MGPCG mg;
     std::unique_ptr<Preconditioner> jacobi;
     std::unique_ptr<LNet> net;             // learned preconditioner (created on first use)
@@ -183,6 +185,8 @@ struct Solver {
         dfree(d_sdf);
         dfree(d_bacc);
         dfree(d_umax);
+        if (h_bacc) hipHostFree(h_bacc);
+        if (body_done) hipEventDestroy(body_done);
         for (auto& e : ev) if (e) hipEventDestroy(e);
         if (st) hipStreamDestroy(st);
     }
@@ -393,14 +397,23 @@ struct Solver {
     }
 
     // updateBody FluidSolver3D.cu:2793-2846 with drag/lift and buoyancy reduced on the GPU
-    int stage_body() {
+    // The force sums are copied to pinned memory; the integration waits for
+    // that copy only, so the step can launch G2P (which does not read the
+    // body) before it.
+    int stage_body_launch() {
         if (!body.active) return FS_OK;
         if (!sorted_valid) return fail(FS_ERR_STATE, "body stage needs the label/sort stage of this step");
         HIPCHK(hipMemsetAsync(d_bacc, 0, sizeof(double) * 8, st));
         launch_body_forces(grid_args(), d_sdf, d_start, PA, d_bacc, st);
-        double acc[8];
-        HIPCHK(hipMemcpyAsync(acc, d_bacc, sizeof(acc), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpyAsync(h_bacc, d_bacc, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(body_done, st));
+        return FS_OK;
+    }
+
+    int stage_body_finish() {
+        if (!body.active) return FS_OK;
+        HIPCHK(hipEventSynchronize(body_done));
+        const double* acc = h_bacc;
         const f3 G = mk(0.0f, cfg.gravity_y, 0.0f);
         f3 totalF = mk((float)acc[0], (float)acc[1], (float)acc[2]);
         f3 totalT = mk((float)acc[3], (float)acc[4], (float)acc[5]);
@@ -415,6 +428,11 @@ struct Solver {
         body.last_sub = sub;
         body.integrate(dt_cur);
         return FS_OK;
+    }
+
+    int stage_body() {
+        int rc = stage_body_launch();
+        return rc ? rc : stage_body_finish();
     }
 
     int stage_g2p_advect() {
@@ -461,10 +479,11 @@ struct Solver {
         HIPCHK(hipEventRecord(ev[3], st));
         stage_project();
         HIPCHK(hipEventRecord(ev[4], st));
-        if ((rc = stage_body())) return rc;
+        if ((rc = stage_body_launch())) return rc;
         HIPCHK(hipEventRecord(ev[5], st));
         if ((rc = stage_g2p_advect())) return rc;
         HIPCHK(hipEventRecord(ev[6], st));
+        if ((rc = stage_body_finish())) return rc;
         HIPCHK(hipEventSynchronize(ev[6]));
         float ms;
         const int slot[6] = {FS_STAGE_LABEL, FS_STAGE_P2G, FS_STAGE_PRESSURE, FS_STAGE_PROJECT, FS_STAGE_BODY,
@@ -693,6 +712,8 @@ int fs_create(const fs_config* cfg, void** out) {
     HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipMalloc(&s->d_bacc, sizeof(double) * 8));
+    HIPCHK(hipHostMalloc((void**)&s->h_bacc, sizeof(double) * 8));
+    HIPCHK(hipEventCreateWithFlags(&s->body_done, hipEventDisableTiming));
     HIPCHK(hipMalloc(&s->d_umax, sizeof(unsigned)));
     s->dt_cur = cfg->dt;
     *out = s.release();

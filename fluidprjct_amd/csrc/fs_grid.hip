@@ -272,18 +272,16 @@ __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __res
 
 // z cells per column thread (4 measured faster than 1, 2, 8 and 16 at 256^3:
 // longer columns serialise more dependent loads per thread)
-static int column_chunk(int, int, int) { return 4; }
+constexpr int COL_KC = 4;
 
 void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st) {
-    const int kc = column_chunk(a.W, a.H, a.D);
-    dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + kc - 1) / kc);
-    hipLaunchKernelGGL(k_rhs, grid, dim3(256), 0, st, a, rhs, kc);
+    dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + COL_KC - 1) / COL_KC);
+    hipLaunchKernelGGL(k_rhs, grid, dim3(256), 0, st, a, rhs, COL_KC);
 }
 
 void launch_project(const GridArgs& a, const float* p, hipStream_t st) {
-    const int kc = column_chunk(a.W, a.H, a.D);
-    dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + kc - 1) / kc);
-    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, st, a, p, kc);
+    dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + COL_KC - 1) / COL_KC);
+    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, st, a, p, COL_KC);
 }
 
 void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P, double* out,
