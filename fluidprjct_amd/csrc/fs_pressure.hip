@@ -1759,7 +1759,10 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     const int extra = (npre >= 2 ? npre - 2 : 0) + npost;
     float* cur = (extra & 1) ? L.xt : L.x;
     float* oth = (cur == L.x) ? L.xt : L.x;
-    const bool v4 = (l == 0) && vec0;   // 4 cells per thread on the fine level
+    // 4 cells per thread on every level this path handles whose rows are a
+    // multiple of 4 cells (the fine level when the box allows, the next ones
+    // above the blocked-smoothing size)
+    const bool v4 = use_vec && L.W % 4 == 0;
     const unsigned nb4 = nblk(L.n / 4);
     if (v4)
         hipLaunchKernelGGL(k_jacobi2_v4, dim3(nb4), dim3(256), 0, st, L, L.b, cur, sweep_omega[0],
