@@ -650,7 +650,11 @@ struct TileFields {
 
     FS_DEV f3 interp(f3 pos) const {
         const float dx = a->dx, inv = a->inv_dx;
-        float rx = div_dx<P2>(pos.x, dx, inv), ry = div_dx<P2>(pos.y, dx, inv), rz = div_dx<P2>(pos.z, dx, inv);
+        return interp_r(div_dx<P2>(pos.x, dx, inv), div_dx<P2>(pos.y, dx, inv), div_dx<P2>(pos.z, dx, inv));
+    }
+
+    // the same sample from r = pos / dx per axis
+    FS_DEV f3 interp_r(float rx, float ry, float rz) const {
         // Fast path: floor(r) in [lo0 + 1, hi - 2] on every axis, lo0 = max(X0, 0)
         // and hi = min(X0 + BX, W).  Then both bases per axis, floor(r) and
         // floor(r - 1/2) (which is floor(r) or floor(r) - 1), lie in the staged
@@ -817,11 +821,14 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
         if (act) {
             p = particle_of(t);
             f3 pos = mk(a.P.px[p], a.P.py[p], a.P.pz[p]);
+            // pos / dx, carried from the cell index of one substep to the first
+            // sample of the next
+            float rx = div_dx<P2>(pos.x, dx, inv), ry = div_dx<P2>(pos.y, dx, inv), rz = div_dx<P2>(pos.z, dx, inv);
             float subT = 0.0f;
             bool finished = false;
             int iter = 0;
             while (!finished && iter++ < 1000) {
-                f3 k1 = F.interp(pos);
+                f3 k1 = F.interp_r(rx, ry, rz);
                 float speed = sqrtf(k1.x * k1.x + k1.y * k1.y + k1.z * k1.z) + 1e-37f + vfloor;
                 float dT = (C * dx) / speed;
                 if (subT + dT >= dt) {
@@ -839,8 +846,8 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
                 pos.z = fmaxf(pos.z, 0.0f); pos.z = fminf(pos.z, zmax);
                 // the clamps leave pos finite and >= 0 (fmaxf returns the non-NaN
                 // operand), so truncation is the floor
-                int cx = (int)div_dx<P2>(pos.x, dx, inv), cy = (int)div_dx<P2>(pos.y, dx, inv),
-                    cz = (int)div_dx<P2>(pos.z, dx, inv);
+                rx = div_dx<P2>(pos.x, dx, inv); ry = div_dx<P2>(pos.y, dx, inv); rz = div_dx<P2>(pos.z, dx, inv);
+                const int cx = (int)rx, cy = (int)ry, cz = (int)rz;
                 // one predicate and one branch (the rare projection) per substep
                 const bool inside = ((unsigned)cx < (unsigned)W) & ((unsigned)cy < (unsigned)H) &
                                     ((unsigned)cz < (unsigned)D);
@@ -849,6 +856,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
                     const bool blocked = (((1u << (L & 3u)) & ((1u << FS_SOLID) | (1u << FS_BODY))) != 0) & (L < 4);
                     if (blocked) {
                         if (!project_particle<P2>(pos, a.labels, W, H, D, dx, inv)) break;
+                        rx = div_dx<P2>(pos.x, dx, inv); ry = div_dx<P2>(pos.y, dx, inv); rz = div_dx<P2>(pos.z, dx, inv);
                     }
                 }
                 if (pos.x < 0.0f || pos.y < 0.0f || pos.z < 0.0f || isnan(pos.x) || isnan(pos.y) || isnan(pos.z))
