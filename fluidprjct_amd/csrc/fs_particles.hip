@@ -853,6 +853,7 @@ __global__ __launch_bounds__(256) void k_g2p_advect_tile(G2PArgs a, const int* _
                                     ((unsigned)cz < (unsigned)D);
                 if (inside) {
                     const uint8_t L = F.label(cx, cy, cz);
+                    // L is SOLID or BODY, as a bit test rather than two branches
                     const bool blocked = (((1u << (L & 3u)) & ((1u << FS_SOLID) | (1u << FS_BODY))) != 0) & (L < 4);
                     if (blocked) {
                         if (!project_particle<P2>(pos, a.labels, W, H, D, dx, inv)) break;
