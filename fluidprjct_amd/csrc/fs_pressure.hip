@@ -202,8 +202,10 @@ __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels
 }
 
 // level-0 types of the solve box (offset ox, oy, oz in the full W x H grid)
-__global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ labels, MGLevel L, int W, int H, int ox,
-                                               int oy, int oz) {
+// and the right-hand side gathered into the box
+__global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ labels, const float* __restrict__ bfull,
+                                               MGLevel L, int W, int H, int ox, int oy, int oz,
+                                               float* __restrict__ bbox) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= L.n) return;
     int i, j, k;
@@ -211,21 +213,10 @@ __global__ __launch_bounds__(256) void k_type0(const uint8_t* __restrict__ label
     i = i + ox;
     j = j + oy;
     k = k + oz;
-    uint8_t lab = labels[(size_t)i + (size_t)j * W + (size_t)k * W * H];
+    const size_t g = (size_t)i + (size_t)j * W + (size_t)k * W * H;
+    const uint8_t lab = labels[g];
     L.type[c] = lab == FS_FLUID ? 1 : (lab == FS_AIR ? 2 : 0);
-}
-
-// b of the solve box from the full-grid right-hand side
-__global__ __launch_bounds__(256) void k_gather_b(const float* __restrict__ bfull, MGLevel L, int W, int H, int ox,
-                                                  int oy, int oz, float* __restrict__ bbox) {
-    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (c >= L.n) return;
-    int i, j, k;
-    split3(c, L.W, L.H, i, j, k);
-    i = i + ox;
-    j = j + oy;
-    k = k + oz;
-    bbox[c] = bfull[(size_t)i + (size_t)j * W + (size_t)k * W * H];
+    bbox[c] = bfull[g];
 }
 
 // full-grid pressure (FP64 and FP32, 0 off fluid) from the solve box
@@ -1545,9 +1536,9 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
 }
 
 // r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partials of r.r (FP64)
-// ZERO_X: x64 is known to be zero (the first pass), so r = b - shift
+// ZERO_X: the first pass, which also sets x64 = 0, so r = b - shift
 template <bool ZERO_X>
-__global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, const double* __restrict__ x64,
+__global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, double* __restrict__ x64,
                                                  double shift, float* __restrict__ r, double* __restrict__ partial) {
     __shared__ double sm[12];
     const long long WH = (long long)L.W * L.H;
@@ -1563,6 +1554,7 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
             rf = (float)rv;
         }
         r[c] = rf;
+        if (ZERO_X) x64[c] = 0.0;
     }
     store_partial(acc, sm, partial);
 }
@@ -1695,8 +1687,8 @@ void MGPCG::configure(int Wb, int Hb, int Db) {
 
 double* MGPCG::slot(int s) { return partial + (size_t)s * RED_BLOCKS; }
 
-void MGPCG::setup(const uint8_t* labels, hipStream_t st) {
-    hipLaunchKernelGGL(k_type0, dim3(nblk(n0)), dim3(256), 0, st, labels, lv[0], W, H, box[0], box[1], box[2]);
+void MGPCG::setup(const uint8_t* labels, const float* b, hipStream_t st) {
+    hipLaunchKernelGGL(k_type0, dim3(nblk(n0)), dim3(256), 0, st, labels, b, lv[0], W, H, box[0], box[1], box[2], bb);
     for (int l = 0; l + 1 < nlev; ++l)
         hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
     for (int l = 0; l < nlev; ++l)
@@ -1977,8 +1969,7 @@ int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
     box[0] = lo[0]; box[1] = lo[1]; box[2] = lo[2];
     configure(hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1);
     hipMemsetAsync(counters + 8, 0, sizeof(int) * 4, st);
-    setup(labels, st);
-    hipLaunchKernelGGL(k_gather_b, dim3(nblk(n0)), dim3(256), 0, st, b, lv[0], W, H, box[0], box[1], box[2], bb);
+    setup(labels, b, st);
     return 0;
 }
 
@@ -2015,7 +2006,6 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     loc.singular = singular ? 1 : 0;
     n_unk = nunk;
     hipMemcpyAsync(&sc->n_unknown, counters + 8, sizeof(int), hipMemcpyDeviceToDevice, st);
-    hipLaunchKernelGGL(k_fill0, dim3(nblk(n0)), dim3(256), 0, st, n0, xb);
     const double shift = singular ? sc_host->bsum / nunk : 0.0;
     const double bn2 =
         singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
@@ -2025,10 +2015,10 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     double rr_true = 0.0;
     for (outer = 0;; ++outer) {
         if (outer == 0)
-            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, (const double*)xb, shift, r,
+            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
                                slot(PS_RES));
         else
-            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, (const double*)xb, shift, r,
+            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
                                slot(PS_RES));
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
                            (const double*)nullptr, nred, sc, tol2b);

@@ -69,8 +69,8 @@ public:
     int prepare(const uint8_t* labels, const float* b, hipStream_t st);
     // level dimensions for a solve box of Wb x Hb x Db cells
     void configure(int Wb, int Hb, int Db);
-    // level codes of the configured box
-    void setup(const uint8_t* labels, hipStream_t st);
+    // level codes of the configured box and its right-hand side (from b)
+    void setup(const uint8_t* labels, const float* b, hipStream_t st);
     MGLevel fine() const;
     // One V-cycle z = M r on level 0, accumulating r.z partial sums.
     void vcycle(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st);
