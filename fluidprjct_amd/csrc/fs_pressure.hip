@@ -1700,8 +1700,9 @@ MGLevel MGPCG::fine() const { return lv[0]; }
 
 float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s) {
     MGLevel& L = lv[l];
-    // small levels (not the fine one, whose sweep accumulates r.z): one workgroup
-    if (use_tail && l > 0 && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
+    // small levels: one workgroup (a fine level that small, as in a thin slab,
+    // too; its r.z partial sums then come from a separate pass)
+    if (use_tail && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
         TailLevels t{};
         t.n = nlev - l;
         for (int q = 0; q < SMOOTH_STEPS; ++q) t.w[q] = sweep_omega[q];
@@ -1715,6 +1716,8 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         if (off <= TAIL_LDS_FLOATS) {
             hipLaunchKernelGGL(k_tail, dim3(1), dim3(TAIL_THREADS), 0, st, t, std::max(nu_pre, 1),
                                std::max(nu_post, 1), coarse_sweeps, omega, (const SolveScalars*)s);
+            if (l == 0 && pdot)
+                hipLaunchKernelGGL(k_dot_rz, dim3(nred), dim3(256), 0, st, L.n, L.b, L.x, pdot, s);
             return L.x;
         }
     }
