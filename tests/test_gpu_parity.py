@@ -189,6 +189,36 @@ def test_g2p_advect_bitexact(hip, oracle, speed, dx):
     assert bad.size == 0, (bad[:10], ph[bad[:3]], po[bad[:3]])
 
 
+# a grid ragged against the G2P tiles on every axis, with BODY cells in the
+# obstacle (the projection search treats them as solid)
+@pytest.mark.parametrize("dx", [1.0 / 40, 1.0 / 32])
+def test_g2p_advect_bitexact_ragged_body(hip, oracle, dx):
+    rng = np.random.default_rng(12)
+    D, H, W = 29, 23, 37
+    lab = tank(0, block=(0.6, 0.5, 1.0), shape=(D, H, W))
+    lab[10:16, 4:9, 12:20] = 3
+    lab[12:14, 5:7, 14:16] = 0
+    pos, vel = random_particles(rng, 50000, W - 2, H - 2, D - 2, dx)
+    pos += dx
+    h, o = pair(hip, oracle, dx=dx, dt=1.0 / 60, pic_weight=0.05)
+    shapes = {FS_GRID_U: (D, H, W + 1), FS_GRID_V: (D, H + 1, W), FS_GRID_W: (D + 1, H, W)}
+    f = {g: (rng.standard_normal(s) * 3.0).astype(np.float32) for g, s in shapes.items()}
+    fs = {g: (a + rng.standard_normal(a.shape).astype(np.float32) * 0.1) for g, a in f.items()}
+    saved = {FS_GRID_U: FS_GRID_USAVED, FS_GRID_V: FS_GRID_VSAVED, FS_GRID_W: FS_GRID_WSAVED}
+    for s in (h, o):
+        s.set_labels(lab)
+        s.set_particles(pos, vel)
+        for g in shapes:
+            s.set_grid(g, f[g])
+            s.set_grid(saved[g], fs[g])
+        s.stage("g2p_advect")
+    ph, vh = h.particles()
+    po, vo = o.particles()
+    assert np.array_equal(vh.view(np.uint32), vo.view(np.uint32))
+    bad = np.flatnonzero((ph.view(np.uint32) != po.view(np.uint32)).any(1))
+    assert bad.size == 0, (bad[:10], ph[bad[:3]], po[bad[:3]])
+
+
 @pytest.mark.parametrize("name", ["dam", "mix"])
 def test_pressure_vs_golden(hip, name):
     g = np.load(os.path.join(GOLDEN, "pressure_golden.npz"))
