@@ -402,7 +402,11 @@ struct Solver {
     // body) before it.
     int stage_body_launch() {
         if (!body.active) return FS_OK;
-        if (!sorted_valid) return fail(FS_ERR_STATE, "body stage needs the label/sort stage of this step");
+        if (!sorted_valid) {
+            // buoyancy looks particles up by cell: sort them (labels unchanged)
+            int rc = sort_only();
+            if (rc) return rc;
+        }
         HIPCHK(hipMemsetAsync(d_bacc, 0, sizeof(double) * 8, st));
         launch_body_forces(grid_args(), d_sdf, d_start, PA, d_bacc, st);
         HIPCHK(hipMemcpyAsync(h_bacc, d_bacc, sizeof(double) * 8, hipMemcpyDeviceToHost, st));

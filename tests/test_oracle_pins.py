@@ -9,6 +9,7 @@ import pytest
 from fluidprjct_amd._abi import (FS_GRID_LABELS, FS_GRID_P, FS_GRID_P64, FS_GRID_RHS, FS_GRID_U, FS_GRID_USAVED,
                                  FS_GRID_V, FS_GRID_VSAVED, FS_GRID_W, FS_GRID_WSAVED)
 from fluidprjct_amd import scenes
+from body_cases import DIRS as _DIRS, one_face_body
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 INPUTS = os.path.join(GOLDEN, "inputdata")
@@ -413,3 +414,171 @@ def test_body_free_fall_in_air(oracle):
     assert st.vel[1] == pytest.approx(-3 * 9.81 * dt, rel=1e-5)
     assert st.pos[1] == pytest.approx(0.7 - 9.81 * dt * dt * 6, rel=1e-5)
     assert st.submerged_volume == 0.0
+
+
+# ---- drag / lift on one BODY-FLUID face (FluidSolver3D.cu:2674-2791) and the
+# ---- rigid-body integrator (RigidBody::integrate utility.cuh:408-484)
+def face_force_expected(n, u_rel, rho=1000.0, dx=0.125, cd=0.1, cl=0.1):
+    """Drag -k|u|u and lift k_l |u| (u (u.n) - n |u|^2) / |n x u| with
+    k = rho C A_eff / 2, A_eff = (n.u/|u| / 2 + 1/2) dx^2 when n.u > 0
+    (the lift direction u x (n x u) / |n x u| written with the BAC-CAB
+    identity).  FP64."""
+    n, u = np.asarray(n, float), np.asarray(u_rel, float)
+    ua = np.linalg.norm(u)
+    nu = float(n @ u)
+    if nu <= 0:
+        return np.zeros(3)
+    area = (0.5 * nu / ua + 0.5) * dx * dx
+    f = -0.5 * rho * cd * area * ua * u
+    nxu = np.linalg.norm(np.cross(n, u))
+    if nxu > 0:
+        f += 0.5 * rho * cl * area * ua * (u * nu - n * ua * ua) / nxu
+    return f
+
+
+@pytest.mark.parametrize("face,urel", [("+x", (2.0, 0.0, 0.0)),      # along n: drag only, lift guard
+                                        ("+x", (0.0, 1.5, 0.0)),      # tangential: n.u = 0, no force
+                                        ("+x", (-1.0, 0.5, 0.0)),     # away from the fluid: no force
+                                        ("+x", (1.0, 1.0, 0.0)),      # 45 degrees in the xy plane
+                                        ("-z", (0.3, -0.4, -1.2)),    # general direction
+                                        ("+y", (-0.7, 0.9, 0.2))])
+def test_drag_lift_one_face_closed_form(oracle, face, urel):
+    U = np.array([0.25, -0.5, 0.75])
+    V = U + np.array(urel)
+    dx, mass = 0.125, 2.0
+    cc = (np.array([2, 2, 2]) + 0.5) * dx
+    pos = cc + np.array([0.03, -0.05, 0.02])
+    s = one_face_body(oracle, [face], U, V, pos, dx=dx, mass=mass)
+    s.stage("body")
+    st = s.body_state()
+    n = np.array(_DIRS[face], float)
+    f = face_force_expected(n, urel, dx=dx)
+    r = cc + 0.5 * dx * n - np.asarray(pos, np.float32).astype(float)
+    # the force is m g + F_fluid summed in FP32: allow the rounding of that sum
+    mg = np.float32(mass) * np.float32(-9.81)
+    fluid_force = np.array(st.force, float) - np.array([0.0, mg, 0.0])
+    tol = 2e-5 * np.linalg.norm(f) + 2e-7 * abs(mg)
+    assert np.abs(fluid_force - f).max() <= tol, (fluid_force, f)
+    assert np.abs(np.array(st.torque) - np.cross(r, f)).max() <= 2e-5 * np.linalg.norm(f)
+    ua = np.array(urel)
+    if np.linalg.norm(f) > 0:
+        drag = -0.5 * 1000.0 * 0.1 * (0.5 * (n @ ua) / np.linalg.norm(ua) + 0.5) * dx * dx * np.linalg.norm(ua) * ua
+        lift = f - drag
+        # lift is normal to the relative velocity and lies in the plane of n and u_rel
+        assert abs(lift @ ua) <= 1e-9 * np.linalg.norm(f) * np.linalg.norm(ua)
+        assert abs(lift @ np.cross(n, ua)) <= 1e-9 * np.linalg.norm(f)
+        # drag opposes the body's motion relative to the fluid
+        assert drag @ ua < 0
+
+
+def test_drag_lift_faces_add(oracle):
+    """Two FLUID neighbours: the force and torque are the sums of the two
+    one-face cases."""
+    U, dx, mass = np.array([0.1, 0.2, -0.3]), 0.125, 2.0
+    urel = np.array([0.8, 0.6, -0.4])
+    cc = (np.array([2, 2, 2]) + 0.5) * dx
+    pos = cc + np.array([-0.01, 0.04, 0.03])
+    s = one_face_body(oracle, ["+x", "-z"], U, U + urel, pos, dx=dx, mass=mass)
+    s.stage("body")
+    st = s.body_state()
+    posf = np.asarray(pos, np.float32).astype(float)
+    fe, te = np.zeros(3), np.zeros(3)
+    for face in ("+x", "-z"):
+        n = np.array(_DIRS[face], float)
+        f = face_force_expected(n, urel, dx=dx)
+        fe += f
+        te += np.cross(cc + 0.5 * dx * n - posf, f)
+    mg = np.float32(mass) * np.float32(-9.81)
+    got = np.array(st.force, float) - np.array([0.0, mg, 0.0])
+    assert np.abs(got - fe).max() <= 2e-5 * np.linalg.norm(fe) + 2e-7 * abs(mg)
+    assert np.abs(np.array(st.torque) - te).max() <= 2e-5 * np.linalg.norm(fe)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_integrator_constant_torque(oracle, axis):
+    """A constant torque about one axis (the body's velocity and position reset
+    before every step, so the drag that makes it stays the same): semi-implicit
+    Euler gives omega_n = n (tau / I) dt and the angle about that axis
+    (phi about x, theta about y, psi about z) = (tau / I) dt^2 n (n + 1) / 2,
+    the other two angles stay 0, and the rotation matrix of the angles is the
+    textbook rotation about that axis.  I = 0.4 m (w cs / 2)^2."""
+    dx, dt, mass, steps = 0.125, 0.01, 2.0, 12
+    a = 1.5
+    # (FLUID face, relative velocity along it, offset of the body centre from the face centre)
+    face, urel, off = [("+y", (0.0, a, 0.0), (0.0, 0.0, -0.04)),
+                       ("+x", (a, 0.0, 0.0), (0.0, 0.0, -0.04)),
+                       ("+x", (a, 0.0, 0.0), (0.0, -0.04, 0.0))][axis]
+    n = np.array(_DIRS[face], float)
+    cc = (np.array([2, 2, 2]) + 0.5) * dx
+    pos = np.asarray(cc + 0.5 * dx * n + np.array(off), np.float32)
+    U = np.zeros(3)
+    s = one_face_body(oracle, [face], U, urel, pos, dx=dx, dt=dt, mass=mass)
+    f = face_force_expected(n, urel, dx=dx)
+    tau = np.cross(cc + 0.5 * dx * n - pos.astype(float), f)
+    assert np.count_nonzero(np.abs(tau) > 1e-12) == 1 and abs(tau[axis]) > 0
+    inertia = 0.4 * mass * (4 * 0.05 / 2) ** 2
+    for _ in range(steps):
+        st = s.body_state()
+        st.pos[:] = pos
+        st.vel[:] = urel
+        s.set_body_state(st)
+        s.stage("body")
+    st = s.body_state()
+    alpha = tau[axis] / inertia
+    om = np.array(st.omega)
+    assert om[axis] == pytest.approx(steps * alpha * dt, rel=1e-5)
+    assert np.count_nonzero(om) == 1
+    ang = {0: st.phi, 1: st.theta, 2: st.psi}
+    want = alpha * dt * dt * steps * (steps + 1) / 2
+    assert ang[axis] == pytest.approx(want, rel=1e-5)
+    assert [ang[q] for q in range(3) if q != axis] == [0.0, 0.0]
+    s.set_body_state(st)                 # rotation matrix of the current angles
+    R = np.array(s.body_state().rotation, float).reshape(3, 3)
+    c, sn = np.cos(want), np.sin(want)
+    Rt = {0: [[1, 0, 0], [0, c, -sn], [0, sn, c]],
+          1: [[c, 0, sn], [0, 1, 0], [-sn, 0, c]],
+          2: [[c, -sn, 0], [sn, c, 0], [0, 0, 1]]}[axis]
+    assert np.abs(R - np.array(Rt)).max() <= 1e-6
+
+
+def test_rotation_matrix_is_a_rotation(oracle):
+    """RigidBody::update_rotation_matrix (utility.cuh:470-484) of three nonzero
+    angles is orthonormal with determinant 1 and equals Ry(theta) Rz(psi)
+    Rx(phi)."""
+    s = one_face_body(oracle, ["+x"], (0, 0, 0), (0, 0, 0), (0.3, 0.3, 0.3))
+    st = s.body_state()
+    st.theta, st.psi, st.phi = 0.4, -0.9, 1.3
+    s.set_body_state(st)
+    R = np.array(s.body_state().rotation, float).reshape(3, 3)
+    assert np.abs(R @ R.T - np.eye(3)).max() <= 1e-6 and np.linalg.det(R) == pytest.approx(1.0, abs=1e-6)
+    th, ps, ph = np.float32(0.4), np.float32(-0.9), np.float32(1.3)
+    Ry = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+    Rz = np.array([[np.cos(ps), -np.sin(ps), 0], [np.sin(ps), np.cos(ps), 0], [0, 0, 1]])
+    Rx = np.array([[1, 0, 0], [0, np.cos(ph), -np.sin(ph)], [0, np.sin(ph), np.cos(ph)]])
+    assert np.abs(R - Ry @ Rz @ Rx).max() <= 1e-6
+
+
+def test_integrator_torque_in_body_frame(oracle):
+    """The torque enters the angular velocity in the body frame, omega +=
+    (dt / I) R^T tau, with R = Ry(theta) Rz(psi) Rx(phi) mapping body to world:
+    the transform the label marking applies to world points
+    (Utility::contains utility.cu:163-206).  One step from a rotated state."""
+    dx, dt, mass = 0.125, 0.01, 2.0
+    cc = (np.array([2, 2, 2]) + 0.5) * dx
+    pos = np.asarray(cc + np.array([0.02, -0.03, 0.05]), np.float32)
+    urel = np.array([0.9, 0.5, -0.3])
+    s = one_face_body(oracle, ["+x"], (0, 0, 0), urel, pos, dx=dx, dt=dt, mass=mass)
+    st = s.body_state()
+    th, ps, ph = 0.3, 0.5, -0.2
+    st.theta, st.psi, st.phi = th, ps, ph
+    s.set_body_state(st)
+    R = np.array(s.body_state().rotation, float).reshape(3, 3)
+    n = np.array(_DIRS["+x"], float)
+    f = face_force_expected(n, urel, dx=dx)
+    tau = np.cross(cc + 0.5 * dx * n - pos.astype(float), f)
+    s.stage("body")
+    om = np.array(s.body_state().omega, float)
+    inertia = 0.4 * mass * 0.1 ** 2
+    want = R.T @ tau * dt / inertia
+    assert np.abs(om - want).max() <= 2e-5 * np.linalg.norm(want)
+    assert np.abs(om - R @ tau * dt / inertia).max() > 1e-2 * np.linalg.norm(want)
