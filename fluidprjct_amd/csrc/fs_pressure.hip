@@ -39,7 +39,7 @@ constexpr long long TB_MAX = 1LL << 18;    // coarse levels up to this size use 
 constexpr uint32_t SELF = 0x8000u;
 
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
-enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_RP, PS_COUNT };
+enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_RP, PS_MEAN, PS_COUNT };
 
 enum { FIN_BNORM = 0, FIN_OUTER };
 
@@ -326,13 +326,21 @@ __global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ count
             uint32_t code = 0;
             if (t[u] == 1) {
                 uint32_t diag = 0;
+                int air = 0;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     if (nt[u][q] == 1) { code |= (1u << q); ++diag; }
-                    else if (nt[u][q] == 2) { ++diag; dir = 1; }
+                    else if (nt[u][q] == 2) { ++diag; air = 1; }
                 }
-                code |= SELF | (diag << 8);
-                ++unk;
+                // a FLUID cell with no FLUID or AIR neighbour has an empty row:
+                // it is no unknown (p = 0, its b is dropped)
+                if (diag) {
+                    code |= SELF | (diag << 8);
+                    ++unk;
+                    dir |= air;
+                } else {
+                    code = 0;
+                }
             }
             L.code[c] = (uint16_t)code;
         }
@@ -1559,6 +1567,57 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
     store_partial(acc, sm, partial);
 }
 
+// Pure-Neumann (singular) systems: the constant vector on the unknowns spans
+// the null space.  The preconditioned residual and the solution are kept
+// orthogonal to it, so the iterates do not drift along it (a drift that grows
+// as the FP32 residual's rounding, which is not orthogonal to the constant,
+// is amplified by the near-singular coarse levels).
+// Partial sums of v over the unknowns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sum_unknowns(MGLevel L, const T* __restrict__ v, double* __restrict__ partial,
+                                                      const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[12];
+    double acc = 0.0;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x)
+        if (L.code[c] & SELF) acc += (double)v[c];
+    store_partial(acc, sm, partial);
+}
+
+// v -= mean of v over the unknowns (the sum from the partials, folded by
+// every block in the same order)
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_mean(MGLevel L, T* __restrict__ v, const double* __restrict__ partial,
+                                                  int np, int n_unknown, const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[12];
+    const double m = sum_partials(partial, np, sm) / (double)n_unknown;
+    const T mt = (T)m;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x)
+        if (L.code[c] & SELF) v[c] -= mt;
+}
+
+// r -= mean of r over the unknowns, and the partial sums of the projected r.r
+// (they replace those of the residual update, for the convergence test)
+__global__ __launch_bounds__(256) void k_sub_mean_rr(MGLevel L, float* __restrict__ r, const double* __restrict__ partial,
+                                                     int np, int n_unknown, double* __restrict__ prr,
+                                                     const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[12];
+    const float m = (float)(sum_partials(partial, np, sm) / (double)n_unknown);
+    double acc = 0.0;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x)
+        if (L.code[c] & SELF) {
+            const float v = r[c] - m;
+            r[c] = v;
+            acc += (double)v * (double)v;
+        }
+    store_partial(acc, sm, prr);
+}
+
 __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -1879,6 +1938,14 @@ void MGPCG::record_batch(Preconditioner* pre, int nb, hipStream_t st) {
         double* pq_new = slot(PS_PQ0 + cur);
         double* pq_old = slot(PS_PQ0 + (cur ^ 1));
         pre->apply(r, z, rz_new, sc, st);   // z = M r and the partial sums of r.z (and z.q when flexible)
+        if (singular_mode) {
+            // z orthogonal to the null space (r.z keeps its partial sums: r is
+            // orthogonal to it up to rounding)
+            hipLaunchKernelGGL(k_sum_unknowns<float>, dim3(nred), dim3(256), 0, st, L0, (const float*)z,
+                               slot(PS_MEAN), (const SolveScalars*)sc);
+            hipLaunchKernelGGL(k_sub_mean<float>, dim3(nred), dim3(256), 0, st, L0, z, (const double*)slot(PS_MEAN),
+                               nred, (int)n_unk, (const SolveScalars*)sc);
+        }
         const double* zq = flex ? (const double*)slot(PS_ZQ) : (const double*)nullptr;
         double* prp = flex ? slot(PS_RP) : (double*)nullptr;
         const double* num = flex ? (const double*)slot(PS_RP) : (const double*)rz_new;
@@ -1894,6 +1961,15 @@ void MGPCG::record_batch(Preconditioner* pre, int nb, hipStream_t st) {
             hipLaunchKernelGGL(k_update, dim3(nred), dim3(256), 0, st, n0, pnew, q, xb, r, num, pq_new, nred,
                                slot(PS_RR), sc);
         }
+        if (singular_mode) {
+            // the FP32 residual update leaves a rounding component along the
+            // null space that no iteration can remove: project it out before
+            // the convergence test
+            hipLaunchKernelGGL(k_sum_unknowns<float>, dim3(nred), dim3(256), 0, st, L0, (const float*)r,
+                               slot(PS_MEAN), (const SolveScalars*)sc);
+            hipLaunchKernelGGL(k_sub_mean_rr, dim3(nred), dim3(256), 0, st, L0, r, (const double*)slot(PS_MEAN), nred,
+                               (int)n_unk, slot(PS_RR), (const SolveScalars*)sc);
+        }
         hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(256), 0, st, rz_new, pq_new, slot(PS_RR), nred, flex ? 1 : 0,
                            flex ? stall_window : 0, sc);
         std::swap(pold, pnew);
@@ -1904,10 +1980,10 @@ void MGPCG::record_batch(Preconditioner* pre, int nb, hipStream_t st) {
 // The batch as a HIP graph, cached per solve-box shape and preconditioner
 // (kernel arguments depend only on those).  The graph holds kernel nodes only.
 hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, int nb, hipStream_t st) {
-    const long long key[8] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
-                              pre->graph_epoch(), nb};
+    const long long key[9] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
+                              pre->graph_epoch(), nb, singular_mode ? (long long)n_unk : -1};
     for (auto& g : graphs)
-        if (std::equal(key, key + 8, g.key)) return g.exec;
+        if (std::equal(key, key + 9, g.key)) return g.exec;
     hipGraph_t graph;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     record_batch(pre, nb, st);
@@ -1923,7 +1999,7 @@ hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, int nb, hipStream_t st) {
         graphs.erase(graphs.begin());
     }
     GraphEntry g;
-    std::copy(key, key + 8, g.key);
+    std::copy(key, key + 9, g.key);
     g.exec = ex;
     graphs.push_back(g);
     return ex;
@@ -2008,6 +2084,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     loc.fluid_cells = nunk;
     loc.singular = singular ? 1 : 0;
     n_unk = nunk;
+    singular_mode = singular;
     hipMemcpyAsync(&sc->n_unknown, counters + 8, sizeof(int), hipMemcpyDeviceToDevice, st);
     const double shift = singular ? sc_host->bsum / nunk : 0.0;
     const double bn2 =
@@ -2031,6 +2108,13 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         total_iters = sc_host->iters;
         if (sc_host->done == 2 || bn2 == 0.0) { converged = true; break; }
         if (outer >= max_outer || total_iters >= maxit) break;
+        if (singular) {
+            // solution orthogonal to the null space before the next pass
+            hipLaunchKernelGGL(k_sum_unknowns<double>, dim3(nred), dim3(256), 0, st, L0, (const double*)xb,
+                               slot(PS_MEAN), (const SolveScalars*)nullptr);
+            hipLaunchKernelGGL(k_sub_mean<double>, dim3(nred), dim3(256), 0, st, L0, xb,
+                               (const double*)slot(PS_MEAN), nred, nunk, (const SolveScalars*)nullptr);
+        }
         // Batches of `batch` iterations; once two checks give a convergence rate,
         // the next batch is sized (8, 4 or 2) to the iterations still expected,
         // since iterations launched after convergence still cost their launches.
@@ -2064,6 +2148,13 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             pre = &mgp;
             loc.fallback = 1;
         }
+    }
+    if (singular) {
+        // the solution orthogonal to the null space (zero mean over the unknowns)
+        hipLaunchKernelGGL(k_sum_unknowns<double>, dim3(nred), dim3(256), 0, st, L0, (const double*)xb,
+                           slot(PS_MEAN), (const SolveScalars*)nullptr);
+        hipLaunchKernelGGL(k_sub_mean<double>, dim3(nred), dim3(256), 0, st, L0, xb, (const double*)slot(PS_MEAN),
+                           nred, nunk, (const SolveScalars*)nullptr);
     }
     hipLaunchKernelGGL(k_scatter_out, dim3(nblk((N + 3) / 4)), dim3(256), 0, st, labels, N, W, H, L0, box[0], box[1], box[2], xb,
                        x64, p);

@@ -118,6 +118,7 @@ public:
     bool use_graph = true;      // replay inner batches as HIP graphs
     bool use_tail = true;
     bool use_tb = true;         // temporally blocked smoothing kernels on small coarse levels
+    bool singular_mode = false; // the current system has no Dirichlet cell (pure Neumann)
     bool use_vec = true;        // 4-cells-per-thread kernels on levels whose width is a multiple of 4
     bool vec0 = false;          // the configured fine level uses them       // levels of at most 32K cells in one workgroup launch
     // partial sums of r.z over the solve box (for preconditioners that do not fuse it)
@@ -129,7 +130,7 @@ public:
 
 private:
     struct GraphEntry {
-        long long key[8];
+        long long key[9];
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs;

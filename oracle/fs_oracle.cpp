@@ -160,6 +160,8 @@ struct Solver {
     float dt_cur = 0.0f;     /* dt of the current step */
     double sim_time = 0.0;
     int64_t nsteps = 0;
+    int64_t nonconv = 0;     /* steps whose pressure solve missed the tolerance */
+    double nonconv_res = 0.0;
 
     size_t C() const { return (size_t)W * H * D; }
     size_t NU() const { return (size_t)(W + 1) * H * D; }
@@ -614,16 +616,53 @@ bool system_singular(const std::vector<uint8_t>& L, int W, int H, int D) {
     return true;
 }
 
+/* Rows of FLUID cells with no FLUID or AIR neighbour are empty (diagonal 0,
+ * no off-diagonal, and no other row refers to them): they are dropped from
+ * the system, so those cells keep p = 0 and their b is not part of the
+ * residual.  The reference would hand such a singular matrix to cuDSS. */
+void drop_empty_rows(Csr& A) {
+    std::vector<int> keep(A.n, -1);
+    int m = 0;
+    for (int r = 0; r < A.n; ++r) {
+        bool empty = true;
+        for (int e = A.off[r]; e < A.off[r + 1]; ++e)
+            if (A.val[e] != 0.0) empty = false;
+        if (!empty) keep[r] = m++;
+    }
+    if (m == A.n) return;
+    Csr B;
+    B.n = m;
+    B.off.assign((size_t)m + 1, 0);
+    for (int r = 0; r < A.n; ++r) {
+        if (keep[r] < 0) continue;
+        for (int e = A.off[r]; e < A.off[r + 1]; ++e) {
+            B.col.push_back(keep[A.col[e]]);
+            B.val.push_back(A.val[e]);
+        }
+        B.off[keep[r] + 1] = (int)B.col.size();
+        B.cell.push_back(A.cell[r]);
+    }
+    A = std::move(B);
+}
+
 /* Solve A p = b for the full-grid b; p64 full grid (0 off fluid). */
 int solve_system(const std::vector<uint8_t>& L, const float* bfull, double* pfull, int W, int H, int D,
                  double tol, int maxit, fs_solve_stats* st) {
     std::vector<int> num;
     Csr A = assemble(L, W, H, D, num);
+    drop_empty_rows(A);
     size_t C = (size_t)W * H * D;
     fs_solve_stats loc{};
     loc.fluid_cells = A.n;
     for (size_t c = 0; c < C; ++c) pfull[c] = 0.0;
-    if (A.n == 0) { if (st) *st = loc; return fail(FS_ERR_NO_FLUID, "No fluid cells found!"); }
+    if (A.n == 0) {
+        if (st) *st = loc;
+        if (num.empty() || std::find_if(num.begin(), num.end(), [](int v) { return v >= 0; }) == num.end())
+            return fail(FS_ERR_NO_FLUID, "No fluid cells found!");
+        loc.converged = 1;
+        if (st) *st = loc;
+        return FS_OK;   /* FLUID cells, but none is an unknown */
+    }
     std::vector<double> b(A.n), x;
     for (int r = 0; r < A.n; ++r) b[r] = (double)bfull[A.cell[r]];
     bool sing = system_singular(L, W, H, D);
@@ -657,6 +696,7 @@ int solve_system(const std::vector<uint8_t>& L, const float* bfull, double* pful
     loc.outer = 1;
     for (int r = 0; r < A.n; ++r) pfull[A.cell[r]] = x[r];
     if (st) *st = loc;
+    if (!loc.converged) return fail(FS_ERR_NOT_CONVERGED, "pressure solve did not reach the tolerance");
     return FS_OK;
 }
 
@@ -1084,7 +1124,14 @@ int step_once(Solver& s) {
     stage_rhs(s);
     int rc = stage_pressure(s);
     s.times[FS_STAGE_PRESSURE] += ms_since(t);
-    if (rc != FS_OK && rc != FS_ERR_NO_FLUID) return rc;
+    /* a solve that misses the tolerance still completes the step (the
+     * reference ignores the solver status); fs_step reports it afterwards */
+    if (rc == FS_ERR_NOT_CONVERGED) {
+        s.nonconv += 1;
+        s.nonconv_res = s.stats.rel_residual;
+    } else if (rc != FS_OK && rc != FS_ERR_NO_FLUID) {
+        return rc;
+    }
     t = clk::now();
     stage_project(s);
     s.times[FS_STAGE_PROJECT] += ms_since(t);
@@ -1097,6 +1144,14 @@ int step_once(Solver& s) {
     s.sim_time += s.dt_cur;
     s.nsteps += 1;
     return FS_OK;
+}
+
+/* FS_ERR_NOT_CONVERGED when a solve of the steps since `before` missed the tolerance */
+int report_nonconv(Solver& s, int64_t before, int64_t nsteps) {
+    if (s.nonconv == before) return FS_OK;
+    return fail(FS_ERR_NOT_CONVERGED, "pressure solve missed the tolerance in " + std::to_string(s.nonconv - before) +
+                                          " of " + std::to_string(nsteps) + " steps (last relative residual " +
+                                          std::to_string(s.nonconv_res) + ")");
 }
 
 /* ---------------- minimal flat-JSON reader for config.json ------------------------ */
@@ -1325,11 +1380,12 @@ int64_t fs_num_particles(void* h) { return ((Solver*)h)->np(); }
 int fs_step(void* h, int nsteps) {
     Solver& s = *(Solver*)h;
     if (s.C() == 0) return fail(FS_ERR_STATE, "no scene loaded");
+    const int64_t before = s.nonconv;
     for (int i = 0; i < nsteps; ++i) {
         int rc = step_once(s);
         if (rc != FS_OK) return rc;
     }
-    return FS_OK;
+    return report_nonconv(s, before, nsteps);
 }
 
 int fs_run(void* h, int frames) {
@@ -1337,6 +1393,7 @@ int fs_run(void* h, int frames) {
     const int ipf = std::max(1, s.cfg.iter_per_frame);
     std::ofstream csv("OutputData/csvRes.csv");
     if (csv.is_open()) csv << "frame,centerX1,centerX2,centerX3,angleX1,angleX2,angleX3\n";
+    const int64_t before = s.nonconv;
     for (int i = 0; i <= frames * ipf; ++i) {
         int rc = step_once(s);
         if (rc != FS_OK) return rc;
@@ -1349,7 +1406,7 @@ int fs_run(void* h, int frames) {
                     << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";
         }
     }
-    return FS_OK;
+    return report_nonconv(s, before, (int64_t)frames * ipf + 1);
 }
 
 int fs_stage_label(void* h) { stage_label(*(Solver*)h); return FS_OK; }
