@@ -38,6 +38,22 @@ int fail(int code, const std::string& m) {
     return code;
 }
 
+// Runs the body of a C entry point: exceptions (a failed host allocation for
+// sizes read from a file or passed in, a parse error) become error codes and
+// never cross the C ABI.
+template <class F>
+int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(FS_ERR_ARG, "host allocation failed (sizes out of range?)");
+    } catch (const std::exception& e) {
+        return fail(FS_ERR_IO, std::string("unexpected error: ") + e.what());
+    } catch (...) {
+        return fail(FS_ERR_IO, "unexpected error");
+    }
+}
+
 #define HIPCHK(expr)                                                                            \
     do {                                                                                        \
         hipError_t e_ = (expr);                                                                 \
@@ -697,220 +713,248 @@ void fs_config_default(fs_config* c) {
 }
 
 int fs_config_load_json(fs_config* c, const char* path) {
-    std::ifstream f(path);
-    if (!f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
-    std::stringstream ss;
-    ss << f.rdbuf();
-    const std::string t = ss.str();
-    std::string v;
-    try {
-        if (json_value(t, "dt", v)) c->dt = std::stof(v);
-        if (json_value(t, "dx", v)) c->dx = std::stof(v);
-        if (json_value(t, "PIC_WEIGHT", v)) c->pic_weight = std::stof(v);
-        if (json_value(t, "iterPerFrame", v)) c->iter_per_frame = std::stoi(v);
-        if (json_value(t, "outputFormat", v)) c->output_format = (v == "PLY") ? FS_FMT_PLY : FS_FMT_OFF;
-        if (json_value(t, "outputTemplate", v)) std::snprintf(c->output_template, 256, "%s", v.c_str());
-        if (json_value(t, "inputLabelData", v)) std::snprintf(c->input_label_data, 256, "%s", v.c_str());
-        if (json_value(t, "frameAmount", v)) c->frame_amount = std::stoi(v);
-        if (json_value(t, "activeBody", v)) c->active_body = (v == "true" || v == "1");
-        if (json_value(t, "body_x", v)) c->body_x = std::stof(v);
-        if (json_value(t, "body_y", v)) c->body_y = std::stof(v);
-        if (json_value(t, "body_z", v)) c->body_z = std::stof(v);
-        if (json_value(t, "body_mass", v)) c->body_mass = std::stof(v);
-        if (json_value(t, "bodySdf", v)) std::snprintf(c->body_sdf_path, 256, "%s", v.c_str());
-    } catch (const std::exception& e) {
-        return fail(FS_ERR_IO, std::string("bad value in ") + path + ": " + e.what());
-    }
-    return FS_OK;
+    return guarded([&]() -> int {
+        std::ifstream f(path);
+        if (!f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
+        std::stringstream ss;
+        ss << f.rdbuf();
+        const std::string t = ss.str();
+        std::string v;
+        try {
+            if (json_value(t, "dt", v)) c->dt = std::stof(v);
+            if (json_value(t, "dx", v)) c->dx = std::stof(v);
+            if (json_value(t, "PIC_WEIGHT", v)) c->pic_weight = std::stof(v);
+            if (json_value(t, "iterPerFrame", v)) c->iter_per_frame = std::stoi(v);
+            if (json_value(t, "outputFormat", v)) c->output_format = (v == "PLY") ? FS_FMT_PLY : FS_FMT_OFF;
+            if (json_value(t, "outputTemplate", v)) std::snprintf(c->output_template, 256, "%s", v.c_str());
+            if (json_value(t, "inputLabelData", v)) std::snprintf(c->input_label_data, 256, "%s", v.c_str());
+            if (json_value(t, "frameAmount", v)) c->frame_amount = std::stoi(v);
+            if (json_value(t, "activeBody", v)) c->active_body = (v == "true" || v == "1");
+            if (json_value(t, "body_x", v)) c->body_x = std::stof(v);
+            if (json_value(t, "body_y", v)) c->body_y = std::stof(v);
+            if (json_value(t, "body_z", v)) c->body_z = std::stof(v);
+            if (json_value(t, "body_mass", v)) c->body_mass = std::stof(v);
+            if (json_value(t, "bodySdf", v)) std::snprintf(c->body_sdf_path, 256, "%s", v.c_str());
+        } catch (const std::exception& e) {
+            return fail(FS_ERR_IO, std::string("bad value in ") + path + ": " + e.what());
+        }
+        return FS_OK;
+    });
 }
 
 int fs_create(const fs_config* cfg, void** out) {
-    if (!cfg || !out) return fail(FS_ERR_ARG, "null argument");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FS_ERR_DEVICE, "no HIP device");
-    std::unique_ptr<Solver> s(new Solver());
-    s->cfg = *cfg;
-    HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
-    for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
-    HIPCHK(hipMalloc(&s->d_bacc, sizeof(double) * 8));
-    HIPCHK(hipHostMalloc((void**)&s->h_bacc, sizeof(double) * 8));
-    HIPCHK(hipEventCreateWithFlags(&s->body_done, hipEventDisableTiming));
-    HIPCHK(hipMalloc(&s->d_umax, sizeof(unsigned)));
-    s->dt_cur = cfg->dt;
-    *out = s.release();
-    return FS_OK;
+    return guarded([&]() -> int {
+        if (!cfg || !out) return fail(FS_ERR_ARG, "null argument");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FS_ERR_DEVICE, "no HIP device");
+        std::unique_ptr<Solver> s(new Solver());
+        s->cfg = *cfg;
+        HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipMalloc(&s->d_bacc, sizeof(double) * 8));
+        HIPCHK(hipHostMalloc((void**)&s->h_bacc, sizeof(double) * 8));
+        HIPCHK(hipEventCreateWithFlags(&s->body_done, hipEventDisableTiming));
+        HIPCHK(hipMalloc(&s->d_umax, sizeof(unsigned)));
+        s->dt_cur = cfg->dt;
+        *out = s.release();
+        return FS_OK;
+    });
 }
 
 void fs_destroy(void* h) { delete (Solver*)h; }
 
 int fs_load_labels_file(void* h, const char* path) {
-    Solver& s = *(Solver*)h;
-    int W, H, D;
-    std::vector<uint8_t> L;
-    int rc = read_label_file(path, W, H, D, L);
-    if (rc) return rc;
-    return upload_scene(s, W, H, D, L);
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        int W, H, D;
+        std::vector<uint8_t> L;
+        int rc = read_label_file(path, W, H, D, L);
+        if (rc) return rc;
+        return upload_scene(s, W, H, D, L);
+    });
 }
 
 int fs_set_labels(void* h, const uint8_t* labels, int W, int H, int D) {
-    Solver& s = *(Solver*)h;
-    if (!labels || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad label grid");
-    std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
-    return upload_scene(s, W, H, D, L);
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!labels || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad label grid");
+        std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
+        return upload_scene(s, W, H, D, L);
+    });
 }
 
 int fs_set_level_sets(void* h, const float* ps, const float* pf, int W, int H, int D) {
-    if (!ps || !pf || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad level sets");
-    std::vector<uint8_t> L((size_t)W * H * D);
-    for (size_t c = 0; c < L.size(); ++c) L[c] = ps[c] < 0 ? FS_SOLID : (pf[c] < 0 ? FS_FLUID : FS_AIR);
-    return fs_set_labels(h, L.data(), W, H, D);
+    return guarded([&]() -> int {
+        if (!ps || !pf || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad level sets");
+        std::vector<uint8_t> L((size_t)W * H * D);
+        for (size_t c = 0; c < L.size(); ++c) L[c] = ps[c] < 0 ? FS_SOLID : (pf[c] < 0 ? FS_FLUID : FS_AIR);
+        return fs_set_labels(h, L.data(), W, H, D);
+    });
 }
 
 int fs_get_dims(void* h, int* W, int* H, int* D) {
-    Solver& s = *(Solver*)h;
-    *W = s.W; *H = s.H; *D = s.D;
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        *W = s.W; *H = s.H; *D = s.D;
+        return FS_OK;
+    });
 }
 
 int fs_set_body_sdf(void* h, const float* sdf, int w, int hh, int d, float ox, float oy, float oz, float cs, float x,
                     float y, float z, float mass) {
-    Solver& s = *(Solver*)h;
-    if (!sdf || w <= 0 || hh <= 0 || d <= 0 || !(cs > 0)) return fail(FS_ERR_ARG, "bad SDF grid");
-    HostBody& b = s.body;
-    b = HostBody();
-    b.active = true;
-    b.sdf.assign(sdf, sdf + (size_t)w * hh * d);
-    b.sw = w; b.sh = hh; b.sd = d;
-    b.file_origin = mk(ox, oy, oz);
-    b.cs = cs;
-    b.size = mk(w * cs, hh * cs, d * cs);
-    f3 center = b.file_origin + b.size * 0.5f;
-    f3 T = mk(x, y, z) - center;
-    b.pos = mk(x, y, z);
-    b.sdf_origin = b.file_origin + T;
-    b.local_com = center;
-    b.mass = mass;
-    float radius = b.size.x / 2.0f;
-    b.inertia = 0.4f * b.mass * radius * radius;
-    b.inv_inertia = 1.0f / b.inertia;
-    b.update_rotation();
-    dfree(s.d_sdf);
-    HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * b.sdf.size()));
-    HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
-    if (s.C > 0) {
-        launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
-        HIPCHK(hipStreamSynchronize(s.st));
-    }
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!sdf || w <= 0 || hh <= 0 || d <= 0 || !(cs > 0)) return fail(FS_ERR_ARG, "bad SDF grid");
+        HostBody& b = s.body;
+        b = HostBody();
+        b.active = true;
+        b.sdf.assign(sdf, sdf + (size_t)w * hh * d);
+        b.sw = w; b.sh = hh; b.sd = d;
+        b.file_origin = mk(ox, oy, oz);
+        b.cs = cs;
+        b.size = mk(w * cs, hh * cs, d * cs);
+        f3 center = b.file_origin + b.size * 0.5f;
+        f3 T = mk(x, y, z) - center;
+        b.pos = mk(x, y, z);
+        b.sdf_origin = b.file_origin + T;
+        b.local_com = center;
+        b.mass = mass;
+        float radius = b.size.x / 2.0f;
+        b.inertia = 0.4f * b.mass * radius * radius;
+        b.inv_inertia = 1.0f / b.inertia;
+        b.update_rotation();
+        dfree(s.d_sdf);
+        HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * b.sdf.size()));
+        HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
+        if (s.C > 0) {
+            launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
+            HIPCHK(hipStreamSynchronize(s.st));
+        }
+        return FS_OK;
+    });
 }
 
 // RigidBody::loadSDF utility.cuh:231-338: "w h d", origin, cell size, then k->j->i values
 int fs_load_body_sdf(void* h, const char* path, float x, float y, float z, float mass) {
-    std::ifstream f(path);
-    if (!f.is_open()) return fail(FS_ERR_IO, std::string("Cannot open SDF file: ") + path);
-    std::string line;
-    int w = 0, hh = 0, d = 0;
-    float ox = 0, oy = 0, oz = 0, cs = 0;
-    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing dimensions");
-    { std::istringstream is(line); is >> w >> hh >> d; }
-    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing origin");
-    { std::istringstream is(line); is >> ox >> oy >> oz; }
-    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing cell size");
-    try { cs = std::stof(line); } catch (...) { return fail(FS_ERR_IO, "SDF: bad cell size"); }
-    if (w <= 0 || hh <= 0 || d <= 0) return fail(FS_ERR_IO, "SDF: bad dimensions");
-    std::vector<float> data((size_t)w * hh * d);
-    for (auto& v : data)
-        if (!(f >> v)) return fail(FS_ERR_IO, "Unexpected end of SDF data");
-    return fs_set_body_sdf(h, data.data(), w, hh, d, ox, oy, oz, cs, x, y, z, mass);
+    return guarded([&]() -> int {
+        std::ifstream f(path);
+        if (!f.is_open()) return fail(FS_ERR_IO, std::string("Cannot open SDF file: ") + path);
+        std::string line;
+        int w = 0, hh = 0, d = 0;
+        float ox = 0, oy = 0, oz = 0, cs = 0;
+        if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing dimensions");
+        { std::istringstream is(line); is >> w >> hh >> d; }
+        if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing origin");
+        { std::istringstream is(line); is >> ox >> oy >> oz; }
+        if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing cell size");
+        try { cs = std::stof(line); } catch (...) { return fail(FS_ERR_IO, "SDF: bad cell size"); }
+        if (w <= 0 || hh <= 0 || d <= 0) return fail(FS_ERR_IO, "SDF: bad dimensions");
+        std::vector<float> data((size_t)w * hh * d);
+        for (auto& v : data)
+            if (!(f >> v)) return fail(FS_ERR_IO, "Unexpected end of SDF data");
+        return fs_set_body_sdf(h, data.data(), w, hh, d, ox, oy, oz, cs, x, y, z, mass);
+    });
 }
 
 int fs_get_body_state(void* h, fs_body_state* o) {
-    const HostBody& b = ((Solver*)h)->body;
-    std::memset(o, 0, sizeof(*o));
-    o->pos[0] = b.pos.x; o->pos[1] = b.pos.y; o->pos[2] = b.pos.z;
-    o->vel[0] = b.vel.x; o->vel[1] = b.vel.y; o->vel[2] = b.vel.z;
-    o->omega[0] = b.omega.x; o->omega[1] = b.omega.y; o->omega[2] = b.omega.z;
-    o->theta = b.theta; o->psi = b.psi; o->phi = b.phi;
-    o->force[0] = b.last_force.x; o->force[1] = b.last_force.y; o->force[2] = b.last_force.z;
-    o->torque[0] = b.last_torque.x; o->torque[1] = b.last_torque.y; o->torque[2] = b.last_torque.z;
-    o->submerged_volume = b.last_sub;
-    for (int i = 0; i < 9; ++i) o->rotation[i] = b.R[i];
-    return FS_OK;
+    return guarded([&]() -> int {
+        const HostBody& b = ((Solver*)h)->body;
+        std::memset(o, 0, sizeof(*o));
+        o->pos[0] = b.pos.x; o->pos[1] = b.pos.y; o->pos[2] = b.pos.z;
+        o->vel[0] = b.vel.x; o->vel[1] = b.vel.y; o->vel[2] = b.vel.z;
+        o->omega[0] = b.omega.x; o->omega[1] = b.omega.y; o->omega[2] = b.omega.z;
+        o->theta = b.theta; o->psi = b.psi; o->phi = b.phi;
+        o->force[0] = b.last_force.x; o->force[1] = b.last_force.y; o->force[2] = b.last_force.z;
+        o->torque[0] = b.last_torque.x; o->torque[1] = b.last_torque.y; o->torque[2] = b.last_torque.z;
+        o->submerged_volume = b.last_sub;
+        for (int i = 0; i < 9; ++i) o->rotation[i] = b.R[i];
+        return FS_OK;
+    });
 }
 
 int fs_set_body_state(void* h, const fs_body_state* in) {
-    HostBody& b = ((Solver*)h)->body;
-    b.pos = mk(in->pos[0], in->pos[1], in->pos[2]);
-    b.vel = mk(in->vel[0], in->vel[1], in->vel[2]);
-    b.omega = mk(in->omega[0], in->omega[1], in->omega[2]);
-    b.theta = in->theta; b.psi = in->psi; b.phi = in->phi;
-    b.update_rotation();
-    return FS_OK;
+    return guarded([&]() -> int {
+        HostBody& b = ((Solver*)h)->body;
+        b.pos = mk(in->pos[0], in->pos[1], in->pos[2]);
+        b.vel = mk(in->vel[0], in->vel[1], in->vel[2]);
+        b.omega = mk(in->omega[0], in->omega[1], in->omega[2]);
+        b.theta = in->theta; b.psi = in->psi; b.phi = in->phi;
+        b.update_rotation();
+        return FS_OK;
+    });
 }
 
 int fs_seed_particles(void* h, int ppc, uint64_t seed) {
-    Solver& s = *(Solver*)h;
-    if (s.C == 0) return fail(FS_ERR_STATE, "no scene loaded");
-    if (ppc <= 0 || ppc > 63) return fail(FS_ERR_ARG, "particles per cell must be in [1,63]");
-    // fluid-cell ranks by an exclusive scan over (label == FLUID) in k->j->i order
-    launch_scan_fluid(s.d_labels, s.C, s.d_bsum, s.d_total, s.d_start, s.st);
-    int nfluid = 0;
-    HIPCHK(hipMemcpyAsync(&nfluid, s.d_total, sizeof(int), hipMemcpyDeviceToHost, s.st));
-    HIPCHK(hipStreamSynchronize(s.st));
-    int rc = s.alloc_particles((long long)nfluid * ppc);
-    if (rc) return rc;
-    launch_seed(s.d_labels, s.d_start, s.C, s.W, s.H, s.cfg.dx, ppc, seed, s.PA, s.st);
-    HIPCHK(hipStreamSynchronize(s.st));
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (s.C == 0) return fail(FS_ERR_STATE, "no scene loaded");
+        if (ppc <= 0 || ppc > 63) return fail(FS_ERR_ARG, "particles per cell must be in [1,63]");
+        // fluid-cell ranks by an exclusive scan over (label == FLUID) in k->j->i order
+        launch_scan_fluid(s.d_labels, s.C, s.d_bsum, s.d_total, s.d_start, s.st);
+        int nfluid = 0;
+        HIPCHK(hipMemcpyAsync(&nfluid, s.d_total, sizeof(int), hipMemcpyDeviceToHost, s.st));
+        HIPCHK(hipStreamSynchronize(s.st));
+        int rc = s.alloc_particles((long long)nfluid * ppc);
+        if (rc) return rc;
+        launch_seed(s.d_labels, s.d_start, s.C, s.W, s.H, s.cfg.dx, ppc, seed, s.PA, s.st);
+        HIPCHK(hipStreamSynchronize(s.st));
+        return FS_OK;
+    });
 }
 
 int fs_set_particles(void* h, const float* pos, const float* vel, int64_t n) {
-    Solver& s = *(Solver*)h;
-    if (n < 0 || (n > 0 && !pos)) return fail(FS_ERR_ARG, "bad particle arrays");
-    int rc = s.alloc_particles(n);
-    if (rc) return rc;
-    std::vector<float> a((size_t)n);
-    std::vector<int> id((size_t)n);
-    float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
-    for (int c = 0; c < 6; ++c) {
-        const float* src = c < 3 ? pos : vel;
-        for (int64_t i = 0; i < n; ++i) a[(size_t)i] = src ? src[3 * i + (c % 3)] : 0.0f;
-        if (n) HIPCHK(hipMemcpy(dst[c], a.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    }
-    for (int64_t i = 0; i < n; ++i) id[(size_t)i] = (int)i;
-    if (n) HIPCHK(hipMemcpy(s.PA.id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (n < 0 || (n > 0 && !pos)) return fail(FS_ERR_ARG, "bad particle arrays");
+        int rc = s.alloc_particles(n);
+        if (rc) return rc;
+        std::vector<float> a((size_t)n);
+        std::vector<int> id((size_t)n);
+        float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
+        for (int c = 0; c < 6; ++c) {
+            const float* src = c < 3 ? pos : vel;
+            for (int64_t i = 0; i < n; ++i) a[(size_t)i] = src ? src[3 * i + (c % 3)] : 0.0f;
+            if (n) HIPCHK(hipMemcpy(dst[c], a.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        }
+        for (int64_t i = 0; i < n; ++i) id[(size_t)i] = (int)i;
+        if (n) HIPCHK(hipMemcpy(s.PA.id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        return FS_OK;
+    });
 }
 
 int fs_get_particles(void* h, float* pos, float* vel, int64_t cap, int64_t* n) {
-    Solver& s = *(Solver*)h;
-    if (n) *n = s.np;
-    if (cap < s.np) return fail(FS_ERR_ARG, "buffer too small");
-    if (s.np == 0) return FS_OK;
-    HIPCHK(hipStreamSynchronize(s.st));
-    std::vector<float> a((size_t)s.np);
-    std::vector<int> id((size_t)s.np);
-    HIPCHK(hipMemcpy(id.data(), s.PA.id, sizeof(int) * s.np, hipMemcpyDeviceToHost));
-    const float* src[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
-    for (int c = 0; c < 6; ++c) {
-        float* out = c < 3 ? pos : vel;
-        if (!out) continue;
-        HIPCHK(hipMemcpy(a.data(), src[c], sizeof(float) * s.np, hipMemcpyDeviceToHost));
-        for (long long i = 0; i < s.np; ++i) out[3 * (size_t)id[i] + (c % 3)] = a[i];
-    }
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (n) *n = s.np;
+        if (cap < s.np) return fail(FS_ERR_ARG, "buffer too small");
+        if (s.np == 0) return FS_OK;
+        HIPCHK(hipStreamSynchronize(s.st));
+        std::vector<float> a((size_t)s.np);
+        std::vector<int> id((size_t)s.np);
+        HIPCHK(hipMemcpy(id.data(), s.PA.id, sizeof(int) * s.np, hipMemcpyDeviceToHost));
+        const float* src[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
+        for (int c = 0; c < 6; ++c) {
+            float* out = c < 3 ? pos : vel;
+            if (!out) continue;
+            HIPCHK(hipMemcpy(a.data(), src[c], sizeof(float) * s.np, hipMemcpyDeviceToHost));
+            for (long long i = 0; i < s.np; ++i) out[3 * (size_t)id[i] + (c % 3)] = a[i];
+        }
+        return FS_OK;
+    });
 }
 
 int64_t fs_num_particles(void* h) { return ((Solver*)h)->np; }
 
 int fs_step(void* h, int nsteps) {
-    Solver& s = *(Solver*)h;
-    const int64_t before = s.nonconv;
-    for (int i = 0; i < nsteps; ++i) {
-        int rc = s.step_once();
-        if (rc) return rc;
-    }
-    return s.report_nonconv(before, nsteps);
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        const int64_t before = s.nonconv;
+        for (int i = 0; i < nsteps; ++i) {
+            int rc = s.step_once();
+            if (rc) return rc;
+        }
+        return s.report_nonconv(before, nsteps);
+    });
 }
 
 // run() FluidSolver3D.cu:3100-3163: max_steps * iterPerFrame + 1 steps, a
@@ -918,42 +962,44 @@ int fs_step(void* h, int nsteps) {
 // to the host after its step; formatting and writing the file runs on a host
 // thread while the following steps run on the GPU (one frame in flight).
 int fs_run(void* h, int frames) {
-    Solver& s = *(Solver*)h;
-    const int ipf = std::max(1, s.cfg.iter_per_frame);
-    std::ofstream csv("OutputData/csvRes.csv");
-    if (csv.is_open()) csv << "frame,centerX1,centerX2,centerX3,angleX1,angleX2,angleX3\n";
-    std::thread writer;
-    std::vector<float> frame_pos;
-    int writer_rc = FS_OK;
-    std::string writer_err;
-    auto join = [&]() {
-        if (writer.joinable()) writer.join();
-        return writer_rc;
-    };
-    int rc = FS_OK;
-    const int64_t before = s.nonconv;
-    for (int i = 0; i <= frames * ipf && rc == FS_OK; ++i) {
-        rc = s.step_once();
-        if (rc) break;
-        if (i % ipf == 0) {
-            if ((rc = join())) break;
-            if ((rc = gather_positions(s, frame_pos))) break;
-            const std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
-                                   (s.cfg.output_format == FS_FMT_OFF ? ".off" : ".ply");
-            const int fmt = s.cfg.output_format;
-            writer = std::thread([&frame_pos, fn, fmt, &writer_rc, &writer_err]() {
-                writer_rc = format_particle_file(frame_pos, fn, fmt, &writer_err);
-            });
-            if (s.body.active) write_body_ply(s, "OutputData/body_" + std::to_string(i / ipf) + ".ply");
-            if (csv.is_open())
-                csv << i / ipf << "," << s.body.pos.x << "," << s.body.pos.y << "," << s.body.pos.z << ","
-                    << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        const int ipf = std::max(1, s.cfg.iter_per_frame);
+        std::ofstream csv("OutputData/csvRes.csv");
+        if (csv.is_open()) csv << "frame,centerX1,centerX2,centerX3,angleX1,angleX2,angleX3\n";
+        std::thread writer;
+        std::vector<float> frame_pos;
+        int writer_rc = FS_OK;
+        std::string writer_err;
+        auto join = [&]() {
+            if (writer.joinable()) writer.join();
+            return writer_rc;
+        };
+        int rc = FS_OK;
+        const int64_t before = s.nonconv;
+        for (int i = 0; i <= frames * ipf && rc == FS_OK; ++i) {
+            rc = s.step_once();
+            if (rc) break;
+            if (i % ipf == 0) {
+                if ((rc = join())) break;
+                if ((rc = gather_positions(s, frame_pos))) break;
+                const std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
+                                       (s.cfg.output_format == FS_FMT_OFF ? ".off" : ".ply");
+                const int fmt = s.cfg.output_format;
+                writer = std::thread([&frame_pos, fn, fmt, &writer_rc, &writer_err]() {
+                    writer_rc = format_particle_file(frame_pos, fn, fmt, &writer_err);
+                });
+                if (s.body.active) write_body_ply(s, "OutputData/body_" + std::to_string(i / ipf) + ".ply");
+                if (csv.is_open())
+                    csv << i / ipf << "," << s.body.pos.x << "," << s.body.pos.y << "," << s.body.pos.z << ","
+                        << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";
+            }
         }
-    }
-    const int wrc = join();
-    if (rc) return rc;
-    if (wrc) return fail(wrc, writer_err);
-    return s.report_nonconv(before, (int64_t)frames * ipf + 1);
+        const int wrc = join();
+        if (rc) return rc;
+        if (wrc) return fail(wrc, writer_err);
+        return s.report_nonconv(before, (int64_t)frames * ipf + 1);
+    });
 }
 
 #define STAGE_SYNC(call)                                                  \
@@ -964,118 +1010,154 @@ int fs_run(void* h, int frames) {
         return FS_OK;                                                     \
     } while (0)
 
-int fs_stage_label(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_sort_label()); }
-int fs_stage_p2g(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_p2g()); }
-int fs_stage_rhs(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_rhs()); }
-int fs_stage_pressure(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_pressure()); }
-int fs_stage_project(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_project()); }
-int fs_stage_body(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_body()); }
-int fs_stage_g2p_advect(void* h) { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_g2p_advect()); }
+int fs_stage_label(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_sort_label());
+    });
+}
+int fs_stage_p2g(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_p2g());
+    });
+}
+int fs_stage_rhs(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_rhs());
+    });
+}
+int fs_stage_pressure(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_pressure());
+    });
+}
+int fs_stage_project(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_project());
+    });
+}
+int fs_stage_body(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_body());
+    });
+}
+int fs_stage_g2p_advect(void* h) {
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_g2p_advect());
+    });
+}
 
 int fs_get_grid(void* h, int which, void* out, int64_t count) {
-    Solver& s = *(Solver*)h;
-    HIPCHK(hipStreamSynchronize(s.st));
-    const void* src = nullptr;
-    long long n = 0;
-    size_t esz = 4;
-    switch (which) {
-        case FS_GRID_U: src = s.d_u; n = s.NU(); break;
-        case FS_GRID_V: src = s.d_v; n = s.NV(); break;
-        case FS_GRID_W: src = s.d_w; n = s.NW(); break;
-        case FS_GRID_USAVED: src = s.d_us; n = s.NU(); break;
-        case FS_GRID_VSAVED: src = s.d_vs; n = s.NV(); break;
-        case FS_GRID_WSAVED: src = s.d_ws; n = s.NW(); break;
-        case FS_GRID_P: src = s.d_p; n = s.C; break;
-        case FS_GRID_RHS: src = s.d_rhs; n = s.C; break;
-        case FS_GRID_LABELS: src = s.d_labels; n = s.C; esz = 1; break;
-        case FS_GRID_SOLID: src = s.d_solid; n = s.C; esz = 1; break;
-        case FS_GRID_P64: src = s.d_p64; n = s.C; esz = 8; break;
-        case FS_GRID_CELLKEY: {
-            n = s.np;
-            if (count < n) return fail(FS_ERR_ARG, "buffer too small");
-            if (n == 0) return FS_OK;
-            if (!s.d_tmpkey) HIPCHK(hipMalloc(&s.d_tmpkey, sizeof(int) * n));
-            launch_cell_keys(s.PA, (int)n, s.cfg.dx, s.W, s.H, s.D, s.d_tmpkey, s.st);
-            std::vector<int> k((size_t)n), id((size_t)n);
-            HIPCHK(hipMemcpyAsync(k.data(), s.d_tmpkey, sizeof(int) * n, hipMemcpyDeviceToHost, s.st));
-            HIPCHK(hipMemcpyAsync(id.data(), s.PA.id, sizeof(int) * n, hipMemcpyDeviceToHost, s.st));
-            HIPCHK(hipStreamSynchronize(s.st));
-            int* o = (int*)out;
-            for (long long i = 0; i < n; ++i) o[id[i]] = k[i];
-            return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        HIPCHK(hipStreamSynchronize(s.st));
+        const void* src = nullptr;
+        long long n = 0;
+        size_t esz = 4;
+        switch (which) {
+            case FS_GRID_U: src = s.d_u; n = s.NU(); break;
+            case FS_GRID_V: src = s.d_v; n = s.NV(); break;
+            case FS_GRID_W: src = s.d_w; n = s.NW(); break;
+            case FS_GRID_USAVED: src = s.d_us; n = s.NU(); break;
+            case FS_GRID_VSAVED: src = s.d_vs; n = s.NV(); break;
+            case FS_GRID_WSAVED: src = s.d_ws; n = s.NW(); break;
+            case FS_GRID_P: src = s.d_p; n = s.C; break;
+            case FS_GRID_RHS: src = s.d_rhs; n = s.C; break;
+            case FS_GRID_LABELS: src = s.d_labels; n = s.C; esz = 1; break;
+            case FS_GRID_SOLID: src = s.d_solid; n = s.C; esz = 1; break;
+            case FS_GRID_P64: src = s.d_p64; n = s.C; esz = 8; break;
+            case FS_GRID_CELLKEY: {
+                n = s.np;
+                if (count < n) return fail(FS_ERR_ARG, "buffer too small");
+                if (n == 0) return FS_OK;
+                if (!s.d_tmpkey) HIPCHK(hipMalloc(&s.d_tmpkey, sizeof(int) * n));
+                launch_cell_keys(s.PA, (int)n, s.cfg.dx, s.W, s.H, s.D, s.d_tmpkey, s.st);
+                std::vector<int> k((size_t)n), id((size_t)n);
+                HIPCHK(hipMemcpyAsync(k.data(), s.d_tmpkey, sizeof(int) * n, hipMemcpyDeviceToHost, s.st));
+                HIPCHK(hipMemcpyAsync(id.data(), s.PA.id, sizeof(int) * n, hipMemcpyDeviceToHost, s.st));
+                HIPCHK(hipStreamSynchronize(s.st));
+                int* o = (int*)out;
+                for (long long i = 0; i < n; ++i) o[id[i]] = k[i];
+                return FS_OK;
+            }
+            default: return fail(FS_ERR_ARG, "unknown grid");
         }
-        default: return fail(FS_ERR_ARG, "unknown grid");
-    }
-    if (count < n) return fail(FS_ERR_ARG, "buffer too small");
-    if (n) HIPCHK(hipMemcpy(out, src, esz * n, hipMemcpyDeviceToHost));
-    return FS_OK;
+        if (count < n) return fail(FS_ERR_ARG, "buffer too small");
+        if (n) HIPCHK(hipMemcpy(out, src, esz * n, hipMemcpyDeviceToHost));
+        return FS_OK;
+    });
 }
 
 int fs_set_grid(void* h, int which, const void* in, int64_t count) {
-    Solver& s = *(Solver*)h;
-    void* dst = nullptr;
-    long long n = 0;
-    size_t esz = 4;
-    switch (which) {
-        case FS_GRID_U: dst = s.d_u; n = s.NU(); break;
-        case FS_GRID_V: dst = s.d_v; n = s.NV(); break;
-        case FS_GRID_W: dst = s.d_w; n = s.NW(); break;
-        case FS_GRID_USAVED: dst = s.d_us; n = s.NU(); break;
-        case FS_GRID_VSAVED: dst = s.d_vs; n = s.NV(); break;
-        case FS_GRID_WSAVED: dst = s.d_ws; n = s.NW(); break;
-        case FS_GRID_P: dst = s.d_p; n = s.C; break;
-        case FS_GRID_RHS: dst = s.d_rhs; n = s.C; break;
-        case FS_GRID_LABELS: dst = s.d_labels; n = s.C; esz = 1; break;
-        case FS_GRID_P64: dst = s.d_p64; n = s.C; esz = 8; break;
-        default: return fail(FS_ERR_ARG, "grid not settable");
-    }
-    if (count != n) return fail(FS_ERR_ARG, "size mismatch");
-    HIPCHK(hipStreamSynchronize(s.st));
-    if (n) HIPCHK(hipMemcpy(dst, in, esz * n, hipMemcpyHostToDevice));
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        void* dst = nullptr;
+        long long n = 0;
+        size_t esz = 4;
+        switch (which) {
+            case FS_GRID_U: dst = s.d_u; n = s.NU(); break;
+            case FS_GRID_V: dst = s.d_v; n = s.NV(); break;
+            case FS_GRID_W: dst = s.d_w; n = s.NW(); break;
+            case FS_GRID_USAVED: dst = s.d_us; n = s.NU(); break;
+            case FS_GRID_VSAVED: dst = s.d_vs; n = s.NV(); break;
+            case FS_GRID_WSAVED: dst = s.d_ws; n = s.NW(); break;
+            case FS_GRID_P: dst = s.d_p; n = s.C; break;
+            case FS_GRID_RHS: dst = s.d_rhs; n = s.C; break;
+            case FS_GRID_LABELS: dst = s.d_labels; n = s.C; esz = 1; break;
+            case FS_GRID_P64: dst = s.d_p64; n = s.C; esz = 8; break;
+            default: return fail(FS_ERR_ARG, "grid not settable");
+        }
+        if (count != n) return fail(FS_ERR_ARG, "size mismatch");
+        HIPCHK(hipStreamSynchronize(s.st));
+        if (n) HIPCHK(hipMemcpy(dst, in, esz * n, hipMemcpyHostToDevice));
+        return FS_OK;
+    });
 }
 
 int fs_get_stage_times(void* h, double* ms, int n) {
-    Solver& s = *(Solver*)h;
-    for (int i = 0; i < n && i < FS_NUM_STAGES; ++i) ms[i] = s.times[i];
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        for (int i = 0; i < n && i < FS_NUM_STAGES; ++i) ms[i] = s.times[i];
+        return FS_OK;
+    });
 }
 
 int fs_get_clock(void* h, double* t, float* dt, int64_t* n) {
-    Solver& s = *(Solver*)h;
-    if (t) *t = s.sim_time;
-    if (dt) *dt = s.dt_cur;
-    if (n) *n = s.nsteps;
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (t) *t = s.sim_time;
+        if (dt) *dt = s.dt_cur;
+        if (n) *n = s.nsteps;
+        return FS_OK;
+    });
 }
 
 int fs_get_solve_stats(void* h, fs_solve_stats* o) {
-    *o = ((Solver*)h)->stats;
-    return FS_OK;
+    return guarded([&]() -> int {
+        *o = ((Solver*)h)->stats;
+        return FS_OK;
+    });
 }
 
-int fs_write_particles(void* h, const char* path, int fmt) { return write_particle_file(*(Solver*)h, path, fmt); }
+int fs_write_particles(void* h, const char* path, int fmt) {
+    return guarded([&]() -> int { return write_particle_file(*(Solver*)h, path, fmt);
+    });
+}
 
 int fs_solve_pressure_system(void* h, const uint8_t* labels, const float* b, double* p, int W, int H, int D,
                              double tol, fs_solve_stats* stats) {
-    Solver& s = *(Solver*)h;
-    if (!labels || !b || !p || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad system");
-    if (s.W != W || s.H != H || s.D != D) {
-        std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
-        int rc = upload_scene(s, W, H, D, L);
-        if (rc) return rc;
-    } else {
-        HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMemcpy(s.d_rhs, b, sizeof(float) * (size_t)W * H * D, hipMemcpyHostToDevice));
-    double save = s.cfg.solver_tol;
-    s.cfg.solver_tol = tol;
-    int rc = s.stage_pressure();
-    s.cfg.solver_tol = save;
-    HIPCHK(hipStreamSynchronize(s.st));
-    HIPCHK(hipMemcpy(p, s.d_p64, sizeof(double) * (size_t)W * H * D, hipMemcpyDeviceToHost));
-    if (stats) *stats = s.stats;
-    return rc;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!labels || !b || !p || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad system");
+        if (s.W != W || s.H != H || s.D != D) {
+            std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
+            int rc = upload_scene(s, W, H, D, L);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
+        }
+        HIPCHK(hipMemcpy(s.d_rhs, b, sizeof(float) * (size_t)W * H * D, hipMemcpyHostToDevice));
+        double save = s.cfg.solver_tol;
+        s.cfg.solver_tol = tol;
+        int rc = s.stage_pressure();
+        s.cfg.solver_tol = save;
+        HIPCHK(hipStreamSynchronize(s.st));
+        HIPCHK(hipMemcpy(p, s.d_p64, sizeof(double) * (size_t)W * H * D, hipMemcpyDeviceToHost));
+        if (stats) *stats = s.stats;
+        return rc;
+    });
 }
 
 }  // extern "C"
@@ -1175,60 +1257,74 @@ int learned_loss_grad(Solver& s, const uint8_t* labels, const float* b, const do
 extern "C" {
 
 int fs_learned_init(void* h, uint64_t seed) {
-    Solver& s = *(Solver*)h;
-    LNet* n = s.get_net();
-    if (!n || n->init_params(seed, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        LNet* n = s.get_net();
+        if (!n || n->init_params(seed, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
+        return FS_OK;
+    });
 }
 
 int fs_learned_get_params(void* h, float* params, int n) {
-    Solver& s = *(Solver*)h;
-    if (!params || n != LN_NPARAM) return fail(FS_ERR_ARG, "expected FSL_NUM_PARAMS parameters");
-    LNet* net = s.get_net();
-    if (!net || net->get_params(params, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter download failed");
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!params || n != LN_NPARAM) return fail(FS_ERR_ARG, "expected FSL_NUM_PARAMS parameters");
+        LNet* net = s.get_net();
+        if (!net || net->get_params(params, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter download failed");
+        return FS_OK;
+    });
 }
 
 int fs_learned_set_params(void* h, const float* params, int n) {
-    Solver& s = *(Solver*)h;
-    if (!params || n != LN_NPARAM) return fail(FS_ERR_ARG, "expected FSL_NUM_PARAMS parameters");
-    for (int i = 0; i < n; ++i)
-        if (!std::isfinite(params[i])) return fail(FS_ERR_ARG, "non-finite parameter");
-    LNet* net = s.get_net();
-    if (!net || net->set_params(params, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!params || n != LN_NPARAM) return fail(FS_ERR_ARG, "expected FSL_NUM_PARAMS parameters");
+        for (int i = 0; i < n; ++i)
+            if (!std::isfinite(params[i])) return fail(FS_ERR_ARG, "non-finite parameter");
+        LNet* net = s.get_net();
+        if (!net || net->set_params(params, s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
+        return FS_OK;
+    });
 }
 
 int fs_learned_apply(void* h, const uint8_t* labels, const float* r, float* z, int W, int H, int D) {
-    Solver& s = *(Solver*)h;
-    if (!z) return fail(FS_ERR_ARG, "null output");
-    DevBuf dl, dr, dz, dzn;
-    double scale;
-    int nf;
-    int rc = learned_forward_host(s, labels, r, W, H, D, dl, dr, dz, dzn, &scale, &nf);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(z, dz.p, sizeof(float) * (size_t)W * H * D, hipMemcpyDeviceToHost, s.st));
-    HIPCHK(hipStreamSynchronize(s.st));
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!z) return fail(FS_ERR_ARG, "null output");
+        DevBuf dl, dr, dz, dzn;
+        double scale;
+        int nf;
+        int rc = learned_forward_host(s, labels, r, W, H, D, dl, dr, dz, dzn, &scale, &nf);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(z, dz.p, sizeof(float) * (size_t)W * H * D, hipMemcpyDeviceToHost, s.st));
+        HIPCHK(hipStreamSynchronize(s.st));
+        return FS_OK;
+    });
 }
 
 int fs_learned_loss_grad(void* h, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
                          double* loss, float* grad) {
-    return learned_loss_grad(*(Solver*)h, labels, b, p, W, H, D, loss, grad, 0.0f, false);
+    return guarded([&]() -> int {
+        return learned_loss_grad(*(Solver*)h, labels, b, p, W, H, D, loss, grad, 0.0f, false);
+    });
 }
 
 int fs_learned_train_step(void* h, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
                           float lr, double* loss) {
-    if (!(lr > 0.0f)) return fail(FS_ERR_ARG, "learning rate must be positive");
-    return learned_loss_grad(*(Solver*)h, labels, b, p, W, H, D, loss, nullptr, lr, true);
+    return guarded([&]() -> int {
+        if (!(lr > 0.0f)) return fail(FS_ERR_ARG, "learning rate must be positive");
+        return learned_loss_grad(*(Solver*)h, labels, b, p, W, H, D, loss, nullptr, lr, true);
+    });
 }
 
 int fs_set_preconditioner(void* h, int precond, int stall_window) {
-    Solver& s = *(Solver*)h;
-    if (precond < FS_PRECOND_MG || precond > FS_PRECOND_JACOBI) return fail(FS_ERR_ARG, "unknown preconditioner");
-    s.cfg.precond = precond;
-    if (stall_window > 0) s.cfg.stall_window = stall_window;
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (precond < FS_PRECOND_MG || precond > FS_PRECOND_JACOBI) return fail(FS_ERR_ARG, "unknown preconditioner");
+        s.cfg.precond = precond;
+        if (stall_window > 0) s.cfg.stall_window = stall_window;
+        return FS_OK;
+    });
 }
 
 }  // extern "C"
@@ -1236,7 +1332,7 @@ int fs_set_preconditioner(void* h, int precond, int stall_window) {
 // ---------------------------------------------------------------- checkpoint
 namespace {
 
-constexpr char kCkptMagic[8] = {'F', 'S', 'C', 'K', 'P', 'T', '0', '1'};
+constexpr char kCkptMagic[8] = {'F', 'S', 'C', 'K', 'P', 'T', '0', '2'};
 
 struct CkptWriter {
     std::ofstream f;
@@ -1252,6 +1348,14 @@ struct CkptWriter {
 struct CkptReader {
     std::ifstream f;
     bool ok = true;
+    // bytes left after the current position
+    long long remaining() {
+        const std::streampos here = f.tellg();
+        f.seekg(0, std::ios::end);
+        const std::streampos end = f.tellg();
+        f.seekg(here);
+        return (here < 0 || end < 0) ? -1 : (long long)(end - here);
+    }
     void raw(void* p, size_t n) {
         if (n) f.read((char*)p, (std::streamsize)n);
         ok = ok && (bool)f;
@@ -1277,111 +1381,145 @@ void body_fields(HostBody& b, IO io) {
 extern "C" {
 
 int fs_save_checkpoint(void* h, const char* path) {
-    Solver& s = *(Solver*)h;
-    if (!path) return fail(FS_ERR_ARG, "null path");
-    if (s.C == 0) return fail(FS_ERR_STATE, "nothing to save: no scene");
-    HIPCHK(hipStreamSynchronize(s.st));
-    CkptWriter w;
-    w.f.open(path, std::ios::binary);
-    if (!w.f.is_open()) return fail(FS_ERR_IO, std::string("cannot write ") + path);
-    w.raw(kCkptMagic, 8);
-    const int32_t cfg_size = (int32_t)sizeof(fs_config);
-    w.put(cfg_size);
-    w.put(s.cfg);
-    w.put(s.W); w.put(s.H); w.put(s.D);
-    w.put(s.dt_cur); w.put(s.sim_time); w.put(s.nsteps);
-    const int64_t np = s.np;
-    w.put(np);
-    std::vector<uint8_t> g((size_t)s.C);
-    HIPCHK(hipMemcpy(g.data(), s.d_solid, (size_t)s.C, hipMemcpyDeviceToHost));
-    w.raw(g.data(), g.size());
-    HIPCHK(hipMemcpy(g.data(), s.d_labels, (size_t)s.C, hipMemcpyDeviceToHost));
-    w.raw(g.data(), g.size());
-    std::vector<float> a((size_t)np);
-    for (float* src : {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz}) {
-        if (np) HIPCHK(hipMemcpy(a.data(), src, sizeof(float) * np, hipMemcpyDeviceToHost));
-        w.raw(a.data(), sizeof(float) * np);
-    }
-    std::vector<int> ids((size_t)np);
-    if (np) HIPCHK(hipMemcpy(ids.data(), s.PA.id, sizeof(int) * np, hipMemcpyDeviceToHost));
-    w.raw(ids.data(), sizeof(int) * np);
-    const int32_t active = s.body.active ? 1 : 0;
-    w.put(active);
-    if (active) {
-        body_fields(s.body, [&](void* p, size_t n) { w.raw(p, n); });
-        w.raw(s.body.sdf.data(), sizeof(float) * s.body.sdf.size());
-    }
-    w.f.close();
-    if (!w.ok || w.f.fail()) return fail(FS_ERR_IO, std::string("write failed: ") + path);
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!path) return fail(FS_ERR_ARG, "null path");
+        if (s.C == 0) return fail(FS_ERR_STATE, "nothing to save: no scene");
+        HIPCHK(hipStreamSynchronize(s.st));
+        CkptWriter w;
+        w.f.open(path, std::ios::binary);
+        if (!w.f.is_open()) return fail(FS_ERR_IO, std::string("cannot write ") + path);
+        w.raw(kCkptMagic, 8);
+        const int32_t cfg_size = (int32_t)sizeof(fs_config);
+        w.put(cfg_size);
+        w.put(s.cfg);
+        w.put(s.W); w.put(s.H); w.put(s.D);
+        w.put(s.dt_cur); w.put(s.sim_time); w.put(s.nsteps);
+        const int64_t np = s.np;
+        w.put(np);
+        std::vector<uint8_t> g((size_t)s.C);
+        HIPCHK(hipMemcpy(g.data(), s.d_solid, (size_t)s.C, hipMemcpyDeviceToHost));
+        w.raw(g.data(), g.size());
+        HIPCHK(hipMemcpy(g.data(), s.d_labels, (size_t)s.C, hipMemcpyDeviceToHost));
+        w.raw(g.data(), g.size());
+        std::vector<float> a((size_t)np);
+        for (float* src : {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz}) {
+            if (np) HIPCHK(hipMemcpy(a.data(), src, sizeof(float) * np, hipMemcpyDeviceToHost));
+            w.raw(a.data(), sizeof(float) * np);
+        }
+        std::vector<int> ids((size_t)np);
+        if (np) HIPCHK(hipMemcpy(ids.data(), s.PA.id, sizeof(int) * np, hipMemcpyDeviceToHost));
+        w.raw(ids.data(), sizeof(int) * np);
+        const int32_t active = s.body.active ? 1 : 0;
+        w.put(active);
+        if (active) {
+            body_fields(s.body, [&](void* p, size_t n) { w.raw(p, n); });
+            w.raw(s.body.sdf.data(), sizeof(float) * s.body.sdf.size());
+        }
+        // the learned preconditioner's parameters (not its optimiser state)
+        const int32_t nparam = s.net ? (int32_t)LN_NPARAM : 0;
+        w.put(nparam);
+        if (nparam) {
+            std::vector<float> prm((size_t)nparam);
+            if (s.net->get_params(prm.data(), s.st) != 0) return fail(FS_ERR_DEVICE, "parameter download failed");
+            w.raw(prm.data(), sizeof(float) * prm.size());
+        }
+        w.f.close();
+        if (!w.ok || w.f.fail()) return fail(FS_ERR_IO, std::string("write failed: ") + path);
+        return FS_OK;
+    });
 }
 
 int fs_load_checkpoint(void* h, const char* path) {
-    Solver& s = *(Solver*)h;
-    if (!path) return fail(FS_ERR_ARG, "null path");
-    CkptReader r;
-    r.f.open(path, std::ios::binary);
-    if (!r.f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
-    char magic[8];
-    r.raw(magic, 8);
-    int32_t cfg_size = 0;
-    r.get(cfg_size);
-    if (!r.ok || std::memcmp(magic, kCkptMagic, 8) != 0 || cfg_size != (int32_t)sizeof(fs_config))
-        return fail(FS_ERR_IO, std::string("not a checkpoint of this version: ") + path);
-    fs_config cfg;
-    r.get(cfg);
-    int W = 0, H = 0, D = 0;
-    r.get(W); r.get(H); r.get(D);
-    float dt_cur = 0.0f;
-    double sim_time = 0.0;
-    int64_t nsteps = 0, np = 0;
-    r.get(dt_cur); r.get(sim_time); r.get(nsteps); r.get(np);
-    if (!r.ok || W <= 0 || H <= 0 || D <= 0 || np < 0 || np > 0x7fffffffLL)
-        return fail(FS_ERR_IO, std::string("corrupt checkpoint header: ") + path);
-    const size_t C = (size_t)W * H * D;
-    std::vector<uint8_t> solid(C), labels(C);
-    r.raw(solid.data(), C);
-    r.raw(labels.data(), C);
-    std::vector<float> arr[6];
-    for (auto& a : arr) {
-        a.resize((size_t)np);
-        r.raw(a.data(), sizeof(float) * np);
-    }
-    std::vector<int> ids((size_t)np);
-    r.raw(ids.data(), sizeof(int) * np);
-    int32_t active = 0;
-    r.get(active);
-    HostBody body;
-    if (r.ok && active) {
-        body.active = true;
-        body_fields(body, [&](void* p, size_t n) { r.raw(p, n); });
-        if (!(body.sw > 0 && body.sh > 0 && body.sd > 0)) return fail(FS_ERR_IO, "corrupt checkpoint body");
-        body.sdf.resize((size_t)body.sw * body.sh * body.sd);
-        r.raw(body.sdf.data(), sizeof(float) * body.sdf.size());
-    }
-    if (!r.ok) return fail(FS_ERR_IO, std::string("truncated checkpoint: ") + path);
-    // the state of the handle is replaced only once the whole file has been read
-    s.cfg = cfg;
-    int rc = s.alloc_grids(W, H, D);
-    if (rc) return rc;
-    HIPCHK(hipMemcpy(s.d_solid, solid.data(), C, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(s.d_labels, labels.data(), C, hipMemcpyHostToDevice));
-    rc = s.alloc_particles(np);
-    if (rc) return rc;
-    float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
-    for (int c = 0; c < 6; ++c)
-        if (np) HIPCHK(hipMemcpy(dst[c], arr[c].data(), sizeof(float) * np, hipMemcpyHostToDevice));
-    if (np) HIPCHK(hipMemcpy(s.PA.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    s.body = std::move(body);
-    dfree(s.d_sdf);
-    if (s.body.active) {
-        HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * s.body.sdf.size()));
-        HIPCHK(hipMemcpy(s.d_sdf, s.body.sdf.data(), sizeof(float) * s.body.sdf.size(), hipMemcpyHostToDevice));
-    }
-    s.dt_cur = dt_cur;
-    s.sim_time = sim_time;
-    s.nsteps = nsteps;
-    return FS_OK;
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!path) return fail(FS_ERR_ARG, "null path");
+        CkptReader r;
+        r.f.open(path, std::ios::binary);
+        if (!r.f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
+        char magic[8];
+        r.raw(magic, 8);
+        int32_t cfg_size = 0;
+        r.get(cfg_size);
+        if (!r.ok || std::memcmp(magic, kCkptMagic, 8) != 0 || cfg_size != (int32_t)sizeof(fs_config))
+            return fail(FS_ERR_IO, std::string("not a checkpoint of this version: ") + path);
+        fs_config cfg;
+        r.get(cfg);
+        int W = 0, H = 0, D = 0;
+        r.get(W); r.get(H); r.get(D);
+        float dt_cur = 0.0f;
+        double sim_time = 0.0;
+        int64_t nsteps = 0, np = 0;
+        r.get(dt_cur); r.get(sim_time); r.get(nsteps); r.get(np);
+        if (!r.ok || W <= 0 || H <= 0 || D <= 0 || np < 0 || np > 0x7fffffffLL)
+            return fail(FS_ERR_IO, std::string("corrupt checkpoint header: ") + path);
+        // sizes from the header are checked against the file before anything is allocated
+        const long long left = r.remaining();
+        const long long cells = (long long)W * H * D;
+        if (left < 0 || cells > (1LL << 40) || (long long)W > left || (long long)H > left ||
+            2 * cells + 28 * np + 8 > left)
+            return fail(FS_ERR_IO, std::string("checkpoint shorter than its header says: ") + path);
+        const size_t C = (size_t)W * H * D;
+        std::vector<uint8_t> solid(C), labels(C);
+        r.raw(solid.data(), C);
+        r.raw(labels.data(), C);
+        std::vector<float> arr[6];
+        for (auto& a : arr) {
+            a.resize((size_t)np);
+            r.raw(a.data(), sizeof(float) * np);
+        }
+        std::vector<int> ids((size_t)np);
+        r.raw(ids.data(), sizeof(int) * np);
+        int32_t active = 0;
+        r.get(active);
+        HostBody body;
+        if (r.ok && active) {
+            body.active = true;
+            body_fields(body, [&](void* p, size_t n) { r.raw(p, n); });
+            if (!(body.sw > 0 && body.sh > 0 && body.sd > 0)) return fail(FS_ERR_IO, "corrupt checkpoint body");
+            const long long nv = (long long)body.sw * body.sh * body.sd;
+            if (!r.ok || 4 * nv + 4 > r.remaining()) return fail(FS_ERR_IO, "corrupt checkpoint body");
+            body.sdf.resize((size_t)nv);
+            r.raw(body.sdf.data(), sizeof(float) * body.sdf.size());
+        }
+        int32_t nparam = 0;
+        r.get(nparam);
+        std::vector<float> prm;
+        if (r.ok && nparam) {
+            if (nparam != LN_NPARAM || 4LL * nparam > r.remaining())
+                return fail(FS_ERR_IO, std::string("corrupt checkpoint parameters: ") + path);
+            prm.resize((size_t)nparam);
+            r.raw(prm.data(), sizeof(float) * prm.size());
+        }
+        if (!r.ok) return fail(FS_ERR_IO, std::string("truncated checkpoint: ") + path);
+        // the state of the handle is replaced only once the whole file has been read
+        s.cfg = cfg;
+        int rc = s.alloc_grids(W, H, D);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(s.d_solid, solid.data(), C, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(s.d_labels, labels.data(), C, hipMemcpyHostToDevice));
+        rc = s.alloc_particles(np);
+        if (rc) return rc;
+        float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};
+        for (int c = 0; c < 6; ++c)
+            if (np) HIPCHK(hipMemcpy(dst[c], arr[c].data(), sizeof(float) * np, hipMemcpyHostToDevice));
+        if (np) HIPCHK(hipMemcpy(s.PA.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+        s.body = std::move(body);
+        dfree(s.d_sdf);
+        if (s.body.active) {
+            HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * s.body.sdf.size()));
+            HIPCHK(hipMemcpy(s.d_sdf, s.body.sdf.data(), sizeof(float) * s.body.sdf.size(), hipMemcpyHostToDevice));
+        }
+        if (!prm.empty()) {
+            LNet* n = s.get_net();
+            if (!n || n->set_params(prm.data(), s.st) != 0) return fail(FS_ERR_DEVICE, "parameter upload failed");
+        }
+        s.learned.reset();
+        s.dt_cur = dt_cur;
+        s.sim_time = sim_time;
+        s.nsteps = nsteps;
+        return FS_OK;
+    });
 }
 
 }  // extern "C"

@@ -6,10 +6,14 @@
  * 13-186).  A checkpoint is one binary file holding everything the next step
  * reads: the configuration, the grid size, the solid mask and labels, the
  * particles (positions, velocities, original ids, in their current order),
- * the rigid body (state and SDF) and the clock (time, step count, current dt).
+ * the rigid body (state and SDF), the clock (time, step count, current dt)
+ * and, when the handle has one, the learned preconditioner's parameters (its
+ * optimiser state is not stored: training is not part of a step).
  * Grid velocities and pressure are not stored: every step rebuilds them from
  * the particles.  The format is host-endian and versioned; loading checks the
- * header and sizes and fails with FS_ERR_IO on a mismatch.
+ * header, and every size it reads against the bytes left in the file before
+ * allocating, and fails with FS_ERR_IO on a mismatch (the handle is then
+ * unchanged).
  */
 #ifndef FLUIDSIM_CHECKPOINT_H
 #define FLUIDSIM_CHECKPOINT_H

@@ -1099,6 +1099,32 @@ int write_particles(const Solver& s, const char* path, int fmt) {
     return FS_OK;
 }
 
+/* RigidBody::generateSurfacePoints(0.5 dx) + exportToPLY utility.cuh:341-400,
+ * written by run() for an active body (FluidSolver3D.cu:3129-3131): the SDF
+ * voxels with |phi| < 0.5 dx, mapped to the world frame, as ASCII PLY */
+void write_body_ply(const Solver& s, const std::string& path) {
+    const Body& b = s.body;
+    const float density = 0.5f * s.cfg.dx;
+    std::vector<f3> pts;
+    for (int k = 0; k < b.sd; ++k)
+        for (int j = 0; j < b.sh; ++j)
+            for (int i = 0; i < b.sw; ++i) {
+                float v = b.sdf[(size_t)i + (size_t)j * b.sw + (size_t)k * b.sw * b.sh];
+                if (!(std::fabs(v) < density)) continue;
+                f3 pl = b.fileOrigin + mk(i * b.cs, j * b.cs, k * b.cs);
+                f3 pc = pl - b.local_com;
+                f3 pr = mk(b.R[0] * pc.x + b.R[1] * pc.y + b.R[2] * pc.z,
+                           b.R[3] * pc.x + b.R[4] * pc.y + b.R[5] * pc.z,
+                           b.R[6] * pc.x + b.R[7] * pc.y + b.R[8] * pc.z);
+                pts.push_back(b.pos + pr);
+            }
+    std::ofstream f(path);
+    if (!f.is_open()) return;
+    f << "ply\nformat ascii 1.0\nelement vertex " << pts.size()
+      << "\nproperty float x\nproperty float y\nproperty float z\nend_header\n";
+    for (auto& p : pts) f << p.x << " " << p.y << " " << p.z << "\n";
+}
+
 /* CFL-limited dt (no reference counterpart): dt = min(dt_max, cfl*dx/max|v|) */
 void choose_dt(Solver& s) {
     s.dt_cur = s.cfg.dt;
@@ -1227,6 +1253,8 @@ int fs_config_load_json(fs_config* c, const char* path) {
     if (json_find(t, "body_y", v)) c->body_y = std::stof(v);
     if (json_find(t, "body_z", v)) c->body_z = std::stof(v);
     if (json_find(t, "body_mass", v)) c->body_mass = std::stof(v);
+    /* the reference hard-codes InputData/ball.sdf (FluidSolver3D.cu:160); this key overrides it */
+    if (json_find(t, "bodySdf", v)) std::snprintf(c->body_sdf_path, sizeof(c->body_sdf_path), "%s", v.c_str());
     return FS_OK;
 }
 
@@ -1401,6 +1429,7 @@ int fs_run(void* h, int frames) {
             std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
                              (s.cfg.output_format == FS_FMT_OFF ? ".off" : ".ply");
             write_particles(s, fn.c_str(), s.cfg.output_format);
+            if (s.body.active) write_body_ply(s, "OutputData/body_" + std::to_string(i / ipf) + ".ply");
             if (csv.is_open())
                 csv << i / ipf << "," << s.body.pos.x << "," << s.body.pos.y << "," << s.body.pos.z << ","
                     << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";

@@ -2,17 +2,21 @@
 
 Bars: save -> load -> save reproduces the file byte for byte; a run resumed
 from a checkpoint follows the uninterrupted run (positions within 1e-4 of the
-unit domain and body position within 1e-5 after three more steps: the P2G
-face sums are FP32 atomics whose order is not fixed, so the runs are not
-bit-identical); a truncated file fails with FS_ERR_IO and leaves the handle
-usable."""
+unit domain and body position within 1e-5 after three more steps: the order
+of the particles within a cell comes from atomic counters, and the P2G sums
+follow that order, so the runs are not bit-identical); a truncated file, or
+one whose header gives sizes the file does not hold, fails with FS_ERR_IO
+before anything is allocated and leaves the handle usable; the learned
+preconditioner's parameters are restored."""
+import ctypes as C
 import os
+import struct
 
 import numpy as np
 import pytest
 
 from fluidprjct_amd import scenes
-from fluidprjct_amd._abi import FsError
+from fluidprjct_amd._abi import FSL_NUM_PARAMS, FsConfig, FsError
 
 pytestmark = pytest.mark.gpu
 
@@ -69,3 +73,36 @@ def test_truncated_checkpoint_is_rejected(hip, tmp_path):
     bad.write_bytes(b"not a checkpoint" + data[16:64])
     with pytest.raises(FsError):
         b.load_checkpoint(bad)
+
+
+def test_header_sizes_checked_before_allocating(hip, tmp_path):
+    a = make(hip)
+    a.step(1)
+    ck = tmp_path / "a.ckpt"
+    a.save_checkpoint(ck)
+    data = ck.read_bytes()
+    off = 8 + 4 + C.sizeof(FsConfig)          # magic, config size, config; then W, H, D
+    b = make(hip)
+    n0 = b.num_particles
+    for fmt, at, vals in (("<iii", off, (100000, 100000, 100000)),       # 1e15 cells
+                          ("<q", off + 12 + 4 + 8 + 8, (2 ** 31 - 1,))):  # particle count
+        bad = bytearray(data)
+        struct.pack_into(fmt, bad, at, *vals)
+        p = tmp_path / "bad.ckpt"
+        p.write_bytes(bytes(bad))
+        with pytest.raises(FsError) as e:
+            b.load_checkpoint(p)
+        assert e.value.code == -2 and "shorter than its header" in str(e.value)
+    assert b.num_particles == n0
+    b.step(1)
+
+
+def test_checkpoint_keeps_learned_parameters(hip, tmp_path):
+    a = make(hip)
+    prm = np.random.default_rng(3).standard_normal(FSL_NUM_PARAMS).astype(np.float32) * 0.1
+    a.set_learned_params(prm)
+    ck = tmp_path / "l.ckpt"
+    a.save_checkpoint(ck)
+    b = hip.solver()
+    b.load_checkpoint(ck)
+    assert np.array_equal(b.learned_params(), prm)

@@ -4,8 +4,8 @@ Bars:
   - bit-exact: seeding, particle -> cell keys, labels, RHS, pressure
     projection, G2P + advection (same inputs on both sides; both sides keep the
     reference's FP32 operation order without contraction);
-  - P2G: relative 2e-6 of the face magnitude (FP32 LDS atomics vs the
-    oracle's FP64 sums, both rounded once);
+  - P2G: relative 2e-6 of the face magnitude (FP32 sums per halo cell, added
+    per face in a fixed order, vs the oracle's FP64 sums in particle order);
   - pressure: ||p - p_ref|| / ||p_ref|| <= 1e-5 against FP64 direct solves
     (scipy spsolve golden vectors and the oracle's band LDL^T);
   - whole steps: particle positions within 2e-5 * domain after 3 steps.
