@@ -121,6 +121,14 @@ _CKPT_SIGS = {
     "fs_load_checkpoint": (C.c_int, [_vp, C.c_char_p]),
 }
 CHECKPOINT_SYMBOLS = tuple(_CKPT_SIGS)
+
+# include/fluidsim_noalloc.h (HIP library only)
+_NOALLOC_SIGS = {
+    "fs_solve_pressure_system_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                   C.POINTER(FsSolveStats)]),
+}
+NOALLOC_SYMBOLS = tuple(_NOALLOC_SIGS)
+FS_NOALLOC_MAX_CELLS = 1 << 20
 FSL_NUM_PARAMS = 27 * 3 * 16 + 16 + 3 * (27 * 16 * 16 + 16) + 27 * 16 + 1
 
 
@@ -138,7 +146,9 @@ class FsLib:
             fn.argtypes = args
         self.has_learned = all(hasattr(self.lib, n) for n in _LEARNED_SIGS)
         self.has_checkpoint = all(hasattr(self.lib, n) for n in _CKPT_SIGS)
-        for present, sigs in ((self.has_learned, _LEARNED_SIGS), (self.has_checkpoint, _CKPT_SIGS)):
+        self.has_noalloc = all(hasattr(self.lib, n) for n in _NOALLOC_SIGS)
+        for present, sigs in ((self.has_learned, _LEARNED_SIGS), (self.has_checkpoint, _CKPT_SIGS),
+                              (self.has_noalloc, _NOALLOC_SIGS)):
             if present:
                 for name, (res, args) in sigs.items():
                     fn = getattr(self.lib, name)
@@ -162,6 +172,20 @@ class FsLib:
 
     def solver(self, cfg: FsConfig | None = None, **kw) -> "Solver":
         return Solver(self, cfg if cfg is not None else self.default_config(**kw))
+
+    def solve_pressure_system_noalloc(self, labels, b, tol):
+        """fs_solve_pressure_system_noalloc (include/fluidsim_noalloc.h): no
+        handle, no device allocation; (p, stats)."""
+        if not self.has_noalloc:
+            raise RuntimeError(f"{self.path} has no allocation-free solve")
+        lab = np.ascontiguousarray(labels, dtype=np.uint8)
+        bb = np.ascontiguousarray(b, dtype=np.float32)
+        D, H, W = lab.shape
+        p = np.zeros((D, H, W), np.float64)
+        st = FsSolveStats()
+        self.check(self.lib.fs_solve_pressure_system_noalloc(_ptr(lab), _ptr(bb), _ptr(p), W, H, D, float(tol),
+                                                             C.byref(st)))
+        return p, st
 
 
 def _ptr(a: np.ndarray):

@@ -21,8 +21,10 @@
 #include "../../include/fluidsim.h"
 #include "../../include/fluidsim_checkpoint.h"
 #include "../../include/fluidsim_learned.h"
+#include "../../include/fluidsim_noalloc.h"
 #include "fs_kernels.hpp"
 #include "fs_learned.hpp"
+#include "fs_noalloc.hpp"
 #include "fs_pressure.hpp"
 
 using fsd::f3;
@@ -1518,6 +1520,93 @@ int fs_load_checkpoint(void* h, const char* path) {
         s.dt_cur = dt_cur;
         s.sim_time = sim_time;
         s.nsteps = nsteps;
+        return FS_OK;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free solve
+namespace {
+
+// Host state of the allocation-free solve on one device: the solver object
+// (its level list reserved at set-up) and the host mirrors of its scalars.
+// Created on the first call on a device and kept until the process ends.
+struct NoAllocState {
+    MGPCG mg;
+    SolveScalars sc_host{};
+    int host_counters[MGPCG::HOST_COUNTERS] = {};
+};
+constexpr int kNoAllocDevices = 64;
+NoAllocState* g_noalloc[kNoAllocDevices] = {};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+int fs_solve_pressure_system_noalloc(const uint8_t* labels, const float* b, double* p, int W, int H, int D,
+                                     double tol, fs_solve_stats* stats) {
+    return guarded([&]() -> int {
+        if (!labels || !b || !p || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad system");
+        const long long C = (long long)W * H * D;
+        if (C > FS_NOALLOC_MAX_CELLS)
+            return fail(FS_ERR_ARG, "allocation-free solve: " + std::to_string(C) + " cells, at most " +
+                                        std::to_string(FS_NOALLOC_MAX_CELLS));
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kNoAllocDevices) return fail(FS_ERR_DEVICE, "device index out of range");
+        hipError_t e = hipSuccess;
+        char* base = (char*)noalloc_scratch(dev, &e);
+        if (!base) return fail(FS_ERR_DEVICE, std::string("workspace address: ") + hipGetErrorString(e));
+        if (!g_noalloc[dev]) {
+            // set-up, once per device: host objects only
+            NoAllocState* st = new NoAllocState();
+            st->mg.lv.reserve(40);
+            g_noalloc[dev] = st;
+        }
+        NoAllocState& st = *g_noalloc[dev];
+        // staging of labels, b, p (FP64) and p (FP32), then the solver's workspace
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            off = align256(off);
+            char* q = base + off;
+            off += bytes;
+            return (void*)q;
+        };
+        uint8_t* dl = (uint8_t*)carve((size_t)C);
+        float* db = (float*)carve(sizeof(float) * (size_t)C);
+        double* dp64 = (double*)carve(sizeof(double) * (size_t)C);
+        float* dp = (float*)carve(sizeof(float) * (size_t)C);
+        off = align256(off);
+        size_t used = 0;
+        int rc = FS_NOALLOC_SCRATCH_BYTES > off
+                     ? st.mg.init_in(W, H, D, base + off, FS_NOALLOC_SCRATCH_BYTES - off, &st.sc_host,
+                                     st.host_counters, &used)
+                     : -2;
+        if (rc == -2)
+            return fail(FS_ERR_ARG, "allocation-free solve: a " + std::to_string(W) + " x " + std::to_string(H) +
+                                        " x " + std::to_string(D) + " system needs " + std::to_string(off + used) +
+                                        " bytes of workspace, the limit is " +
+                                        std::to_string((size_t)FS_NOALLOC_SCRATCH_BYTES));
+        if (rc) return fail(FS_ERR_DEVICE, "allocation-free solve: workspace set-up failed");
+        HIPCHK(hipMemcpy(dl, labels, (size_t)C, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(db, b, sizeof(float) * (size_t)C, hipMemcpyHostToDevice));
+        // the settings of a default handle (fs_config_default), launched eagerly
+        // (the same bits as the graph replays of fs_solve_pressure_system)
+        st.mg.stall_window = 5;
+        st.mg.use_graph = false;
+        st.mg.use_tail = true;
+        st.mg.use_tb = true;
+        fs_solve_stats loc{};
+        rc = st.mg.solve(dl, db, dp64, dp, tol, 500, nullptr, (hipStream_t)0, &loc);
+        if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
+        HIPCHK(hipStreamSynchronize((hipStream_t)0));
+        HIPCHK(hipMemcpy(p, dp64, sizeof(double) * (size_t)C, hipMemcpyDeviceToHost));
+        if (stats) *stats = loc;
+        if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");
+        if (rc == -4) return fail(FS_ERR_NOT_CONVERGED, "pressure solve did not reach the tolerance");
         return FS_OK;
     });
 }

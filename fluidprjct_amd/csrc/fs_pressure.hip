@@ -154,6 +154,7 @@ FS_DEV void store_partial(double v, double* sm, double* __restrict__ partial) {
 // a multiple of 16 cells), splitting only a chunk's first index into (i, j, k)
 // and stepping along the rows from there.
 constexpr int BBOX_BLOCKS = 256;
+static_assert(MGPCG::HOST_COUNTERS == MGPCG::COUNTERS_BBOX + 6 * BBOX_BLOCKS, "host counter mirror size");
 template <bool VEC16>
 __global__ __launch_bounds__(256) void k_bbox(const uint8_t* __restrict__ labels, int W, int H, long long n,
                                               int* __restrict__ part) {
@@ -1658,18 +1659,20 @@ MGPCG::~MGPCG() { free_all(); }
 
 void MGPCG::free_all() {
     clear_graphs();
-    for (size_t l = 0; l < lv.size(); ++l) {
-        MGLevel& L = lv[l];
-        hipFree(L.code); hipFree(L.type);
-        if (l > 0) { hipFree(L.b); hipFree(L.x); }
-        hipFree(L.xt); hipFree(L.r);
+    if (owns_memory) {
+        for (size_t l = 0; l < lv.size(); ++l) {
+            MGLevel& L = lv[l];
+            hipFree(L.code); hipFree(L.type);
+            if (l > 0) { hipFree(L.b); hipFree(L.x); }
+            hipFree(L.xt); hipFree(L.r);
+        }
+        hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(bb); hipFree(xb);
+        hipFree(partial); hipFree(sc); hipFree(counters);
+        if (sc_host) hipHostFree(sc_host);
+        if (host_counters) hipHostFree(host_counters);
     }
     lv.clear();
     nlev = 0;
-    hipFree(r); hipFree(z); hipFree(pa); hipFree(pb); hipFree(q); hipFree(bb); hipFree(xb);
-    hipFree(partial); hipFree(sc); hipFree(counters);
-    if (sc_host) hipHostFree(sc_host);
-    if (host_counters) hipHostFree(host_counters);
     r = z = pa = pb = q = bb = nullptr;
     xb = nullptr;
     partial = nullptr;
@@ -1691,15 +1694,59 @@ static int level_dims(int W, int H, int D, long long coarse_max, int dims[][3], 
     return n;
 }
 
-int MGPCG::init(int W_, int H_, int D_) {
+int MGPCG::init(int W_, int H_, int D_) { return init_impl(W_, H_, D_, nullptr, 0, nullptr, nullptr, nullptr); }
+
+int MGPCG::init_in(int W_, int H_, int D_, void* base, size_t bytes, SolveScalars* sc_host_buf,
+                   int* host_counters_buf, size_t* used) {
+    if (!base || !sc_host_buf || !host_counters_buf) return -1;
+    return init_impl(W_, H_, D_, base, bytes, sc_host_buf, host_counters_buf, used);
+}
+
+// Device buffers from hipMalloc (base == nullptr), or carved in order from
+// [base, base + cap) at 256-byte alignment with the host mirrors given by the
+// caller (nothing allocated; *used = the bytes the carving needs, also when
+// they exceed cap, in which case nothing is carved and -2 is returned).
+int MGPCG::init_impl(int W_, int H_, int D_, void* base, size_t cap, SolveScalars* sc_host_buf,
+                     int* host_counters_buf, size_t* used) {
     free_all();
+    owns_memory = base == nullptr;
     coarse_max = std::min<long long>(coarse_max, COARSE_MAX);
     W = W_; H = H_; D = D_;
     N = (long long)W * H * D;
     int dims[40][3];
     int n = level_dims(W, H, D, coarse_max, dims, 40);
     bool ok = true;
-    auto A = [&](void** p, size_t bytes) { if (hipMalloc(p, bytes) != hipSuccess) ok = false; };
+    size_t off = 0;
+    if (base) {
+        // the size of the carving first, so that a workspace that is too small
+        // leaves every pointer null
+        size_t need = 0;
+        auto add = [&](size_t b) { need = ((need + 255) & ~(size_t)255) + b; };
+        for (int l = 0; l < n; ++l) {
+            const size_t ln = (size_t)dims[l][0] * dims[l][1] * dims[l][2];
+            add(sizeof(uint16_t) * ln);
+            add(ln);
+            if (l > 0) { add(sizeof(float) * ln); add(sizeof(float) * ln); }
+            add(sizeof(float) * ln);
+            add(sizeof(float) * ln);
+        }
+        for (int v = 0; v < 6; ++v) add(sizeof(float) * (size_t)N);
+        add(sizeof(double) * (size_t)N);
+        add(sizeof(double) * RED_BLOCKS * PS_COUNT);
+        add(sizeof(SolveScalars));
+        add(sizeof(int) * (COUNTERS_BBOX + 6 * BBOX_BLOCKS));
+        if (used) *used = need;
+        if (need > cap) return -2;
+    }
+    auto A = [&](void** p, size_t bytes) {
+        if (base) {
+            off = (off + 255) & ~(size_t)255;
+            *p = (char*)base + off;
+            off += bytes;
+        } else if (hipMalloc(p, bytes) != hipSuccess) {
+            ok = false;
+        }
+    };
     lv.resize(n);
     for (int l = 0; l < n; ++l) {
         MGLevel& L = lv[l];
@@ -1720,10 +1767,17 @@ int MGPCG::init(int W_, int H_, int D_) {
     A((void**)&partial, sizeof(double) * RED_BLOCKS * PS_COUNT);
     A((void**)&sc, sizeof(SolveScalars));
     A((void**)&counters, sizeof(int) * (COUNTERS_BBOX + 6 * BBOX_BLOCKS));
-    if (hipHostMalloc((void**)&sc_host, sizeof(SolveScalars)) != hipSuccess) ok = false;
-    if (hipHostMalloc((void**)&host_counters, sizeof(int) * (COUNTERS_BBOX + 6 * BBOX_BLOCKS)) != hipSuccess) ok = false;
+    if (base) {
+        sc_host = sc_host_buf;
+        host_counters = host_counters_buf;
+    } else {
+        if (hipHostMalloc((void**)&sc_host, sizeof(SolveScalars)) != hipSuccess) ok = false;
+        if (hipHostMalloc((void**)&host_counters, sizeof(int) * (COUNTERS_BBOX + 6 * BBOX_BLOCKS)) != hipSuccess)
+            ok = false;
+    }
     if (!ok) { free_all(); return -1; }
     // CG direction buffers start finite: the first update multiplies them by beta = 0
+    // (carved memory holds the previous call's contents: cleared the same way)
     hipMemset(pa, 0, sizeof(float) * N);
     hipMemset(pb, 0, sizeof(float) * N);
     hipMemset(partial, 0, sizeof(double) * RED_BLOCKS * PS_COUNT);

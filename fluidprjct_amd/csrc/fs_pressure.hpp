@@ -60,6 +60,12 @@ public:
     ~MGPCG();
     // allocate the hierarchy for a W x H x D grid
     int init(int W, int H, int D);
+    // the same hierarchy carved from a caller's device workspace of `bytes`
+    // bytes, with caller-owned host mirrors; nothing is allocated (the level
+    // list only grows its host vector up to the capacity reserved before).
+    // *used: the bytes needed.  -2 when they exceed `bytes`.
+    int init_in(int W, int H, int D, void* base, size_t bytes, SolveScalars* sc_host_buf, int* host_counters_buf,
+                size_t* used);
     // Solve A x = b (b: full-grid float, 0 off fluid).  x64 (full grid, double)
     // and p (float, 0 off fluid) are written.  Returns 0 on convergence.
     int solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
@@ -107,6 +113,7 @@ public:
     float* bb = nullptr;           // b on the solve box
     double* xb = nullptr;          // FP64 solution on the solve box
     static constexpr int COUNTERS_BBOX = 16;   // per-block bounding boxes from here on
+    static constexpr int HOST_COUNTERS = COUNTERS_BBOX + 6 * 256;   // ints in the host mirror
     int* counters = nullptr;       // device: unknown count [8], Dirichlet flag [9], per-block bbox [16..]
     int* host_counters = nullptr;  // pinned mirror
     double* partial = nullptr;     // partial-sum slots of the reducing kernels
@@ -138,6 +145,9 @@ private:
     hipGraphExec_t batch_graph(Preconditioner* pre, int nb, hipStream_t st);
     void clear_graphs();
     void free_all();
+    int init_impl(int W, int H, int D, void* base, size_t bytes, SolveScalars* sc_host_buf, int* host_counters_buf,
+                  size_t* used);
+    bool owns_memory = true;   // buffers from hipMalloc (else carved from a caller's workspace)
     float* vcycle_level(int l, hipStream_t st, double* partial_dot, SolveScalars* sc);
 };
 

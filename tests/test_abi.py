@@ -13,6 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fluidsim.h")
 LEARNED_HEADER = os.path.join(ROOT, "include", "fluidsim_learned.h")
 CKPT_HEADER = os.path.join(ROOT, "include", "fluidsim_checkpoint.h")
+NOALLOC_HEADER = os.path.join(ROOT, "include", "fluidsim_noalloc.h")
 
 
 def header_functions(path=HEADER):
@@ -25,6 +26,10 @@ def test_header_and_ctypes_agree():
     assert header_functions() == sorted(_abi.ABI_SYMBOLS)
     assert header_functions(LEARNED_HEADER) == sorted(_abi.LEARNED_SYMBOLS)
     assert header_functions(CKPT_HEADER) == sorted(_abi.CHECKPOINT_SYMBOLS)
+    assert header_functions(NOALLOC_HEADER) == sorted(_abi.NOALLOC_SYMBOLS)
+    txt = open(NOALLOC_HEADER).read()
+    assert int(eval(__import__("re").search(r"#define FS_NOALLOC_MAX_CELLS (.*)", txt).group(1))) == \
+        _abi.FS_NOALLOC_MAX_CELLS
 
 
 def test_learned_parameter_count_matches_header():
@@ -36,10 +41,11 @@ def test_learned_parameter_count_matches_header():
 def test_hip_library_exports_learned_symbols():
     path = os.path.join(ROOT, "fluidprjct_amd", "libfluidsim_hip.so")
     dll = C.CDLL(path)
-    missing = [f for f in header_functions(LEARNED_HEADER) + header_functions(CKPT_HEADER) if not hasattr(dll, f)]
+    missing = [f for f in header_functions(LEARNED_HEADER) + header_functions(CKPT_HEADER) +
+               header_functions(NOALLOC_HEADER) if not hasattr(dll, f)]
     assert not missing, missing
     fl = _abi.FsLib(path)
-    assert fl.has_learned and fl.has_checkpoint
+    assert fl.has_learned and fl.has_checkpoint and fl.has_noalloc
 
 
 @pytest.mark.parametrize("lib", ["hip", "oracle"])
