@@ -83,13 +83,61 @@ def stage_delta(a, b):
     return {k: b[k] - a[k] for k in a}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def face_gradient_norm(dp, lab):
+    """L2 norm of the pressure differences across the faces that the projection
+    updates with p (both cells FLUID or AIR, FluidSolver3D.cu:2072-2336)."""
+    ok = (lab == 1) | (lab == 2)
+    tot = 0.0
+    for ax in (2, 1, 0):
+        a = [slice(None)] * 3
+        b = [slice(None)] * 3
+        a[ax], b[ax] = slice(1, None), slice(None, -1)
+        m = ok[tuple(a)] & ok[tuple(b)]
+        d = (dp[tuple(a)] - dp[tuple(b)])[m]
+        tot += float(d @ d)
+    return tot ** 0.5
+
+
+def projected_velocity_norm(s, lab):
+    from fluidprjct_amd._abi import FS_GRID_U, FS_GRID_V, FS_GRID_W
+    tot = 0.0
+    fl = lab == 1
+    for g, ax in ((FS_GRID_U, 2), (FS_GRID_V, 1), (FS_GRID_W, 0)):
+        f = s.get_grid(g).astype(np.float64)
+        a = [slice(None)] * 3
+        b = [slice(None)] * 3
+        a[ax], b[ax] = slice(1, None), slice(None, -1)
+        # faces with a FLUID cell on either side
+        m = np.zeros(f.shape, bool)
+        m[tuple(a)] |= fl
+        m[tuple(b)] |= fl
+        tot += float((f[m] ** 2).sum())
+    return tot ** 0.5
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--full-steps", type=int, default=200,
+                    help="length of the whole workload run after the timed steps (c3: 200 steps); 0 skips it")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the inner PCG iterations eagerly")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
@@ -117,18 +165,37 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
         return
     from fluidprjct_amd import load_library
+    from fluidprjct_amd._abi import FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS
     hip = load_library()
 
     N, lab, body, ppc, seed, desc = build_scene(args.config)
+    log(f"{args.config}: scene {lab.shape}, seeding")
     s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0)
     Np = s.num_particles
-    s.step(args.warmup)
+    step_ms = []          # every step of the run, host wall clock (each fs_step returns synchronised)
+    submerged = []
+    t_run0 = time.perf_counter()
+
+    def run_steps(n):
+        for _ in range(n):
+            t = time.perf_counter()
+            s.step(1)
+            step_ms.append(1e3 * (time.perf_counter() - t))
+            if body is not None:
+                submerged.append(s.body_state().submerged_volume)
+
+    log(f"{Np} particles; {args.warmup} warmup steps")
+    run_steps(args.warmup)
     torch.cuda.synchronize()
     t_stage0 = s.stage_times()
+    log(f"{args.steps} timed steps")
     t0 = time.perf_counter()
     s.step(args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    step_ms.extend([1e3 * elapsed / args.steps] * args.steps)
+    if body is not None:
+        submerged.extend([s.body_state().submerged_volume] * args.steps)
     t_stage = stage_delta(t_stage0, s.stage_times())
     K = args.steps
     st = s.solve_stats()
@@ -137,7 +204,7 @@ def main():
     D, H, W = lab.shape
     C = W * H * D
     F = ((W + 1) * H * D + W * (H + 1) * D + W * H * (D + 1)) // 3   # faces per component (mean)
-    labs = s.get_grid(7)
+    labs = s.get_grid(FS_GRID_LABELS)
     nf = int((labs == 1).sum())
     ms = {k: v / K for k, v in t_stage.items()}
     # algorithmic bytes per launch (SoA FP32 particles 24 B, uint8 labels, FP32 faces)
@@ -175,9 +242,42 @@ def main():
                      "traffic": measured_traffic(KERNEL_NAMES[dom], args.config), "alg_bytes": alg[dom],
                      "avg_launch_ms": ms[dom]},
     }
+    # the rest of the workload (c3: 200 steps from the released sphere, through
+    # the water impact), continuing the same run; steps/s over every step
+    if args.full_steps > len(step_ms):
+        log(f"continuing to step {args.full_steps}")
+        run_steps(args.full_steps - len(step_ms))
+        torch.cuda.synchronize()
+        tot_ms = sum(step_ms)
+        tall = s.stage_times()   # accumulated over every step of the handle
+        full = {"steps": len(step_ms), "steps_per_s": len(step_ms) / (tot_ms * 1e-3),
+                "wall_s_incl_readback": time.perf_counter() - t_run0,
+                "stage_ms_mean": {k: round(v / len(step_ms), 4) for k, v in tall.items()},
+                "note": "the timed steps are steps warmup+1..warmup+steps of this run; host wall clock per "
+                        "step (each step returns synchronised; the body state is read after every step "
+                        "outside the timed block)"}
+        if submerged:
+            hit = next((i for i, v in enumerate(submerged) if v > 0), None)
+            full["impact_step"] = hit
+            if hit:
+                pre, post = step_ms[1:hit], step_ms[hit:]
+                full["pre_impact_steps_per_s"] = len(pre) / (sum(pre) * 1e-3) if pre else None
+                full["post_impact_steps_per_s"] = len(post) / (sum(post) * 1e-3) if post else None
+            full["body_y_final"] = s.body_state().pos[1]
+        st2 = s.solve_stats()
+        full["pressure_iters_last"] = st2.iterations
+        full["pressure_rel_residual_last"] = st2.rel_residual
+        result["full_run"] = full
+    # the system of the last step, kept for the FP64 check below
+    sys_lab = s.get_grid(FS_GRID_LABELS)
+    sys_b = s.get_grid(FS_GRID_RHS)
+    sys_p = s.get_grid(FS_GRID_P64)
+    sys_dt = s.clock()[1]
+    u_norm = projected_velocity_norm(s, sys_lab)
     # learned vs plain: the same workload continued with the learned
     # preconditioner (flexible PCG, multigrid fallback after 5 stalled iterations)
     if not args.no_learned:
+        log("learned-preconditioner steps")
         from fluidprjct_amd.learned import DEFAULT_PARAMS, use_learned
         use_learned(s, DEFAULT_PARAMS, 5)
         s.step(1)
@@ -201,17 +301,43 @@ def main():
     torch.cuda.empty_cache()
 
     if not args.no_cpu:
+        # the CPU leg: the oracle timed on a bounded sample of the workload, and
+        # the oracle's FP64 PCG as the checker of the last GPU pressure system
         from fluidprjct_amd._abi import FsLib
         orc = FsLib(os.path.join(ROOT, "oracle", "_build", "libfs_oracle.so"))
+        threads = int(orc.lib.oracle_num_threads()) if hasattr(orc.lib, "oracle_num_threads") else 1
+        log(f"FP64 reference solve of the last step's system (oracle, {threads} threads)")
+        t0 = time.perf_counter()
+        o = orc.solver()
+        pref, sref = o.solve_pressure_system(sys_lab, sys_b, 1e-12)
+        tref = time.perf_counter() - t0
+        fl = sys_lab == 1
+        dp = np.where(fl, sys_p - pref, 0.0)
+        scale = sys_dt / (1000.0 * (1.0 / N))   # dt / (rho dx): the projection's pressure-to-velocity factor
+        result["error_vs_fp64"] = {
+            "pressure_rel_l2": float(np.linalg.norm(dp[fl]) / np.linalg.norm(pref[fl])),
+            "velocity_rel_l2": float(scale * face_gradient_norm(dp, sys_lab) / u_norm) if u_norm > 0 else None,
+            "reference": f"oracle FP64 MIC(0)-PCG to 1e-12 relative residual ({sref.iterations} iterations, "
+                         f"{tref:.1f} s); system of the last GPU step (step {len(step_ms)})",
+            "velocity_note": "dt/(rho dx) |grad(p_gpu - p_fp64)| over the faces the projection updates, "
+                             "relative to |u| on the faces of fluid cells after projection",
+        }
+        del o
+        log(f"CPU baseline: {args.cpu_steps} oracle step(s)")
         o = make_solver(orc, N, lab, body, ppc, seed)
         t0 = time.perf_counter()
-        o.step(args.cpu_steps)
+        for i in range(args.cpu_steps):
+            o.step(1)
+            log(f"  oracle step {i + 1} done")
         ct = time.perf_counter() - t0
-        threads = int(orc.lib.oracle_num_threads()) if hasattr(orc.lib, "oracle_num_threads") else 1
         result["cpu_baseline"] = {
             "value": args.cpu_steps / ct, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"first {args.cpu_steps} step(s) of the same workload from the same seeded state "
-                      f"(oracle C++ -O3, OpenMP {threads} threads, FP64 MIC(0)-PCG to 1e-6)",
+            "sample": f"first {args.cpu_steps} step(s) of the same workload from the same seeded state (oracle "
+                      f"C++ -O3; OpenMP {threads} threads in labels, RHS, projection, G2P/advection and the "
+                      f"CG's SpMV and dots; P2G, the MIC(0) triangular solves and the body sums serial; FP64 "
+                      f"MIC(0)-PCG to 1e-6)",
+            "cpu_model": cpu_model(),
+            "host_cpus": len(os.sched_getaffinity(0)),
             "stage_ms": {k: round(v / args.cpu_steps, 1) for k, v in o.stage_times().items()},
         }
     print(json.dumps(result))
