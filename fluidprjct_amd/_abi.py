@@ -18,6 +18,7 @@ FS_PRECOND_MG, FS_PRECOND_LEARNED, FS_PRECOND_JACOBI = 0, 1, 2
 FS_SOLVER_NO_GRAPH = 1
 FS_SOLVER_NO_TAIL = 2
 FS_SOLVER_NO_FUSE = 4
+FS_SOLVER_COLD_START = 8
 (FS_GRID_U, FS_GRID_V, FS_GRID_W, FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED, FS_GRID_P,
  FS_GRID_LABELS, FS_GRID_RHS, FS_GRID_P64, FS_GRID_CELLKEY, FS_GRID_SOLID) = range(12)
 STAGES = ("label", "p2g", "pressure", "project", "body", "g2p_advect", "sort")

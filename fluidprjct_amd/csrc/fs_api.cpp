@@ -192,6 +192,7 @@ struct Solver {
     int64_t nsteps = 0;
     int64_t nonconv = 0;        // steps whose pressure solve missed the tolerance
     double nonconv_res = 0.0;
+    bool p64_from_step = false; // d_p64 holds the last step's pressure (the next solve's initial guess)
     unsigned* d_umax = nullptr; // max |v| of the particle velocities (float bits)
     bool umax_valid = false;
 
@@ -255,6 +256,7 @@ struct Solver {
         HIPCHK(hipMemset(d_rhs, 0, sizeof(float) * C));
         HIPCHK(hipMemset(d_p64, 0, sizeof(double) * C));
         if (mg.init(W, H, D) != 0) return fail(FS_ERR_DEVICE, "pressure solver allocation failed");
+        p64_from_step = false;
         jacobi.reset(make_jacobi_preconditioner(&mg));
         keys_valid = sorted_valid = false;
         return FS_OK;
@@ -388,7 +390,8 @@ struct Solver {
         return net.get();
     }
 
-    int stage_pressure() {
+    // warm: start from the previous step's pressure (d_p64) when it is one
+    int stage_pressure(bool warm = false) {
         Preconditioner* pre = nullptr;
         if (cfg.precond == FS_PRECOND_JACOBI) pre = jacobi.get();
         if (cfg.precond == FS_PRECOND_LEARNED) {
@@ -404,7 +407,10 @@ struct Solver {
         mg.use_graph = !(cfg.solver_flags & FS_SOLVER_NO_GRAPH);
         mg.use_tail = !(cfg.solver_flags & FS_SOLVER_NO_TAIL);
         mg.use_tb = !(cfg.solver_flags & FS_SOLVER_NO_FUSE);
-        int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats);
+        const bool use_x0 = warm && p64_from_step && !(cfg.solver_flags & FS_SOLVER_COLD_START);
+        int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats,
+                          use_x0 ? d_p64 : nullptr);
+        p64_from_step = warm && rc != -5;
         if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
         if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");
         if (rc == -4) return fail(FS_ERR_NOT_CONVERGED, "pressure solve did not reach the tolerance");
@@ -506,7 +512,7 @@ struct Solver {
         if ((rc = stage_p2g())) return rc;
         HIPCHK(hipEventRecord(ev[2], st));
         stage_rhs();
-        rc = stage_pressure();
+        rc = stage_pressure(true);
         // a solve that misses the tolerance still completes the step (the
         // reference ignores the solver status); fs_step reports it afterwards
         if (rc == FS_ERR_NOT_CONVERGED) {
@@ -850,7 +856,8 @@ int fs_load_body_sdf(void* h, const char* path, float x, float y, float z, float
         { std::istringstream is(line); is >> ox >> oy >> oz; }
         if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing cell size");
         try { cs = std::stof(line); } catch (...) { return fail(FS_ERR_IO, "SDF: bad cell size"); }
-        if (w <= 0 || hh <= 0 || d <= 0) return fail(FS_ERR_IO, "SDF: bad dimensions");
+        if (w <= 0 || hh <= 0 || d <= 0 || (long long)w * hh * d > (1LL << 31))
+            return fail(FS_ERR_IO, "SDF: bad dimensions");
         std::vector<float> data((size_t)w * hh * d);
         for (auto& v : data)
             if (!(f >> v)) return fail(FS_ERR_IO, "Unexpected end of SDF data");
@@ -1025,7 +1032,7 @@ int fs_stage_rhs(void* h) {
     });
 }
 int fs_stage_pressure(void* h) {
-    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_pressure());
+    return guarded([&]() -> int { Solver& s = *(Solver*)h; STAGE_SYNC(s.stage_pressure(true));
     });
 }
 int fs_stage_project(void* h) {

@@ -1619,6 +1619,20 @@ __global__ __launch_bounds__(256) void k_sub_mean_rr(MGLevel L, float* __restric
     store_partial(acc, sm, prr);
 }
 
+// x = x0 (full grid) on the unknowns of the box, 0 elsewhere
+__global__ __launch_bounds__(256) void k_gather_x0(MGLevel L, const double* __restrict__ x0, int W, int H, int ox,
+                                                   int oy, int oz, double* __restrict__ x) {
+    const long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (c >= L.n) return;
+    double v = 0.0;
+    if (L.code[c] & SELF) {
+        int i, j, k;
+        split3(c, L.W, L.H, i, j, k);
+        v = x0[(size_t)(i + ox) + (size_t)(j + oy) * W + (size_t)(k + oz) * W * H];
+    }
+    x[c] = v;
+}
+
 __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__ x64) {
     long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -2048,7 +2062,7 @@ hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, int nb, hipStream_t st) {
         return nullptr;
     }
     hipGraphDestroy(graph);
-    if (graphs.size() >= 16) {
+    if (graphs.size() >= 32) {
         hipGraphExecDestroy(graphs.front().exec);
         graphs.erase(graphs.begin());
     }
@@ -2107,7 +2121,7 @@ int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
 }
 
 int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
-                 Preconditioner* pre, hipStream_t st, fs_solve_stats* stats) {
+                 Preconditioner* pre, hipStream_t st, fs_solve_stats* stats, const double* x0) {
     auto t0 = std::chrono::steady_clock::now();
     MGPreconditioner mgp(this);
     if (!pre) pre = &mgp;
@@ -2146,9 +2160,21 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     const double tol2b = tol * tol * bn2;
     int total_iters = 0, outer = 0;
     bool converged = false;
-    double rr_true = 0.0;
+    double rr_true = 0.0, rr_first = 0.0;
     for (outer = 0;; ++outer) {
-        if (outer == 0)
+        if (outer == 0 && x0 && bn2 > 0.0) {
+            // warm start: the initial guess on the unknowns of the box (0 elsewhere)
+            hipLaunchKernelGGL(k_gather_x0, dim3(nblk(n0)), dim3(256), 0, st, L0, x0, W, H, box[0], box[1], box[2],
+                               xb);
+            if (singular) {
+                hipLaunchKernelGGL(k_sum_unknowns<double>, dim3(nred), dim3(256), 0, st, L0, (const double*)xb,
+                                   slot(PS_MEAN), (const SolveScalars*)nullptr);
+                hipLaunchKernelGGL(k_sub_mean<double>, dim3(nred), dim3(256), 0, st, L0, xb,
+                                   (const double*)slot(PS_MEAN), nred, nunk, (const SolveScalars*)nullptr);
+            }
+            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
+                               slot(PS_RES));
+        } else if (outer == 0)
             hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
                                slot(PS_RES));
         else
@@ -2158,7 +2184,16 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
                            (const double*)nullptr, nred, sc, tol2b);
         hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
         if (hipStreamSynchronize(st) != hipSuccess) return -5;
+        if (outer == 0 && x0 && bn2 > 0.0 && sc_host->rr_true >= bn2) {
+            // the initial guess is no better than 0: start from 0
+            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, slot(PS_RES));
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
+                               (const double*)nullptr, nred, sc, tol2b);
+            hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) return -5;
+        }
         rr_true = sc_host->rr_true;
+        if (outer == 0) rr_first = rr_true;
         total_iters = sc_host->iters;
         if (sc_host->done == 2 || bn2 == 0.0) { converged = true; break; }
         if (outer >= max_outer || total_iters >= maxit) break;
@@ -2169,12 +2204,22 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             hipLaunchKernelGGL(k_sub_mean<double>, dim3(nred), dim3(256), 0, st, L0, xb,
                                (const double*)slot(PS_MEAN), nred, nunk, (const SolveScalars*)nullptr);
         }
-        // Batches of `batch` iterations; once two checks give a convergence rate,
-        // the next batch is sized (8, 4 or 2) to the iterations still expected,
-        // since iterations launched after convergence still cost their launches.
+        // Batches of at most `batch` iterations, each sized (an even count, so
+        // that every batch starts from the same direction buffer) to the
+        // iterations still expected: from the rate measured between two checks
+        // of this solve, and for the first batch from the previous solve's
+        // rate.  An iteration launched after convergence still costs its
+        // launches (a few microseconds each), one more host check tens.
+        auto batch_for = [&](double need) {
+            if (!(need > 0.0) || !std::isfinite(need)) return batch;
+            const int n = 2 * (int)std::ceil(need / 2.0);
+            return std::max(2, std::min(batch, n));
+        };
         double rr_prev = sc_host->rr;
         int it_prev = sc_host->iters;
         int nb = batch;
+        if (rate_per_iter < 0.0 && sc_host->rr > 0.0 && sc_host->target > 0.0)
+            nb = batch_for(std::log(sc_host->target / sc_host->rr) / rate_per_iter);
         while (true) {
             if (use_graph) {
                 hipGraphExec_t ex = batch_graph(pre, nb, st);
@@ -2190,9 +2235,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             const int dit = sc_host->iters - it_prev;
             if (dit > 0 && sc_host->rr > 0.0 && rr_prev > sc_host->rr && sc_host->target > 0.0) {
                 const double per_it = std::log(sc_host->rr / rr_prev) / dit;   // < 0
-                const double need = std::log(sc_host->target / sc_host->rr) / per_it;
-                if (need <= 2.0) nb = 2;
-                else if (need <= 4.0) nb = std::min(batch, 4);
+                nb = batch_for(std::log(sc_host->target / sc_host->rr) / per_it);
             }
             rr_prev = sc_host->rr;
             it_prev = sc_host->iters;
@@ -2212,6 +2255,9 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     }
     hipLaunchKernelGGL(k_scatter_out, dim3(nblk((N + 3) / 4)), dim3(256), 0, st, labels, N, W, H, L0, box[0], box[1], box[2], xb,
                        x64, p);
+    // mean reduction of ||r||^2 per iteration, for sizing the next solve's first batch
+    if (converged && total_iters > 0 && rr_true > 0.0 && rr_first > 0.0 && rr_true < rr_first)
+        rate_per_iter = std::log(rr_true / rr_first) / total_iters;
     last_iters = total_iters;
     last_outer = outer + 1;
     loc.iterations = total_iters;

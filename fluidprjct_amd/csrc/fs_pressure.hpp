@@ -68,8 +68,10 @@ public:
                 size_t* used);
     // Solve A x = b (b: full-grid float, 0 off fluid).  x64 (full grid, double)
     // and p (float, 0 off fluid) are written.  Returns 0 on convergence.
+    // x0 (optional, full grid, may be x64 itself): initial guess on the unknowns
+    // (the previous step's pressure); null starts from 0.
     int solve(const uint8_t* labels, const float* b, double* x64, float* p, double tol, int maxit,
-              Preconditioner* pre, hipStream_t st, struct fs_solve_stats* stats);
+              Preconditioner* pre, hipStream_t st, struct fs_solve_stats* stats, const double* x0 = nullptr);
     // Fluid bounding box of `labels` (+1 cell), level set-up for that box and
     // the gathered right-hand side bb.  Returns -3 when there is no fluid.
     int prepare(const uint8_t* labels, const float* b, hipStream_t st);
@@ -122,6 +124,7 @@ public:
     SolveScalars* sc_host = nullptr;  // pinned
     int nred = 0;   // number of partial-sum slots used by reducing kernels
     int last_iters = 0, last_outer = 0;
+    double rate_per_iter = 0.0;   // log(||r||^2 reduction) per iteration of the last converged solve (< 0)
     bool use_graph = true;      // replay inner batches as HIP graphs
     bool use_tail = true;
     bool use_tb = true;         // temporally blocked smoothing kernels on small coarse levels

@@ -113,6 +113,7 @@ typedef struct fs_config {
 #define FS_SOLVER_NO_GRAPH 1     /* launch the inner PCG iterations eagerly instead of as a HIP graph */
 #define FS_SOLVER_NO_TAIL  2     /* run every multigrid level as separate launches (no single-workgroup tail) */
 #define FS_SOLVER_NO_FUSE  4     /* one launch per sweep on every level (no temporally blocked smoothing) */
+#define FS_SOLVER_COLD_START 8   /* start every step's solve from p = 0 instead of the previous step's pressure */
 
 typedef struct fs_body_state {
     float pos[3], vel[3], omega[3];

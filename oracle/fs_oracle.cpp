@@ -1240,6 +1240,7 @@ int fs_config_load_json(fs_config* c, const char* path) {
     std::stringstream ss;
     ss << f.rdbuf();
     std::string t = ss.str(), v;
+    try {
     if (json_find(t, "dt", v)) c->dt = std::stof(v);
     if (json_find(t, "dx", v)) c->dx = std::stof(v);
     if (json_find(t, "PIC_WEIGHT", v)) c->pic_weight = std::stof(v);
@@ -1255,6 +1256,9 @@ int fs_config_load_json(fs_config* c, const char* path) {
     if (json_find(t, "body_mass", v)) c->body_mass = std::stof(v);
     /* the reference hard-codes InputData/ball.sdf (FluidSolver3D.cu:160); this key overrides it */
     if (json_find(t, "bodySdf", v)) std::snprintf(c->body_sdf_path, sizeof(c->body_sdf_path), "%s", v.c_str());
+    } catch (const std::exception& e) {
+        return fail(FS_ERR_IO, std::string("bad value in ") + path + ": " + e.what());
+    }
     return FS_OK;
 }
 
@@ -1295,20 +1299,29 @@ int fs_get_dims(void* h, int* W, int* H, int* D) {
     return FS_OK;
 }
 
+/* RigidBody::loadSDF utility.cuh:231-338.  The reference throws
+ * std::runtime_error (or std::invalid_argument from std::stof) on a bad file;
+ * here each of those cases returns FS_ERR_IO, and dimensions are checked
+ * before the value buffer is sized. */
 int fs_load_body_sdf(void* h, const char* path, float x, float y, float z, float mass) {
     std::ifstream f(path);
     if (!f.is_open()) return fail(FS_ERR_IO, std::string("Cannot open SDF file: ") + path);
     std::string line;
-    std::getline(f, line);
-    std::istringstream d1(line);
-    int w, hh, d;
-    d1 >> w >> hh >> d;
-    std::getline(f, line);
-    std::istringstream d2(line);
-    f3 o;
-    d2 >> o.x >> o.y >> o.z;
-    std::getline(f, line);
-    float cs = std::stof(line);
+    int w = 0, hh = 0, d = 0;
+    f3 o{0, 0, 0};
+    float cs = 0.0f;
+    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing dimensions");
+    { std::istringstream d1(line); d1 >> w >> hh >> d; }
+    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing origin");
+    { std::istringstream d2(line); d2 >> o.x >> o.y >> o.z; }
+    if (!std::getline(f, line)) return fail(FS_ERR_IO, "SDF: missing cell size");
+    try {
+        cs = std::stof(line);
+    } catch (const std::exception&) {
+        return fail(FS_ERR_IO, "SDF: bad cell size");
+    }
+    if (w <= 0 || hh <= 0 || d <= 0 || (long long)w * hh * d > (1LL << 31))
+        return fail(FS_ERR_IO, "SDF: bad dimensions");
     std::vector<float> data((size_t)w * hh * d);
     for (size_t q = 0; q < data.size(); ++q)
         if (!(f >> data[q])) return fail(FS_ERR_IO, "Unexpected end of SDF data");
