@@ -19,6 +19,7 @@ FS_SOLVER_NO_GRAPH = 1
 FS_SOLVER_NO_TAIL = 2
 FS_SOLVER_NO_FUSE = 4
 FS_SOLVER_COLD_START = 8
+FS_STEP_DETERMINISTIC, FS_STEP_BODY_WALLS, FS_STEP_BODY_INERTIA = 1, 2, 4
 (FS_GRID_U, FS_GRID_V, FS_GRID_W, FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED, FS_GRID_P,
  FS_GRID_LABELS, FS_GRID_RHS, FS_GRID_P64, FS_GRID_CELLKEY, FS_GRID_SOLID) = range(12)
 STAGES = ("label", "p2g", "pressure", "project", "body", "g2p_advect", "sort")
@@ -35,7 +36,7 @@ class FsConfig(C.Structure):
         ("particles_per_cell", C.c_int), ("seed", C.c_uint64), ("gravity_y", C.c_float),
         ("density", C.c_float), ("advect_c", C.c_float), ("solver_tol", C.c_double),
         ("solver_max_iter", C.c_int), ("precond", C.c_int), ("stall_window", C.c_int),
-        ("cfl", C.c_float), ("solver_flags", C.c_int),
+        ("cfl", C.c_float), ("solver_flags", C.c_int), ("step_flags", C.c_int),
     ]
 
 

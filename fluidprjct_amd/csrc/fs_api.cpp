@@ -172,6 +172,7 @@ struct Solver {
     long long np = 0;
     ParticlesView PA{}, PB{};
     int *d_key = nullptr, *d_rank = nullptr, *d_tmpkey = nullptr;
+    int* d_dkey = nullptr;      // cell of each sorted slot (deterministic order only)
     bool keys_valid = false;   // d_key/d_rank/d_count hold the counting pass of the current order
     bool sorted_valid = false; // d_start matches the current particle order
     bool count_clean = true;   // d_count is all zero
@@ -221,7 +222,7 @@ struct Solver {
     void free_particles() {
         free_view(PA);
         free_view(PB);
-        dfree(d_key); dfree(d_rank); dfree(d_tmpkey);
+        dfree(d_key); dfree(d_rank); dfree(d_tmpkey); dfree(d_dkey);
         np = 0;
     }
 
@@ -346,8 +347,15 @@ struct Solver {
     }
 
     int finish_sort() {
-        launch_scatter(PA, PB, (int)np, d_key, d_rank, d_start, st);
-        std::swap(PA, PB);
+        if (cfg.step_flags & FS_STEP_DETERMINISTIC) {
+            // scatter into PB, then each cell's particles back into PA in id order
+            if (!d_dkey) HIPCHK(hipMalloc(&d_dkey, sizeof(int) * (size_t)std::max<long long>(np, 1)));
+            launch_scatter(PA, PB, (int)np, d_key, d_rank, d_start, d_dkey, st);
+            launch_canonical(PB, PA, (int)np, d_dkey, d_start, st);
+        } else {
+            launch_scatter(PA, PB, (int)np, d_key, d_rank, d_start, nullptr, st);
+            std::swap(PA, PB);
+        }
         keys_valid = false;  // counts were consumed (and cleared) by the scan
         count_clean = true;
         sorted_valid = true;
@@ -1425,6 +1433,14 @@ int fs_save_checkpoint(void* h, const char* path) {
             body_fields(s.body, [&](void* p, size_t n) { w.raw(p, n); });
             w.raw(s.body.sdf.data(), sizeof(float) * s.body.sdf.size());
         }
+        // the last step's pressure, the next solve's initial guess
+        const int32_t has_p = s.p64_from_step ? 1 : 0;
+        w.put(has_p);
+        if (has_p) {
+            std::vector<double> pv((size_t)s.C);
+            HIPCHK(hipMemcpy(pv.data(), s.d_p64, sizeof(double) * (size_t)s.C, hipMemcpyDeviceToHost));
+            w.raw(pv.data(), sizeof(double) * pv.size());
+        }
         // the learned preconditioner's parameters (not its optimiser state)
         const int32_t nparam = s.net ? (int32_t)LN_NPARAM : 0;
         w.put(nparam);
@@ -1491,6 +1507,14 @@ int fs_load_checkpoint(void* h, const char* path) {
             body.sdf.resize((size_t)nv);
             r.raw(body.sdf.data(), sizeof(float) * body.sdf.size());
         }
+        int32_t has_p = 0;
+        r.get(has_p);
+        std::vector<double> pv;
+        if (r.ok && has_p) {
+            if (8LL * (long long)C + 4 > r.remaining()) return fail(FS_ERR_IO, std::string("corrupt checkpoint: ") + path);
+            pv.resize(C);
+            r.raw(pv.data(), sizeof(double) * C);
+        }
         int32_t nparam = 0;
         r.get(nparam);
         std::vector<float> prm;
@@ -1518,6 +1542,10 @@ int fs_load_checkpoint(void* h, const char* path) {
         if (s.body.active) {
             HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * s.body.sdf.size()));
             HIPCHK(hipMemcpy(s.d_sdf, s.body.sdf.data(), sizeof(float) * s.body.sdf.size(), hipMemcpyHostToDevice));
+        }
+        if (!pv.empty()) {
+            HIPCHK(hipMemcpy(s.d_p64, pv.data(), sizeof(double) * C, hipMemcpyHostToDevice));
+            s.p64_from_step = true;
         }
         if (!prm.empty()) {
             LNet* n = s.get_net();

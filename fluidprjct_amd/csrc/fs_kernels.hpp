@@ -102,7 +102,10 @@ void launch_scan_fluid(const uint8_t* labels, long long C, int* bsum, int* total
 void launch_seed(const uint8_t* labels, const int* frank, long long C, int W, int H, float dx, int ppc, uint64_t seed,
                  const ParticlesView& P, hipStream_t st);
 void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, const int* key, const int* rank,
-                    const int* start, hipStream_t st);
+                    const int* start, int* dkey, hipStream_t st);
+// particles of each cell segment of src to their rank by id in dst (dkey: cell of each slot)
+void launch_canonical(const ParticlesView& src, const ParticlesView& dst, int n, const int* dkey, const int* start,
+                      hipStream_t st);
 void launch_p2g(const P2GArgs& a, hipStream_t st);
 void launch_g2p_advect(const G2PArgs& a, const int* start, hipStream_t st);
 void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st);

@@ -186,12 +186,15 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_final(int* __restrict__ cnt, co
 }
 
 // ---------------------------------------------------------------------------
+// dkey (optional): the cell of each destination slot, for k_canonical
 __global__ __launch_bounds__(256) void k_scatter(ParticlesView src, ParticlesView dst, int n,
                                                  const int* __restrict__ key, const int* __restrict__ rank,
-                                                 const int* __restrict__ start) {
+                                                 const int* __restrict__ start, int* __restrict__ dkey) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int d = start[key[i]] + rank[i];
+    const int k = key[i];
+    int d = start[k] + rank[i];
+    if (dkey) dkey[d] = k;
     dst.px[d] = src.px[i];
     dst.py[d] = src.py[i];
     dst.pz[d] = src.pz[i];
@@ -199,6 +202,31 @@ __global__ __launch_bounds__(256) void k_scatter(ParticlesView src, ParticlesVie
     dst.vy[d] = src.vy[i];
     dst.vz[d] = src.vz[i];
     dst.id[d] = src.id[i];
+}
+
+// Canonical order within each cell: the atomic ranks of the counting pass
+// place a cell's particles in an order that can change from run to run; this
+// pass moves each particle of the cell-sorted src to its rank by original id
+// within its cell's segment of dst, so that the particle order - and with it
+// every floating-point sum over a cell's particles - depends on the state
+// only.  One thread per slot; a cell's ids are read from the cache.
+__global__ __launch_bounds__(256) void k_canonical(ParticlesView src, ParticlesView dst, int n,
+                                                   const int* __restrict__ dkey, const int* __restrict__ start) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const int k = dkey[d];
+    const int s = start[k], e = start[k + 1];
+    const int id = src.id[d];
+    int r = 0;
+    for (int q = s; q < e; ++q) r += src.id[q] < id;
+    const int t = s + r;
+    dst.px[t] = src.px[d];
+    dst.py[t] = src.py[d];
+    dst.pz[t] = src.pz[d];
+    dst.vx[t] = src.vx[d];
+    dst.vy[t] = src.vy[d];
+    dst.vz[t] = src.vz[d];
+    dst.id[t] = id;
 }
 
 // ---------------------------------------------------------------------------
@@ -920,9 +948,15 @@ void launch_seed(const uint8_t* labels, const int* frank, long long C, int W, in
 }
 
 void launch_scatter(const ParticlesView& src, const ParticlesView& dst, int n, const int* key, const int* rank,
-                    const int* start, hipStream_t st) {
+                    const int* start, int* dkey, hipStream_t st) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n, key, rank, start);
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n, key, rank, start, dkey);
+}
+
+void launch_canonical(const ParticlesView& src, const ParticlesView& dst, int n, const int* dkey, const int* start,
+                      hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_canonical, dim3((n + 255) / 256), dim3(256), 0, st, src, dst, n, dkey, start);
 }
 
 void launch_p2g(const P2GArgs& a, hipStream_t st) {

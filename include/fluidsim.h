@@ -107,6 +107,7 @@ typedef struct fs_config {
     float    cfl;                /* > 0: dt = min(dt, cfl*dx/max|v_particle|) each step; 0: fixed dt
                                     (no reference counterpart: the reference's dt is fixed) */
     int      solver_flags;       /* FS_SOLVER_* bits */
+    int      step_flags;         /* FS_STEP_* bits (0: the reference's behaviour) */
 } fs_config;
 
 /* solver_flags */
@@ -114,6 +115,15 @@ typedef struct fs_config {
 #define FS_SOLVER_NO_TAIL  2     /* run every multigrid level as separate launches (no single-workgroup tail) */
 #define FS_SOLVER_NO_FUSE  4     /* one launch per sweep on every level (no temporally blocked smoothing) */
 #define FS_SOLVER_COLD_START 8   /* start every step's solve from p = 0 instead of the previous step's pressure */
+
+/* step_flags (options with no counterpart in the reference's default path) */
+#define FS_STEP_DETERMINISTIC 1  /* particles of a cell kept in original-id order: runs repeat bit for bit
+                                    (one more pass over the particles per step) */
+#define FS_STEP_BODY_WALLS    2  /* handleBodyWallCollisions (FluidSolver3D.cu:2848-2895, which the
+                                    reference has switched off) after each body update */
+#define FS_STEP_BODY_INERTIA  4  /* full inertia tensor of the body's SDF voxels and a quaternion
+                                    orientation in place of the scalar sphere inertia and the Euler-angle
+                                    rates of RigidBody::integrate (utility.cuh:408-467) */
 
 typedef struct fs_body_state {
     float pos[3], vel[3], omega[3];

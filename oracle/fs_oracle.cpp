@@ -34,6 +34,7 @@
  * brute-force tiny cases, scipy spsolve of the assembled system.
  */
 #include "../include/fluidsim.h"
+#include "../include/fluidsim_checkpoint.h"
 
 #include <algorithm>
 #include <chrono>
@@ -1594,6 +1595,170 @@ int64_t oracle_assemble_upper_csr(const uint8_t* labels, int W, int H, int D, in
         if (off) off[r + 1] = (int)nnz;
     }
     return nnz;
+}
+
+}  // extern "C"
+
+/* ---- checkpoint (include/fluidsim_checkpoint.h; no reference counterpart):
+ * the file format of the HIP library, so that a run saved by either library
+ * continues in the other.  Particles are stored in the oracle's order (by
+ * original id); a loaded file's particles are put back into id order. */
+namespace {
+const char kCkptMagic[8] = {'F', 'S', 'C', 'K', 'P', 'T', '0', '2'};
+
+template <class IO>
+void body_ckpt_fields(Body& b, IO io) {
+    auto f = [&](auto& v) { io((void*)&v, sizeof(v)); };
+    f(b.sw); f(b.sh); f(b.sd);
+    f(b.fileOrigin); f(b.sdf_origin); f(b.local_com); f(b.size); f(b.cs);
+    f(b.pos); f(b.vel); f(b.force); f(b.omega); f(b.torque);
+    f(b.mass); f(b.inertia); f(b.inv_inertia); f(b.theta); f(b.psi); f(b.phi);
+    f(b.R); f(b.last_force); f(b.last_torque); f(b.last_sub);
+}
+}  // namespace
+
+extern "C" {
+
+
+int fs_save_checkpoint(void* h, const char* path) {
+    Solver& s = *(Solver*)h;
+    if (!path) return fail(FS_ERR_ARG, "null path");
+    if (s.C() == 0) return fail(FS_ERR_STATE, "nothing to save: no scene");
+    std::ofstream f(path, std::ios::binary);
+    if (!f.is_open()) return fail(FS_ERR_IO, std::string("cannot write ") + path);
+    auto raw = [&](const void* p, size_t n) { if (n) f.write((const char*)p, (std::streamsize)n); };
+    raw(kCkptMagic, 8);
+    const int32_t cfg_size = (int32_t)sizeof(fs_config);
+    raw(&cfg_size, 4);
+    raw(&s.cfg, sizeof(fs_config));
+    raw(&s.W, 4); raw(&s.H, 4); raw(&s.D, 4);
+    raw(&s.dt_cur, 4); raw(&s.sim_time, 8); raw(&s.nsteps, 8);
+    const int64_t np = s.np();
+    raw(&np, 8);
+    raw(s.solid.data(), s.C());
+    raw(s.labels.data(), s.C());
+    std::vector<float> a((size_t)np);
+    for (int c = 0; c < 6; ++c) {
+        const std::vector<float>& src = c < 3 ? s.pos : s.vel;
+        for (int64_t q = 0; q < np; ++q) a[(size_t)q] = src[3 * (size_t)q + (c % 3)];
+        raw(a.data(), sizeof(float) * (size_t)np);
+    }
+    std::vector<int> ids((size_t)np);
+    for (int64_t q = 0; q < np; ++q) ids[(size_t)q] = (int)q;
+    raw(ids.data(), sizeof(int) * (size_t)np);
+    const int32_t active = s.body.active ? 1 : 0;
+    raw(&active, 4);
+    if (active) {
+        body_ckpt_fields(s.body, [&](void* p, size_t n) { raw(p, n); });
+        raw(s.body.sdf.data(), sizeof(float) * s.body.sdf.size());
+    }
+    const int32_t has_p = 1;   /* the last solution (the HIP library's next initial guess) */
+    raw(&has_p, 4);
+    raw(s.p64.data(), sizeof(double) * s.C());
+    const int32_t nparam = 0;
+    raw(&nparam, 4);
+    f.close();
+    if (f.fail()) return fail(FS_ERR_IO, std::string("write failed: ") + path);
+    return FS_OK;
+}
+
+int fs_load_checkpoint(void* h, const char* path) {
+    Solver& s = *(Solver*)h;
+    if (!path) return fail(FS_ERR_ARG, "null path");
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) return fail(FS_ERR_IO, std::string("cannot open ") + path);
+    f.seekg(0, std::ios::end);
+    const long long size = (long long)f.tellg();
+    f.seekg(0);
+    bool ok = true;
+    auto raw = [&](void* p, size_t n) {
+        if (n) f.read((char*)p, (std::streamsize)n);
+        ok = ok && (bool)f;
+    };
+    auto left = [&]() { return size - (long long)f.tellg(); };
+    char magic[8];
+    int32_t cfg_size = 0;
+    raw(magic, 8);
+    raw(&cfg_size, 4);
+    if (!ok || std::memcmp(magic, kCkptMagic, 8) != 0 || cfg_size != (int32_t)sizeof(fs_config))
+        return fail(FS_ERR_IO, std::string("not a checkpoint of this version: ") + path);
+    fs_config cfg;
+    raw(&cfg, sizeof(cfg));
+    int W = 0, H = 0, D = 0;
+    raw(&W, 4); raw(&H, 4); raw(&D, 4);
+    float dt_cur = 0;
+    double sim_time = 0;
+    int64_t nsteps = 0, np = 0;
+    raw(&dt_cur, 4); raw(&sim_time, 8); raw(&nsteps, 8); raw(&np, 8);
+    const long long cells = (long long)W * H * D;
+    if (!ok || W <= 0 || H <= 0 || D <= 0 || np < 0 || np > 0x7fffffffLL || cells > (1LL << 40) ||
+        2 * cells + 28 * np + 8 > left())
+        return fail(FS_ERR_IO, std::string("corrupt checkpoint header: ") + path);
+    std::vector<uint8_t> solid((size_t)cells), labels((size_t)cells);
+    raw(solid.data(), solid.size());
+    raw(labels.data(), labels.size());
+    std::vector<float> arr[6];
+    for (auto& a : arr) {
+        a.resize((size_t)np);
+        raw(a.data(), sizeof(float) * (size_t)np);
+    }
+    std::vector<int> ids((size_t)np);
+    raw(ids.data(), sizeof(int) * (size_t)np);
+    int32_t active = 0;
+    raw(&active, 4);
+    Body body;
+    if (ok && active) {
+        body.active = true;
+        body_ckpt_fields(body, [&](void* p, size_t n) { raw(p, n); });
+        const long long nv = (long long)body.sw * body.sh * body.sd;
+        if (!ok || body.sw <= 0 || body.sh <= 0 || body.sd <= 0 || 4 * nv + 4 > left())
+            return fail(FS_ERR_IO, "corrupt checkpoint body");
+        body.sdf.resize((size_t)nv);
+        raw(body.sdf.data(), sizeof(float) * (size_t)nv);
+    }
+    int32_t has_p = 0;
+    raw(&has_p, 4);
+    std::vector<double> pv;
+    if (ok && has_p) {
+        if (8 * cells + 4 > left()) return fail(FS_ERR_IO, "corrupt checkpoint pressure");
+        pv.resize((size_t)cells);
+        raw(pv.data(), sizeof(double) * (size_t)cells);
+    }
+    int32_t nparam = 0;
+    raw(&nparam, 4);
+    if (!ok) return fail(FS_ERR_IO, std::string("truncated checkpoint: ") + path);
+    if (nparam > 0) {   /* learned parameters: skipped (the oracle has no network in its step) */
+        if (4LL * nparam > left()) return fail(FS_ERR_IO, "corrupt checkpoint parameters");
+        f.seekg(4LL * nparam, std::ios::cur);
+    }
+    /* particles back into original-id order (ids are a permutation of 0..np-1) */
+    std::vector<int64_t> where((size_t)np, -1);
+    for (int64_t q = 0; q < np; ++q) {
+        const int id = ids[(size_t)q];
+        if (id < 0 || id >= np || where[(size_t)id] >= 0) return fail(FS_ERR_IO, "corrupt checkpoint particle ids");
+        where[(size_t)id] = q;
+    }
+    s.cfg = cfg;
+    s.W = W; s.H = H; s.D = D;
+    s.alloc_grids();
+    s.solid = solid;
+    s.labels = labels;
+    s.pos.assign(3 * (size_t)np, 0.f);
+    s.vel.assign(3 * (size_t)np, 0.f);
+    for (int64_t id = 0; id < np; ++id) {
+        const size_t q = (size_t)where[(size_t)id];
+        for (int c = 0; c < 3; ++c) {
+            s.pos[3 * (size_t)id + c] = arr[c][q];
+            s.vel[3 * (size_t)id + c] = arr[3 + c][q];
+        }
+    }
+    s.key.clear();
+    if (!pv.empty()) s.p64 = pv;
+    s.body = std::move(body);
+    s.dt_cur = dt_cur;
+    s.sim_time = sim_time;
+    s.nsteps = nsteps;
+    return FS_OK;
 }
 
 int oracle_num_threads(void) {
