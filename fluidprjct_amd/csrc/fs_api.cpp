@@ -80,6 +80,10 @@ struct HostBody {
     float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     f3 last_force{0, 0, 0}, last_torque{0, 0, 0};
     float last_sub = 0;
+    // FS_STEP_BODY_INERTIA: body-frame inertia tensor about local_com, its
+    // inverse, and the orientation quaternion (w, x, y, z), body to world
+    double Ib[9] = {0}, Ibinv[9] = {0};
+    double q[4] = {1, 0, 0, 0};
 
     // RigidBody::update_rotation_matrix utility.cuh:470-484
     void update_rotation() {
@@ -119,6 +123,141 @@ struct HostBody {
         sdf_origin = pos + ro;
         force = mk(0, 0, 0);
         torque = mk(0, 0, 0);
+    }
+
+    // ---- FS_STEP_BODY_INERTIA (no reference counterpart) -----------------
+    static void quat_mul(const double* a, const double* b, double* o) {
+        o[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+        o[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+        o[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+        o[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    }
+    void rotation_of_quaternion() {
+        const double w = q[0], x = q[1], y = q[2], z = q[3];
+        R[0] = (float)(1.0 - 2.0 * (y * y + z * z));
+        R[1] = (float)(2.0 * (x * y - w * z));
+        R[2] = (float)(2.0 * (x * z + w * y));
+        R[3] = (float)(2.0 * (x * y + w * z));
+        R[4] = (float)(1.0 - 2.0 * (x * x + z * z));
+        R[5] = (float)(2.0 * (y * z - w * x));
+        R[6] = (float)(2.0 * (x * z - w * y));
+        R[7] = (float)(2.0 * (y * z + w * x));
+        R[8] = (float)(1.0 - 2.0 * (x * x + y * y));
+    }
+    // theta, psi, phi of R = Ry(theta) Rz(psi) Rx(phi), the reference's convention
+    void angles_of_rotation() {
+        psi = (float)std::asin(std::clamp((double)R[3], -1.0, 1.0));
+        theta = (float)std::atan2(-(double)R[6], (double)R[0]);
+        phi = (float)std::atan2(-(double)R[5], (double)R[4]);
+    }
+    void quaternion_of_angles() {
+        const double a = 0.5 * theta, b = 0.5 * psi, c = 0.5 * phi;
+        const double qy[4] = {std::cos(a), 0, std::sin(a), 0}, qz[4] = {std::cos(b), 0, 0, std::sin(b)},
+                     qx[4] = {std::cos(c), std::sin(c), 0, 0};
+        double t[4];
+        quat_mul(qy, qz, t);
+        quat_mul(t, qx, q);
+    }
+    // second moments of the SDF's inside voxels (partial weights as the
+    // buoyancy's, FluidSolver3D.cu:2979-2983) about local_com, scaled to the
+    // mass; inverse by cofactors
+    void inertia_tensor() {
+        double wsum = 0, A[9] = {0};
+        for (int k = 0; k < sd; ++k)
+            for (int j = 0; j < sh; ++j)
+                for (int i = 0; i < sw; ++i) {
+                    const float v = sdf[(size_t)i + (size_t)j * sw + (size_t)k * sw * sh];
+                    if (v > 0.0f) continue;
+                    double wv = 1.0;
+                    if (v > -cs) wv = 0.5 - 0.5 * (double)v / cs;
+                    const double rx = file_origin.x + (double)i * cs - local_com.x;
+                    const double ry = file_origin.y + (double)j * cs - local_com.y;
+                    const double rz = file_origin.z + (double)k * cs - local_com.z;
+                    const double r2 = rx * rx + ry * ry + rz * rz;
+                    A[0] += wv * (r2 - rx * rx); A[1] -= wv * rx * ry; A[2] -= wv * rx * rz;
+                    A[4] += wv * (r2 - ry * ry); A[5] -= wv * ry * rz;
+                    A[8] += wv * (r2 - rz * rz);
+                    wsum += wv;
+                }
+        A[3] = A[1]; A[6] = A[2]; A[7] = A[5];
+        for (int e = 0; e < 9; ++e) Ib[e] = wsum > 0 ? (double)mass * A[e] / wsum : 0.0;
+        const double* M = Ib;
+        const double c0 = M[4] * M[8] - M[5] * M[7], c1 = M[5] * M[6] - M[3] * M[8], c2 = M[3] * M[7] - M[4] * M[6];
+        const double det = M[0] * c0 + M[1] * c1 + M[2] * c2;
+        if (!(std::fabs(det) > 0)) {
+            for (double& e : Ibinv) e = 0;
+            return;
+        }
+        Ibinv[0] = c0 / det;
+        Ibinv[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+        Ibinv[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+        Ibinv[3] = c1 / det;
+        Ibinv[4] = (M[0] * M[8] - M[2] * M[6]) / det;
+        Ibinv[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+        Ibinv[6] = c2 / det;
+        Ibinv[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+        Ibinv[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+    }
+    // translation as integrate(); rotation by Euler's equations in the body
+    // frame (omega held in the world frame), quaternion step then normalised
+    void integrate_tensor(float dt) {
+        f3 accel = force / mass;
+        vel = vel + accel * dt;
+        pos = pos + vel * dt;
+        rotation_of_quaternion();
+        double Rm[9];
+        for (int e = 0; e < 9; ++e) Rm[e] = R[e];
+        const double tw[3] = {torque.x, torque.y, torque.z}, ow[3] = {omega.x, omega.y, omega.z};
+        double tb[3], wb[3], Iw[3], g[3], d[3];
+        for (int a = 0; a < 3; ++a) {
+            tb[a] = Rm[a] * tw[0] + Rm[3 + a] * tw[1] + Rm[6 + a] * tw[2];
+            wb[a] = Rm[a] * ow[0] + Rm[3 + a] * ow[1] + Rm[6 + a] * ow[2];
+        }
+        for (int a = 0; a < 3; ++a) Iw[a] = Ib[3 * a] * wb[0] + Ib[3 * a + 1] * wb[1] + Ib[3 * a + 2] * wb[2];
+        g[0] = wb[1] * Iw[2] - wb[2] * Iw[1];
+        g[1] = wb[2] * Iw[0] - wb[0] * Iw[2];
+        g[2] = wb[0] * Iw[1] - wb[1] * Iw[0];
+        for (int a = 0; a < 3; ++a) d[a] = tb[a] - g[a];
+        for (int a = 0; a < 3; ++a)
+            wb[a] += (double)dt * (Ibinv[3 * a] * d[0] + Ibinv[3 * a + 1] * d[1] + Ibinv[3 * a + 2] * d[2]);
+        const double wq[4] = {0, wb[0], wb[1], wb[2]};
+        double dq[4];
+        quat_mul(q, wq, dq);
+        for (int e = 0; e < 4; ++e) q[e] += 0.5 * (double)dt * dq[e];
+        const double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (double& e : q) e /= qn;
+        omega = mk((float)(Rm[0] * wb[0] + Rm[1] * wb[1] + Rm[2] * wb[2]),
+                   (float)(Rm[3] * wb[0] + Rm[4] * wb[1] + Rm[5] * wb[2]),
+                   (float)(Rm[6] * wb[0] + Rm[7] * wb[1] + Rm[8] * wb[2]));
+        rotation_of_quaternion();
+        angles_of_rotation();
+        f3 lo = file_origin;
+        f3 ro = mk(R[0] * lo.x + R[1] * lo.y + R[2] * lo.z, R[3] * lo.x + R[4] * lo.y + R[5] * lo.z,
+                   R[6] * lo.x + R[7] * lo.y + R[8] * lo.z);
+        sdf_origin = pos + ro;
+        force = mk(0, 0, 0);
+        torque = mk(0, 0, 0);
+    }
+    // FluidSolver3D::handleBodyWallCollisions FluidSolver3D.cu:2848-2895 (off
+    // in the reference, :2841); its assert dx == SDF cell size is not enforced
+    void collide_walls(int W, int H, int D, float dx) {
+        const float r = 0.5f * std::fmax(std::fmax(size.x, size.y), size.z);
+        const float half = 0.5f * dx;
+        const f3 lo = mk(half, half, half), hi = mk(W * dx - half, H * dx - half, D * dx - half);
+        auto axis = [&](float& p, float pMin, float pMax, f3 n) {
+            const float distMin = p - pMin, distMax = pMax - p;
+            const float dist = std::fmin(distMin, distMax);
+            const float pen = r - dist;
+            if (pen > 0.0f) {
+                p = p + pen * (distMin < distMax ? +1.0f : -1.0f);
+                const float vn = vel.x * n.x + vel.y * n.y + vel.z * n.z;
+                if (vn < 0.0f) vel = vel - vn * n;
+            }
+        };
+        axis(pos.x, lo.x + r, hi.x - r, mk(1, 0, 0));
+        axis(pos.y, lo.y + r, hi.y - r, mk(0, 1, 0));
+        axis(pos.z, lo.z + r, hi.z - r, mk(0, 0, 1));
+        sdf_origin = pos - size * 0.5f;
     }
 
     fsd::BodyParams params() const {
@@ -464,7 +603,11 @@ struct Solver {
         body.last_force = body.force;
         body.last_torque = body.torque;
         body.last_sub = sub;
-        body.integrate(dt_cur);
+        if (cfg.step_flags & FS_STEP_BODY_INERTIA)
+            body.integrate_tensor(dt_cur);
+        else
+            body.integrate(dt_cur);
+        if (cfg.step_flags & FS_STEP_BODY_WALLS) body.collide_walls(W, H, D, cfg.dx);
         return FS_OK;
     }
 
@@ -839,6 +982,7 @@ int fs_set_body_sdf(void* h, const float* sdf, int w, int hh, int d, float ox, f
         b.inertia = 0.4f * b.mass * radius * radius;
         b.inv_inertia = 1.0f / b.inertia;
         b.update_rotation();
+        b.inertia_tensor();
         dfree(s.d_sdf);
         HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * b.sdf.size()));
         HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
@@ -897,6 +1041,7 @@ int fs_set_body_state(void* h, const fs_body_state* in) {
         b.omega = mk(in->omega[0], in->omega[1], in->omega[2]);
         b.theta = in->theta; b.psi = in->psi; b.phi = in->phi;
         b.update_rotation();
+        b.quaternion_of_angles();
         return FS_OK;
     });
 }
@@ -1391,6 +1536,7 @@ void body_fields(HostBody& b, IO io) {
     f(b.pos); f(b.vel); f(b.force); f(b.omega); f(b.torque);
     f(b.mass); f(b.inertia); f(b.inv_inertia); f(b.theta); f(b.psi); f(b.phi);
     f(b.R); f(b.last_force); f(b.last_torque); f(b.last_sub);
+    f(b.Ib); f(b.Ibinv); f(b.q);
 }
 
 }  // namespace

@@ -101,6 +101,11 @@ struct Body {
     float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     f3 last_force{0, 0, 0}, last_torque{0, 0, 0};
     float last_sub = 0;
+    /* FS_STEP_BODY_INERTIA (no reference counterpart): inertia tensor of the
+     * SDF's inside voxels about local_com in the body frame, its inverse, and
+     * the orientation as a unit quaternion (w, x, y, z), body to world */
+    double Ib[9] = {0}, Ibinv[9] = {0};
+    double q[4] = {1, 0, 0, 0};
 
     /* RigidBody::update_rotation_matrix utility.cuh:470-484 */
     void update_rotation_matrix() {
@@ -143,6 +148,154 @@ struct Body {
         sdf_origin = pos + ro;
         force = mk(0, 0, 0);
         torque = mk(0, 0, 0);
+    }
+
+    /* ---- FS_STEP_BODY_INERTIA ---------------------------------------------- */
+    /* R of the quaternion (body to world), row-major as R[] */
+    void rotation_from_quaternion() {
+        const double w = q[0], x = q[1], y = q[2], z = q[3];
+        R[0] = (float)(1.0 - 2.0 * (y * y + z * z));
+        R[1] = (float)(2.0 * (x * y - w * z));
+        R[2] = (float)(2.0 * (x * z + w * y));
+        R[3] = (float)(2.0 * (x * y + w * z));
+        R[4] = (float)(1.0 - 2.0 * (x * x + z * z));
+        R[5] = (float)(2.0 * (y * z - w * x));
+        R[6] = (float)(2.0 * (x * z - w * y));
+        R[7] = (float)(2.0 * (y * z + w * x));
+        R[8] = (float)(1.0 - 2.0 * (x * x + y * y));
+    }
+    /* the reference's angles of R = Ry(theta) Rz(psi) Rx(phi) (utility.cuh:470-484):
+     * R[3] = sin psi, R[0] = cos theta cos psi, R[6] = -sin theta cos psi,
+     * R[4] = cos phi cos psi, R[5] = -sin phi cos psi */
+    void angles_from_rotation() {
+        psi = (float)std::asin(std::clamp((double)R[3], -1.0, 1.0));
+        theta = (float)std::atan2(-(double)R[6], (double)R[0]);
+        phi = (float)std::atan2(-(double)R[5], (double)R[4]);
+    }
+    /* quaternion of Ry(theta) Rz(psi) Rx(phi): q_y(theta) q_z(psi) q_x(phi) */
+    void quaternion_from_angles() {
+        const double a = 0.5 * theta, b = 0.5 * psi, c = 0.5 * phi;
+        const double qy[4] = {std::cos(a), 0, std::sin(a), 0}, qz[4] = {std::cos(b), 0, 0, std::sin(b)},
+                     qx[4] = {std::cos(c), std::sin(c), 0, 0};
+        double t[4];
+        qmul(qy, qz, t);
+        qmul(t, qx, q);
+    }
+    static void qmul(const double* a, const double* b, double* o) {
+        o[0] = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+        o[1] = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+        o[2] = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+        o[3] = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    }
+    /* Ib = sum over voxels with phi <= 0 (weights as applyBuoyancy,
+     * FluidSolver3D.cu:2979-2983) of m_v (|r|^2 I - r r^T), r from local_com,
+     * m_v = mass * weight / total weight; inverted by cofactors */
+    void compute_inertia_tensor() {
+        double Wsum = 0, A[9] = {0};
+        for (int k = 0; k < sd; ++k)
+            for (int j = 0; j < sh; ++j)
+                for (int i = 0; i < sw; ++i) {
+                    const float v = sdf[(size_t)i + (size_t)j * sw + (size_t)k * sw * sh];
+                    if (v > 0.0f) continue;
+                    double wv = 1.0;
+                    if (v > -cs) wv = 0.5 - 0.5 * (double)v / cs;
+                    const double rx = fileOrigin.x + (double)i * cs - local_com.x;
+                    const double ry = fileOrigin.y + (double)j * cs - local_com.y;
+                    const double rz = fileOrigin.z + (double)k * cs - local_com.z;
+                    const double r2 = rx * rx + ry * ry + rz * rz;
+                    A[0] += wv * (r2 - rx * rx); A[1] -= wv * rx * ry; A[2] -= wv * rx * rz;
+                    A[4] += wv * (r2 - ry * ry); A[5] -= wv * ry * rz;
+                    A[8] += wv * (r2 - rz * rz);
+                    Wsum += wv;
+                }
+        A[3] = A[1]; A[6] = A[2]; A[7] = A[5];
+        for (int e = 0; e < 9; ++e) Ib[e] = Wsum > 0 ? (double)mass * A[e] / Wsum : 0.0;
+        const double* M = Ib;
+        const double c0 = M[4] * M[8] - M[5] * M[7], c1 = M[5] * M[6] - M[3] * M[8], c2 = M[3] * M[7] - M[4] * M[6];
+        const double det = M[0] * c0 + M[1] * c1 + M[2] * c2;
+        if (!(std::fabs(det) > 0)) {
+            for (double& e : Ibinv) e = 0;
+            return;
+        }
+        Ibinv[0] = c0 / det;
+        Ibinv[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+        Ibinv[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+        Ibinv[3] = c1 / det;
+        Ibinv[4] = (M[0] * M[8] - M[2] * M[6]) / det;
+        Ibinv[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+        Ibinv[6] = c2 / det;
+        Ibinv[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+        Ibinv[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+    }
+    /* semi-implicit Euler as integrate() for the translation; rotation by
+     * Euler's equations in the body frame, w_b += dt Ib^-1 (tau_b - w_b x Ib w_b),
+     * with omega kept in the world frame (the frame V + omega x r uses), and
+     * the quaternion advanced by q += dt q (0, w_b) / 2, then normalised */
+    void integrate_tensor(float dt) {
+        f3 accel = force / mass;
+        vel = vel + accel * dt;
+        pos = pos + vel * dt;
+        rotation_from_quaternion();
+        double Rm[9];
+        for (int e = 0; e < 9; ++e) Rm[e] = R[e];
+        const double tw[3] = {torque.x, torque.y, torque.z}, ow[3] = {omega.x, omega.y, omega.z};
+        double tb[3], wb[3], Iw[3], g[3], d[3];
+        for (int a = 0; a < 3; ++a) {   /* R^T */
+            tb[a] = Rm[a] * tw[0] + Rm[3 + a] * tw[1] + Rm[6 + a] * tw[2];
+            wb[a] = Rm[a] * ow[0] + Rm[3 + a] * ow[1] + Rm[6 + a] * ow[2];
+        }
+        for (int a = 0; a < 3; ++a) Iw[a] = Ib[3 * a] * wb[0] + Ib[3 * a + 1] * wb[1] + Ib[3 * a + 2] * wb[2];
+        g[0] = wb[1] * Iw[2] - wb[2] * Iw[1];
+        g[1] = wb[2] * Iw[0] - wb[0] * Iw[2];
+        g[2] = wb[0] * Iw[1] - wb[1] * Iw[0];
+        for (int a = 0; a < 3; ++a) d[a] = tb[a] - g[a];
+        for (int a = 0; a < 3; ++a)
+            wb[a] += (double)dt * (Ibinv[3 * a] * d[0] + Ibinv[3 * a + 1] * d[1] + Ibinv[3 * a + 2] * d[2]);
+        const double wq[4] = {0, wb[0], wb[1], wb[2]};
+        double dq[4];
+        qmul(q, wq, dq);
+        for (int e = 0; e < 4; ++e) q[e] += 0.5 * (double)dt * dq[e];
+        const double qn = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (double& e : q) e /= qn;
+        /* omega in the world frame with the rotation the step started from */
+        omega = mk((float)(Rm[0] * wb[0] + Rm[1] * wb[1] + Rm[2] * wb[2]),
+                   (float)(Rm[3] * wb[0] + Rm[4] * wb[1] + Rm[5] * wb[2]),
+                   (float)(Rm[6] * wb[0] + Rm[7] * wb[1] + Rm[8] * wb[2]));
+        rotation_from_quaternion();
+        angles_from_rotation();
+        f3 lo = fileOrigin;
+        f3 ro = mk(R[0] * lo.x + R[1] * lo.y + R[2] * lo.z, R[3] * lo.x + R[4] * lo.y + R[5] * lo.z,
+                   R[6] * lo.x + R[7] * lo.y + R[8] * lo.z);
+        sdf_origin = pos + ro;
+        force = mk(0, 0, 0);
+        torque = mk(0, 0, 0);
+    }
+
+    /* FluidSolver3D::handleBodyWallCollisions FluidSolver3D.cu:2848-2895 (the
+     * reference has the call switched off, :2841): per axis, with r = half the
+     * largest SDF box side and walls at the centres of the border cells, a
+     * body closer than r to its clamp position is pushed back by the
+     * difference and a negative velocity along the axis is removed; the
+     * reference's assert dx == sdf cell size is not enforced */
+    void collide_walls(int W, int H, int D, float dx) {
+        const float r = 0.5f * fmaxf(fmaxf(size.x, size.y), size.z);
+        const float half = 0.5f * dx;
+        const f3 minWall = mk(half, half, half);
+        const f3 maxWall = mk(W * dx - half, H * dx - half, D * dx - half);
+        auto axis = [&](float& p, float pMin, float pMax, f3 n) {
+            const float distMin = p - pMin, distMax = pMax - p;
+            const float dist = fminf(distMin, distMax);
+            const float penetration = r - dist;
+            if (penetration > 0.0f) {
+                p = p + penetration * (distMin < distMax ? +1.0f : -1.0f);
+                const float vn = dot(vel, n);
+                if (vn < 0.0f) vel = vel - vn * n;
+            }
+        };
+        axis(pos.x, minWall.x + r, maxWall.x - r, mk(1, 0, 0));
+        axis(pos.y, minWall.y + r, maxWall.y - r, mk(0, 1, 0));
+        axis(pos.z, minWall.z + r, maxWall.z - r, mk(0, 0, 1));
+        sdf_origin = pos - size * 0.5f;
     }
 };
 
@@ -1019,7 +1172,11 @@ void stage_body(Solver& s) {
     b.last_force = b.force;
     b.last_torque = b.torque;
     b.last_sub = subf;
-    b.integrate(s.dt_cur);
+    if (s.cfg.step_flags & FS_STEP_BODY_INERTIA)
+        b.integrate_tensor(s.dt_cur);
+    else
+        b.integrate(s.dt_cur);
+    if (s.cfg.step_flags & FS_STEP_BODY_WALLS) b.collide_walls(s.W, s.H, s.D, s.cfg.dx);
 }
 
 /* ---------------- scene IO ------------------------------------------------------ */
@@ -1080,6 +1237,7 @@ void init_body(Solver& s, const float* sdf, int w, int h, int d, f3 origin, floa
     b.torque = mk(0, 0, 0);
     b.theta = b.psi = b.phi = 0;
     b.update_rotation_matrix();
+    b.compute_inertia_tensor();
     if (!s.labels.empty()) mark_body(s);
 }
 
@@ -1360,6 +1518,7 @@ int fs_set_body_state(void* h, const fs_body_state* in) {
     b.omega = mk(in->omega[0], in->omega[1], in->omega[2]);
     b.theta = in->theta; b.psi = in->psi; b.phi = in->phi;
     b.update_rotation_matrix();
+    b.quaternion_from_angles();
     return FS_OK;
 }
 
@@ -1614,6 +1773,7 @@ void body_ckpt_fields(Body& b, IO io) {
     f(b.pos); f(b.vel); f(b.force); f(b.omega); f(b.torque);
     f(b.mass); f(b.inertia); f(b.inv_inertia); f(b.theta); f(b.psi); f(b.phi);
     f(b.R); f(b.last_force); f(b.last_torque); f(b.last_sub);
+    f(b.Ib); f(b.Ibinv); f(b.q);
 }
 }  // namespace
 

@@ -144,3 +144,53 @@ def test_falling_sphere_through_impact(hip, oracle):
     assert bo.force[1] > np.float32(-9.81) * np.float32(scenes.sphere_mass(r, 500.0))
     lh, lo = h.get_grid(FS_GRID_LABELS), o.get_grid(FS_GRID_LABELS)
     assert (lh != lo).sum() <= 0.001 * lh.size
+
+
+def test_body_options_match_oracle(hip, oracle):
+    """FS_STEP_BODY_INERTIA (inertia tensor of a turned box's voxels, quaternion
+    orientation) on the one-cell case, and FS_STEP_BODY_WALLS with it for a
+    box that falls on the floor of an empty tank: the integrated state against
+    the oracle's, step by step."""
+    from body_cases import box_sdf
+    from fluidprjct_amd._abi import FS_STEP_BODY_INERTIA, FS_STEP_BODY_WALLS
+    cs = 0.005
+    sdf, origin = box_sdf((0.2, 0.12, 0.08), cs, 0.6)
+    U = np.array([0.25, -0.5, 0.75])
+    pos = (np.array([2, 2, 2]) + 0.5) * 0.125 + np.array([0.03, -0.05, 0.02])
+    res = []
+    for lib in (hip, oracle):
+        s = one_face_body(lib, ["+y", "-x", "+z"], U, U + np.array([-0.7, 0.9, 0.2]), pos, sdf=(sdf, origin, cs),
+                          step_flags=FS_STEP_BODY_INERTIA)
+        st = s.body_state()
+        st.theta, st.psi, st.phi = -0.4, 0.1, 0.6
+        st.omega[:] = (0.5, -1.0, 0.25)
+        s.set_body_state(st)
+        for _ in range(5):
+            s.stage("body")
+        res.append(state_arrays(s.body_state()))
+    h, o = res
+    for k in h:
+        assert close(h[k], o[k], 1e-6, 1e-6), (k, h[k], o[k])
+    # a box dropped in an empty tank, spinning, with wall collisions
+    N = 32
+    sdf, origin = box_sdf((0.16, 0.1, 0.08), 1.0 / N, 0.3)
+    sols = []
+    for lib in (hip, oracle):
+        s = lib.solver(dx=1.0 / N, dt=1.0 / 120, step_flags=FS_STEP_BODY_INERTIA | FS_STEP_BODY_WALLS)
+        s.set_labels(scenes.bordered(N, N, N))
+        s.set_body_sdf(sdf, origin, 1.0 / N, (0.5, 0.35, 0.5), 1.0)
+        st = s.body_state()
+        st.omega[:] = (2.0, 1.0, -3.0)
+        s.set_body_state(st)
+        s.seed_particles(1, 1)
+        sols.append(s)
+    floor = None
+    for step in range(40):
+        for s in sols:
+            s.step(1)
+        bh, bo = state_arrays(sols[0].body_state()), state_arrays(sols[1].body_state())
+        for k in ("pos", "vel", "omega", "rotation"):
+            assert close(bh[k], bo[k], 1e-5, 1e-3), (step, k, bh[k], bo[k])
+        if bo["vel"][1] == 0.0 and floor is None:
+            floor = step
+    assert floor is not None    # the walls stopped the fall

@@ -619,3 +619,135 @@ def test_step_reports_unconverged_solve(oracle):
         s.step(2)
     assert e.value.code == -4 and "2 of 2 steps" in str(e.value)
     assert s.clock()[2] == 2
+
+
+# ---- options beyond the reference's default path: inertia tensor with a
+# ---- quaternion orientation (FS_STEP_BODY_INERTIA), wall collision (FS_STEP_BODY_WALLS)
+def box_inertia(m, sides, angle_z=0.0):
+    """Closed form of a solid box, turned by angle_z about z."""
+    a, b, c = sides
+    I = m / 12.0 * np.diag([b * b + c * c, a * a + c * c, a * a + b * b])
+    ca, sa = np.cos(angle_z), np.sin(angle_z)
+    Rz = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1]])
+    return Rz @ I @ Rz.T
+
+
+@pytest.mark.parametrize("angle", [0.0, 0.6])
+def test_inertia_tensor_of_a_box(oracle, angle):
+    """One step from rest under a torque tau: omega = dt I^-1 tau (the
+    gyroscopic term vanishes at rest), with I the box's closed form (voxel
+    discretisation at 1/32 of the shortest side: 3%).  The turned box has
+    products of inertia."""
+    from body_cases import box_sdf
+    from fluidprjct_amd._abi import FS_STEP_BODY_INERTIA
+    dx, dt, mass = 0.125, 0.01, 2.0
+    sides = (0.2, 0.12, 0.08)
+    cs = 0.0025
+    sdf = box_sdf(sides, cs, angle)
+    cc = (np.array([2, 2, 2]) + 0.5) * dx
+    urel = np.array([0.0, 1.5, 0.0])
+    n = np.array(_DIRS["+y"], float)
+    pos = np.asarray(cc + 0.5 * dx * n + np.array([0.0, 0.0, -0.04]), np.float32)
+    s = one_face_body(oracle, ["+y"], (0, 0, 0), urel, pos, dx=dx, dt=dt, mass=mass, sdf=(sdf[0], sdf[1], cs),
+                      step_flags=FS_STEP_BODY_INERTIA)
+    f = face_force_expected(n, urel, dx=dx)
+    tau = np.cross(cc + 0.5 * dx * n - pos.astype(float), f)
+    s.stage("body")
+    om = np.array(s.body_state().omega, float)
+    want = dt * np.linalg.solve(box_inertia(mass, sides, angle), tau)
+    assert np.abs(om - want).max() <= 0.03 * np.abs(want).max(), (om, want)
+    if angle:
+        assert abs(want[1]) > 0.1 * abs(want[0])    # the turned box couples x and y
+
+
+def test_quaternion_rotation_about_a_principal_axis(oracle):
+    """Constant torque about a principal axis (x): omega_n = n (tau / Ixx) dt
+    and the angle about x = sum of 2 atan(omega_k dt / 2) (the normalised
+    quaternion step), which phi reports; R stays a rotation about x."""
+    from body_cases import box_sdf
+    from fluidprjct_amd._abi import FS_STEP_BODY_INERTIA
+    dx, dt, mass, steps = 0.125, 0.01, 2.0, 10
+    sides = (0.2, 0.12, 0.08)
+    sdf = box_sdf(sides, 0.005)
+    cc = (np.array([2, 2, 2]) + 0.5) * dx
+    urel = np.array([0.0, 1.5, 0.0])
+    n = np.array(_DIRS["+y"], float)
+    pos = np.asarray(cc + 0.5 * dx * n + np.array([0.0, 0.0, -0.04]), np.float32)
+    s = one_face_body(oracle, ["+y"], (0, 0, 0), urel, pos, dx=dx, dt=dt, mass=mass, sdf=(sdf[0], sdf[1], 0.005),
+                      step_flags=FS_STEP_BODY_INERTIA)
+    s.stage("body")
+    w1 = s.body_state().omega[0]               # dt tau / Ixx (the tensor is pinned above)
+    for _ in range(steps - 1):
+        st = s.body_state()
+        st.pos[:] = pos
+        st.vel[:] = urel
+        s.set_body_state(st)
+        s.stage("body")
+    st = s.body_state()
+    assert st.omega[0] == pytest.approx(steps * w1, rel=1e-5)
+    assert abs(st.omega[1]) <= 1e-6 * abs(st.omega[0]) and abs(st.omega[2]) <= 1e-6 * abs(st.omega[0])
+    want = sum(2 * np.arctan(k * w1 * dt / 2) for k in range(1, steps + 1))
+    assert st.phi == pytest.approx(want, rel=1e-4)
+    assert abs(st.theta) <= 1e-6 and abs(st.psi) <= 1e-6
+    R = np.array(st.rotation, float).reshape(3, 3)
+    c, sn = np.cos(want), np.sin(want)
+    assert np.abs(R - np.array([[1, 0, 0], [0, c, -sn], [0, sn, c]])).max() <= 1e-5
+
+
+def test_torque_free_spin_conserves_angular_momentum(oracle):
+    """A turned box spinning about a general axis in air (no fluid next to
+    it, so no drag and no torque): the world-frame angular momentum
+    R I_b R^T omega, with I_b the box's closed form, stays within 3.5% over
+    40 small steps (3% for the voxel tensor against the closed form, 0.5% for
+    the integration), while omega itself precesses."""
+    from body_cases import box_sdf
+    from fluidprjct_amd._abi import FS_STEP_BODY_INERTIA
+    N, dt, mass = 16, 2e-3, 1.0
+    sides = (0.3, 0.18, 0.1)
+    sdf, origin = box_sdf(sides, 0.01, 0.3)
+    s = oracle.solver(dx=1.0 / N, dt=dt, step_flags=FS_STEP_BODY_INERTIA)
+    s.set_labels(scenes.bordered(N, N, N))
+    s.set_body_sdf(sdf, origin, 0.01, (0.5, 0.5, 0.5), mass)
+    st = s.body_state()
+    st.omega[:] = (3.0, -4.0, 2.0)
+    s.set_body_state(st)
+    Ib = box_inertia(mass, sides, 0.3)
+
+    def L(st):
+        R = np.array(st.rotation, float).reshape(3, 3)
+        return R @ Ib @ R.T @ np.array(st.omega, float)
+
+    L0 = L(s.body_state())
+    om0 = np.array(s.body_state().omega, float)
+    for _ in range(40):
+        s.stage("body")
+    st = s.body_state()
+    assert np.linalg.norm(L(st) - L0) <= 0.035 * np.linalg.norm(L0)
+    assert np.linalg.norm(np.array(st.omega, float) - om0) > 1e-3 * np.linalg.norm(om0)
+
+
+def test_wall_collision(oracle):
+    """handleBodyWallCollisions (FluidSolver3D.cu:2848-2895): with r = half the
+    largest SDF box side and the wall at the centre of the border cell, a
+    body within r of its clamp position pMin = dx/2 + r is moved out by the
+    difference, and its velocity into the wall is removed; along the other
+    axes it is left alone."""
+    from fluidprjct_amd._abi import FS_STEP_BODY_WALLS
+    N, dt = 32, 1e-3
+    dx = 1.0 / N
+    sdf, origin, cs = scenes.sphere_sdf(0.1, dx)          # box side w * cs
+    r = 0.5 * sdf.shape[2] * cs
+    pmin = 0.5 * dx + r
+    y0 = pmin + 0.3 * r                                     # 0.3 r past the clamp: pushed by 0.7 r
+    s = oracle.solver(dx=dx, dt=dt, step_flags=FS_STEP_BODY_WALLS)
+    s.set_labels(scenes.bordered(N, N, N))
+    s.set_body_sdf(sdf, origin, cs, (0.5, y0, 0.5), 1.0)
+    st = s.body_state()
+    st.vel[:] = (0.2, -1.0, 0.0)
+    s.set_body_state(st)
+    s.stage("body")
+    st = s.body_state()
+    y1 = y0 + (-1.0 - 9.81 * dt) * dt                      # after the free-fall step
+    assert st.pos[1] == pytest.approx(y1 + (r - (y1 - pmin)), rel=1e-5)
+    assert st.vel[1] == 0.0 and st.vel[0] == pytest.approx(0.2)
+    assert st.pos[0] == pytest.approx(0.5 + 0.2 * dt, rel=1e-5)
