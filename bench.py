@@ -142,6 +142,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch the inner PCG iterations eagerly")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--learned-steps", type=int, default=5)
+    ap.add_argument("--learned-params", default="", help="parameters of the learned preconditioner (.npy)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a float32 y = x + 1 kernel over MIB MiB (read MIB, write MIB): the "
                          "reference for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -279,7 +280,8 @@ def main():
     if not args.no_learned:
         log("learned-preconditioner steps")
         from fluidprjct_amd.learned import DEFAULT_PARAMS, use_learned
-        use_learned(s, DEFAULT_PARAMS, 5)
+        params = args.learned_params or DEFAULT_PARAMS
+        use_learned(s, params, 5)
         s.step(1)
         torch.cuda.synchronize()
         t0s = s.stage_times()
@@ -294,7 +296,7 @@ def main():
         tl = stage_delta(t0s, s.stage_times())
         result["pressure_solve_ms_learned"] = tl["pressure"] / args.learned_steps
         result["learned"] = {"steps": args.learned_steps, "fallbacks": fb, "iterations": iters,
-                             "params": os.path.relpath(DEFAULT_PARAMS, ROOT),
+                             "params": os.path.relpath(params, ROOT),
                              "note": "same workload continued after the timed steps; flexible PCG with the "
                                      "BF16 conv net, multigrid fallback after 5 stalled iterations"}
     del s

@@ -41,7 +41,7 @@ constexpr uint32_t SELF = 0x8000u;
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
 enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_RP, PS_MEAN, PS_COUNT };
 
-enum { FIN_BNORM = 0, FIN_OUTER };
+enum { FIN_BNORM = 0, FIN_OUTER, FIN_OUTER_REL };
 
 // 1/d for the smoothers (d = 1..6): the hardware reciprocal (1 ulp) instead of
 // a correctly rounded division.  The preconditioner only needs one fixed
@@ -1544,6 +1544,114 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
     store_partial(a1, sm, p1);
 }
 
+// Level-0 set-up in one pass over the solve box (offset ox, oy, oz in the full
+// W x H grid): the type and stencil code of each cell from the labels (as
+// k_type0 + k_code on level 0: neighbours outside the box count as none), b
+// gathered into the box, the unknown count and Dirichlet flag (counters[0],
+// counters[1]) and the partial sums of b^2 and b over the unknowns, with the
+// cells visited in k_bnorm's order (same partial sums, same bits).
+__global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ labels, const float* __restrict__ bfull,
+                                                MGLevel L, int W, int H, int ox, int oy, int oz,
+                                                float* __restrict__ bbox, double* __restrict__ p2,
+                                                double* __restrict__ p1, int* __restrict__ counters) {
+    __shared__ double sm[12];
+    __shared__ int s_cnt[2];
+    if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
+    __syncthreads();
+    const long long WHf = (long long)W * H;
+    double a2 = 0.0, a1 = 0.0;
+    int unk = 0, dir = 0;
+    auto kind = [](uint8_t lab) { return lab == FS_FLUID ? 1 : (lab == FS_AIR ? 2 : 0); };
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        split3(c, L.W, L.H, i, j, k);
+        const long long g = (long long)(i + ox) + (long long)(j + oy) * W + (long long)(k + oz) * WHf;
+        const int t = kind(labels[g]);
+        const float bv = bfull[g];
+        L.type[c] = (uint8_t)t;
+        bbox[c] = bv;
+        uint32_t code = 0;
+        if (t == 1) {
+            int nt[6] = {0, 0, 0, 0, 0, 0};
+            if (i + 1 < L.W) nt[0] = kind(labels[g + 1]);
+            if (i > 0) nt[1] = kind(labels[g - 1]);
+            if (j + 1 < L.H) nt[2] = kind(labels[g + W]);
+            if (j > 0) nt[3] = kind(labels[g - W]);
+            if (k + 1 < L.D) nt[4] = kind(labels[g + WHf]);
+            if (k > 0) nt[5] = kind(labels[g - WHf]);
+            uint32_t diag = 0;
+            int air = 0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (nt[q] == 1) { code |= (1u << q); ++diag; }
+                else if (nt[q] == 2) { ++diag; air = 1; }
+            }
+            // an empty row (no FLUID or AIR neighbour) is no unknown
+            if (diag) {
+                code |= SELF | (diag << 8);
+                ++unk;
+                dir |= air;
+                const double v = bv;
+                a2 += v * v;
+                a1 += v;
+            } else {
+                code = 0;
+            }
+        }
+        L.code[c] = (uint16_t)code;
+    }
+    unk = wave_sum(unk);
+    dir = wave_sum(dir);
+    if ((threadIdx.x & 63) == 0) {
+        if (unk) atomicAdd(&s_cnt[0], unk);
+        if (dir) atomicOr(&s_cnt[1], 1);
+    }
+    store_partial(a2, sm, p2);
+    store_partial(a1, sm, p1);
+    if (threadIdx.x == 0) {
+        if (s_cnt[0]) atomicAdd(&counters[0], s_cnt[0]);
+        if (s_cnt[1]) atomicOr(&counters[1], 1);
+    }
+}
+
+// The warm start's first residual: x = x0 (full grid, box offset ox, oy, oz) on
+// the unknowns, 0 elsewhere, written to x64, and r = b - shift - A x with the
+// operations of k_resid64 (the same bits as gathering x0 first)
+__global__ __launch_bounds__(256) void k_resid64_x0(MGLevel L, const float* __restrict__ b,
+                                                    const double* __restrict__ x0, int W, int H, int ox, int oy,
+                                                    int oz, double shift, float* __restrict__ r,
+                                                    double* __restrict__ partial, double* __restrict__ x64) {
+    __shared__ double sm[12];
+    const long long WHf = (long long)W * H;
+    double acc = 0.0;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const uint32_t code = L.code[c];
+        float rf = 0.0f;
+        double xc = 0.0;
+        if (code & SELF) {
+            int i, j, k;
+            split3(c, L.W, L.H, i, j, k);
+            const long long g = (long long)(i + ox) + (long long)(j + oy) * W + (long long)(k + oz) * WHf;
+            xc = x0[g];
+            double sx = (double)((code >> 8) & 7u) * xc;
+            if (code & 1u) sx -= x0[g + 1];
+            if (code & 2u) sx -= x0[g - 1];
+            if (code & 4u) sx -= x0[g + W];
+            if (code & 8u) sx -= x0[g - W];
+            if (code & 16u) sx -= x0[g + WHf];
+            if (code & 32u) sx -= x0[g - WHf];
+            const double rv = ((double)b[c] - shift) - sx;
+            acc += rv * rv;
+            rf = (float)rv;
+        }
+        r[c] = rf;
+        x64[c] = xc;
+    }
+    store_partial(acc, sm, partial);
+}
+
 // r = (b - shift) - A x64 in FP64 on unknowns; r stored FP32; partials of r.r (FP64)
 // ZERO_X: the first pass, which also sets x64 = 0, so r = b - shift
 template <bool ZERO_X>
@@ -1650,6 +1758,8 @@ __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __rest
         sc->bnorm2 = s1;
         sc->bsum = s2;
     } else {
+        // FIN_OUTER_REL: tol2b holds tol^2, relative to the b norm of FIN_BNORM
+        if (mode == FIN_OUTER_REL) tol2b *= sc->bnorm2;
         sc->rr_true = s1;
         sc->tol2b = tol2b;
         if (s1 <= tol2b) {
@@ -1815,12 +1925,14 @@ void MGPCG::configure(int Wb, int Hb, int Db) {
 double* MGPCG::slot(int s) { return partial + (size_t)s * RED_BLOCKS; }
 
 void MGPCG::setup(const uint8_t* labels, const float* b, hipStream_t st) {
-    hipLaunchKernelGGL(k_type0, dim3(nblk(n0)), dim3(256), 0, st, labels, b, lv[0], W, H, box[0], box[1], box[2], bb);
+    // level 0: types, codes, b and the partial sums of the b norm in one pass
+    hipLaunchKernelGGL(k_setup0, dim3(nred), dim3(256), 0, st, labels, b, lv[0], W, H, box[0], box[1], box[2], bb,
+                       slot(PS_B2), slot(PS_B1), counters + 8);
     for (int l = 0; l + 1 < nlev; ++l)
         hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
-    for (int l = 0; l < nlev; ++l)
+    for (int l = 1; l < nlev; ++l)
         hipLaunchKernelGGL(k_code, dim3((unsigned)std::min<long long>(1024, nblk(lv[l].n))), dim3(256), 0, st, lv[l],
-                           counters + (l == 0 ? 8 : 10));
+                           counters + 10);
 }
 
 MGLevel MGPCG::fine() const { return lv[0]; }
@@ -2142,8 +2254,20 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     init.maxit = maxit;
     init.inner_rtol2 = inner_rtol * inner_rtol;
     hipMemcpyAsync(sc, &init, sizeof(SolveScalars), hipMemcpyHostToDevice, st);
-    hipLaunchKernelGGL(k_bnorm, dim3(nred), dim3(256), 0, st, L0, bb, slot(PS_B2), slot(PS_B1));
+    // the b norm's partial sums came from the level-0 set-up pass
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_BNORM, slot(PS_B2), slot(PS_B1), nred, sc, 0.0);
+    // The first residual is launched before the host knows whether the system
+    // is singular (no Dirichlet cell: then b is shifted by its mean and the
+    // warm start's mean removed, and this pass is redone below), with the
+    // tolerance relative to the b norm on the device: one host round trip.
+    hipMemcpyAsync(&sc->n_unknown, counters + 8, sizeof(int), hipMemcpyDeviceToDevice, st);
+    if (x0)
+        hipLaunchKernelGGL(k_resid64_x0, dim3(nred), dim3(256), 0, st, L0, bb, x0, W, H, box[0], box[1], box[2], 0.0,
+                           r, slot(PS_RES), xb);
+    else
+        hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, 0.0, r, slot(PS_RES));
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER_REL, slot(PS_RES),
+                       (const double*)nullptr, nred, sc, tol * tol);
     hipMemcpyAsync(host_counters + 8, counters + 8, sizeof(int) * 2, hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
@@ -2153,16 +2277,21 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     loc.singular = singular ? 1 : 0;
     n_unk = nunk;
     singular_mode = singular;
-    hipMemcpyAsync(&sc->n_unknown, counters + 8, sizeof(int), hipMemcpyDeviceToDevice, st);
     const double shift = singular ? sc_host->bsum / nunk : 0.0;
     const double bn2 =
         singular ? std::max(0.0, sc_host->bnorm2 - sc_host->bsum * sc_host->bsum / nunk) : sc_host->bnorm2;
     const double tol2b = tol * tol * bn2;
+    // the first pass is redone when the system is singular, when b is zero
+    // (the solution is then 0, whatever the initial guess), and when the
+    // initial guess is no better than 0
+    const bool first_done = !singular && bn2 > 0.0 && !(x0 && sc_host->rr_true >= bn2);
     int total_iters = 0, outer = 0;
     bool converged = false;
     double rr_true = 0.0, rr_first = 0.0;
     for (outer = 0;; ++outer) {
-        if (outer == 0 && x0 && bn2 > 0.0) {
+        if (outer == 0 && first_done) {
+            // the speculative first pass stands
+        } else if (outer == 0 && x0 && bn2 > 0.0 && singular) {
             // warm start: the initial guess on the unknowns of the box (0 elsewhere)
             hipLaunchKernelGGL(k_gather_x0, dim3(nblk(n0)), dim3(256), 0, st, L0, x0, W, H, box[0], box[1], box[2],
                                xb);
@@ -2180,11 +2309,13 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         else
             hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
                                slot(PS_RES));
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
-                           (const double*)nullptr, nred, sc, tol2b);
-        hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
-        if (hipStreamSynchronize(st) != hipSuccess) return -5;
-        if (outer == 0 && x0 && bn2 > 0.0 && sc_host->rr_true >= bn2) {
+        if (!(outer == 0 && first_done)) {
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
+                               (const double*)nullptr, nred, sc, tol2b);
+            hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) return -5;
+        }
+        if (outer == 0 && x0 && bn2 > 0.0 && singular && sc_host->rr_true >= bn2) {
             // the initial guess is no better than 0: start from 0
             hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, slot(PS_RES));
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
