@@ -26,10 +26,11 @@
 #endif
 
 // P2G LDS tile (cells); one 256-thread block per tile.  (TX + 2) x (TY + 2)
-// halo cells per z-plane, one per thread of the window phase.  Stage ms at
-// 256^3: 18x10x8 1.93, 22x8x8 1.96, 16x12x8 2.09, 20x9x8 2.11, 16x10x8 2.13,
-// 14x14x8 2.13, 16x12x12 2.21, 30x6x8 2.24, 16x12x6 2.30, 62x2x8 3.08; within
-// 3% of each other (18x10x8, 22x8x8, 16x12x8) in a developed 128^3 dam break.
+// halo cells per z-plane, one per thread of the window phase; CAP particles
+// of a plane staged in LDS per batch (24 bytes each; with the window exchange
+// two blocks fit a CU).  Stage ms at 256^3, 8 ppc at rest: TZ 16 CAP 1920
+// 0.99, TZ 32 CAP 1960 0.99, TZ 8 CAP 1920 1.08, TZ 16 CAP 1280 1.31, TZ 24
+// 1.29 (ragged in z).
 #ifndef P2G_TX
 #define P2G_TX 18
 #endif
@@ -37,7 +38,10 @@
 #define P2G_TY 10
 #endif
 #ifndef P2G_TZ
-#define P2G_TZ 8
+#define P2G_TZ 16
+#endif
+#ifndef P2G_CAP
+#define P2G_CAP 1920
 #endif
 
 namespace fsk {

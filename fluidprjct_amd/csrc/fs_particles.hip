@@ -274,222 +274,319 @@ __global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels
 // {ck-1..ck+1}; v and w alike with the 2-wide axis along y or z).  Along each
 // axis a particle's base slot s gets 1 - f and slot s+1 gets f; a slot beyond
 // the window is a face beyond the domain, which the reference skips.
-// One component at a time, the halo is walked plane by plane along z:
-//   A. one thread per halo cell of the plane sums the weights and weighted
-//      velocities of its particles into its window in registers (weights and
-//      products as AccumulateVelocities :468-594) and stores the window in LDS;
-//   B. one thread per owned face in the planes those windows reach adds the
-//      window slots that land on it (at most 3 x 3 cells of the plane).
-// No LDS atomics: every face sum has one writer per phase, so the result is
-// also the same from run to run.  Each owned face is finally normalised
-// (ComputeVelocityFunc :598-605), saved (saveVelocities) and given gravity
-// (ApplyScalarForce :687-698) in one write.
+// The halo is walked plane by plane along z; per plane kc:
+//   S. the plane's particles (one contiguous range per halo row, the cells
+//      being sorted) are copied to LDS by direct global->LDS loads, issued
+//      while the previous plane is still being reduced (phase B), so their
+//      latency is hidden; a plane with more than CAP particles takes further
+//      batches;
+//   A. one thread per halo cell sums the weights and weighted velocities of
+//      its particles, in their sorted order, into its three windows in
+//      registers (weights and products as AccumulateVelocities :468-594);
+//   B. per component, the windows go to LDS and one thread per owned face
+//      column adds the window slots that land on its faces (at most 3 x 3
+//      cells of the plane) into registers that hold the column's faces of
+//      the face planes the current plane reaches.
+// A face plane is complete once the cell planes either side of it are done;
+// its faces are then normalised (ComputeVelocityFunc :598-605), saved
+// (saveVelocities) and given gravity (ApplyScalarForce :687-698) in one
+// write.  No atomics: every face sum has one writer and a fixed order, so the
+// result is also the same from run to run.
 constexpr int P2G_SLOTS = 18;
+typedef __attribute__((address_space(1))) const void gvoid_t;
+typedef __attribute__((address_space(3))) void lvoid_t;
 
-// window of one cell for one component: slot x + NX (y + NY z)
-template <int COMP, bool P2>
-FS_DEV void p2g_cell_window(const P2GArgs& a, int ci, int cj, int ck, int beg, int end, float (&num)[P2G_SLOTS],
-                            float (&den)[P2G_SLOTS]) {
+// Barrier that orders LDS reads and writes only: the direct global->LDS loads
+// of the next plane stay in flight (a __syncthreads would wait for them).
+FS_DEV void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Weights of one axis of a window: base slot s gets a0 = 1 - f and slot s + 1
+// gets a1 = f.  s is 0 or 1 for every particle: the cell index and the base
+// index are floors of the same quotient (or of it minus 1/2), clamped alike.
+template <int N>
+FS_DEV void p2g_axis(int s, float f, float (&w)[N]) {
+    const bool s0 = s == 0;
+    const float a0 = 1.0f - f, a1 = f;
+    w[0] = s0 ? a0 : 0.0f;
+    w[1] = s0 ? a1 : a0;
+    if (N == 3) w[N - 1] = s0 ? 0.0f : a1;
+}
+
+// Adds one particle (rx, ry, rz = position / dx) of cell (ci,cj,ck) to that
+// cell's window of one component: slot x + NX (y + NY z) gains
+// (wx wy)(vc wz) in num and (wx wy) wz in den.
+template <int COMP>
+FS_DEV void p2g_add(const P2GArgs& a, int ci, int cj, int ck, float rx, float ry, float rz, float vc,
+                    float (&num)[P2G_SLOTS], float (&den)[P2G_SLOTS]) {
     constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
-    const int W = a.W, H = a.H, D = a.D;
-    const float dx = a.dx, inv = a.inv_dx;
-    const float* __restrict__ vel = COMP == 0 ? a.P.vx : (COMP == 1 ? a.P.vy : a.P.vz);
     const int ox = (COMP == 0) ? ci : ci - 1, oy = (COMP == 1) ? cj : cj - 1, oz = (COMP == 2) ? ck : ck - 1;
+    const Base b = mac_base<COMP>(rx, ry, rz, a.W, a.H, a.D);
+    float wx[NX], wy[NY], wz[NZ], vz[NZ];
+    p2g_axis<NX>(b.i - ox, b.fx, wx);
+    p2g_axis<NY>(b.j - oy, b.fy, wy);
+    p2g_axis<NZ>(b.k - oz, b.fz, wz);
 #pragma unroll
-    for (int q = 0; q < P2G_SLOTS; ++q) { num[q] = 0.f; den[q] = 0.f; }
-    if (beg >= end) return;
-    float px = a.P.px[beg], py = a.P.py[beg], pz = a.P.pz[beg], vc = vel[beg];
-    for (int p = beg; p < end; ++p) {
-        const int q = p + 1 < end ? p + 1 : p;
-        const float npx = a.P.px[q], npy = a.P.py[q], npz = a.P.pz[q], nvc = vel[q];
-        const float rx = div_dx<P2>(px, dx, inv), ry = div_dx<P2>(py, dx, inv), rz = div_dx<P2>(pz, dx, inv);
-        const Base b = mac_base<COMP>(rx, ry, rz, W, H, D);
-        const int sx = b.i - ox, sy = b.j - oy, sz = b.k - oz;
-        const float ax0 = 1.0f - b.fx, ax1 = b.fx, ay0 = 1.0f - b.fy, ay1 = b.fy, az0 = 1.0f - b.fz, az1 = b.fz;
-        float wx[NX], wy[NY], wz[NZ];
+    for (int z = 0; z < NZ; ++z) vz[z] = vc * wz[z];
 #pragma unroll
-        for (int t = 0; t < NX; ++t) wx[t] = t == sx ? ax0 : (t == sx + 1 ? ax1 : 0.0f);
+    for (int y = 0; y < NY; ++y)
 #pragma unroll
-        for (int t = 0; t < NY; ++t) wy[t] = t == sy ? ay0 : (t == sy + 1 ? ay1 : 0.0f);
+        for (int x = 0; x < NX; ++x) {
+            const float wxy = wx[x] * wy[y];
 #pragma unroll
-        for (int t = 0; t < NZ; ++t) wz[t] = t == sz ? az0 : (t == sz + 1 ? az1 : 0.0f);
-#pragma unroll
-        for (int z = 0; z < NZ; ++z)
-#pragma unroll
-            for (int y = 0; y < NY; ++y)
-#pragma unroll
-                for (int x = 0; x < NX; ++x) {
-                    const int sl = x + NX * (y + NY * z);
-                    const float wgt = wx[x] * wy[y] * wz[z];
-                    num[sl] += vc * wgt;
-                    den[sl] += wgt;
-                }
-        px = npx; py = npy; pz = npz; vc = nvc;
-    }
-}
-
-// One component on one halo plane kc: phases A and B.  A window is handed
-// over one z-slice at a time (the slots that land on one face plane), so
-// swn/swd hold [slot in slice][cell] for one slice of the plane; sfn/sfd are
-// the sums of this component's owned faces.
-template <int COMP, int TX, int TY, int TZ, bool P2>
-FS_DEV void p2g_plane(const P2GArgs& a, int x0, int y0, int z0, int kc, int beg, int end, float* __restrict__ swn,
-                      float* __restrict__ swd, float* __restrict__ sfn, float* __restrict__ sfd) {
-    constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
-    constexpr int NXY = NX * NY;
-    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
-    static_assert(HP <= 256, "one thread per halo cell of a plane");
-    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
-    const int W = a.W, H = a.H, D = a.D;
-    // owned faces: [x0, hix] x [y0, hiy] x [z0, hiz]
-    const int hix = (COMP == 0) ? ((x0 + TX >= W) ? W : x0 + TX - 1) : min(x0 + TX, W) - 1;
-    const int hiy = (COMP == 1) ? ((y0 + TY >= H) ? H : y0 + TY - 1) : min(y0 + TY, H) - 1;
-    const int hiz = (COMP == 2) ? ((z0 + TZ >= D) ? D : z0 + TZ - 1) : min(z0 + TZ, D) - 1;
-    const int t = threadIdx.x;
-    const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
-    // ---- A: the window of this thread's halo cell of plane kc
-    float num[P2G_SLOTS], den[P2G_SLOTS];
-    p2g_cell_window<COMP, P2>(a, ci, cj, kc, beg, end, num, den);
-    // ---- B, one z-slice z of the windows at a time: face plane k = kc - (NZ == 3) + z
-#pragma unroll
-    for (int z = 0; z < NZ; ++z) {
-        if (t < HP) {
-#pragma unroll
-            for (int q = 0; q < NXY; ++q) {
-                swn[q * HP + t] = num[q + NXY * z];
-                swd[q * HP + t] = den[q + NXY * z];
+            for (int z = 0; z < NZ; ++z) {
+                const int sl = x + NX * (y + NY * z);
+                num[sl] = fmaf(wxy, vz[z], num[sl]);
+                den[sl] = fmaf(wxy, wz[z], den[sl]);
             }
         }
-        __syncthreads();
-        const int k = kc - (NZ == 3 ? 1 : 0) + z;
-        if (k >= z0 && k <= hiz) {
-            for (int f = t; f < LX * LY; f += blockDim.x) {
-                const int li = f % LX, lj = f / LX;
-                const int i = x0 + li, j = y0 + lj;
-                if (i > hix || j > hiy) continue;
-                float sn = 0.0f, sd = 0.0f;
-                // cells with window slot x = i - ox(ci), ox = ci - (NX == 3); y alike
-#pragma unroll
-                for (int y = 0; y < NY; ++y) {
-                    const int hy = j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
-                    if (hy < 0 || hy >= HY) continue;
-#pragma unroll
-                    for (int x = 0; x < NX; ++x) {
-                        const int hx = i - x + (NX == 3 ? 1 : 0) - (x0 - 1);
-                        if (hx < 0 || hx >= HX) continue;
-                        const int q = x + NX * y, cell = hx + HX * hy;
-                        sn += swn[q * HP + cell];
-                        sd += swd[q * HP + cell];
-                    }
-                }
-                const int l = li + LX * (lj + LY * (k % 3));
-                sfn[l] += sn;
-                sfd[l] += sd;
-            }
-        }
-        __syncthreads();
-    }
 }
 
-// Normalise, save and add gravity to the owned faces of face plane k of one
-// component (sums in ring slot k % 3), then clear the slot for plane k + 3.
+// Owned faces of one component: columns (li, lj) of [x0, hix] x [y0, hiy],
+// planes [z0, hiz]; LX x LY columns, one per thread.
 template <int COMP, int TX, int TY, int TZ>
-FS_DEV void p2g_write_plane(const P2GArgs& a, int x0, int y0, int k, float* __restrict__ sfn,
-                            float* __restrict__ sfd) {
-    constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
-    const int nx = a.W + (COMP == 0), ny = a.H + (COMP == 1);
-    const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
-    const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
-    float* __restrict__ g = COMP == 0 ? a.u : COMP == 1 ? a.v : a.w;
-    float* __restrict__ gs = COMP == 0 ? a.us : COMP == 1 ? a.vs : a.ws;
-    const float acc = COMP == 1 ? a.gravity_y : 0.0f;
-    const float vel_unknown = -2147483648.0f;  // (float)INT_MIN, utility.cuh:56
-    for (int f = threadIdx.x; f < LX * LY; f += blockDim.x) {
-        const int li = f % LX, lj = f / LX;
-        const int l = li + LX * (lj + LY * (k % 3));
-        const int i = x0 + li, j = y0 + lj;
-        if (i <= hix && j <= hiy) {
-            const float num = sfn[l], den = sfd[l];
-            const float val = (den > 1e-8f) ? num / den : 0.0f;
-            const size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
-            gs[idx] = val;
-            g[idx] = (val > vel_unknown) ? val + a.dt * acc : val;
+struct P2GFaces {
+    static constexpr int NX = (COMP == 0) ? 2 : 3, NY = (COMP == 1) ? 2 : 3, NZ = (COMP == 2) ? 2 : 3;
+    static constexpr int LX = TX + (COMP == 0), LY = TY + (COMP == 1);
+    int i, j, hiz;
+    bool own;
+    FS_DEV P2GFaces(const P2GArgs& a, int x0, int y0, int z0) {
+        const int t = threadIdx.x;
+        const int hix = (COMP == 0) ? ((x0 + TX >= a.W) ? a.W : x0 + TX - 1) : min(x0 + TX, a.W) - 1;
+        const int hiy = (COMP == 1) ? ((y0 + TY >= a.H) ? a.H : y0 + TY - 1) : min(y0 + TY, a.H) - 1;
+        hiz = (COMP == 2) ? ((z0 + TZ >= a.D) ? a.D : z0 + TZ - 1) : min(z0 + TZ, a.D) - 1;
+        i = x0 + t % LX;
+        j = y0 + t / LX;
+        own = t < LX * LY && i <= hix && j <= hiy;
+    }
+    // normalise, save and add gravity to face (i, j, k)
+    FS_DEV void store(const P2GArgs& a, int k, float num, float den) const {
+        const int nx = a.W + (COMP == 0), ny = a.H + (COMP == 1);
+        float* __restrict__ g = COMP == 0 ? a.u : COMP == 1 ? a.v : a.w;
+        float* __restrict__ gs = COMP == 0 ? a.us : COMP == 1 ? a.vs : a.ws;
+        const float acc = COMP == 1 ? a.gravity_y : 0.0f;
+        const float vel_unknown = -2147483648.0f;  // (float)INT_MIN, utility.cuh:56
+        const float val = (den > 1e-8f) ? num / den : 0.0f;
+        const size_t idx = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
+        gs[idx] = val;
+        g[idx] = (val > vel_unknown) ? val + a.dt * acc : val;
+    }
+};
+
+// Phase B of one component for the current cell plane: the windows of the
+// plane's halo cells go to LDS (xch: [num | den][slot][halo cell]); each owned
+// face column adds the slots that land on its face in slice z, face plane
+// kc - (NZ == 3) + z, to rn[z] / rd[z].
+template <int COMP, int TX, int TY, int TZ>
+FS_DEV void p2g_gather(const P2GFaces<COMP, TX, TY, TZ>& F, int x0, int y0, const float (&num)[P2G_SLOTS],
+                       const float (&den)[P2G_SLOTS], float* __restrict__ xch, float (&rn)[3], float (&rd)[3]) {
+    typedef P2GFaces<COMP, TX, TY, TZ> FT;
+    constexpr int NX = FT::NX, NY = FT::NY, NZ = FT::NZ, NXY = NX * NY;
+    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    const int t = threadIdx.x;
+    if (t < HP) {
+#pragma unroll
+        for (int q = 0; q < P2G_SLOTS; ++q) {
+            xch[q * HP + t] = num[q];
+            xch[(P2G_SLOTS + q) * HP + t] = den[q];
         }
-        sfn[l] = 0.0f;
-        sfd[l] = 0.0f;
+    }
+    lds_barrier();
+    if (F.own) {
+        float sn[NZ], sd[NZ];
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) { sn[z] = 0.0f; sd[z] = 0.0f; }
+        // cells with window slot x = i - ox(ci), ox = ci - (NX == 3); y alike
+#pragma unroll
+        for (int y = 0; y < NY; ++y) {
+            const int hy = F.j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
+            if (hy < 0 || hy >= HY) continue;
+#pragma unroll
+            for (int x = 0; x < NX; ++x) {
+                const int hx = F.i - x + (NX == 3 ? 1 : 0) - (x0 - 1);
+                if (hx < 0 || hx >= HX) continue;
+                const int cell = hx + HX * hy;
+#pragma unroll
+                for (int z = 0; z < NZ; ++z) {
+                    const int q = x + NX * y + NXY * z;
+                    sn[z] += xch[q * HP + cell];
+                    sd[z] += xch[(P2G_SLOTS + q) * HP + cell];
+                }
+            }
+        }
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) { rn[z] += sn[z]; rd[z] += sd[z]; }
+    }
+    lds_barrier();
+}
+
+// Halo rows of one cell plane: lanes q < HY of every wave hold row q's first
+// particle (rb) and count (cnt); all other lanes hold 0.
+struct P2GRows {
+    int rb, cnt;
+};
+
+template <int TX, int TY>
+FS_DEV P2GRows p2g_rows_load(const P2GArgs& a, int x0, int y0, int kc) {
+    constexpr int HY = TY + 2;
+    P2GRows r{0, 0};
+    const int q = threadIdx.x & 63, rj = y0 - 1 + q;
+    if (q < HY && rj >= 0 && rj < a.H) {
+        const int lo = max(x0 - 1, 0), hi = min(x0 + TX, a.W - 1);
+        const long long row = (long long)rj * a.W + (long long)kc * a.W * a.H;
+        r.rb = a.start[row + lo];
+        r.cnt = a.start[row + hi + 1] - r.rb;
+    }
+    return r;
+}
+
+// Copies particles [c0, c0 + CAP) of the plane's list (rows back to back) to
+// stg by direct global->LDS loads: each wave takes every fourth row, 64
+// particles per instruction.  excl: row offsets (exclusive scan, lane q).
+template <int TY, int CAP>
+FS_DEV void p2g_stage(const P2GArgs& a, float* __restrict__ stg, int c0, const P2GRows& R, int excl) {
+    constexpr int HY = TY + 2;
+    const float* __restrict__ src[6] = {a.P.px, a.P.py, a.P.pz, a.P.vx, a.P.vy, a.P.vz};
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = blockDim.x >> 6;
+    for (int q = wave; q < HY; q += nw) {
+        const int ex = __builtin_amdgcn_readlane(excl, q), cn = __builtin_amdgcn_readlane(R.cnt, q);
+        const int rb = __builtin_amdgcn_readlane(R.rb, q);
+        const int s0 = max(ex, c0), s1 = min(ex + cn, c0 + CAP);
+        for (int g0 = s0; g0 < s1; g0 += 64) {
+            const int ge = g0 + lane;
+            if (ge < s1) {
+                const int gi = rb + (ge - ex);
+                float* dst = stg + (g0 - c0);
+#pragma unroll
+                for (int s = 0; s < 6; ++s)
+                    __builtin_amdgcn_global_load_lds((gvoid_t*)(src[s] + gi), (lvoid_t*)(dst + s * CAP), 4, 0, 0);
+            }
+        }
     }
 }
 
-template <int TX, int TY, int TZ, bool P2>
+// CAP: particles of a plane staged per batch.
+template <int TX, int TY, int TZ, int CAP, bool P2>
 __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
-    constexpr int HX = TX + 2, HP = HX * (TY + 2);
-    // face sums: a ring of three face planes per component (slot k % 3)
-    constexpr int RU = 3 * (TX + 1) * TY, RV = 3 * TX * (TY + 1), RW = 3 * TX * TY;
-    __shared__ float swn[9 * HP], swd[9 * HP];   // 9: largest z-slice of a window
-    __shared__ float sun[RU], sud[RU], svn[RV], svd[RV], swfn[RW], swfd[RW];
+    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    static_assert(HP <= 256 && HY <= 64, "one thread per halo cell of a plane; halo rows in one wave");
+    static_assert((TX + 1) * TY <= 256 && TX * (TY + 1) <= 256, "one thread per owned face column");
+    __shared__ float stg[6 * CAP];
+    __shared__ float xch[2 * P2G_SLOTS * HP];
 
     const int W = a.W, H = a.H, D = a.D;
     const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = bid % ntx, ty = (bid / ntx) % nty, tz = bid / (ntx * nty);
     const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
-    for (int l = threadIdx.x; l < RU; l += blockDim.x) { sun[l] = 0.f; sud[l] = 0.f; }
-    for (int l = threadIdx.x; l < RV; l += blockDim.x) { svn[l] = 0.f; svd[l] = 0.f; }
-    for (int l = threadIdx.x; l < RW; l += blockDim.x) { swfn[l] = 0.f; swfd[l] = 0.f; }
-    // owned face planes: u, v k in [z0, hiz]; w k in [z0, hizw]
-    const int hiz = min(z0 + TZ, D) - 1, hizw = (z0 + TZ >= D) ? D : z0 + TZ - 1;
-    // particles in the halo box: one thread per halo row (cj, ck)
+    const P2GFaces<0, TX, TY, TZ> FU(a, x0, y0, z0);
+    const P2GFaces<1, TX, TY, TZ> FV(a, x0, y0, z0);
+    const P2GFaces<2, TX, TY, TZ> FW(a, x0, y0, z0);
+    const int lo = max(x0 - 1, 0), hi = min(x0 + TX, W - 1);
+    // particles in the halo box, over its halo rows (cj, ck)
     bool any = false;
-    {
-        constexpr int HY = TY + 2, HZ = TZ + 2;
-        static_assert(HY * HZ <= 256, "one thread per halo row");
-        const int t = threadIdx.x;
-        const int cj = y0 - 1 + t % HY, ck = z0 - 1 + t / HY;
-        if (t < HY * HZ && cj >= 0 && cj < H && ck >= 0 && ck < D) {
-            const int lo = max(x0 - 1, 0), hi = min(x0 + TX, W - 1);
+    for (int r = threadIdx.x; r < HY * (TZ + 2); r += blockDim.x) {
+        const int cj = y0 - 1 + r % HY, ck = z0 - 1 + r / HY;
+        if (cj >= 0 && cj < H && ck >= 0 && ck < D) {
             const long long row = (long long)cj * W + (long long)ck * W * H;
-            any = a.start[row + hi + 1] > a.start[row + lo];
+            any = any || a.start[row + hi + 1] > a.start[row + lo];
         }
     }
-    // (the barrier also orders the zero fill before the plane passes)
     if (!__syncthreads_or(any)) {
         // no particle reaches the tile: every owned face is empty
-        for (int k = z0; k <= hiz; ++k) {
-            p2g_write_plane<0, TX, TY, TZ>(a, x0, y0, k, sun, sud);
-            p2g_write_plane<1, TX, TY, TZ>(a, x0, y0, k, svn, svd);
+        for (int k = z0; k <= FU.hiz; ++k) {
+            if (FU.own) FU.store(a, k, 0.0f, 0.0f);
+            if (FV.own) FV.store(a, k, 0.0f, 0.0f);
         }
-        for (int k = z0; k <= hizw; ++k) p2g_write_plane<2, TX, TY, TZ>(a, x0, y0, k, swfn, swfd);
+        if (FW.own)
+            for (int k = z0; k <= FW.hiz; ++k) FW.store(a, k, 0.0f, 0.0f);
         return;
     }
-    // halo planes in z; the three components of a plane back to back, so that
-    // the second and third reads of its particles hit the caches.  After cell
-    // plane kc, u and v face plane kc - 1 and w face plane kc are complete.
     const int t = threadIdx.x;
     const int ci = x0 - 1 + t % HX, cj = y0 - 1 + t / HX;
     const bool cell_in = t < HP && ci >= 0 && ci < W && cj >= 0 && cj < H;
     const int kc0 = max(z0 - 1, 0), kc1 = min(z0 + TZ, D - 1);
-    for (int kc = kc0; kc <= kc1; ++kc) {
-        int beg = 0, end = 0;
+    const float dx = a.dx, inv = a.inv_dx;
+
+    // this cell's particles in plane kc as positions in the plane's list;
+    // returns the plane's particle count (uniform)
+    auto locate = [&](int kc, const P2GRows& R, int& excl, int& beg, int& end) -> int {
+        int inc = R.cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(inc, off, 64);
+            if ((t & 63) >= off) inc += y;
+        }
+        excl = inc - R.cnt;
+        const int sh = __shfl(excl - R.rb, t / HX, 64);
+        beg = end = 0;
         if (cell_in) {
             const long long c = (long long)ci + (long long)cj * W + (long long)kc * W * H;
-            beg = a.start[c];
-            end = a.start[c + 1];
+            beg = a.start[c] + sh;
+            end = a.start[c + 1] + sh;
         }
-        p2g_plane<0, TX, TY, TZ, P2>(a, x0, y0, z0, kc, beg, end, swn, swd, sun, sud);
-        p2g_plane<1, TX, TY, TZ, P2>(a, x0, y0, z0, kc, beg, end, swn, swd, svn, svd);
-        p2g_plane<2, TX, TY, TZ, P2>(a, x0, y0, z0, kc, beg, end, swn, swd, swfn, swfd);
-        if (kc - 1 >= z0 && kc - 1 <= hiz) {
-            p2g_write_plane<0, TX, TY, TZ>(a, x0, y0, kc - 1, sun, sud);
-            p2g_write_plane<1, TX, TY, TZ>(a, x0, y0, kc - 1, svn, svd);
+        return __builtin_amdgcn_readlane(inc, HY - 1);
+    };
+
+    // face sums of the owned columns: u, v planes kc - 1, kc, kc + 1; w planes
+    // kc, kc + 1 (relative to the current cell plane)
+    float un[3] = {0.f, 0.f, 0.f}, ud[3] = {0.f, 0.f, 0.f}, vn[3] = {0.f, 0.f, 0.f}, vd[3] = {0.f, 0.f, 0.f};
+    float wn[3] = {0.f, 0.f, 0.f}, wd[3] = {0.f, 0.f, 0.f};
+
+    // first plane: rows, ranges and its first batch; rows of the second
+    P2GRows R = p2g_rows_load<TX, TY>(a, x0, y0, kc0);
+    int excl, beg, end;
+    int total = locate(kc0, R, excl, beg, end);
+    p2g_stage<TY, CAP>(a, stg, 0, R, excl);
+    P2GRows Rn = kc0 < kc1 ? p2g_rows_load<TX, TY>(a, x0, y0, kc0 + 1) : P2GRows{0, 0};
+    for (int kc = kc0; kc <= kc1; ++kc) {
+        float nu[P2G_SLOTS], du[P2G_SLOTS], nv[P2G_SLOTS], dv[P2G_SLOTS], nw[P2G_SLOTS], dw[P2G_SLOTS];
+#pragma unroll
+        for (int q = 0; q < P2G_SLOTS; ++q) {
+            nu[q] = 0.f; du[q] = 0.f; nv[q] = 0.f; dv[q] = 0.f; nw[q] = 0.f; dw[q] = 0.f;
         }
-        if (kc >= z0 && kc <= hizw) p2g_write_plane<2, TX, TY, TZ>(a, x0, y0, kc, swfn, swfd);
-        // (the next plane's first barrier orders these clears before its sums)
+        for (int c0 = 0; c0 < total; c0 += CAP) {
+            if (c0 > 0) {  // a further batch of a crowded plane
+                lds_barrier();
+                p2g_stage<TY, CAP>(a, stg, c0, R, excl);
+            }
+            __syncthreads();  // this batch has landed
+            // A: this cell's particles of the batch, in order
+            const int pb = max(beg, c0), pe = min(end, c0 + CAP);
+            for (int p = pb; p < pe; ++p) {
+                const int e = p - c0;
+                const float rx = div_dx<P2>(stg[e], dx, inv), ry = div_dx<P2>(stg[CAP + e], dx, inv),
+                            rz = div_dx<P2>(stg[2 * CAP + e], dx, inv);
+                p2g_add<0>(a, ci, cj, kc, rx, ry, rz, stg[3 * CAP + e], nu, du);
+                p2g_add<1>(a, ci, cj, kc, rx, ry, rz, stg[4 * CAP + e], nv, dv);
+                p2g_add<2>(a, ci, cj, kc, rx, ry, rz, stg[5 * CAP + e], nw, dw);
+            }
+        }
+        lds_barrier();  // stg is free
+        if (kc < kc1) {  // the next plane's first batch flies during phase B
+            R = Rn;
+            total = locate(kc + 1, R, excl, beg, end);
+            p2g_stage<TY, CAP>(a, stg, 0, R, excl);
+            Rn = kc + 1 < kc1 ? p2g_rows_load<TX, TY>(a, x0, y0, kc + 2) : P2GRows{0, 0};
+        }
+        // B
+        p2g_gather(FU, x0, y0, nu, du, xch, un, ud);
+        p2g_gather(FV, x0, y0, nv, dv, xch, vn, vd);
+        p2g_gather(FW, x0, y0, nw, dw, xch, wn, wd);
+        // u, v face plane kc - 1 and w face plane kc are complete
+        if (FU.own && kc - 1 >= z0 && kc - 1 <= FU.hiz) FU.store(a, kc - 1, un[0], ud[0]);
+        if (FV.own && kc - 1 >= z0 && kc - 1 <= FV.hiz) FV.store(a, kc - 1, vn[0], vd[0]);
+        if (FW.own && kc >= z0 && kc <= FW.hiz) FW.store(a, kc, wn[0], wd[0]);
+        un[0] = un[1]; un[1] = un[2]; un[2] = 0.f;
+        ud[0] = ud[1]; ud[1] = ud[2]; ud[2] = 0.f;
+        vn[0] = vn[1]; vn[1] = vn[2]; vn[2] = 0.f;
+        vd[0] = vd[1]; vd[1] = vd[2]; vd[2] = 0.f;
+        wn[0] = wn[1]; wn[1] = 0.f;
+        wd[0] = wd[1]; wd[1] = 0.f;
     }
-    if (kc1 >= z0 && kc1 <= hiz) {
-        p2g_write_plane<0, TX, TY, TZ>(a, x0, y0, kc1, sun, sud);
-        p2g_write_plane<1, TX, TY, TZ>(a, x0, y0, kc1, svn, svd);
-    }
-    if (kc1 + 1 >= z0 && kc1 + 1 <= hizw) p2g_write_plane<2, TX, TY, TZ>(a, x0, y0, kc1 + 1, swfn, swfd);
+    if (FU.own && kc1 >= z0 && kc1 <= FU.hiz) FU.store(a, kc1, un[0], ud[0]);
+    if (FV.own && kc1 >= z0 && kc1 <= FV.hiz) FV.store(a, kc1, vn[0], vd[0]);
+    if (FW.own && kc1 + 1 >= z0 && kc1 + 1 <= FW.hiz) FW.store(a, kc1 + 1, wn[0], wd[0]);
 }
 
 // ---------------------------------------------------------------------------
@@ -960,12 +1057,12 @@ void launch_canonical(const ParticlesView& src, const ParticlesView& dst, int n,
 }
 
 void launch_p2g(const P2GArgs& a, hipStream_t st) {
-    constexpr int TX = P2G_TX, TY = P2G_TY, TZ = P2G_TZ;
+    constexpr int TX = P2G_TX, TY = P2G_TY, TZ = P2G_TZ, CAP = P2G_CAP;
     int nb = ((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * ((a.D + TZ - 1) / TZ);
     if (a.inv_dx > 0.0f)
-        hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ, true>), dim3(nb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ, CAP, true>), dim3(nb), dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ, false>), dim3(nb), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((k_p2g_tile<TX, TY, TZ, CAP, false>), dim3(nb), dim3(256), 0, st, a);
 }
 
 void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st) {
