@@ -26,22 +26,23 @@
 #endif
 
 // P2G LDS tile (cells); one 256-thread block per tile.  (TX + 2) x (TY + 2)
-// halo cells per z-plane, one per thread of the window phase; CAP particles
-// of a plane staged in LDS per batch (24 bytes each; with the window exchange
-// two blocks fit a CU).  Stage ms at 256^3, 8 ppc at rest: TZ 16 CAP 1920
-// 0.99, TZ 32 CAP 1960 0.99, TZ 8 CAP 1920 1.08, TZ 16 CAP 1280 1.31, TZ 24
-// 1.29 (ragged in z).
+// = 16 x 16 halo cells per z-plane, one per thread of the window phase, each
+// halo row one 16-lane DPP row; CAP particles of a plane staged in LDS per
+// batch (24 bytes each; with the window exchange two blocks fit a CU).  Stage
+// ms at 256^3, 8 ppc at rest: 14x14x16 CAP 2048 0.85, CAP 2560 0.85; with the
+// exchange of whole windows (no lane sums) 18x10x16 CAP 1920 0.99, 18x10x32
+// 0.99, 18x10x8 1.08, 18x10x16 CAP 1280 1.31.
 #ifndef P2G_TX
-#define P2G_TX 18
+#define P2G_TX 14
 #endif
 #ifndef P2G_TY
-#define P2G_TY 10
+#define P2G_TY 14
 #endif
 #ifndef P2G_TZ
 #define P2G_TZ 16
 #endif
 #ifndef P2G_CAP
-#define P2G_CAP 1920
+#define P2G_CAP 2048
 #endif
 
 namespace fsk {

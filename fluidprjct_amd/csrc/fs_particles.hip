@@ -283,10 +283,11 @@ __global__ __launch_bounds__(256) void k_seed(const uint8_t* __restrict__ labels
 //   A. one thread per halo cell sums the weights and weighted velocities of
 //      its particles, in their sorted order, into its three windows in
 //      registers (weights and products as AccumulateVelocities :468-594);
-//   B. per component, the windows go to LDS and one thread per owned face
-//      column adds the window slots that land on its faces (at most 3 x 3
-//      cells of the plane) into registers that hold the column's faces of
-//      the face planes the current plane reaches.
+//   B. per component, the windows are summed along x across neighbouring
+//      lanes (a halo row is one 16-lane DPP row), the sums go to LDS, and one
+//      thread per owned face column adds those of the 2 or 3 cells in y that
+//      land on its faces into registers that hold the column's faces of the
+//      face planes the current plane reaches.
 // A face plane is complete once the cell planes either side of it are done;
 // its faces are then normalised (ComputeVelocityFunc :598-605), saved
 // (saveVelocities) and given gravity (ApplyScalarForce :687-698) in one
@@ -372,51 +373,71 @@ struct P2GFaces {
     }
 };
 
-// Phase B of one component for the current cell plane: the windows of the
-// plane's halo cells go to LDS (xch: [num | den][slot][halo cell]); each owned
-// face column adds the slots that land on its face in slice z, face plane
+// Lane l gets v of lane l - 1 (SH = 1) or l + 1 (SH = -1) within its row of 16
+// lanes; 0 past the row's ends.
+template <int SH>
+FS_DEV float row_shift(float v) {
+    constexpr int ctrl = SH > 0 ? 0x110 + SH : 0x100 - SH;  // row_shr:SH / row_shl:-SH
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+// Phase B of one component for the current cell plane, in two halves with a
+// block barrier between them.  The halo rows are 16 cells wide, one DPP row of
+// lanes each, so p2g_xsum first sums the windows along x across neighbouring
+// lanes: the lane of cell (ci, cj) then holds, per window slot (y, z), the sum
+// for face column i = ci, and these NY x NZ sums go to LDS (x: [num | den]
+// [y z slot][halo cell]).  p2g_ysum then lets each owned face column add the
+// sums of the NY cells in y that land on its face in slice z, face plane
 // kc - (NZ == 3) + z, to rn[z] / rd[z].
 template <int COMP, int TX, int TY, int TZ>
-FS_DEV void p2g_gather(const P2GFaces<COMP, TX, TY, TZ>& F, int x0, int y0, const float (&num)[P2G_SLOTS],
-                       const float (&den)[P2G_SLOTS], float* __restrict__ xch, float (&rn)[3], float (&rd)[3]) {
+FS_DEV void p2g_xsum(const float (&num)[P2G_SLOTS], const float (&den)[P2G_SLOTS], float* __restrict__ x) {
     typedef P2GFaces<COMP, TX, TY, TZ> FT;
-    constexpr int NX = FT::NX, NY = FT::NY, NZ = FT::NZ, NXY = NX * NY;
+    constexpr int NX = FT::NX, NYZ = FT::NY * FT::NZ;
     constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    static_assert(HX == 16 && HP == 256, "halo rows are DPP rows");
     const int t = threadIdx.x;
-    if (t < HP) {
+    // u face i gets slot 0 of cell i and slot 1 of cell i - 1; v and w face i
+    // slot 0 of cell i + 1, slot 1 of cell i and slot 2 of cell i - 1
 #pragma unroll
-        for (int q = 0; q < P2G_SLOTS; ++q) {
-            xch[q * HP + t] = num[q];
-            xch[(P2G_SLOTS + q) * HP + t] = den[q];
+    for (int yz = 0; yz < NYZ; ++yz) {
+        float sn, sd;
+        if (NX == 2) {
+            sn = num[NX * yz] + row_shift<1>(num[NX * yz + 1]);
+            sd = den[NX * yz] + row_shift<1>(den[NX * yz + 1]);
+        } else {
+            sn = row_shift<-1>(num[NX * yz]) + num[NX * yz + 1] + row_shift<1>(num[NX * yz + 2]);
+            sd = row_shift<-1>(den[NX * yz]) + den[NX * yz + 1] + row_shift<1>(den[NX * yz + 2]);
+        }
+        x[yz * HP + t] = sn;
+        x[(NYZ + yz) * HP + t] = sd;
+    }
+}
+
+template <int COMP, int TX, int TY, int TZ>
+FS_DEV void p2g_ysum(const P2GFaces<COMP, TX, TY, TZ>& F, int x0, int y0, const float* __restrict__ x,
+                     float (&rn)[3], float (&rd)[3]) {
+    typedef P2GFaces<COMP, TX, TY, TZ> FT;
+    constexpr int NY = FT::NY, NZ = FT::NZ, NYZ = NY * NZ;
+    constexpr int HX = TX + 2, HY = TY + 2, HP = HX * HY;
+    if (!F.own) return;
+    // cells with window slot y = j - oy(cj), oy = cj - (NY == 3)
+    const int hx = F.i - (x0 - 1);
+    float sn[NZ], sd[NZ];
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) { sn[z] = 0.0f; sd[z] = 0.0f; }
+#pragma unroll
+    for (int y = 0; y < NY; ++y) {
+        const int hy = F.j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
+        if (hy < 0 || hy >= HY) continue;
+        const int cell = hx + HX * hy;
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            sn[z] += x[(y + NY * z) * HP + cell];
+            sd[z] += x[(NYZ + y + NY * z) * HP + cell];
         }
     }
-    lds_barrier();
-    if (F.own) {
-        float sn[NZ], sd[NZ];
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) { sn[z] = 0.0f; sd[z] = 0.0f; }
-        // cells with window slot x = i - ox(ci), ox = ci - (NX == 3); y alike
-#pragma unroll
-        for (int y = 0; y < NY; ++y) {
-            const int hy = F.j - y + (NY == 3 ? 1 : 0) - (y0 - 1);
-            if (hy < 0 || hy >= HY) continue;
-#pragma unroll
-            for (int x = 0; x < NX; ++x) {
-                const int hx = F.i - x + (NX == 3 ? 1 : 0) - (x0 - 1);
-                if (hx < 0 || hx >= HX) continue;
-                const int cell = hx + HX * hy;
-#pragma unroll
-                for (int z = 0; z < NZ; ++z) {
-                    const int q = x + NX * y + NXY * z;
-                    sn[z] += xch[q * HP + cell];
-                    sd[z] += xch[(P2G_SLOTS + q) * HP + cell];
-                }
-            }
-        }
-#pragma unroll
-        for (int z = 0; z < NZ; ++z) { rn[z] += sn[z]; rd[z] += sd[z]; }
-    }
-    lds_barrier();
+    for (int z = 0; z < NZ; ++z) { rn[z] += sn[z]; rd[z] += sd[z]; }
 }
 
 // Halo rows of one cell plane: lanes q < HY of every wave hold row q's first
@@ -472,7 +493,7 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
     static_assert(HP <= 256 && HY <= 64, "one thread per halo cell of a plane; halo rows in one wave");
     static_assert((TX + 1) * TY <= 256 && TX * (TY + 1) <= 256, "one thread per owned face column");
     __shared__ float stg[6 * CAP];
-    __shared__ float xch[2 * P2G_SLOTS * HP];
+    __shared__ float xch[2 * (9 + 6) * HP];
 
     const int W = a.W, H = a.H, D = a.D;
     const int ntx = (W + TX - 1) / TX, nty = (H + TY - 1) / TY;
@@ -569,10 +590,18 @@ __global__ __launch_bounds__(256) void k_p2g_tile(P2GArgs a) {
             p2g_stage<TY, CAP>(a, stg, 0, R, excl);
             Rn = kc + 1 < kc1 ? p2g_rows_load<TX, TY>(a, x0, y0, kc + 2) : P2GRows{0, 0};
         }
-        // B
-        p2g_gather(FU, x0, y0, nu, du, xch, un, ud);
-        p2g_gather(FV, x0, y0, nv, dv, xch, vn, vd);
-        p2g_gather(FW, x0, y0, nw, dw, xch, wn, wd);
+        // B: u and v, then w (xch holds 2 x (9 + 6) sums per halo cell); the
+        // barriers of the next plane's phase A order its exchange after these
+        // reads
+        p2g_xsum<0, TX, TY, TZ>(nu, du, xch);
+        p2g_xsum<1, TX, TY, TZ>(nv, dv, xch + 18 * HP);
+        lds_barrier();
+        p2g_ysum(FU, x0, y0, xch, un, ud);
+        p2g_ysum(FV, x0, y0, xch + 18 * HP, vn, vd);
+        lds_barrier();
+        p2g_xsum<2, TX, TY, TZ>(nw, dw, xch);
+        lds_barrier();
+        p2g_ysum(FW, x0, y0, xch, wn, wd);
         // u, v face plane kc - 1 and w face plane kc are complete
         if (FU.own && kc - 1 >= z0 && kc - 1 <= FU.hiz) FU.store(a, kc - 1, un[0], ud[0]);
         if (FV.own && kc - 1 >= z0 && kc - 1 <= FV.hiz) FV.store(a, kc - 1, vn[0], vd[0]);
