@@ -54,9 +54,9 @@ def build_scene(name):
     raise SystemExit(f"unknown config {name}")
 
 
-def make_solver(lib, N, lab, body, ppc, seed, tol=1e-6, flags=0):
+def make_solver(lib, N, lab, body, ppc, seed, tol=1e-6, flags=0, step_flags=0):
     s = lib.solver(dt=1.0 / 120, dx=1.0 / N, pic_weight=0.05, cfl=1.0, solver_tol=tol, solver_max_iter=400,
-                   particles_per_cell=ppc, seed=seed, solver_flags=flags)
+                   particles_per_cell=ppc, seed=seed, solver_flags=flags, step_flags=step_flags)
     s.set_labels(lab)
     if body is not None:
         s.set_body_sdf(body["sdf"], body["origin"], body["cell_size"], body["pos"], body["mass"])
@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the inner PCG iterations eagerly")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="FS_STEP_DETERMINISTIC: particles of a cell in id order (bit-reproducible runs)")
     ap.add_argument("--learned-steps", type=int, default=5)
     ap.add_argument("--learned-params", default="", help="parameters of the learned preconditioner (.npy)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -166,12 +168,13 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
         return
     from fluidprjct_amd import load_library
-    from fluidprjct_amd._abi import FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS
+    from fluidprjct_amd._abi import FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS, FS_STEP_DETERMINISTIC
     hip = load_library()
 
     N, lab, body, ppc, seed, desc = build_scene(args.config)
     log(f"{args.config}: scene {lab.shape}, seeding")
-    s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0)
+    s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0,
+                    step_flags=FS_STEP_DETERMINISTIC if args.deterministic else 0)
     Np = s.num_particles
     step_ms = []          # every step of the run, host wall clock (each fs_step returns synchronised)
     submerged = []
@@ -229,7 +232,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": args.config + ": " + desc, "grid": [W, H, D], "particles": Np,
+        "config": {"workload": args.config + ": " + desc + (", deterministic cell order" if args.deterministic else ""),
+                   "grid": [W, H, D], "particles": Np,
                    "fluid_cells": nf, "precision": "FP32 transfers/stencil, FP64 dots + refinement"},
         "pressure_solve_ms_plain": ms["pressure"],
         "pressure_iters_last": st.iterations,
