@@ -256,6 +256,88 @@ def test_p2g_single_particle_weights(oracle):
     assert np.array_equal(s.get_grid(FS_GRID_U), us)
 
 
+def test_p2g_two_particles_hat_weights(oracle):
+    """Two particles of different velocities: every face value is the
+    weighted mean sum(w v) / sum(w) with the trilinear hat weights
+    w = prod over axes of max(0, 1 - |x_p - x_f| / dx), x_f the face centre
+    (u at (i, j + 1/2, k + 1/2) dx, v and w alike), computed here from the
+    positions alone (particlesToGrid FluidSolver3D.cu:607-685)."""
+    dx = 0.25
+    pos = np.array([[1.3, 2.6, 1.9], [1.75, 2.2, 1.45]], np.float32) * np.float32(dx)
+    vel = np.array([[1.0, 2.0, 3.0], [-2.0, 0.5, -1.0]], np.float32)
+    s = oracle.solver(dx=dx, dt=0.01)
+    s.set_labels(np.full((4, 4, 4), 2, np.uint8))
+    s.set_particles(pos, vel)
+    s.stage("label")
+    s.stage("p2g")
+    p = pos.astype(float) / dx
+    for comp, g in enumerate((FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED)):
+        got = s.get_grid(g)
+        D_, H_, W_ = got.shape
+        k, j, i = np.meshgrid(np.arange(D_), np.arange(H_), np.arange(W_), indexing="ij")
+        fc = [i + (0.0 if comp == 0 else 0.5), j + (0.0 if comp == 1 else 0.5), k + (0.0 if comp == 2 else 0.5)]
+        num = np.zeros(got.shape)
+        den = np.zeros(got.shape)
+        for q in range(2):
+            w = np.ones(got.shape)
+            for ax in range(3):
+                w *= np.maximum(0.0, 1.0 - np.abs(p[q, ax] - fc[ax]))
+            num += w * vel[q, comp]
+            den += w
+        want = np.where(den > 1e-8, num / np.maximum(den, 1e-300), 0.0)
+        assert np.abs(got - want).max() <= 1e-6 * np.abs(vel).max(), comp
+        assert np.count_nonzero(den) > 8   # the windows overlap
+
+
+def test_rhs_of_a_linear_field(oracle):
+    """constructRHS (FluidSolver3D.cu:1360-1474): b = -(rho dx / dt) div u per
+    FLUID cell, the divergence taken with the faces a SOLID cell shares at
+    velocity 0 and those a BODY cell shares at the body velocity.  Face
+    velocities linear in the face index (u = a i, v = b j, w = c k), one SOLID
+    cell and one moving BODY cell inside the fluid."""
+    dx, dt, rho = 0.125, 0.01, 1000.0
+    a, b, c = 0.3, -0.7, 0.45
+    V = np.array([0.8, -0.25, 0.6])
+    s = one_face_body(oracle, list(_DIRS), (0, 0, 0), V, (2.5 * dx, 2.5 * dx, 2.5 * dx), dx=dx, dt=dt, N=8)
+    lab = np.full((8, 8, 8), 2, np.uint8)
+    lab[1:-1, 1:-1, 1:-1] = 1
+    lab[2, 2, 2] = 3          # the body's cell (i, j, k) = (2, 2, 2)
+    lab[4, 5, 4] = 0          # a SOLID cell at (4, 5, 4)
+    s.set_grid(FS_GRID_LABELS, lab)
+    for g, f in ((FS_GRID_U, lambda k, j, i: a * i), (FS_GRID_V, lambda k, j, i: b * j),
+                 (FS_GRID_W, lambda k, j, i: c * k)):
+        shape, _ = s.grid_shape(g)
+        s.set_grid(g, np.fromfunction(f, shape).astype(np.float32))
+    s.stage("rhs")
+    rhs = s.get_grid(FS_GRID_RHS)
+    scale = rho * dx / dt
+
+    def face(comp, i, j, k, side):
+        # velocity of the face of cell (i, j, k) on side -1 / +1 of axis comp
+        n = [i, j, k]
+        n[comp] += side
+        L = lab[n[2], n[1], n[0]]
+        if L == 0:
+            return 0.0
+        if L == 3:
+            return V[comp]
+        idx = [i, j, k]
+        idx[comp] += (1 if side > 0 else 0)
+        return (a, b, c)[comp] * idx[comp]
+
+    checked = 0
+    for k in range(8):
+        for j in range(8):
+            for i in range(8):
+                if lab[k, j, i] != 1:
+                    assert rhs[k, j, i] == 0.0
+                    continue
+                div = sum(face(q, i, j, k, 1) - face(q, i, j, k, -1) for q in range(3))
+                assert rhs[k, j, i] == pytest.approx(-scale * div, rel=2e-5, abs=1e-4 * scale), (i, j, k)
+                checked += 1
+    assert checked > 200
+
+
 def test_p2g_uniform_velocity_is_reproduced(oracle):
     rng = np.random.default_rng(3)
     N, dx = 10, 0.1
@@ -582,6 +664,37 @@ def test_integrator_torque_in_body_frame(oracle):
     want = R.T @ tau * dt / inertia
     assert np.abs(om - want).max() <= 2e-5 * np.linalg.norm(want)
     assert np.abs(om - R @ tau * dt / inertia).max() > 1e-2 * np.linalg.norm(want)
+
+
+def test_euler_angle_rates(oracle):
+    """The Euler-angle update of RigidBody::integrate (utility.cuh:455-460),
+    restated from the reference's source: with no torque (a body in air),
+    one step adds dt * (wy cos(phi) - wz sin(phi)) to theta,
+    dt * (wy sin(phi) + wz cos(phi)) / cos(theta) to psi and
+    dt * (wx + dpsi sin(theta)) to phi, from the angles before the step.
+    These rates are not the body-frame rates of the reference's own
+    R = Ry(theta) Rz(psi) Rx(phi) (R^T dR/dt != [omega]x); the build keeps the
+    reference's equations, and the second check records the difference."""
+    dt = 0.01
+    s = one_face_body(oracle, [], (0, 0, 0), (0, 0, 0), (0.3, 0.3, 0.3), dt=dt)
+    st = s.body_state()
+    st.theta, st.psi, st.phi = 0.4, -0.3, 0.7
+    st.omega[:] = (0.5, -1.2, 0.8)
+    s.set_body_state(st)
+    s.stage("body")
+    st2 = s.body_state()
+    wx, wy, wz = (float(v) for v in np.float32([0.5, -1.2, 0.8]))
+    th, ps, ph = (float(v) for v in np.float32([0.4, -0.3, 0.7]))
+    assert tuple(st2.omega) == pytest.approx((wx, wy, wz), abs=0)
+    dth = wy * np.cos(ph) - wz * np.sin(ph)
+    dps = (wy * np.sin(ph) + wz * np.cos(ph)) / np.cos(th)
+    dph = wx + dps * np.sin(th)
+    got = np.array([st2.theta - th, st2.psi - ps, st2.phi - ph]) / dt
+    assert np.abs(got - np.array([dth, dps, dph])).max() <= 2e-4
+    # the body-frame rates of R = Ry Rz Rx would differ
+    body_dth = (wy * np.cos(ph) - wz * np.sin(ph)) / np.cos(ps)
+    body_dps = wy * np.sin(ph) + wz * np.cos(ph)
+    assert abs(body_dps - dps) > 5e-3 and abs(body_dth - dth) > 5e-3
 
 
 def test_isolated_fluid_cell_is_no_unknown(oracle):
