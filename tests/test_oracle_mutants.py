@@ -92,6 +92,18 @@ MUTANTS = [
      "f3 Farch = (-rho * subf) * G;",
      "f3 Farch = (rho * subf) * G;",
      [lambda lib: pins.test_body_sphere_labels_and_buoyancy(lib)]),
+    ("RK3 second stage at the wrong fraction",
+     "f3 k2 = interp_vel(F, pos + k1 * 0.5f * dT);",
+     "f3 k2 = interp_vel(F, pos + k1 * 0.75f * dT);",
+     [lambda lib: pins.test_advection_rk3_in_a_rotation_field(lib)]),
+    ("RK3 weights swapped",
+     "pos = pos + (2.0f / 9.0f) * dT * k1 + (3.0f / 9.0f) * dT * k2 + (4.0f / 9.0f) * dT * k3;",
+     "pos = pos + (2.0f / 9.0f) * dT * k1 + (4.0f / 9.0f) * dT * k2 + (3.0f / 9.0f) * dT * k3;",
+     [lambda lib: pins.test_advection_rk3_in_a_rotation_field(lib)]),
+    ("seeding octant offset",
+     "const float off = 0.25f * dx;",
+     "const float off = 0.3f * dx;",
+     [lambda lib: pins.test_seeding_matches_reference_init_dump(lib)]),
     ("FLIP blend weights swapped",
      "float nu = alpha * pic.x + (1.0f - alpha) * (s.vel[3 * q] + flip.x);",
      "float nu = (1.0f - alpha) * pic.x + alpha * (s.vel[3 * q] + flip.x);",
