@@ -297,6 +297,30 @@ def test_tail_matches_level_launches(hip, N, block):
     assert np.array_equal(pt.view(np.uint32), pn.view(np.uint32))
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 1), (40, 40, 2)])
+def test_tail_matches_level_launches_thin_slab(hip, shape):
+    """A solve box of at most 4096 cells runs its whole V-cycle, the fine level
+    included, in the LDS tail (r.z from a separate pass): the same iterations
+    as the per-level launches and the solution within the rounding of the
+    differently ordered r.z partial sums."""
+    from fluidprjct_amd import scenes
+    from fluidprjct_amd._abi import FS_SOLVER_NO_TAIL
+    W, H, D = shape
+    lab = np.full((D, H, W), 2, np.uint8)
+    lab[:, 0, :] = 0
+    lab[:, :, 0] = 0
+    lab[:, :, -1] = 0
+    lab[:, 1:H // 2, 1:W // 2] = 1
+    assert (lab == 1).sum() * 1.0 <= 4096
+    rng = np.random.default_rng(W)
+    b = (rng.standard_normal(lab.shape) * 1e3).astype(np.float32) * (lab == 1)
+    pt, st = hip.solver().solve_pressure_system(lab, b, 1e-7)
+    pn, sn = hip.solver(solver_flags=FS_SOLVER_NO_TAIL).solve_pressure_system(lab, b, 1e-7)
+    assert st.converged == 1 and sn.converged == 1
+    assert st.iterations == sn.iterations
+    assert np.abs(pt - pn).max() <= 1e-9 * np.abs(pn).max()
+
+
 def test_c1_slab_steps_close(hip, oracle):
     """The 2D dam break (128^2, 4 ppc) as a one-cell-deep slab: the HIP step
     and the oracle stay within the whole-step bars for 5 steps, and the slab
