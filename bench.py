@@ -143,6 +143,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--deterministic", action="store_true",
                     help="FS_STEP_DETERMINISTIC: particles of a cell in id order (bit-reproducible runs)")
+    ap.add_argument("--face-fractions", action="store_true",
+                    help="FS_STEP_FACE_FRACTIONS: the body couples through the open fractions of the faces "
+                         "its SDF cuts (the reference voxelises it at cell centres)")
     ap.add_argument("--learned-steps", type=int, default=5)
     ap.add_argument("--learned-params", default="", help="parameters of the learned preconditioner (.npy)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -168,20 +171,25 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
         return
     from fluidprjct_amd import load_library
-    from fluidprjct_amd._abi import FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS, FS_STEP_DETERMINISTIC
+    from fluidprjct_amd._abi import (FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS, FS_STEP_DETERMINISTIC,
+                                     FS_STEP_FACE_FRACTIONS)
     hip = load_library()
 
     N, lab, body, ppc, seed, desc = build_scene(args.config)
     log(f"{args.config}: scene {lab.shape}, seeding")
-    s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0,
-                    step_flags=FS_STEP_DETERMINISTIC if args.deterministic else 0)
+    step_flags = ((FS_STEP_DETERMINISTIC if args.deterministic else 0) |
+                  (FS_STEP_FACE_FRACTIONS if args.face_fractions and body is not None else 0))
+    s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0, step_flags=step_flags)
     Np = s.num_particles
     step_ms = []          # every step of the run, host wall clock (each fs_step returns synchronised)
     submerged = []
     t_run0 = time.perf_counter()
 
+    last_pre_body = [None]   # body state at the start of the last run_steps step (its pressure solve's)
+
     def run_steps(n):
         for _ in range(n):
+            last_pre_body[0] = s.body_state() if body is not None else None
             t = time.perf_counter()
             s.step(1)
             step_ms.append(1e3 * (time.perf_counter() - t))
@@ -232,7 +240,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": args.config + ": " + desc + (", deterministic cell order" if args.deterministic else ""),
+        "config": {"workload": args.config + ": " + desc + (", deterministic cell order" if args.deterministic else "")
+                   + (", SDF face fractions" if step_flags & FS_STEP_FACE_FRACTIONS else ""),
                    "grid": [W, H, D], "particles": Np,
                    "fluid_cells": nf, "precision": "FP32 transfers/stencil, FP64 dots + refinement"},
         "pressure_solve_ms_plain": ms["pressure"],
@@ -273,11 +282,15 @@ def main():
         full["pressure_iters_last"] = st2.iterations
         full["pressure_rel_residual_last"] = st2.rel_residual
         result["full_run"] = full
-    # the system of the last step, kept for the FP64 check below
+    # the system of the last step, kept for the FP64 check below (with the
+    # body state its pressure solve saw: the step moves the body afterwards)
+    if args.full_steps <= args.warmup + args.steps:
+        run_steps(1)
     sys_lab = s.get_grid(FS_GRID_LABELS)
     sys_b = s.get_grid(FS_GRID_RHS)
     sys_p = s.get_grid(FS_GRID_P64)
     sys_dt = s.clock()[1]
+    sys_body = last_pre_body[0]
     u_norm = projected_velocity_norm(s, sys_lab)
     # learned vs plain: the same workload continued with the learned
     # preconditioner (flexible PCG, multigrid fallback after 5 stalled iterations)
@@ -314,8 +327,19 @@ def main():
         threads = int(orc.lib.oracle_num_threads()) if hasattr(orc.lib, "oracle_num_threads") else 1
         log(f"FP64 reference solve of the last step's system (oracle, {threads} threads)")
         t0 = time.perf_counter()
-        o = orc.solver()
-        pref, sref = o.solve_pressure_system(sys_lab, sys_b, 1e-12)
+        if step_flags & FS_STEP_FACE_FRACTIONS:
+            # the fraction-weighted system of the body's final state
+            o = orc.solver(dx=1.0 / N, solver_tol=1e-12, solver_max_iter=100000, step_flags=step_flags)
+            o.set_labels(lab)
+            o.set_body_sdf(body["sdf"], body["origin"], body["cell_size"], body["pos"], body["mass"])
+            o.set_body_state(sys_body)
+            o.set_grid(FS_GRID_LABELS, sys_lab)
+            o.set_grid(FS_GRID_RHS, sys_b)
+            o.stage("pressure")
+            pref, sref = o.get_grid(FS_GRID_P64), o.solve_stats()
+        else:
+            o = orc.solver()
+            pref, sref = o.solve_pressure_system(sys_lab, sys_b, 1e-12)
         tref = time.perf_counter() - t0
         fl = sys_lab == 1
         dp = np.where(fl, sys_p - pref, 0.0)
@@ -330,7 +354,7 @@ def main():
         }
         del o
         log(f"CPU baseline: {args.cpu_steps} oracle step(s)")
-        o = make_solver(orc, N, lab, body, ppc, seed)
+        o = make_solver(orc, N, lab, body, ppc, seed, step_flags=step_flags)
         t0 = time.perf_counter()
         for i in range(args.cpu_steps):
             o.step(1)

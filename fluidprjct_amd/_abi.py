@@ -19,9 +19,10 @@ FS_SOLVER_NO_GRAPH = 1
 FS_SOLVER_NO_TAIL = 2
 FS_SOLVER_NO_FUSE = 4
 FS_SOLVER_COLD_START = 8
-FS_STEP_DETERMINISTIC, FS_STEP_BODY_WALLS, FS_STEP_BODY_INERTIA = 1, 2, 4
+FS_STEP_DETERMINISTIC, FS_STEP_BODY_WALLS, FS_STEP_BODY_INERTIA, FS_STEP_FACE_FRACTIONS = 1, 2, 4, 8
 (FS_GRID_U, FS_GRID_V, FS_GRID_W, FS_GRID_USAVED, FS_GRID_VSAVED, FS_GRID_WSAVED, FS_GRID_P,
- FS_GRID_LABELS, FS_GRID_RHS, FS_GRID_P64, FS_GRID_CELLKEY, FS_GRID_SOLID) = range(12)
+ FS_GRID_LABELS, FS_GRID_RHS, FS_GRID_P64, FS_GRID_CELLKEY, FS_GRID_SOLID,
+ FS_GRID_OPEN_U, FS_GRID_OPEN_V, FS_GRID_OPEN_W) = range(15)
 STAGES = ("label", "p2g", "pressure", "project", "body", "g2p_advect", "sort")
 
 
@@ -291,11 +292,11 @@ class Solver:
     # grids ----------------------------------------------------------------
     def grid_shape(self, which):
         W, H, D = self.dims
-        if which in (FS_GRID_U, FS_GRID_USAVED):
+        if which in (FS_GRID_U, FS_GRID_USAVED, FS_GRID_OPEN_U):
             return (D, H, W + 1), np.float32
-        if which in (FS_GRID_V, FS_GRID_VSAVED):
+        if which in (FS_GRID_V, FS_GRID_VSAVED, FS_GRID_OPEN_V):
             return (D, H + 1, W), np.float32
-        if which in (FS_GRID_W, FS_GRID_WSAVED):
+        if which in (FS_GRID_W, FS_GRID_WSAVED, FS_GRID_OPEN_W):
             return (D + 1, H, W), np.float32
         if which in (FS_GRID_P, FS_GRID_RHS):
             return (D, H, W), np.float32

@@ -29,6 +29,7 @@
 
 using fsd::f3;
 using fsd::mk;
+using fsd::FaceFrac;
 using namespace fsk;
 
 namespace {
@@ -335,6 +336,11 @@ struct Solver {
     bool p64_from_step = false; // d_p64 holds the last step's pressure (the next solve's initial guess)
     unsigned* d_umax = nullptr; // max |v| of the particle velocities (float bits)
     bool umax_valid = false;
+    // face fractions: box tables of the current body pose (FaceFrac::th)
+    float* d_theta[3] = {nullptr, nullptr, nullptr};
+    long long theta_cap[3] = {0, 0, 0};
+    bool theta_valid = false;   // cleared whenever the SDF is replaced
+    FaceFrac theta_of{};        // the pose, box and SDF the tables were computed for
 
     long long NU() const { return (long long)(W + 1) * H * D; }
     long long NV() const { return (long long)W * (H + 1) * D; }
@@ -346,6 +352,7 @@ struct Solver {
         dfree(d_sdf);
         dfree(d_bacc);
         dfree(d_umax);
+        for (float*& t : d_theta) dfree(t);
         if (h_bacc) hipHostFree(h_bacc);
         if (body_done) hipEventDestroy(body_done);
         for (auto& e : ev) if (e) hipEventDestroy(e);
@@ -456,6 +463,50 @@ struct Solver {
         return la;
     }
 
+    // The body's face fractions (FS_STEP_FACE_FRACTIONS with a body loaded),
+    // with the box table of the current pose (computed when the pose, the box
+    // or the SDF changed); inactive otherwise: the reference's operator.
+    int face_frac(FaceFrac& f) {
+        f = FaceFrac{};
+        f.body = body.params();
+        if (!((cfg.step_flags & FS_STEP_FACE_FRACTIONS) && body.active && d_sdf)) return FS_OK;
+        f.active = 1;
+        f.dx = cfg.dx;
+        f.sdf = d_sdf;
+        const double far = std::sqrt((double)f.body.brad2) + (double)cfg.dx;
+        f.far2 = (float)(far * far * 1.0001);
+        // every face whose centre lies within far of bcen, with a cell of margin
+        const double c[3] = {f.body.bcen.x, f.body.bcen.y, f.body.bcen.z};
+        const int dims[3] = {W, H, D};
+        for (int q = 0; q < 3; ++q) {
+            const double a = std::floor((c[q] - far) / cfg.dx) - 1.0, b = std::ceil((c[q] + far) / cfg.dx) + 1.0;
+            const int lo = (int)std::max(0.0, std::min(a, (double)dims[q]));
+            const int hi = (int)std::max((double)lo, std::min(b, (double)dims[q]));
+            f.lo[q] = lo;
+            f.n[q] = hi - lo;
+        }
+        for (int comp = 0; comp < 3; ++comp) {
+            const long long need =
+                std::max(1LL, (long long)(f.n[0] + (comp == 0)) * (f.n[1] + (comp == 1)) * (f.n[2] + (comp == 2)));
+            if (need > theta_cap[comp]) {
+                dfree(d_theta[comp]);
+                HIPCHK(hipMalloc(&d_theta[comp], sizeof(float) * need));
+                theta_cap[comp] = need;
+                theta_valid = false;
+            }
+            f.th[comp] = d_theta[comp];
+        }
+        const bool same = theta_valid && std::memcmp(&theta_of.body, &f.body, sizeof(f.body)) == 0 &&
+                          theta_of.dx == f.dx && theta_of.far2 == f.far2 && theta_of.sdf == f.sdf &&
+                          std::equal(f.lo, f.lo + 3, theta_of.lo) && std::equal(f.n, f.n + 3, theta_of.n);
+        if (!same) {
+            launch_face_theta(f, d_theta, st);
+            theta_of = f;
+            theta_valid = true;
+        }
+        return FS_OK;
+    }
+
     GridArgs grid_args() {
         GridArgs g{};
         g.W = W; g.H = H; g.D = D;
@@ -523,7 +574,9 @@ struct Solver {
     }
 
     int stage_rhs() {
-        launch_rhs(grid_args(), d_rhs, st);
+        FaceFrac ff;
+        if (int rc = face_frac(ff)) return rc;
+        launch_rhs(grid_args(), ff, d_rhs, st);
         return FS_OK;
     }
 
@@ -537,8 +590,10 @@ struct Solver {
         return net.get();
     }
 
-    // warm: start from the previous step's pressure (d_p64) when it is one
-    int stage_pressure(bool warm = false) {
+    // warm: start from the previous step's pressure (d_p64) when it is one;
+    // faces: the operator of the step (face fractions when the option is on),
+    // else the reference's matrix of the labels
+    int stage_pressure(bool warm = false, bool faces = true) {
         Preconditioner* pre = nullptr;
         if (cfg.precond == FS_PRECOND_JACOBI) pre = jacobi.get();
         if (cfg.precond == FS_PRECOND_LEARNED) {
@@ -555,8 +610,13 @@ struct Solver {
         mg.use_tail = !(cfg.solver_flags & FS_SOLVER_NO_TAIL);
         mg.use_tb = !(cfg.solver_flags & FS_SOLVER_NO_FUSE);
         const bool use_x0 = warm && p64_from_step && !(cfg.solver_flags & FS_SOLVER_COLD_START);
+        mg.face = FaceFrac{};
+        if (faces) {
+            if (int rc0 = face_frac(mg.face)) return rc0;
+        }
         int rc = mg.solve(d_labels, d_rhs, d_p64, d_p, cfg.solver_tol, cfg.solver_max_iter, pre, st, &stats,
                           use_x0 ? d_p64 : nullptr);
+        mg.face.active = 0;
         p64_from_step = warm && rc != -5;
         if (rc == -5) return fail(FS_ERR_DEVICE, std::string("pressure solve: ") + hipGetErrorString(hipGetLastError()));
         if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");
@@ -565,7 +625,9 @@ struct Solver {
     }
 
     int stage_project() {
-        launch_project(grid_args(), d_p, st);
+        FaceFrac ff;
+        if (int rc = face_frac(ff)) return rc;
+        launch_project(grid_args(), ff, d_p, st);
         return FS_OK;
     }
 
@@ -662,7 +724,7 @@ struct Solver {
         HIPCHK(hipEventRecord(ev[1], st));
         if ((rc = stage_p2g())) return rc;
         HIPCHK(hipEventRecord(ev[2], st));
-        stage_rhs();
+        if ((rc = stage_rhs())) return rc;
         rc = stage_pressure(true);
         // a solve that misses the tolerance still completes the step (the
         // reference ignores the solver status); fs_step reports it afterwards
@@ -673,7 +735,7 @@ struct Solver {
             return rc;
         }
         HIPCHK(hipEventRecord(ev[3], st));
-        stage_project();
+        if ((rc = stage_project())) return rc;
         HIPCHK(hipEventRecord(ev[4], st));
         if ((rc = stage_body_launch())) return rc;
         HIPCHK(hipEventRecord(ev[5], st));
@@ -984,6 +1046,7 @@ int fs_set_body_sdf(void* h, const float* sdf, int w, int hh, int d, float ox, f
         b.update_rotation();
         b.inertia_tensor();
         dfree(s.d_sdf);
+        s.theta_valid = false;
         HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * b.sdf.size()));
         HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
         if (s.C > 0) {
@@ -1220,6 +1283,26 @@ int fs_get_grid(void* h, int which, void* out, int64_t count) {
             case FS_GRID_LABELS: src = s.d_labels; n = s.C; esz = 1; break;
             case FS_GRID_SOLID: src = s.d_solid; n = s.C; esz = 1; break;
             case FS_GRID_P64: src = s.d_p64; n = s.C; esz = 8; break;
+            case FS_GRID_OPEN_U:
+            case FS_GRID_OPEN_V:
+            case FS_GRID_OPEN_W: {
+                const int comp = which - FS_GRID_OPEN_U;
+                n = comp == 0 ? s.NU() : comp == 1 ? s.NV() : s.NW();
+                if (count < n) return fail(FS_ERR_ARG, "buffer too small");
+                float* tmp = nullptr;
+                HIPCHK(hipMalloc(&tmp, sizeof(float) * n));
+                FaceFrac ff;
+                if (int rc = s.face_frac(ff)) {
+                    hipFree(tmp);
+                    return rc;
+                }
+                launch_face_open(s.grid_args(), ff, comp, tmp, s.st);
+                const hipError_t e1 = hipStreamSynchronize(s.st);
+                const hipError_t e2 = e1 == hipSuccess ? hipMemcpy(out, tmp, sizeof(float) * n, hipMemcpyDeviceToHost) : e1;
+                hipFree(tmp);
+                if (e2 != hipSuccess) return fail(FS_ERR_DEVICE, hipGetErrorString(e2));
+                return FS_OK;
+            }
             case FS_GRID_CELLKEY: {
                 n = s.np;
                 if (count < n) return fail(FS_ERR_ARG, "buffer too small");
@@ -1313,7 +1396,7 @@ int fs_solve_pressure_system(void* h, const uint8_t* labels, const float* b, dou
         HIPCHK(hipMemcpy(s.d_rhs, b, sizeof(float) * (size_t)W * H * D, hipMemcpyHostToDevice));
         double save = s.cfg.solver_tol;
         s.cfg.solver_tol = tol;
-        int rc = s.stage_pressure();
+        int rc = s.stage_pressure(false, false);
         s.cfg.solver_tol = save;
         HIPCHK(hipStreamSynchronize(s.st));
         HIPCHK(hipMemcpy(p, s.d_p64, sizeof(double) * (size_t)W * H * D, hipMemcpyDeviceToHost));
@@ -1685,6 +1768,7 @@ int fs_load_checkpoint(void* h, const char* path) {
         if (np) HIPCHK(hipMemcpy(s.PA.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
         s.body = std::move(body);
         dfree(s.d_sdf);
+        s.theta_valid = false;
         if (s.body.active) {
             HIPCHK(hipMalloc(&s.d_sdf, sizeof(float) * s.body.sdf.size()));
             HIPCHK(hipMemcpy(s.d_sdf, s.body.sdf.data(), sizeof(float) * s.body.sdf.size(), hipMemcpyHostToDevice));

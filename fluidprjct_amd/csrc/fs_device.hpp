@@ -140,6 +140,122 @@ FS_DEV bool body_contains(const BodyParams& b, const float* __restrict__ sdf, f3
     return sdf[(size_t)i + (size_t)j * b.sw + (size_t)k * b.sw * b.sh] <= 0.0f;
 }
 
+// ---- face fractions of the rigid body (an option with no reference
+// counterpart: the reference voxelises the body at cell centres)
+//
+// Signed distance of the body at a world point: the SDF samples (at the
+// points file_origin + cs (i, j, k) of the body frame) interpolated
+// trilinearly; FACE_FAR (outside) beyond the sampled box.
+constexpr float FACE_FAR = 1.0e30f;
+FS_DEV float body_phi(const BodyParams& b, const float* __restrict__ sdf, f3 wp) {
+    f3 pr = wp - b.pos;
+    f3 plc;
+    plc.x = b.R[0] * pr.x + b.R[3] * pr.y + b.R[6] * pr.z;
+    plc.y = b.R[1] * pr.x + b.R[4] * pr.y + b.R[7] * pr.z;
+    plc.z = b.R[2] * pr.x + b.R[5] * pr.y + b.R[8] * pr.z;
+    f3 pl = plc + b.local_com;
+    const float sx = (pl.x - b.file_origin.x) / b.cs, sy = (pl.y - b.file_origin.y) / b.cs,
+                sz = (pl.z - b.file_origin.z) / b.cs;
+    if (!(sx >= 0.0f && sx <= (float)(b.sw - 1) && sy >= 0.0f && sy <= (float)(b.sh - 1) && sz >= 0.0f &&
+          sz <= (float)(b.sd - 1)))
+        return FACE_FAR;
+    const int i = min((int)sx, b.sw - 2 < 0 ? 0 : b.sw - 2), j = min((int)sy, b.sh - 2 < 0 ? 0 : b.sh - 2),
+              k = min((int)sz, b.sd - 2 < 0 ? 0 : b.sd - 2);
+    const float fx = sx - (float)i, fy = sy - (float)j, fz = sz - (float)k;
+    const size_t sw = (size_t)b.sw, swh = (size_t)b.sw * b.sh;
+    const size_t o = (size_t)i + (size_t)j * sw + (size_t)k * swh;
+    const size_t ox = b.sw > 1 ? 1 : 0, oy = b.sh > 1 ? sw : 0, oz = b.sd > 1 ? swh : 0;
+    const float c00 = sdf[o] * (1.0f - fx) + sdf[o + ox] * fx;
+    const float c10 = sdf[o + oy] * (1.0f - fx) + sdf[o + oy + ox] * fx;
+    const float c01 = sdf[o + oz] * (1.0f - fx) + sdf[o + oz + ox] * fx;
+    const float c11 = sdf[o + oz + oy] * (1.0f - fx) + sdf[o + oz + oy + ox] * fx;
+    const float c0 = c00 * (1.0f - fy) + c10 * fy, c1 = c01 * (1.0f - fy) + c11 * fy;
+    return c0 * (1.0f - fz) + c1 * fz;
+}
+
+// Fraction of a triangle inside the body (phi < 0) for the signed distances
+// a, b, c at its corners, phi linear over the triangle.
+FS_HD float tri_inside(float a, float b, float c) {
+    const int n = (a < 0.0f) + (b < 0.0f) + (c < 0.0f);
+    if (n == 0) return 0.0f;
+    if (n == 3) return 1.0f;
+    // order so that the corner that differs is first
+    if (n == 1) {
+        const float p = a < 0.0f ? a : (b < 0.0f ? b : c);
+        const float q = a < 0.0f ? b : (b < 0.0f ? c : a);
+        const float r = a < 0.0f ? c : (b < 0.0f ? a : b);
+        return (p / (p - q)) * (p / (p - r));
+    }
+    const float p = a >= 0.0f ? a : (b >= 0.0f ? b : c);
+    const float q = a >= 0.0f ? b : (b >= 0.0f ? c : a);
+    const float r = a >= 0.0f ? c : (b >= 0.0f ? a : b);
+    return 1.0f - (p / (p - q)) * (p / (p - r));
+}
+
+// Open (fluid) fraction of a square face from the distances at its corners
+// (p00, p10, p01, p11 in the face's two axes): the square as two triangles
+// (00, 10, 11) and (00, 11, 01), phi linear on each; below FACE_MIN_OPEN the
+// face counts as closed.
+constexpr float FACE_MIN_OPEN = 1.0e-3f;
+FS_HD float face_open(float p00, float p10, float p01, float p11) {
+    const float inside = 0.5f * (tri_inside(p00, p10, p11) + tri_inside(p00, p11, p01));
+    const float open = 1.0f - inside;
+    return open < FACE_MIN_OPEN ? 0.0f : open;
+}
+
+// What the face-fraction kernels need of the body.  active = 0 makes every
+// face between two FLUID/AIR cells fully open (the reference's operator).
+// far2: faces whose centre lies farther than sqrt(far2) from body.bcen have
+// all four corners outside the SDF box (body_sphere_has's radius plus one
+// cell), so they are open without a lookup.  th[c]: the open fractions of
+// the faces of component c in the box of cells [lo, lo + n) (the faces of
+// those cells, (n0 + (c == 0)) x (n1 + (c == 1)) x (n2 + (c == 2)) of them,
+// i-fastest), computed once per body position by face_open_sdf; the box
+// holds every face within sqrt(far2) of bcen.
+struct FaceFrac {
+    int active;
+    float dx, far2;
+    const float* sdf;
+    BodyParams body;
+    int lo[3], n[3];
+    const float* th[3];
+};
+
+// Open fraction of the face of component comp (0: u at x = fi dx, 1: v at
+// y = fj dx, 2: w at z = fk dx) from the body's SDF at its four corners.
+// Only faces between two FLUID/AIR cells are asked; wall faces are closed by
+// the callers' label tests.
+FS_DEV float face_open_sdf(const FaceFrac& f, int comp, int fi, int fj, int fk) {
+    if (!f.active) return 1.0f;
+    const float dx = f.dx;
+    const float x0 = (float)fi * dx, y0 = (float)fj * dx, z0 = (float)fk * dx;
+    const float x1 = (float)(fi + 1) * dx, y1 = (float)(fj + 1) * dx, z1 = (float)(fk + 1) * dx;
+    const f3 c = comp == 0 ? mk(x0, (fj + 0.5f) * dx, (fk + 0.5f) * dx)
+               : comp == 1 ? mk((fi + 0.5f) * dx, y0, (fk + 0.5f) * dx)
+                           : mk((fi + 0.5f) * dx, (fj + 0.5f) * dx, z0);
+    const float ex = c.x - f.body.bcen.x, ey = c.y - f.body.bcen.y, ez = c.z - f.body.bcen.z;
+    if (ex * ex + ey * ey + ez * ez > f.far2) return 1.0f;
+    f3 p00, p10, p01, p11;
+    if (comp == 0) {
+        p00 = mk(x0, y0, z0); p10 = mk(x0, y1, z0); p01 = mk(x0, y0, z1); p11 = mk(x0, y1, z1);
+    } else if (comp == 1) {
+        p00 = mk(x0, y0, z0); p10 = mk(x1, y0, z0); p01 = mk(x0, y0, z1); p11 = mk(x1, y0, z1);
+    } else {
+        p00 = mk(x0, y0, z0); p10 = mk(x1, y0, z0); p01 = mk(x0, y1, z0); p11 = mk(x1, y1, z0);
+    }
+    return face_open(body_phi(f.body, f.sdf, p00), body_phi(f.body, f.sdf, p10), body_phi(f.body, f.sdf, p01),
+                     body_phi(f.body, f.sdf, p11));
+}
+
+// The same fraction read from the box table th (1 outside the box)
+FS_DEV float face_open_box(const FaceFrac& f, int comp, int fi, int fj, int fk) {
+    if (!f.active) return 1.0f;
+    const int nx = f.n[0] + (comp == 0), ny = f.n[1] + (comp == 1), nz = f.n[2] + (comp == 2);
+    const unsigned a = (unsigned)(fi - f.lo[0]), b = (unsigned)(fj - f.lo[1]), c = (unsigned)(fk - f.lo[2]);
+    if (a >= (unsigned)nx || b >= (unsigned)ny || c >= (unsigned)nz) return 1.0f;
+    return f.th[comp][a + (size_t)nx * (b + (size_t)ny * c)];
+}
+
 // Rigid velocity V + Omega x r at a face (FluidSolver3D.cu:1386-1393, :2125-2135).
 FS_DEV f3 body_velocity(const BodyParams& b, f3 pos) {
     f3 r = mk(pos.x - b.pos.x, pos.y - b.pos.y, pos.z - b.pos.z);

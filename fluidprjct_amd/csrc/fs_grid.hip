@@ -30,7 +30,11 @@ constexpr int COL_TX = 64, COL_TY = 4;
 
 FS_DEV bool is_wall(int L) { return L == FS_SOLID || L == FS_BODY; }
 
-__global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs, int kc) {
+// FF: faces between two FLUID/AIR cells that the body's SDF covers in part
+// (open fraction th < 1) carry th u + (1 - th) u_body into the divergence;
+// wall faces are those of the reference (th = 0).
+template <bool FF>
+__global__ __launch_bounds__(256) void k_rhs(GridArgs a, FaceFrac ff, float* __restrict__ rhs, int kc) {
     const int W = a.W, H = a.H, D = a.D;
     const int i = blockIdx.x * COL_TX + (int)(threadIdx.x % COL_TX);
     const int j = blockIdx.y * COL_TY + (int)(threadIdx.x / COL_TX);
@@ -58,6 +62,12 @@ __global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs
                 if (is_wall(L)) {
                     float s = (L == FS_BODY) ? body_velocity(a.body, mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
                     r -= scale * (u[ui0] - s);
+                } else if (FF) {
+                    const float th = face_open_box(ff, 0, i, j, k);
+                    if (th < 1.0f) {
+                        float s = body_velocity(a.body, mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x;
+                        r -= scale * ((1.0f - th) * (u[ui0] - s));
+                    }
                 }
             }
             {
@@ -65,6 +75,12 @@ __global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs
                 if (is_wall(L)) {
                     float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x : 0.0f;
                     r += scale * (u[ui0 + 1] - s);
+                } else if (FF) {
+                    const float th = face_open_box(ff, 0, i + 1, j, k);
+                    if (th < 1.0f) {
+                        float s = body_velocity(a.body, mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx)).x;
+                        r += scale * ((1.0f - th) * (u[ui0 + 1] - s));
+                    }
                 }
             }
             if (j - 1 >= 0) {
@@ -72,6 +88,12 @@ __global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs
                 if (is_wall(L)) {
                     float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx)).y : 0.0f;
                     r -= scale * (v[vi0] - s);
+                } else if (FF) {
+                    const float th = face_open_box(ff, 1, i, j, k);
+                    if (th < 1.0f) {
+                        float s = body_velocity(a.body, mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx)).y;
+                        r -= scale * ((1.0f - th) * (v[vi0] - s));
+                    }
                 }
             }
             {
@@ -79,15 +101,33 @@ __global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs
                 if (is_wall(L)) {
                     float s = (L == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx)).y : 0.0f;
                     r += scale * (v[vi0 + W] - s);
+                } else if (FF) {
+                    const float th = face_open_box(ff, 1, i, j + 1, k);
+                    if (th < 1.0f) {
+                        float s = body_velocity(a.body, mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx)).y;
+                        r += scale * ((1.0f - th) * (v[vi0 + W] - s));
+                    }
                 }
             }
             if (k - 1 >= 0 && is_wall(Lm)) {
                 float s = (Lm == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx)).z : 0.0f;
                 r -= scale * (w[wi0] - s);
+            } else if (FF && k - 1 >= 0) {
+                const float th = face_open_box(ff, 2, i, j, k);
+                if (th < 1.0f) {
+                    float s = body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx)).z;
+                    r -= scale * ((1.0f - th) * (w[wi0] - s));
+                }
             }
             if (is_wall(Lp)) {
                 float s = (Lp == FS_BODY) ? body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx)).z : 0.0f;
                 r += scale * (w[wi0 + (size_t)WH] - s);
+            } else if (FF) {
+                const float th = face_open_box(ff, 2, i, j, k + 1);
+                if (th < 1.0f) {
+                    float s = body_velocity(a.body, mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx)).z;
+                    r += scale * ((1.0f - th) * (w[wi0 + (size_t)WH] - s));
+                }
             }
             rhs[idx] = r;
         }
@@ -98,15 +138,26 @@ __global__ __launch_bounds__(256) void k_rhs(GridArgs a, float* __restrict__ rhs
 
 // One face of component COMP at face index (fi, fj, fk) between the cells
 // with labels Ll, Lr and pressures pl, pr.
-template <int COMP>
-FS_DEV void project_face(const GridArgs& a, float* __restrict__ G, size_t idx, int Ll, float pl, int Lr, float pr,
-                         int fi, int fj, int fk, float scale, float invScale) {
+// FF: a face between two FLUID/AIR cells that the body's SDF closes
+// completely (open fraction 0) takes the body's velocity, as a BODY wall face
+// on its plus side would.
+template <int COMP, bool FF>
+FS_DEV void project_face(const GridArgs& a, const FaceFrac& ff, float* __restrict__ G, size_t idx, int Ll, float pl,
+                         int Lr, float pr, int fi, int fj, int fk, float scale, float invScale) {
     const float dx = a.dx;
     const float gv = G[idx];
     float p0, p1;
     if ((Ll == FS_FLUID || Ll == FS_AIR) && (Lr == FS_FLUID || Lr == FS_AIR)) {
         p0 = pl;
         p1 = pr;
+        if (FF && face_open_box(ff, COMP, fi, fj, fk) == 0.0f) {
+            const f3 fp = COMP == 0 ? mk(fi * dx, (fj + 0.5f) * dx, (fk + 0.5f) * dx)
+                        : COMP == 1 ? mk((fi + 0.5f) * dx, fj * dx, (fk + 0.5f) * dx)
+                                    : mk((fi + 0.5f) * dx, (fj + 0.5f) * dx, fk * dx);
+            const f3 vb = body_velocity(a.body, fp);
+            const float sol = COMP == 0 ? vb.x : COMP == 1 ? vb.y : vb.z;
+            p1 = pl + invScale * (gv - sol);
+        }
     } else if (is_wall(Ll) || is_wall(Lr)) {
         const int Lw = is_wall(Ll) ? Ll : Lr;
         float sol = 0.0f;
@@ -135,7 +186,8 @@ FS_DEV void project_face(const GridArgs& a, float* __restrict__ G, size_t idx, i
 // and k = D by the threads of the last column, row and plane.  The cells on
 // either side of a face are at linear indices c - 1 | c, c - W | c and
 // c - WH | c (wrapped past a row or slab end as the reference indexes them).
-__global__ __launch_bounds__(256) void k_project(GridArgs a, const float* __restrict__ p, int kc) {
+template <bool FF>
+__global__ __launch_bounds__(256) void k_project(GridArgs a, FaceFrac ff, const float* __restrict__ p, int kc) {
     const int W = a.W, H = a.H, D = a.D;
     const int i = blockIdx.x * COL_TX + (int)(threadIdx.x % COL_TX);
     const int j = blockIdx.y * COL_TY + (int)(threadIdx.x / COL_TX);
@@ -152,17 +204,17 @@ __global__ __launch_bounds__(256) void k_project(GridArgs a, const float* __rest
         const float pc = p_at(p, c, C), px = p_at(p, c - 1, C), py = p_at(p, c - W, C);
         const size_t ui = (size_t)i + (size_t)j * (W + 1) + (size_t)k * (W + 1) * H;
         const size_t vi = (size_t)i + (size_t)j * W + (size_t)k * W * (H + 1);
-        project_face<0>(a, a.u, ui, Lx, px, Lc, pc, i, j, k, scale, invScale);
-        project_face<1>(a, a.v, vi, Ly, py, Lc, pc, i, j, k, scale, invScale);
-        project_face<2>(a, a.w, (size_t)c, Lb, pb, Lc, pc, i, j, k, scale, invScale);
+        project_face<0, FF>(a, ff, a.u, ui, Lx, px, Lc, pc, i, j, k, scale, invScale);
+        project_face<1, FF>(a, ff, a.v, vi, Ly, py, Lc, pc, i, j, k, scale, invScale);
+        project_face<2, FF>(a, ff, a.w, (size_t)c, Lb, pb, Lc, pc, i, j, k, scale, invScale);
         if (i == W - 1)
-            project_face<0>(a, a.u, ui + 1, Lc, pc, label_at(a.labels, c + 1, C), p_at(p, c + 1, C), i + 1, j, k,
+            project_face<0, FF>(a, ff, a.u, ui + 1, Lc, pc, label_at(a.labels, c + 1, C), p_at(p, c + 1, C), i + 1, j, k,
                             scale, invScale);
         if (j == H - 1)
-            project_face<1>(a, a.v, vi + W, Lc, pc, label_at(a.labels, c + W, C), p_at(p, c + W, C), i, j + 1, k,
+            project_face<1, FF>(a, ff, a.v, vi + W, Lc, pc, label_at(a.labels, c + W, C), p_at(p, c + W, C), i, j + 1, k,
                             scale, invScale);
         if (k == D - 1)
-            project_face<2>(a, a.w, (size_t)(c + WH), Lc, pc, label_at(a.labels, c + WH, C), p_at(p, c + WH, C), i,
+            project_face<2, FF>(a, ff, a.w, (size_t)(c + WH), Lc, pc, label_at(a.labels, c + WH, C), p_at(p, c + WH, C), i,
                             j, k + 1, scale, invScale);
         Lb = Lc;
         pb = pc;
@@ -274,14 +326,63 @@ __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __res
 // longer columns serialise more dependent loads per thread)
 constexpr int COL_KC = 4;
 
-void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st) {
+void launch_rhs(const GridArgs& a, const FaceFrac& ff, float* rhs, hipStream_t st) {
     dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + COL_KC - 1) / COL_KC);
-    hipLaunchKernelGGL(k_rhs, grid, dim3(256), 0, st, a, rhs, COL_KC);
+    if (ff.active)
+        hipLaunchKernelGGL(k_rhs<true>, grid, dim3(256), 0, st, a, ff, rhs, COL_KC);
+    else
+        hipLaunchKernelGGL(k_rhs<false>, grid, dim3(256), 0, st, a, ff, rhs, COL_KC);
 }
 
-void launch_project(const GridArgs& a, const float* p, hipStream_t st) {
+void launch_project(const GridArgs& a, const FaceFrac& ff, const float* p, hipStream_t st) {
     dim3 grid((a.W + COL_TX - 1) / COL_TX, (a.H + COL_TY - 1) / COL_TY, (a.D + COL_KC - 1) / COL_KC);
-    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, st, a, p, COL_KC);
+    if (ff.active)
+        hipLaunchKernelGGL(k_project<true>, grid, dim3(256), 0, st, a, ff, p, COL_KC);
+    else
+        hipLaunchKernelGGL(k_project<false>, grid, dim3(256), 0, st, a, ff, p, COL_KC);
+}
+
+// Open fractions of every face of the three components (1 between FLUID/AIR
+// cells away from the body, 0 where a wall cell or the domain boundary
+// closes the face), for read-back.
+__global__ __launch_bounds__(256) void k_face_open(GridArgs a, FaceFrac ff, int comp, float* __restrict__ out,
+                                                   long long n) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int W = a.W, H = a.H, D = a.D;
+    const int FW = W + (comp == 0), FH = H + (comp == 1);
+    const int fi = (int)(q % FW), fj = (int)((q / FW) % FH), fk = (int)(q / ((long long)FW * FH));
+    int li = fi, lj = fj, lk = fk;   // cell on the minus side
+    if (comp == 0) li -= 1; else if (comp == 1) lj -= 1; else lk -= 1;
+    float th = 0.0f;
+    if (li >= 0 && lj >= 0 && lk >= 0 && fi < W && fj < H && fk < D) {
+        const long long WH = (long long)W * H;
+        const int Ll = a.labels[li + (long long)lj * W + lk * WH], Lr = a.labels[fi + (long long)fj * W + fk * WH];
+        if ((Ll == FS_FLUID || Ll == FS_AIR) && (Lr == FS_FLUID || Lr == FS_AIR)) th = face_open_box(ff, comp, fi, fj, fk);
+    }
+    out[q] = th;
+}
+
+// the box table of face_open_box: face (a, b, c) of component comp in the box
+__global__ __launch_bounds__(256) void k_face_theta(FaceFrac ff, int comp, float* __restrict__ th, long long n) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int nx = ff.n[0] + (comp == 0), ny = ff.n[1] + (comp == 1);
+    const int a = (int)(q % nx), b = (int)((q / nx) % ny), c = (int)(q / ((long long)nx * ny));
+    th[q] = face_open_sdf(ff, comp, ff.lo[0] + a, ff.lo[1] + b, ff.lo[2] + c);
+}
+
+void launch_face_theta(const FaceFrac& ff, float* const th[3], hipStream_t st) {
+    for (int comp = 0; comp < 3; ++comp) {
+        const long long n = (long long)(ff.n[0] + (comp == 0)) * (ff.n[1] + (comp == 1)) * (ff.n[2] + (comp == 2));
+        if (n > 0)
+            hipLaunchKernelGGL(k_face_theta, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ff, comp, th[comp], n);
+    }
+}
+
+void launch_face_open(const GridArgs& a, const FaceFrac& ff, int comp, float* out, hipStream_t st) {
+    const long long n = (long long)(a.W + (comp == 0)) * (a.H + (comp == 1)) * (a.D + (comp == 2));
+    hipLaunchKernelGGL(k_face_open, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ff, comp, out, n);
 }
 
 void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P, double* out,

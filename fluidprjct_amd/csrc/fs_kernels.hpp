@@ -116,8 +116,10 @@ void launch_g2p_advect(const G2PArgs& a, const int* start, hipStream_t st);
 void launch_max_speed(const ParticlesView& P, int n, unsigned* umax, hipStream_t st);
 
 // grid stages
-void launch_rhs(const GridArgs& a, float* rhs, hipStream_t st);
-void launch_project(const GridArgs& a, const float* p, hipStream_t st);
+void launch_rhs(const GridArgs& a, const fsd::FaceFrac& ff, float* rhs, hipStream_t st);
+void launch_project(const GridArgs& a, const fsd::FaceFrac& ff, const float* p, hipStream_t st);
+void launch_face_open(const GridArgs& a, const fsd::FaceFrac& ff, int comp, float* out, hipStream_t st);
+void launch_face_theta(const fsd::FaceFrac& ff, float* const th[3], hipStream_t st);
 // drag/lift + buoyancy partial sums: out[0..2] force, [3..5] torque, [6] submerged volume (FP64)
 void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P,
                         double* out, hipStream_t st);

@@ -37,6 +37,7 @@ constexpr int COARSE_MAX = 4096;      // cells of the LDS-resident coarsest leve
 constexpr long long FUSE_MAX = 1LL << 16;  // levels up to this size use the fused (recomputing) transfer kernels
 constexpr long long TB_MAX = 1LL << 18;    // coarse levels up to this size use the temporally blocked smoothers
 constexpr uint32_t SELF = 0x8000u;
+constexpr uint32_t WBIT = 0x4000u;   // level 0: the cell has a partly open face (weighted stencil)
 
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
 enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_RP, PS_MEAN, PS_COUNT };
@@ -47,6 +48,10 @@ enum { FIN_BNORM = 0, FIN_OUTER, FIN_OUTER_REL };
 // a correctly rounded division.  The preconditioner only needs one fixed
 // diagonal scaling, the same in every sweep, which keeps it symmetric.
 FS_DEV float inv_diag(uint32_t d) { return __builtin_amdgcn_rcpf((float)d); }
+// the diagonal of cell c as a float: the weighted sum for a WBIT cell of level 0
+FS_DEV float diag_L(const float* __restrict__ wd, long long c, uint32_t code) {
+    return (code & WBIT) ? wd[c] : (float)((code >> 8) & 7u);
+}
 
 FS_DEV float stencil_Ax(const float* __restrict__ x, long long c, uint32_t code, int W, long long WH) {
     float s = (float)((code >> 8) & 7u) * x[c];
@@ -67,6 +72,36 @@ FS_DEV double stencil_Ax64(const double* __restrict__ x, long long c, uint32_t c
     if (code & 8u) s -= x[c - W];
     if (code & 16u) s -= x[c + WH];
     if (code & 32u) s -= x[c - WH];
+    return s;
+}
+
+// A x at an unknown c with the face weights: d_c x_c minus the open fraction
+// times the neighbour's value on each coupled face (nb(q): the neighbour's
+// value across face q = +x, -x, +y, -y, +z, -z, as a double).  With every
+// fraction 1 it is stencil_Ax64's sum, term for term.
+template <class Nb>
+FS_DEV double stencil_Ax64_w(const MGLevel& L, long long c, uint32_t code, long long WH, double xc, Nb nb) {
+    double s = (double)L.wd[c] * xc;
+    if (code & 1u) s -= (double)L.wx[c] * nb(0);
+    if (code & 2u) s -= (double)L.wx[c - 1] * nb(1);
+    if (code & 4u) s -= (double)L.wy[c] * nb(2);
+    if (code & 8u) s -= (double)L.wy[c - L.W] * nb(3);
+    if (code & 16u) s -= (double)L.wz[c] * nb(4);
+    if (code & 32u) s -= (double)L.wz[c - WH] * nb(5);
+    return s;
+}
+
+// FP32 A x and 1/diag at cell c of level L: the unit stencil, or the weighted
+// one for a cell with WBIT (level 0 with face fractions)
+FS_DEV float stencil_Ax_L(const MGLevel& L, const float* __restrict__ x, long long c, uint32_t code, long long WH) {
+    if (!(code & WBIT)) return stencil_Ax(x, c, code, L.W, WH);
+    float s = L.wd[c] * x[c];
+    if (code & 1u) s -= L.wx[c] * x[c + 1];
+    if (code & 2u) s -= L.wx[c - 1] * x[c - 1];
+    if (code & 4u) s -= L.wy[c] * x[c + L.W];
+    if (code & 8u) s -= L.wy[c - L.W] * x[c - L.W];
+    if (code & 16u) s -= L.wz[c] * x[c + WH];
+    if (code & 32u) s -= L.wz[c - WH] * x[c - WH];
     return s;
 }
 
@@ -376,7 +411,8 @@ __global__ __launch_bounds__(256) void k_jacobi(MGLevel L, const float* __restri
         uint32_t d = (code >> 8) & 7u;
         if ((code & SELF) && d) {
             float bc = b[c];
-            xn = xin[c] + omega * (bc - stencil_Ax(xin, c, code, L.W, WH)) * inv_diag(d);
+            const float id = (code & WBIT) ? __builtin_amdgcn_rcpf(L.wd[c]) : inv_diag(d);
+            xn = xin[c] + omega * (bc - stencil_Ax_L(L, xin, c, code, WH)) * id;
             if (DOT) acc += (double)bc * (double)xn;
         }
         xout[c] = xn;
@@ -394,7 +430,8 @@ __global__ __launch_bounds__(256) void k_diag(MGLevel L, const float* __restrict
          c += (long long)gridDim.x * blockDim.x) {
         uint32_t code = L.code[c];
         uint32_t d = (code >> 8) & 7u;
-        float zn = ((code & SELF) && d) ? r[c] * inv_diag(d) : 0.0f;
+        const float id = (code & WBIT) ? __builtin_amdgcn_rcpf(L.wd[c]) : inv_diag(d);
+        float zn = ((code & SELF) && d) ? r[c] * id : 0.0f;
         z[c] = zn;
         acc += (double)r[c] * (double)zn;
     }
@@ -409,6 +446,12 @@ FS_DEV float jacobi_first(const float* __restrict__ b, const uint16_t* __restric
     uint32_t d = (cd >> 8) & 7u;
     return ((cd & SELF) && d) ? omega * b[c] * inv_diag(d) : 0.0f;
 }
+// the same with the weighted diagonal of a WBIT cell (level 0 with face fractions)
+FS_DEV float jacobi_first_L(const MGLevel& L, const float* __restrict__ b, long long c, float omega) {
+    uint32_t cd = L.code[c];
+    if (!(cd & WBIT)) return jacobi_first(b, L.code, c, omega);
+    return (cd & SELF) ? omega * b[c] * __builtin_amdgcn_rcpf(L.wd[c]) : 0.0f;
+}
 
 __global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restrict__ b, float* __restrict__ xout,
                                                  float omega, float omega2, const SolveScalars* __restrict__ sc) {
@@ -418,18 +461,41 @@ __global__ __launch_bounds__(256) void k_jacobi2(MGLevel L, const float* __restr
     uint32_t code = L.code[c];
     float xn = 0.0f;
     uint32_t d = (code >> 8) & 7u;
-    if ((code & SELF) && d) {
+    if ((code & SELF) && d && !(code & WBIT)) {
         const long long WH = (long long)L.W * L.H;
         float bc = b[c];
         float x1 = omega * bc * inv_diag(d);
         float ax = (float)d * x1;
-        if (code & 1u) ax -= jacobi_first(b, L.code, c + 1, omega);
-        if (code & 2u) ax -= jacobi_first(b, L.code, c - 1, omega);
-        if (code & 4u) ax -= jacobi_first(b, L.code, c + L.W, omega);
-        if (code & 8u) ax -= jacobi_first(b, L.code, c - L.W, omega);
-        if (code & 16u) ax -= jacobi_first(b, L.code, c + WH, omega);
-        if (code & 32u) ax -= jacobi_first(b, L.code, c - WH, omega);
+        if (L.wd) {   // neighbours may be WBIT cells
+            if (code & 1u) ax -= jacobi_first_L(L, b, c + 1, omega);
+            if (code & 2u) ax -= jacobi_first_L(L, b, c - 1, omega);
+            if (code & 4u) ax -= jacobi_first_L(L, b, c + L.W, omega);
+            if (code & 8u) ax -= jacobi_first_L(L, b, c - L.W, omega);
+            if (code & 16u) ax -= jacobi_first_L(L, b, c + WH, omega);
+            if (code & 32u) ax -= jacobi_first_L(L, b, c - WH, omega);
+        } else {
+            if (code & 1u) ax -= jacobi_first(b, L.code, c + 1, omega);
+            if (code & 2u) ax -= jacobi_first(b, L.code, c - 1, omega);
+            if (code & 4u) ax -= jacobi_first(b, L.code, c + L.W, omega);
+            if (code & 8u) ax -= jacobi_first(b, L.code, c - L.W, omega);
+            if (code & 16u) ax -= jacobi_first(b, L.code, c + WH, omega);
+            if (code & 32u) ax -= jacobi_first(b, L.code, c - WH, omega);
+        }
         xn = x1 + omega2 * (bc - ax) * inv_diag(d);
+    } else if ((code & SELF) && d) {
+        // a WBIT cell: the weighted diagonal and couplings
+        const long long WH = (long long)L.W * L.H;
+        float bc = b[c];
+        const float id = __builtin_amdgcn_rcpf(L.wd[c]);
+        float x1 = omega * bc * id;
+        float ax = L.wd[c] * x1;
+        if (code & 1u) ax -= L.wx[c] * jacobi_first_L(L, b, c + 1, omega);
+        if (code & 2u) ax -= L.wx[c - 1] * jacobi_first_L(L, b, c - 1, omega);
+        if (code & 4u) ax -= L.wy[c] * jacobi_first_L(L, b, c + L.W, omega);
+        if (code & 8u) ax -= L.wy[c - L.W] * jacobi_first_L(L, b, c - L.W, omega);
+        if (code & 16u) ax -= L.wz[c] * jacobi_first_L(L, b, c + WH, omega);
+        if (code & 32u) ax -= L.wz[c - WH] * jacobi_first_L(L, b, c - WH, omega);
+        xn = x1 + omega2 * (bc - ax) * id;
     }
     xout[c] = xn;
 }
@@ -441,7 +507,7 @@ __global__ __launch_bounds__(256) void k_residual(MGLevel L, const float* __rest
     if (c >= L.n) return;
     uint32_t code = L.code[c];
     float bc = b ? b[c] : 0.0f;
-    r[c] = (code & SELF) ? bc - stencil_Ax(x, c, code, L.W, (long long)L.W * L.H) : 0.0f;
+    r[c] = (code & SELF) ? bc - stencil_Ax_L(L, x, c, code, (long long)L.W * L.H) : 0.0f;
 }
 
 // coarse b = 1/2 * P^T r with trilinear weights 1/4,3/4,3/4,1/4 per axis.
@@ -544,15 +610,26 @@ __global__ __launch_bounds__(256) void k_prolong_smooth(MGLevel F, MGLevel Cc, c
             int i, j, k;
             split3(c, F.W, F.H, i, j, k);
             float xp = x[c] + prolong_at(Cc, xc, i, j, k);
-            float ax = (float)d * xp;
-            if (code & 1u) ax -= x[c + 1] + prolong_at(Cc, xc, i + 1, j, k);
-            if (code & 2u) ax -= x[c - 1] + prolong_at(Cc, xc, i - 1, j, k);
-            if (code & 4u) ax -= x[c + F.W] + prolong_at(Cc, xc, i, j + 1, k);
-            if (code & 8u) ax -= x[c - F.W] + prolong_at(Cc, xc, i, j - 1, k);
-            if (code & 16u) ax -= x[c + WH] + prolong_at(Cc, xc, i, j, k + 1);
-            if (code & 32u) ax -= x[c - WH] + prolong_at(Cc, xc, i, j, k - 1);
             float bc = b[c];
-            xn = xp + omega * (bc - ax) * inv_diag(d);
+            if (code & WBIT) {   // level 0 with face fractions: the weighted stencil
+                float ax = F.wd[c] * xp;
+                if (code & 1u) ax -= F.wx[c] * (x[c + 1] + prolong_at(Cc, xc, i + 1, j, k));
+                if (code & 2u) ax -= F.wx[c - 1] * (x[c - 1] + prolong_at(Cc, xc, i - 1, j, k));
+                if (code & 4u) ax -= F.wy[c] * (x[c + F.W] + prolong_at(Cc, xc, i, j + 1, k));
+                if (code & 8u) ax -= F.wy[c - F.W] * (x[c - F.W] + prolong_at(Cc, xc, i, j - 1, k));
+                if (code & 16u) ax -= F.wz[c] * (x[c + WH] + prolong_at(Cc, xc, i, j, k + 1));
+                if (code & 32u) ax -= F.wz[c - WH] * (x[c - WH] + prolong_at(Cc, xc, i, j, k - 1));
+                xn = xp + omega * (bc - ax) * __builtin_amdgcn_rcpf(F.wd[c]);
+            } else {
+                float ax = (float)d * xp;
+                if (code & 1u) ax -= x[c + 1] + prolong_at(Cc, xc, i + 1, j, k);
+                if (code & 2u) ax -= x[c - 1] + prolong_at(Cc, xc, i - 1, j, k);
+                if (code & 4u) ax -= x[c + F.W] + prolong_at(Cc, xc, i, j + 1, k);
+                if (code & 8u) ax -= x[c - F.W] + prolong_at(Cc, xc, i, j - 1, k);
+                if (code & 16u) ax -= x[c + WH] + prolong_at(Cc, xc, i, j, k + 1);
+                if (code & 32u) ax -= x[c - WH] + prolong_at(Cc, xc, i, j, k - 1);
+                xn = xp + omega * (bc - ax) * inv_diag(d);
+            }
             if (DOT) acc += (double)bc * (double)xn;
         }
         xout[c] = xn;
@@ -773,26 +850,48 @@ FS_DEV float ax4(const Nbhd4& v, uint32_t code, int q) {
     return s;
 }
 
+// the same with the weights of a WBIT cell c (level 0 with face fractions),
+// in stencil_Ax_L's order
+FS_DEV float ax4_L(const MGLevel& L, const Nbhd4& v, uint32_t code, int q, long long c, long long WH) {
+    if (!(code & WBIT)) return ax4(v, code, q);
+    float s = L.wd[c] * v.ctr[q];
+    if (code & 1u) s -= L.wx[c] * v.xp[q];
+    if (code & 2u) s -= L.wx[c - 1] * v.xm[q];
+    if (code & 4u) s -= L.wy[c] * v.yp[q];
+    if (code & 8u) s -= L.wy[c - L.W] * v.ym[q];
+    if (code & 16u) s -= L.wz[c] * v.zp[q];
+    if (code & 32u) s -= L.wz[c - WH] * v.zm[q];
+    return s;
+}
+
+// 1/diagonal of cell c: the weighted one of a WBIT cell
+FS_DEV float inv_diag_L(const MGLevel& L, uint32_t code, long long c) {
+    return (code & WBIT) ? __builtin_amdgcn_rcpf(L.wd[c]) : inv_diag((code >> 8) & 7u);
+}
+
 struct PlainField {
     const float* __restrict__ p;
     FS_DEV float4 vec(long long c) const { return ld4(p, c); }
     FS_DEV float one(long long c) const { return p[c]; }
 };
 
-// omega b / d on unknowns (the first sweep from zero), 0 elsewhere
+// omega b / d on unknowns (the first sweep from zero), 0 elsewhere (wd: the
+// weighted diagonals of WBIT cells, null without face fractions)
 struct FirstSweepField {
     const float* __restrict__ b;
     const uint16_t* __restrict__ code;
     float omega;
-    FS_DEV float one(long long c) const { return jacobi_first(b, code, c, omega); }
+    const float* __restrict__ wd;
+    FS_DEV float f(float bb, uint32_t cd, long long c) const {
+        if (cd & WBIT) return (cd & SELF) ? omega * bb * __builtin_amdgcn_rcpf(wd[c]) : 0.0f;
+        uint32_t d = (cd >> 8) & 7u;
+        return ((cd & SELF) && d) ? omega * bb * inv_diag(d) : 0.0f;
+    }
+    FS_DEV float one(long long c) const { return f(b[c], code[c], c); }
     FS_DEV float4 vec(long long c) const {
         const float4 bv = ld4(b, c);
         const ushort4 cv = *reinterpret_cast<const ushort4*>(code + c);
-        auto f = [&](float bb, uint32_t cd) {
-            uint32_t d = (cd >> 8) & 7u;
-            return ((cd & SELF) && d) ? omega * bb * inv_diag(d) : 0.0f;
-        };
-        return make_float4(f(bv.x, cv.x), f(bv.y, cv.y), f(bv.z, cv.z), f(bv.w, cv.w));
+        return make_float4(f(bv.x, cv.x, c), f(bv.y, cv.y, c + 1), f(bv.z, cv.z, c + 2), f(bv.w, cv.w, c + 3));
     }
 };
 
@@ -816,7 +915,7 @@ __global__ __launch_bounds__(256) void k_jacobi2_v4(MGLevel L, const float* __re
     const Codes4 k = load_codes4(L.code, c0);
     const long long WH = (long long)L.W * L.H;
     Nbhd4 v;
-    gather4(FirstSweepField{b, L.code, omega}, c0, k, L.W, WH, v);
+    gather4(FirstSweepField{b, L.code, omega, L.wd}, c0, k, L.W, WH, v);
     const float4 bv = ld4(b, c0);
     float out[4];
 #pragma unroll
@@ -826,8 +925,8 @@ __global__ __launch_bounds__(256) void k_jacobi2_v4(MGLevel L, const float* __re
         if ((code & SELF) && d) {
             const float bc = f4(bv, q);
             const float x1 = v.ctr[q];
-            const float ax = ax4(v, code, q);
-            xn = x1 + omega2 * (bc - ax) * inv_diag(d);
+            const float ax = ax4_L(L, v, code, q, c0 + q, WH);
+            xn = x1 + omega2 * (bc - ax) * inv_diag_L(L, code, c0 + q);
         }
         out[q] = xn;
     }
@@ -855,7 +954,7 @@ __global__ __launch_bounds__(256) void k_jacobi_v4(MGLevel L, const float* __res
             float xn = 0.0f;
             if ((code & SELF) && d) {
                 const float bc = f4(bv, q);
-                xn = v.ctr[q] + omega * (bc - ax4(v, code, q)) * inv_diag(d);
+                xn = v.ctr[q] + omega * (bc - ax4_L(L, v, code, q, c0 + q, WH)) * inv_diag_L(L, code, c0 + q);
                 if (DOT) acc += (double)bc * (double)xn;
             }
             out[q] = xn;
@@ -876,7 +975,8 @@ __global__ __launch_bounds__(256) void k_residual_v4(MGLevel L, const float* __r
     const float4 bv = b ? ld4(b, c0) : make_float4(0.f, 0.f, 0.f, 0.f);
     float out[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[q] = (k.c[q] & SELF) ? f4(bv, q) - ax4(v, k.c[q], q) : 0.0f;
+    for (int q = 0; q < 4; ++q)
+        out[q] = (k.c[q] & SELF) ? f4(bv, q) - ax4_L(L, v, k.c[q], q, c0 + q, (long long)L.W * L.H) : 0.0f;
     *reinterpret_cast<float4*>(r + c0) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
@@ -1031,13 +1131,19 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv_v4(MGLevel L, const float*
             qc[t] = 0.0f;
             if (code & SELF) {
                 pn[t] = v.ctr[t];
-                double qd = (double)((code >> 8) & 7u) * (double)pn[t];
-                if (code & 1u) qd -= (double)v.xp[t];
-                if (code & 2u) qd -= (double)v.xm[t];
-                if (code & 4u) qd -= (double)v.yp[t];
-                if (code & 8u) qd -= (double)v.ym[t];
-                if (code & 16u) qd -= (double)v.zp[t];
-                if (code & 32u) qd -= (double)v.zm[t];
+                double qd;
+                if (code & WBIT) {   // the weighted row, in stencil_Ax64_w's order
+                    const float nb[6] = {v.xp[t], v.xm[t], v.yp[t], v.ym[t], v.zp[t], v.zm[t]};
+                    qd = stencil_Ax64_w(L, c0 + t, code, WH, (double)pn[t], [&](int m) { return (double)nb[m]; });
+                } else {
+                    qd = (double)((code >> 8) & 7u) * (double)pn[t];
+                    if (code & 1u) qd -= (double)v.xp[t];
+                    if (code & 2u) qd -= (double)v.xm[t];
+                    if (code & 4u) qd -= (double)v.yp[t];
+                    if (code & 8u) qd -= (double)v.ym[t];
+                    if (code & 16u) qd -= (double)v.zp[t];
+                    if (code & 32u) qd -= (double)v.zm[t];
+                }
                 qc[t] = (float)qd;
                 acc += (double)pn[t] * qd;
                 if (prp) acc_rp += (double)f4(rv, t) * (double)pn[t];
@@ -1424,13 +1530,20 @@ __global__ __launch_bounds__(256) void k_pupdate_spmv(MGLevel L, const float* __
             // A p of the stored (rounded) p, summed in FP64: p is smooth and large
             // next to A p, and an FP32 sum would lose A p to cancellation
             pn = z[c] + beta * pold[c];
-            double qd = (double)((code >> 8) & 7u) * (double)pn;
-            if (code & 1u) qd -= (double)(z[c + 1] + beta * pold[c + 1]);
-            if (code & 2u) qd -= (double)(z[c - 1] + beta * pold[c - 1]);
-            if (code & 4u) qd -= (double)(z[c + L.W] + beta * pold[c + L.W]);
-            if (code & 8u) qd -= (double)(z[c - L.W] + beta * pold[c - L.W]);
-            if (code & 16u) qd -= (double)(z[c + WH] + beta * pold[c + WH]);
-            if (code & 32u) qd -= (double)(z[c - WH] + beta * pold[c - WH]);
+            double qd;
+            if (code & WBIT) {
+                const long long off[6] = {1, -1, L.W, -(long long)L.W, WH, -WH};
+                qd = stencil_Ax64_w(L, c, code, WH, (double)pn,
+                                    [&](int k) { return (double)(z[c + off[k]] + beta * pold[c + off[k]]); });
+            } else {
+                qd = (double)((code >> 8) & 7u) * (double)pn;
+                if (code & 1u) qd -= (double)(z[c + 1] + beta * pold[c + 1]);
+                if (code & 2u) qd -= (double)(z[c - 1] + beta * pold[c - 1]);
+                if (code & 4u) qd -= (double)(z[c + L.W] + beta * pold[c + L.W]);
+                if (code & 8u) qd -= (double)(z[c - L.W] + beta * pold[c - L.W]);
+                if (code & 16u) qd -= (double)(z[c + WH] + beta * pold[c + WH]);
+                if (code & 32u) qd -= (double)(z[c - WH] + beta * pold[c - WH]);
+            }
             qc = (float)qd;
             acc += (double)pn * qd;
             if (prp) acc_rp += (double)r[c] * (double)pn;
@@ -1550,10 +1663,15 @@ __global__ __launch_bounds__(256) void k_bnorm(MGLevel L, const float* __restric
 // gathered into the box, the unknown count and Dirichlet flag (counters[0],
 // counters[1]) and the partial sums of b^2 and b over the unknowns, with the
 // cells visited in k_bnorm's order (same partial sums, same bits).
+// FF: a face to a FLUID/AIR neighbour counts only when its open fraction
+// (face_open_sdf) is positive; a cell with such a face partly open gets WBIT
+// and its diagonal in L.wd (sum of the open fractions of its faces); every
+// FLUID cell writes the fractions of its +x/+y/+z faces to L.wx/wy/wz.
+template <bool FF>
 __global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ labels, const float* __restrict__ bfull,
                                                 MGLevel L, int W, int H, int ox, int oy, int oz,
                                                 float* __restrict__ bbox, double* __restrict__ p2,
-                                                double* __restrict__ p1, int* __restrict__ counters) {
+                                                double* __restrict__ p1, int* __restrict__ counters, FaceFrac ff) {
     __shared__ double sm[12];
     __shared__ int s_cnt[2];
     if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
@@ -1580,6 +1698,19 @@ __global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ labe
             if (j > 0) nt[3] = kind(labels[g - W]);
             if (k + 1 < L.D) nt[4] = kind(labels[g + WHf]);
             if (k > 0) nt[5] = kind(labels[g - WHf]);
+            float th[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+            if (FF) {
+                const int gi = i + ox, gj = j + oy, gk = k + oz;
+                if (nt[0]) th[0] = face_open_box(ff, 0, gi + 1, gj, gk);
+                if (nt[1]) th[1] = face_open_box(ff, 0, gi, gj, gk);
+                if (nt[2]) th[2] = face_open_box(ff, 1, gi, gj + 1, gk);
+                if (nt[3]) th[3] = face_open_box(ff, 1, gi, gj, gk);
+                if (nt[4]) th[4] = face_open_box(ff, 2, gi, gj, gk + 1);
+                if (nt[5]) th[5] = face_open_box(ff, 2, gi, gj, gk);
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+                    if (!nt[q] || th[q] == 0.0f) { nt[q] = 0; th[q] = 0.0f; }
+            }
             uint32_t diag = 0;
             int air = 0;
 #pragma unroll
@@ -1587,8 +1718,20 @@ __global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ labe
                 if (nt[q] == 1) { code |= (1u << q); ++diag; }
                 else if (nt[q] == 2) { ++diag; air = 1; }
             }
+            bool part = false;
+            if (FF) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) part |= th[q] > 0.0f && th[q] < 1.0f;
+                // every FLUID cell's plus faces (a WBIT neighbour reads its
+                // minus faces there), the diagonal of the WBIT cells
+                const_cast<float*>(L.wx)[c] = th[0];
+                const_cast<float*>(L.wy)[c] = th[2];
+                const_cast<float*>(L.wz)[c] = th[4];
+                if (part) const_cast<float*>(L.wd)[c] = ((((th[0] + th[1]) + th[2]) + th[3]) + th[4]) + th[5];
+            }
             // an empty row (no FLUID or AIR neighbour) is no unknown
             if (diag) {
+                if (part) code |= WBIT;
                 code |= SELF | (diag << 8);
                 ++unk;
                 dir |= air;
@@ -1635,13 +1778,19 @@ __global__ __launch_bounds__(256) void k_resid64_x0(MGLevel L, const float* __re
             split3(c, L.W, L.H, i, j, k);
             const long long g = (long long)(i + ox) + (long long)(j + oy) * W + (long long)(k + oz) * WHf;
             xc = x0[g];
-            double sx = (double)((code >> 8) & 7u) * xc;
-            if (code & 1u) sx -= x0[g + 1];
-            if (code & 2u) sx -= x0[g - 1];
-            if (code & 4u) sx -= x0[g + W];
-            if (code & 8u) sx -= x0[g - W];
-            if (code & 16u) sx -= x0[g + WHf];
-            if (code & 32u) sx -= x0[g - WHf];
+            double sx;
+            if (code & WBIT) {
+                const long long off[6] = {1, -1, W, -(long long)W, WHf, -WHf};
+                sx = stencil_Ax64_w(L, c, code, (long long)L.W * L.H, xc, [&](int k) { return x0[g + off[k]]; });
+            } else {
+                sx = (double)((code >> 8) & 7u) * xc;
+                if (code & 1u) sx -= x0[g + 1];
+                if (code & 2u) sx -= x0[g - 1];
+                if (code & 4u) sx -= x0[g + W];
+                if (code & 8u) sx -= x0[g - W];
+                if (code & 16u) sx -= x0[g + WHf];
+                if (code & 32u) sx -= x0[g - WHf];
+            }
             const double rv = ((double)b[c] - shift) - sx;
             acc += rv * rv;
             rf = (float)rv;
@@ -1666,7 +1815,14 @@ __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restr
         float rf = 0.0f;
         if (code & SELF) {
             double rv = (double)b[c] - shift;
-            if (!ZERO_X) rv -= stencil_Ax64(x64, c, code, L.W, WH);
+            if (!ZERO_X) {
+                if (code & WBIT) {
+                    const long long off[6] = {1, -1, L.W, -(long long)L.W, WH, -WH};
+                    rv -= stencil_Ax64_w(L, c, code, WH, x64[c], [&](int k) { return x64[c + off[k]]; });
+                } else {
+                    rv -= stencil_Ax64(x64, c, code, L.W, WH);
+                }
+            }
             acc += rv * rv;
             rf = (float)rv;
         }
@@ -1799,6 +1955,8 @@ void MGPCG::free_all() {
     nlev = 0;
     r = z = pa = pb = q = bb = nullptr;
     xb = nullptr;
+    if (fw) hipFree(fw);
+    fw = nullptr;
     partial = nullptr;
     sc = nullptr;
     sc_host = nullptr;
@@ -1926,8 +2084,18 @@ double* MGPCG::slot(int s) { return partial + (size_t)s * RED_BLOCKS; }
 
 void MGPCG::setup(const uint8_t* labels, const float* b, hipStream_t st) {
     // level 0: types, codes, b and the partial sums of the b norm in one pass
-    hipLaunchKernelGGL(k_setup0, dim3(nred), dim3(256), 0, st, labels, b, lv[0], W, H, box[0], box[1], box[2], bb,
-                       slot(PS_B2), slot(PS_B1), counters + 8);
+    MGLevel& L0 = lv[0];
+    const bool weighted = face.active && fw;
+    L0.wd = weighted ? fw : nullptr;
+    L0.wx = weighted ? fw + N : nullptr;
+    L0.wy = weighted ? fw + 2 * N : nullptr;
+    L0.wz = weighted ? fw + 3 * N : nullptr;
+    if (weighted)
+        hipLaunchKernelGGL(k_setup0<true>, dim3(nred), dim3(256), 0, st, labels, b, L0, W, H, box[0], box[1], box[2],
+                           bb, slot(PS_B2), slot(PS_B1), counters + 8, face);
+    else
+        hipLaunchKernelGGL(k_setup0<false>, dim3(nred), dim3(256), 0, st, labels, b, L0, W, H, box[0], box[1],
+                           box[2], bb, slot(PS_B2), slot(PS_B1), counters + 8, face);
     for (int l = 0; l + 1 < nlev; ++l)
         hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
     for (int l = 1; l < nlev; ++l)
@@ -1937,11 +2105,13 @@ void MGPCG::setup(const uint8_t* labels, const float* b, hipStream_t st) {
 
 MGLevel MGPCG::fine() const { return lv[0]; }
 
+
+
 float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s) {
     MGLevel& L = lv[l];
     // small levels: one workgroup (a fine level that small, as in a thin slab,
     // too; its r.z partial sums then come from a separate pass)
-    if (use_tail && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS) {
+    if (use_tail && l < nlev - 1 && L.n <= TAIL_MAX && nlev - l <= TAIL_LEVELS && !L.wd) {
         TailLevels t{};
         t.n = nlev - l;
         for (int q = 0; q < SMOOTH_STEPS; ++q) t.w[q] = sweep_omega[q];
@@ -2160,10 +2330,10 @@ void MGPCG::record_batch(Preconditioner* pre, int nb, hipStream_t st) {
 // The batch as a HIP graph, cached per solve-box shape and preconditioner
 // (kernel arguments depend only on those).  The graph holds kernel nodes only.
 hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, int nb, hipStream_t st) {
-    const long long key[9] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
-                              pre->graph_epoch(), nb, singular_mode ? (long long)n_unk : -1};
+    const long long key[10] = {lv[0].W, lv[0].H, lv[0].D, nlev, (long long)(uintptr_t)pre->graph_key(), stall_window,
+                               pre->graph_epoch(), nb, singular_mode ? (long long)n_unk : -1, face.active ? 1 : 0};
     for (auto& g : graphs)
-        if (std::equal(key, key + 9, g.key)) return g.exec;
+        if (std::equal(key, key + 10, g.key)) return g.exec;
     hipGraph_t graph;
     if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     record_batch(pre, nb, st);
@@ -2179,7 +2349,7 @@ hipGraphExec_t MGPCG::batch_graph(Preconditioner* pre, int nb, hipStream_t st) {
         graphs.erase(graphs.begin());
     }
     GraphEntry g;
-    std::copy(key, key + 9, g.key);
+    std::copy(key, key + 10, g.key);
     g.exec = ex;
     graphs.push_back(g);
     return ex;
@@ -2238,6 +2408,19 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     MGPreconditioner mgp(this);
     if (!pre) pre = &mgp;
     fs_solve_stats loc{};
+    if (face.active && !fw) {
+        if (!owns_memory || hipMalloc((void**)&fw, sizeof(float) * 4 * (size_t)N) != hipSuccess) {
+            fw = nullptr;
+            return -5;
+        }
+    }
+    // r = (b - shift) - A x64 (zero: x64 = 0 first)
+    auto launch_resid64 = [&](bool zero, double sh, hipStream_t s) {
+        if (zero)
+            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, s, lv[0], bb, xb, sh, r, slot(PS_RES));
+        else
+            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, s, lv[0], bb, xb, sh, r, slot(PS_RES));
+    };
     int rc = prepare(labels, b, st);
     if (rc == -5) return -5;
     if (rc == -3) {
@@ -2265,7 +2448,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         hipLaunchKernelGGL(k_resid64_x0, dim3(nred), dim3(256), 0, st, L0, bb, x0, W, H, box[0], box[1], box[2], 0.0,
                            r, slot(PS_RES), xb);
     else
-        hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, 0.0, r, slot(PS_RES));
+        launch_resid64(true, 0.0, st);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER_REL, slot(PS_RES),
                        (const double*)nullptr, nred, sc, tol * tol);
     hipMemcpyAsync(host_counters + 8, counters + 8, sizeof(int) * 2, hipMemcpyDeviceToHost, st);
@@ -2301,14 +2484,11 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
                 hipLaunchKernelGGL(k_sub_mean<double>, dim3(nred), dim3(256), 0, st, L0, xb,
                                    (const double*)slot(PS_MEAN), nred, nunk, (const SolveScalars*)nullptr);
             }
-            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
-                               slot(PS_RES));
+            launch_resid64(false, shift, st);
         } else if (outer == 0)
-            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
-                               slot(PS_RES));
+            launch_resid64(true, shift, st);
         else
-            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r,
-                               slot(PS_RES));
+            launch_resid64(false, shift, st);
         if (!(outer == 0 && first_done)) {
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
                                (const double*)nullptr, nred, sc, tol2b);
@@ -2317,7 +2497,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         }
         if (outer == 0 && x0 && bn2 > 0.0 && singular && sc_host->rr_true >= bn2) {
             // the initial guess is no better than 0: start from 0
-            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, st, L0, bb, xb, shift, r, slot(PS_RES));
+            launch_resid64(true, shift, st);
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
                                (const double*)nullptr, nred, sc, tol2b);
             hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);

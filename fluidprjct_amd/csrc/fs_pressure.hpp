@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fs_device.hpp"
+
 #include <vector>
 
 namespace fsk {
@@ -26,12 +28,17 @@ struct SolveScalars {
 struct MGLevel {
     int W, H, D;
     long long n;
-    uint16_t* code;   // bits 0-5 neighbour unknown (+x,-x,+y,-y,+z,-z), 8-10 diag, 15 self unknown
+    uint16_t* code;   // bits 0-5 neighbour unknown (+x,-x,+y,-y,+z,-z), 8-10 diag, 14 weighted, 15 self unknown
     uint8_t* type;    // 0 none, 1 unknown, 2 Dirichlet
     float* b;         // right-hand side (level 0: aliases the CG residual)
     float* x;         // current iterate
     float* xt;        // Jacobi ping-pong partner
     float* r;         // residual scratch
+    // level 0 with the body's face fractions: the open fractions of the +x,
+    // +y and +z faces of the FLUID cells and, for the cells whose code has
+    // bit 14 (a face to a FLUID/AIR neighbour partly open), the diagonal (sum
+    // of the open fractions of the faces to FLUID/AIR neighbours); null otherwise
+    const float *wd, *wx, *wy, *wz;
 };
 
 // A preconditioner that maps r -> z on the fine grid.  `apply` must leave z = 0
@@ -66,6 +73,14 @@ public:
     // *used: the bytes needed.  -2 when they exceed `bytes`.
     int init_in(int W, int H, int D, void* base, size_t bytes, SolveScalars* sc_host_buf, int* host_counters_buf,
                 size_t* used);
+    // Open fractions of the faces between FLUID/AIR cells from the body's SDF
+    // (face.active = 0: every such face open, the reference's matrix).  When
+    // active, A's coefficient on a face is its open fraction: the CG operator,
+    // the FP64 residuals and the fine level of the V-cycle use the fractions
+    // (on the cells that have a partly open face; the others keep unit
+    // coefficients); the coarse levels are re-discretised with unit
+    // coefficients on the faces that are open at all.
+    fsd::FaceFrac face{};
     // Solve A x = b (b: full-grid float, 0 off fluid).  x64 (full grid, double)
     // and p (float, 0 off fluid) are written.  Returns 0 on convergence.
     // x0 (optional, full grid, may be x64 itself): initial guess on the unknowns
@@ -140,7 +155,7 @@ public:
 
 private:
     struct GraphEntry {
-        long long key[9];
+        long long key[10];
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs;
@@ -151,6 +166,7 @@ private:
     int init_impl(int W, int H, int D, void* base, size_t bytes, SolveScalars* sc_host_buf, int* host_counters_buf,
                   size_t* used);
     bool owns_memory = true;   // buffers from hipMalloc (else carved from a caller's workspace)
+    float* fw = nullptr;       // face weights of the solve box, 4 x N floats (allocated on first use)
     float* vcycle_level(int l, hipStream_t st, double* partial_dot, SolveScalars* sc);
 };
 

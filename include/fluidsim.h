@@ -66,6 +66,11 @@ extern "C" {
 #define FS_GRID_P64      9   /* double W*H*D, pressure solution before rounding      */
 #define FS_GRID_CELLKEY 10   /* int32 Np, cell index of each particle's current position (id order) */
 #define FS_GRID_SOLID   11   /* uint8 W*H*D, static SOLID mask (1 = SOLID)            */
+#define FS_GRID_OPEN_U  12   /* float, as U: open fraction of each face in the pressure operator (read only;
+                                0 where a SOLID/BODY cell or the domain boundary closes it, the body's
+                                SDF fraction with FS_STEP_FACE_FRACTIONS, else 1; no reference counterpart) */
+#define FS_GRID_OPEN_V  13   /* float, as V */
+#define FS_GRID_OPEN_W  14   /* float, as W */
 
 /* per-stage timer slots, in the reference's order (FluidSolver3D.cu:3030-3098) */
 #define FS_STAGE_LABEL    0
@@ -124,6 +129,12 @@ typedef struct fs_config {
 #define FS_STEP_BODY_INERTIA  4  /* full inertia tensor of the body's SDF voxels and a quaternion
                                     orientation in place of the scalar sphere inertia and the Euler-angle
                                     rates of RigidBody::integrate (utility.cuh:408-467) */
+#define FS_STEP_FACE_FRACTIONS 8 /* the body couples to the pressure solve through the open fractions of
+                                    the faces between FLUID/AIR cells (from its SDF at the face corners):
+                                    A's coefficient on such a face is its fraction th, the divergence
+                                    takes th u + (1 - th) u_body, fully closed faces take u_body.  BODY
+                                    cells (centre inside, MarkBodyCellsFunctor FluidSolver3D.cu:45-78)
+                                    stay walls.  Off: the reference's cell-centre voxels only. */
 
 typedef struct fs_body_state {
     float pos[3], vel[3], omega[3];

@@ -26,6 +26,12 @@
  *     LDL^T direct solve for small systems, else PCG) instead of cuDSS FP32;
  *   - lift is skipped when |n x u_rel| == 0 (the reference divides by zero,
  *     :2774-2775).
+ * Options with no reference counterpart (FS_STEP_* flags) are restated here
+ * too; the face-fraction option (FS_STEP_FACE_FRACTIONS) is pinned by
+ * tests/test_oracle_face_fractions.py (plane cuts against exact polygon
+ * areas, the hydrostatic balance with a resting body, the weighted divergence
+ * after the projection, a plate thinner than a cell, scipy on the weighted
+ * matrix).
  *
  * Parity pins (tests/test_oracle_*.py): label-file fixtures (counts), the
  * particles_-1.ply seeding fixture (layout/order/offset bounds), the cuDSS 5x5
@@ -37,6 +43,7 @@
 #include "../include/fluidsim_checkpoint.h"
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -370,6 +377,104 @@ bool body_contains(const Body& b, f3 world_pos) {
     return b.sdf[(size_t)i + (size_t)j * b.sw + (size_t)k * b.sw * b.sh] <= 0.0f;
 }
 
+/* ---------------- SDF face fractions (FS_STEP_FACE_FRACTIONS) ----------------
+ * No reference counterpart: the reference voxelises the body at cell centres
+ * (MarkBodyCellsFunctor, below).  The option's algorithm, stated once here:
+ *   phi(x): the body's SDF samples, taken at the points fileOrigin + cs (i, j, k)
+ *           of the body frame, interpolated trilinearly (lerps a (1 - t) + b t);
+ *           outside the sampled box the point is outside (phi = FACE_FAR);
+ *   a face's open fraction th: the square split along its (00)-(11) diagonal
+ *           into two triangles, phi linear on each; the inside part of a
+ *           triangle with one corner p on its own side of zero is
+ *           (p / (p - q)) (p / (p - r)) (that corner inside) or 1 minus it
+ *           (that corner outside); th = 1 - mean of the two; th < 1e-3 -> 0.
+ *   Faces between two FLUID/AIR cells get th; faces next to SOLID/BODY cells
+ *   keep the reference's wall treatment (th = 0). */
+const float FACE_FAR = 1.0e30f, FACE_MIN_OPEN = 1.0e-3f;
+
+float body_phi(const Body& b, f3 wp) {
+    f3 pr = wp - b.pos;
+    const float* R = b.R;
+    f3 plc;
+    plc.x = R[0] * pr.x + R[3] * pr.y + R[6] * pr.z;
+    plc.y = R[1] * pr.x + R[4] * pr.y + R[7] * pr.z;
+    plc.z = R[2] * pr.x + R[5] * pr.y + R[8] * pr.z;
+    f3 pl = plc + b.local_com;
+    const float sx = (pl.x - b.fileOrigin.x) / b.cs, sy = (pl.y - b.fileOrigin.y) / b.cs,
+                sz = (pl.z - b.fileOrigin.z) / b.cs;
+    if (!(sx >= 0.0f && sx <= (float)(b.sw - 1) && sy >= 0.0f && sy <= (float)(b.sh - 1) && sz >= 0.0f &&
+          sz <= (float)(b.sd - 1)))
+        return FACE_FAR;
+    const int i = std::min((int)sx, std::max(b.sw - 2, 0)), j = std::min((int)sy, std::max(b.sh - 2, 0)),
+              k = std::min((int)sz, std::max(b.sd - 2, 0));
+    const float fx = sx - (float)i, fy = sy - (float)j, fz = sz - (float)k;
+    auto S = [&](int a, int bb, int c) {
+        a = std::min(a, b.sw - 1); bb = std::min(bb, b.sh - 1); c = std::min(c, b.sd - 1);
+        return b.sdf[(size_t)a + (size_t)bb * b.sw + (size_t)c * b.sw * b.sh];
+    };
+    const float c00 = S(i, j, k) * (1.0f - fx) + S(i + 1, j, k) * fx;
+    const float c10 = S(i, j + 1, k) * (1.0f - fx) + S(i + 1, j + 1, k) * fx;
+    const float c01 = S(i, j, k + 1) * (1.0f - fx) + S(i + 1, j, k + 1) * fx;
+    const float c11 = S(i, j + 1, k + 1) * (1.0f - fx) + S(i + 1, j + 1, k + 1) * fx;
+    const float c0 = c00 * (1.0f - fy) + c10 * fy, c1 = c01 * (1.0f - fy) + c11 * fy;
+    return c0 * (1.0f - fz) + c1 * fz;
+}
+
+/* inside fraction of a triangle with distances a, b, c at its corners */
+float tri_inside(float a, float b, float c) {
+    const float v[3] = {a, b, c};
+    int neg = 0;
+    for (float e : v) neg += e < 0.0f;
+    if (neg == 0) return 0.0f;
+    if (neg == 3) return 1.0f;
+    /* the corner alone on its side of zero, then the other two in cyclic order */
+    int o = 0;
+    for (int t = 0; t < 3; ++t)
+        if ((v[t] < 0.0f) == (neg == 1)) { o = t; break; }
+    const float p = v[o], q = v[(o + 1) % 3], r = v[(o + 2) % 3];
+    const float part = (p / (p - q)) * (p / (p - r));
+    return neg == 1 ? part : 1.0f - part;
+}
+
+float face_open(float p00, float p10, float p01, float p11) {
+    const float inside = 0.5f * (tri_inside(p00, p10, p11) + tri_inside(p00, p11, p01));
+    const float open = 1.0f - inside;
+    return open < FACE_MIN_OPEN ? 0.0f : open;
+}
+
+bool face_fractions_on(const Solver& s) { return (s.cfg.step_flags & FS_STEP_FACE_FRACTIONS) && s.body.active; }
+
+/* open fraction of face (fi, fj, fk) of component comp from the SDF at its corners */
+float face_open_sdf(const Solver& s, int comp, int fi, int fj, int fk) {
+    if (!face_fractions_on(s)) return 1.0f;
+    const float dx = s.cfg.dx;
+    const float x0 = (float)fi * dx, y0 = (float)fj * dx, z0 = (float)fk * dx;
+    const float x1 = (float)(fi + 1) * dx, y1 = (float)(fj + 1) * dx, z1 = (float)(fk + 1) * dx;
+    f3 c[4];
+    if (comp == 0) { c[0] = mk(x0, y0, z0); c[1] = mk(x0, y1, z0); c[2] = mk(x0, y0, z1); c[3] = mk(x0, y1, z1); }
+    else if (comp == 1) { c[0] = mk(x0, y0, z0); c[1] = mk(x1, y0, z0); c[2] = mk(x0, y0, z1); c[3] = mk(x1, y0, z1); }
+    else { c[0] = mk(x0, y0, z0); c[1] = mk(x1, y0, z0); c[2] = mk(x0, y1, z0); c[3] = mk(x1, y1, z0); }
+    return face_open(body_phi(s.body, c[0]), body_phi(s.body, c[1]), body_phi(s.body, c[2]), body_phi(s.body, c[3]));
+}
+
+/* the operator's open fraction of every face of component comp */
+std::vector<float> open_fractions(const Solver& s, int comp) {
+    const int W = s.W, H = s.H, D = s.D;
+    const int FW = W + (comp == 0), FH = H + (comp == 1), FD = D + (comp == 2);
+    std::vector<float> th((size_t)FW * FH * FD, 0.0f);
+#pragma omp parallel for schedule(static)
+    for (long long q = 0; q < (long long)th.size(); ++q) {
+        const int fi = (int)(q % FW), fj = (int)((q / FW) % FH), fk = (int)(q / ((long long)FW * FH));
+        const int li = fi - (comp == 0), lj = fj - (comp == 1), lk = fk - (comp == 2);
+        if (li < 0 || lj < 0 || lk < 0 || fi >= W || fj >= H || fk >= D) continue;
+        const int Ll = s.labels[(size_t)li + (size_t)lj * W + (size_t)lk * W * H];
+        const int Lr = s.labels[(size_t)fi + (size_t)fj * W + (size_t)fk * W * H];
+        if ((Ll == FS_FLUID || Ll == FS_AIR) && (Lr == FS_FLUID || Lr == FS_AIR))
+            th[(size_t)q] = face_open_sdf(s, comp, fi, fj, fk);
+    }
+    return th;
+}
+
 /* MarkBodyCellsFunctor FluidSolver3D.cu:45-78 */
 void mark_body(Solver& s) {
     const float dx = s.cfg.dx;
@@ -499,6 +604,7 @@ void stage_p2g(Solver& s) {
 /* RHSCalculator3D FluidSolver3D.cu:1360-1474 with scale = rho*dx/dt (:1520) */
 void stage_rhs(Solver& s) {
     const int W = s.W, H = s.H;
+    const bool ff = face_fractions_on(s);
     const float dx = s.cfg.dx;
     const float scale = (s.cfg.density * dx) / s.dt_cur;
     const f3 vel_com = s.body.vel, omega = s.body.omega, com = s.body.pos;
@@ -524,6 +630,12 @@ void stage_rhs(Solver& s) {
                 f3 fp = mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx);
                 float us_ = (L == FS_BODY) ? body_vel(fp).x : 0.0f;
                 rhs_val -= scale * (u[ui0] - us_);
+            } else if (ff) {   /* a FLUID/AIR neighbour: the SDF's open fraction of the face */
+                const float th = face_open_sdf(s, 0, i, j, k);
+                if (th < 1.0f) {
+                    f3 fp = mk(i * dx, (j + 0.5f) * dx, (k + 0.5f) * dx);
+                    rhs_val -= scale * ((1.0f - th) * (u[ui0] - body_vel(fp).x));
+                }
             }
         }
         {   /* i < W always holds: the plus-side read is unguarded (:1406) */
@@ -532,6 +644,12 @@ void stage_rhs(Solver& s) {
                 f3 fp = mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx);
                 float us_ = (L == FS_BODY) ? body_vel(fp).x : 0.0f;
                 rhs_val += scale * (u[ui0 + 1] - us_);
+            } else if (ff) {
+                const float th = face_open_sdf(s, 0, i + 1, j, k);
+                if (th < 1.0f) {
+                    f3 fp = mk((i + 1) * dx, (j + 0.5f) * dx, (k + 0.5f) * dx);
+                    rhs_val += scale * ((1.0f - th) * (u[ui0 + 1] - body_vel(fp).x));
+                }
             }
         }
         if (j - 1 >= 0) {
@@ -540,6 +658,12 @@ void stage_rhs(Solver& s) {
                 f3 fp = mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx);
                 float vs_ = (L == FS_BODY) ? body_vel(fp).y : 0.0f;
                 rhs_val -= scale * (v[vi0] - vs_);
+            } else if (ff) {
+                const float th = face_open_sdf(s, 1, i, j, k);
+                if (th < 1.0f) {
+                    f3 fp = mk((i + 0.5f) * dx, j * dx, (k + 0.5f) * dx);
+                    rhs_val -= scale * ((1.0f - th) * (v[vi0] - body_vel(fp).y));
+                }
             }
         }
         {
@@ -548,6 +672,12 @@ void stage_rhs(Solver& s) {
                 f3 fp = mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx);
                 float vs_ = (L == FS_BODY) ? body_vel(fp).y : 0.0f;
                 rhs_val += scale * (v[vi0 + W] - vs_);
+            } else if (ff) {
+                const float th = face_open_sdf(s, 1, i, j + 1, k);
+                if (th < 1.0f) {
+                    f3 fp = mk((i + 0.5f) * dx, (j + 1) * dx, (k + 0.5f) * dx);
+                    rhs_val += scale * ((1.0f - th) * (v[vi0 + W] - body_vel(fp).y));
+                }
             }
         }
         if (k - 1 >= 0) {
@@ -556,6 +686,12 @@ void stage_rhs(Solver& s) {
                 f3 fp = mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx);
                 float ws_ = (L == FS_BODY) ? body_vel(fp).z : 0.0f;
                 rhs_val -= scale * (w[wi0] - ws_);
+            } else if (ff) {
+                const float th = face_open_sdf(s, 2, i, j, k);
+                if (th < 1.0f) {
+                    f3 fp = mk((i + 0.5f) * dx, (j + 0.5f) * dx, k * dx);
+                    rhs_val -= scale * ((1.0f - th) * (w[wi0] - body_vel(fp).z));
+                }
             }
         }
         {
@@ -564,6 +700,12 @@ void stage_rhs(Solver& s) {
                 f3 fp = mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx);
                 float ws_ = (L == FS_BODY) ? body_vel(fp).z : 0.0f;
                 rhs_val += scale * (w[wi0 + (size_t)W * H] - ws_);
+            } else if (ff) {
+                const float th = face_open_sdf(s, 2, i, j, k + 1);
+                if (th < 1.0f) {
+                    f3 fp = mk((i + 0.5f) * dx, (j + 0.5f) * dx, (k + 1) * dx);
+                    rhs_val += scale * ((1.0f - th) * (w[wi0 + (size_t)W * H] - body_vel(fp).z));
+                }
             }
         }
         s.rhs[idx] = rhs_val;
@@ -578,9 +720,16 @@ struct Csr {
     std::vector<int> cell;          /* fluid number -> cell index */
 };
 
+/* Open fraction of the face between cell (i, j, k) and its neighbour n (order
+ * +x, -x, +y, -y, +z, -z) for the face-fraction operator; null: every face 1. */
+using FaceFn = std::function<float(int i, int j, int k, int n)>;
+
 /* exclusive-scan numbering (:1830-1853) and the SPD matrix of MatrixFiller3D
- * (:1656-1716): diag = #in-domain FLUID/AIR neighbours, -1 per FLUID neighbour */
-Csr assemble(const std::vector<uint8_t>& L, int W, int H, int D, std::vector<int>& num) {
+ * (:1656-1716): diag = #in-domain FLUID/AIR neighbours, -1 per FLUID neighbour.
+ * With open fractions th (faces to FLUID/AIR neighbours with th > 0 only):
+ * diag = their FP32 sum in the neighbour order above, -th per FLUID neighbour. */
+Csr assemble(const std::vector<uint8_t>& L, int W, int H, int D, std::vector<int>& num,
+             const FaceFn& open = nullptr) {
     size_t C = (size_t)W * H * D;
     num.assign(C, -1);
     Csr A;
@@ -592,15 +741,20 @@ Csr assemble(const std::vector<uint8_t>& L, int W, int H, int D, std::vector<int
         int c = A.cell[r];
         int i = c % W, j = (c / W) % H, k = c / (W * H);
         double diag = 0;
+        float fdiag = 0.0f;
         std::vector<std::pair<int, double>> row;
         for (int n = 0; n < 6; ++n) {
             int ni = i + off[n][0], nj = j + off[n][1], nk = k + off[n][2];
             if (ni < 0 || ni >= W || nj < 0 || nj >= H || nk < 0 || nk >= D) continue;
             size_t nidx = (size_t)ni + (size_t)nj * W + (size_t)nk * W * H;
-            if (L[nidx] == FS_FLUID) { row.push_back({num[nidx], -1.0}); diag += 1.0; }
-            else if (L[nidx] == FS_AIR) diag += 1.0;
+            if (L[nidx] != FS_FLUID && L[nidx] != FS_AIR) continue;
+            const float th = open ? open(i, j, k, n) : 1.0f;
+            if (th == 0.0f) continue;
+            if (L[nidx] == FS_FLUID) row.push_back({num[nidx], -(double)th});
+            diag += 1.0;
+            fdiag = fdiag + th;
         }
-        row.push_back({r, diag});
+        row.push_back({r, open ? (double)fdiag : diag});
         std::sort(row.begin(), row.end());
         for (auto& e : row) { A.col.push_back(e.first); A.val.push_back(e.second); }
         A.off[r + 1] = (int)A.col.size();
@@ -753,18 +907,21 @@ int pcg64(const Csr& A, const std::vector<double>& b, std::vector<double>& x, do
     return it;
 }
 
-bool system_singular(const std::vector<uint8_t>& L, int W, int H, int D) {
-    /* the constant vector is in the null space iff no fluid cell has an AIR neighbour */
+bool system_singular(const std::vector<uint8_t>& L, int W, int H, int D, const FaceFn& open = nullptr) {
+    /* the constant vector is in the null space iff no fluid cell has an AIR
+     * neighbour (through an open face) */
     for (int k = 0; k < D; ++k)
         for (int j = 0; j < H; ++j)
             for (int i = 0; i < W; ++i) {
                 size_t c = (size_t)i + (size_t)j * W + (size_t)k * W * H;
                 if (L[c] != FS_FLUID) continue;
                 const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-                for (auto& o : off) {
-                    int ni = i + o[0], nj = j + o[1], nk = k + o[2];
+                for (int n = 0; n < 6; ++n) {
+                    int ni = i + off[n][0], nj = j + off[n][1], nk = k + off[n][2];
                     if (ni < 0 || ni >= W || nj < 0 || nj >= H || nk < 0 || nk >= D) continue;
-                    if (L[(size_t)ni + (size_t)nj * W + (size_t)nk * W * H] == FS_AIR) return false;
+                    if (L[(size_t)ni + (size_t)nj * W + (size_t)nk * W * H] == FS_AIR &&
+                        (!open || open(i, j, k, n) > 0.0f))
+                        return false;
                 }
             }
     return true;
@@ -801,9 +958,9 @@ void drop_empty_rows(Csr& A) {
 
 /* Solve A p = b for the full-grid b; p64 full grid (0 off fluid). */
 int solve_system(const std::vector<uint8_t>& L, const float* bfull, double* pfull, int W, int H, int D,
-                 double tol, int maxit, fs_solve_stats* st) {
+                 double tol, int maxit, fs_solve_stats* st, const FaceFn& open = nullptr) {
     std::vector<int> num;
-    Csr A = assemble(L, W, H, D, num);
+    Csr A = assemble(L, W, H, D, num, open);
     drop_empty_rows(A);
     size_t C = (size_t)W * H * D;
     fs_solve_stats loc{};
@@ -819,7 +976,7 @@ int solve_system(const std::vector<uint8_t>& L, const float* bfull, double* pful
     }
     std::vector<double> b(A.n), x;
     for (int r = 0; r < A.n; ++r) b[r] = (double)bfull[A.cell[r]];
-    bool sing = system_singular(L, W, H, D);
+    bool sing = system_singular(L, W, H, D, open);
     loc.singular = sing ? 1 : 0;
     auto t0 = clk::now();
     double rr = 0;
@@ -855,9 +1012,25 @@ int solve_system(const std::vector<uint8_t>& L, const float* bfull, double* pful
 }
 
 /* pressureSolve minus RHS (FluidSolver3D.cu:1815-2062): p = solution on FLUID, 0 elsewhere */
+/* the face-fraction operator's open fraction of the face between cell
+ * (i, j, k) and its neighbour n (+x, -x, +y, -y, +z, -z) */
+FaceFn face_fn(const Solver& s) {
+    if (!face_fractions_on(s)) return nullptr;
+    return [&s](int i, int j, int k, int n) {
+        switch (n) {
+            case 0: return face_open_sdf(s, 0, i + 1, j, k);
+            case 1: return face_open_sdf(s, 0, i, j, k);
+            case 2: return face_open_sdf(s, 1, i, j + 1, k);
+            case 3: return face_open_sdf(s, 1, i, j, k);
+            case 4: return face_open_sdf(s, 2, i, j, k + 1);
+            default: return face_open_sdf(s, 2, i, j, k);
+        }
+    };
+}
+
 int stage_pressure(Solver& s) {
     int rc = solve_system(s.labels, s.rhs.data(), s.p64.data(), s.W, s.H, s.D, s.cfg.solver_tol,
-                          s.cfg.solver_max_iter, &s.stats);
+                          s.cfg.solver_max_iter, &s.stats, face_fn(s));
     for (size_t c = 0; c < s.C(); ++c) s.p[c] = (s.labels[c] == FS_FLUID) ? (float)s.p64[c] : 0.0f;
     return rc;
 }
@@ -865,6 +1038,7 @@ int stage_pressure(Solver& s) {
 /* UFunctor/VFunctor/WFunctor FluidSolver3D.cu:2072-2336, scale = dt/(rho*dx) (:2341) */
 void stage_project(Solver& s) {
     const int W = s.W, H = s.H;
+    const bool ff = face_fractions_on(s);
     const float dx = s.cfg.dx;
     const float scale = s.dt_cur / (s.cfg.density * dx);
     const f3 vel_com = s.body.vel, omega = s.body.omega, com = s.body.pos;
@@ -908,6 +1082,8 @@ void stage_project(Solver& s) {
             float p0, p1, g = G[idx];
             if ((Ll == FS_FLUID || Ll == FS_AIR) && (Lr == FS_FLUID || Lr == FS_AIR)) {
                 p0 = p_at(s, left); p1 = p_at(s, right);
+                /* face fractions: a face the SDF closes completely takes the body's velocity */
+                if (ff && face_open_sdf(s, comp, fi, fj, fk) == 0.0f) p1 = p0 + invScale * (g - vbc);
             } else if (Ll == FS_SOLID || Ll == FS_BODY) {
                 float sol = (Ll == FS_BODY) ? vbc : 0.0f;
                 p0 = p_at(s, right) - invScale * (g - sol);
@@ -1644,6 +1820,12 @@ int fs_get_grid(void* h, int which, void* out, int64_t count) {
             return cp(k.data(), k.size(), 4);
         }
         case FS_GRID_SOLID: return cp(s.solid.data(), s.solid.size(), 1);
+        case FS_GRID_OPEN_U:
+        case FS_GRID_OPEN_V:
+        case FS_GRID_OPEN_W: {
+            std::vector<float> th = open_fractions(s, which - FS_GRID_OPEN_U);
+            return cp(th.data(), th.size(), 4);
+        }
         default: return fail(FS_ERR_ARG, "unknown grid");
     }
 }

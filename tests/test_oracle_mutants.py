@@ -3,7 +3,8 @@
 Each case plants one mistake in a copy of oracle/fs_oracle.cpp (a flipped
 sign, a dropped term, a transposed operand, a swapped weight, a wrong
 staggering offset), builds that copy into a temporary library and runs the
-pins of tests/test_oracle_pins.py that cover the function against it: at
+pins of tests/test_oracle_pins.py (and test_oracle_face_fractions.py) that
+cover the function against it: at
 least one of them must fail with an assertion.  The copies live in pytest's
 temporary directory; the oracle itself is not modified.
 """
@@ -13,8 +14,9 @@ import subprocess
 
 import pytest
 
+import test_oracle_face_fractions as ff
 import test_oracle_pins as pins
-from fluidprjct_amd._abi import FsLib
+from fluidprjct_amd._abi import FS_STEP_FACE_FRACTIONS, FsLib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle")
@@ -84,8 +86,8 @@ MUTANTS = [
      "G[idx] = g + scale * (p1 - p0);",
      [lambda lib: pins.test_hydrostatic_pool(lib)]),
     ("AIR neighbour left out of the diagonal",
-     "else if (L[nidx] == FS_AIR) diag += 1.0;",
-     "else if (L[nidx] == FS_AIR) diag += 0.0;",
+     "            diag += 1.0;\n            fdiag = fdiag + th;",
+     "            if (L[nidx] == FS_FLUID) diag += 1.0;\n            fdiag = fdiag + th;",
      [lambda lib: pins.test_reference_csr_of_enclosed_toy(lib), lambda lib: pins.test_hydrostatic_pool(lib),
       lambda lib: pins.test_direct_solve_vs_scipy(lib, "dam")]),
     ("buoyancy sign",
@@ -108,6 +110,31 @@ MUTANTS = [
      "float nu = alpha * pic.x + (1.0f - alpha) * (s.vel[3 * q] + flip.x);",
      "float nu = (1.0f - alpha) * pic.x + alpha * (s.vel[3 * q] + flip.x);",
      [lambda lib: pins.test_g2p_uniform_fields_and_advection(lib)]),
+    ("triangle fraction with one edge ratio twice",
+     "const float part = (p / (p - q)) * (p / (p - r));",
+     "const float part = (p / (p - q)) * (p / (p - q));",
+     [lambda lib: ff.test_open_fraction_of_a_plane_cut(lib)]),
+    ("SDF lerp weights swapped along y",
+     "const float c0 = c00 * (1.0f - fy) + c10 * fy, c1 = c01 * (1.0f - fy) + c11 * fy;",
+     "const float c0 = c00 * fy + c10 * (1.0f - fy), c1 = c01 * fy + c11 * (1.0f - fy);",
+     [lambda lib: ff.test_open_fraction_of_a_plane_cut(lib)]),
+    ("cut face without the body velocity",
+     "rhs_val -= scale * ((1.0f - th) * (u[ui0] - body_vel(fp).x));",
+     "rhs_val -= scale * ((1.0f - th) * u[ui0]);",
+     [lambda lib: ff.test_projection_makes_the_weighted_divergence_vanish(lib, FS_STEP_FACE_FRACTIONS)]),
+    ("open fraction in place of the closed one",
+     "rhs_val += scale * ((1.0f - th) * (v[vi0 + W] - body_vel(fp).y));",
+     "rhs_val += scale * (th * (v[vi0 + W] - body_vel(fp).y));",
+     [lambda lib: ff.test_projection_makes_the_weighted_divergence_vanish(lib, FS_STEP_FACE_FRACTIONS)]),
+    ("unweighted diagonal with weighted neighbours",
+     "row.push_back({r, open ? (double)fdiag : diag});",
+     "row.push_back({r, diag});",
+     [lambda lib: ff.test_hydrostatic_pool_with_a_resting_body(lib),
+      lambda lib: ff.test_pressure_matches_scipy_on_the_weighted_matrix(lib)]),
+    ("closed face keeps its velocity",
+     "if (ff && face_open_sdf(s, comp, fi, fj, fk) == 0.0f) p1 = p0 + invScale * (g - vbc);",
+     "if (ff && face_open_sdf(s, comp, fi, fj, fk) < 0.0f) p1 = p0 + invScale * (g - vbc);",
+     [lambda lib: ff.test_thin_plate_closes_the_faces_it_covers(lib)]),
 ]
 
 
