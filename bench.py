@@ -143,9 +143,12 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--deterministic", action="store_true",
                     help="FS_STEP_DETERMINISTIC: particles of a cell in id order (bit-reproducible runs)")
-    ap.add_argument("--face-fractions", action="store_true",
-                    help="FS_STEP_FACE_FRACTIONS: the body couples through the open fractions of the faces "
-                         "its SDF cuts (the reference voxelises it at cell centres)")
+    ap.add_argument("--voxel-coupling", action="store_true",
+                    help="couple the body as the reference does (BODY cells at cell centres only) instead of "
+                         "through its SDF face fractions (FS_STEP_FACE_FRACTIONS), which the 256^3 "
+                         "sphere-drop workload names")
+    ap.add_argument("--no-other-coupling", action="store_true",
+                    help="skip the leg that times the same workload with the other body coupling")
     ap.add_argument("--learned-steps", type=int, default=5)
     ap.add_argument("--learned-params", default="", help="parameters of the learned preconditioner (.npy)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -178,7 +181,7 @@ def main():
     N, lab, body, ppc, seed, desc = build_scene(args.config)
     log(f"{args.config}: scene {lab.shape}, seeding")
     step_flags = ((FS_STEP_DETERMINISTIC if args.deterministic else 0) |
-                  (FS_STEP_FACE_FRACTIONS if args.face_fractions and body is not None else 0))
+                  (FS_STEP_FACE_FRACTIONS if not args.voxel_coupling and body is not None else 0))
     s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0, step_flags=step_flags)
     Np = s.num_particles
     step_ms = []          # every step of the run, host wall clock (each fs_step returns synchronised)
@@ -284,7 +287,7 @@ def main():
         result["full_run"] = full
     # the system of the last step, kept for the FP64 check below (with the
     # body state its pressure solve saw: the step moves the body afterwards)
-    if args.full_steps <= args.warmup + args.steps:
+    if not args.no_cpu and args.full_steps <= args.warmup + args.steps:
         run_steps(1)
     sys_lab = s.get_grid(FS_GRID_LABELS)
     sys_b = s.get_grid(FS_GRID_RHS)
@@ -319,6 +322,38 @@ def main():
     del s
     torch.cuda.empty_cache()
 
+    # the same workload with the other body coupling (the reference's voxel
+    # coupling when the line is quoted with face fractions, and vice versa)
+    if body is not None and not args.no_other_coupling:
+        other = step_flags ^ FS_STEP_FACE_FRACTIONS
+        log("other body coupling: " + ("SDF face fractions" if other & FS_STEP_FACE_FRACTIONS else "voxels"))
+        s2 = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0, step_flags=other)
+        for _ in range(args.warmup):
+            s2.step(1)
+        torch.cuda.synchronize()
+        a0 = s2.stage_times()
+        t0 = time.perf_counter()
+        s2.step(args.steps)
+        torch.cuda.synchronize()
+        e2 = time.perf_counter() - t0
+        a1 = s2.stage_times()
+        n_done = args.warmup + args.steps
+        t1 = time.perf_counter()
+        if args.full_steps > n_done:
+            s2.step(args.full_steps - n_done)
+            torch.cuda.synchronize()
+        e_rest = time.perf_counter() - t1
+        result["other_coupling"] = {
+            "coupling": "SDF face fractions" if other & FS_STEP_FACE_FRACTIONS else
+                        "voxels at cell centres (the reference's)",
+            "value": args.steps / e2, "unit": "steps/s", "ms_per_step": 1e3 * e2 / args.steps,
+            "stage_ms": {k: round(v / args.steps, 4) for k, v in stage_delta(a0, a1).items()},
+            "full_run_steps_per_s": (max(args.full_steps, n_done) - args.warmup) / (e2 + e_rest),
+            "note": "same scene and seed on a fresh handle; full_run_steps_per_s over the steps after the "
+                    "warmup (fs_step batches, one synchronisation per batch)"}
+        del s2
+        torch.cuda.empty_cache()
+
     if not args.no_cpu:
         # the CPU leg: the oracle timed on a bounded sample of the workload, and
         # the oracle's FP64 PCG as the checker of the last GPU pressure system
@@ -333,6 +368,11 @@ def main():
             o.set_labels(lab)
             o.set_body_sdf(body["sdf"], body["origin"], body["cell_size"], body["pos"], body["mass"])
             o.set_body_state(sys_body)
+            # the rotation the step used (it lags the angles by one integration)
+            import ctypes
+            fn = orc.lib.oracle_set_body_rotation
+            fn.restype, fn.argtypes = ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]
+            orc.check(fn(o.h, (ctypes.c_float * 9)(*sys_body.rotation)))
             o.set_grid(FS_GRID_LABELS, sys_lab)
             o.set_grid(FS_GRID_RHS, sys_b)
             o.stage("pressure")

@@ -2103,6 +2103,17 @@ int fs_load_checkpoint(void* h, const char* path) {
     return FS_OK;
 }
 
+/* Sets the body's rotation matrix without touching its angles: the matrix
+ * lags the Euler angles by one integration (RigidBody::integrate
+ * utility.cuh:408-467 forms R before it advances the angles), so a state read
+ * back mid-run is restored exactly only with both.  Test infrastructure. */
+int oracle_set_body_rotation(void* h, const float* R) {
+    Solver& s = *(Solver*)h;
+    if (!R) return fail(FS_ERR_ARG, "null rotation");
+    for (int i = 0; i < 9; ++i) s.body.R[i] = R[i];
+    return FS_OK;
+}
+
 int oracle_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
