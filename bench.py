@@ -132,11 +132,12 @@ def projected_velocity_norm(s, lab):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed steps from the workload's initial state (the 256^3 sphere drop is 200 steps)")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed steps on a separate handle first")
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--full-steps", type=int, default=200,
-                    help="length of the whole workload run after the timed steps (c3: 200 steps); 0 skips it")
+    ap.add_argument("--no-phases", action="store_true",
+                    help="skip the per-step run that splits the workload into its phases (before/after impact)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the inner PCG iterations eagerly")
@@ -179,38 +180,35 @@ def main():
     hip = load_library()
 
     N, lab, body, ppc, seed, desc = build_scene(args.config)
-    log(f"{args.config}: scene {lab.shape}, seeding")
+    log(f"{args.config}: scene {lab.shape}")
     step_flags = ((FS_STEP_DETERMINISTIC if args.deterministic else 0) |
                   (FS_STEP_FACE_FRACTIONS if not args.voxel_coupling and body is not None else 0))
-    s = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0, step_flags=step_flags)
+    solver_flags = 1 if args.no_graph else 0
+
+    # warmup: the same workload's first steps on a handle of its own (code
+    # objects loaded, pressure graphs of the workload's box shapes captured
+    # once per process), so that the timed run starts from the released state
+    if args.warmup > 0:
+        w = make_solver(hip, N, lab, body, ppc, seed, flags=solver_flags, step_flags=step_flags)
+        log(f"{w.num_particles} particles; {args.warmup} warmup steps on a separate handle")
+        w.step(args.warmup)
+        torch.cuda.synchronize()
+        del w
+        torch.cuda.empty_cache()
+
+    # the timed run: exactly K steps of the workload from its initial state
+    s = make_solver(hip, N, lab, body, ppc, seed, flags=solver_flags, step_flags=step_flags)
     Np = s.num_particles
-    step_ms = []          # every step of the run, host wall clock (each fs_step returns synchronised)
-    submerged = []
-    t_run0 = time.perf_counter()
-
-    last_pre_body = [None]   # body state at the start of the last run_steps step (its pressure solve's)
-
-    def run_steps(n):
-        for _ in range(n):
-            last_pre_body[0] = s.body_state() if body is not None else None
-            t = time.perf_counter()
-            s.step(1)
-            step_ms.append(1e3 * (time.perf_counter() - t))
-            if body is not None:
-                submerged.append(s.body_state().submerged_volume)
-
-    log(f"{Np} particles; {args.warmup} warmup steps")
-    run_steps(args.warmup)
     torch.cuda.synchronize()
     t_stage0 = s.stage_times()
-    log(f"{args.steps} timed steps")
+    log(f"{args.steps} timed steps from the initial state")
     t0 = time.perf_counter()
-    s.step(args.steps)
+    rc_steps = s.lib.fs_step(s.h, args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    step_ms.extend([1e3 * elapsed / args.steps] * args.steps)
-    if body is not None:
-        submerged.extend([s.body_state().submerged_volume] * args.steps)
+    not_converged = rc_steps == -4
+    if rc_steps not in (0, -4):
+        s.fl.check(rc_steps)
     t_stage = stage_delta(t_stage0, s.stage_times())
     K = args.steps
     st = s.solve_stats()
@@ -246,111 +244,123 @@ def main():
         "config": {"workload": args.config + ": " + desc + (", deterministic cell order" if args.deterministic else "")
                    + (", SDF face fractions" if step_flags & FS_STEP_FACE_FRACTIONS else ""),
                    "grid": [W, H, D], "particles": Np,
-                   "fluid_cells": nf, "precision": "FP32 transfers/stencil, FP64 dots + refinement"},
+                   "fluid_cells_last": nf, "precision": "FP32 transfers/stencil, FP64 dots + refinement",
+                   "timed": f"steps 1..{K} of the workload from its initial state (one fs_step call); the "
+                            f"{args.warmup} warmup steps ran on a separate handle"},
         "pressure_solve_ms_plain": ms["pressure"],
         "pressure_iters_last": st.iterations,
         "pressure_outer_last": st.outer,
         "pressure_rel_residual_last": st.rel_residual,
+        "pressure_not_converged": bool(not_converged),
         "stage_ms": {k: round(v, 4) for k, v in ms.items()},
         "particles_per_s_transfers": Np / ((ms["p2g"] + ms["g2p_advect"]) * 1e-3),
         "dt_last": dt_last,
+        "sim_time": simt,
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom],
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(KERNEL_NAMES[dom], args.config), "alg_bytes": alg[dom],
-                     "avg_launch_ms": ms[dom]},
+                     "avg_launch_ms": ms[dom],
+                     "note": "alg_bytes per launch / the stage's mean HIP-event time over the timed steps"},
     }
-    # the rest of the workload (c3: 200 steps from the released sphere, through
-    # the water impact), continuing the same run; steps/s over every step
-    if args.full_steps > len(step_ms):
-        log(f"continuing to step {args.full_steps}")
-        run_steps(args.full_steps - len(step_ms))
-        torch.cuda.synchronize()
-        tot_ms = sum(step_ms)
-        tall = s.stage_times()   # accumulated over every step of the handle
-        full = {"steps": len(step_ms), "steps_per_s": len(step_ms) / (tot_ms * 1e-3),
-                "wall_s_incl_readback": time.perf_counter() - t_run0,
-                "stage_ms_mean": {k: round(v / len(step_ms), 4) for k, v in tall.items()},
-                "note": "the timed steps are steps warmup+1..warmup+steps of this run; host wall clock per "
-                        "step (each step returns synchronised; the body state is read after every step "
-                        "outside the timed block)"}
-        if submerged:
-            hit = next((i for i, v in enumerate(submerged) if v > 0), None)
-            full["impact_step"] = hit
-            if hit:
-                pre, post = step_ms[1:hit], step_ms[hit:]
-                full["pre_impact_steps_per_s"] = len(pre) / (sum(pre) * 1e-3) if pre else None
-                full["post_impact_steps_per_s"] = len(post) / (sum(post) * 1e-3) if post else None
-            full["body_y_final"] = s.body_state().pos[1]
-        st2 = s.solve_stats()
-        full["pressure_iters_last"] = st2.iterations
-        full["pressure_rel_residual_last"] = st2.rel_residual
-        result["full_run"] = full
-    # the system of the last step, kept for the FP64 check below (with the
-    # body state its pressure solve saw: the step moves the body afterwards)
-    if not args.no_cpu and args.full_steps <= args.warmup + args.steps:
-        run_steps(1)
+    if body is not None:
+        result["body_y_final"] = s.body_state().pos[1]
+
+    # one more step whose system (labels, right-hand side, body pose) the FP64
+    # check below solves again
+    sys_body = s.body_state() if body is not None else None
+    s.step(1)
     sys_lab = s.get_grid(FS_GRID_LABELS)
     sys_b = s.get_grid(FS_GRID_RHS)
     sys_p = s.get_grid(FS_GRID_P64)
     sys_dt = s.clock()[1]
-    sys_body = last_pre_body[0]
     u_norm = projected_velocity_norm(s, sys_lab)
-    # learned vs plain: the same workload continued with the learned
-    # preconditioner (flexible PCG, multigrid fallback after 5 stalled iterations)
+
+    # learned vs plain, after the workload: the same run continued for a few
+    # steps with each preconditioner, then one system solved by both from zero
     if not args.no_learned:
-        log("learned-preconditioner steps")
+        from fluidprjct_amd._abi import FS_PRECOND_MG
         from fluidprjct_amd.learned import DEFAULT_PARAMS, use_learned
         params = args.learned_params or DEFAULT_PARAMS
-        use_learned(s, params, 5)
-        s.step(1)
-        torch.cuda.synchronize()
-        t0s = s.stage_times()
-        fb = 0
-        iters = []
-        for _ in range(args.learned_steps):
-            s.step(1)
-            st = s.solve_stats()
-            fb += st.fallback
-            iters.append(st.iterations)
-        torch.cuda.synchronize()
-        tl = stage_delta(t0s, s.stage_times())
-        result["pressure_solve_ms_learned"] = tl["pressure"] / args.learned_steps
-        result["learned"] = {"steps": args.learned_steps, "fallbacks": fb, "iterations": iters,
-                             "params": os.path.relpath(params, ROOT),
-                             "note": "same workload continued after the timed steps; flexible PCG with the "
-                                     "BF16 conv net, multigrid fallback after 5 stalled iterations"}
+        legs = {}
+        for name in ("plain", "learned"):
+            if name == "learned":
+                use_learned(s, params, 5)
+                s.step(1)
+            torch.cuda.synchronize()
+            t0s = s.stage_times()
+            fb, iters = 0, []
+            for _ in range(args.learned_steps):
+                s.step(1)
+                stl = s.solve_stats()
+                fb += stl.fallback
+                iters.append(stl.iterations)
+            torch.cuda.synchronize()
+            tl = stage_delta(t0s, s.stage_times())
+            legs[name] = {"pressure_ms": tl["pressure"] / args.learned_steps, "iterations": iters, "fallbacks": fb}
+            log(f"{name} steps: {legs[name]}")
+        same = {}
+        for name, pc in (("plain", FS_PRECOND_MG), ("learned", 1)):
+            s.set_preconditioner(pc, 5)
+            s.solve_pressure_system(sys_lab, sys_b, 1e-6)
+            _, sts = s.solve_pressure_system(sys_lab, sys_b, 1e-6)
+            same[name] = {"ms": sts.solve_ms, "iterations": sts.iterations,
+                          "fallback": sts.fallback, "rel_residual": sts.rel_residual}
+        result["pressure_solve_ms_learned"] = legs["learned"]["pressure_ms"]
+        result["learned"] = {
+            "params": os.path.relpath(params, ROOT), "steps": legs, "same_system": same,
+            "note": f"steps: the workload continued after step {K + 1} for {args.learned_steps} steps with "
+                    f"each preconditioner (pressure stage mean); same_system: the system of step {K + 1} "
+                    f"solved from p = 0 by both (the reference's matrix; the second solve's device time, without the host copies)"}
     del s
     torch.cuda.empty_cache()
 
+    # phases of the same workload on a fresh handle, one fs_step per step:
+    # the resting column before the sphere reaches the water and after it
+    if not args.no_phases:
+        ph = make_solver(hip, N, lab, body, ppc, seed, flags=solver_flags, step_flags=step_flags)
+        step_ms, submerged = [], []
+        log(f"phases: {K} steps one at a time")
+        for _ in range(K):
+            t = time.perf_counter()
+            ph.step(1)
+            step_ms.append(1e3 * (time.perf_counter() - t))
+            if body is not None:
+                submerged.append(ph.body_state().submerged_volume)
+        phases = {"steps_per_s": K / (sum(step_ms) * 1e-3),
+                  "steps_4_23_per_s": 20 / (sum(step_ms[3:23]) * 1e-3) if K >= 23 else None,
+                  "note": "host wall clock per step (each fs_step returns synchronised; the body state is read "
+                          "after every step outside the clock)"}
+        if submerged:
+            hit = next((i for i, v in enumerate(submerged) if v > 0), None)
+            phases["impact_step"] = hit
+            if hit:
+                pre, post = step_ms[1:hit], step_ms[hit:]
+                phases["pre_impact_steps_per_s"] = len(pre) / (sum(pre) * 1e-3) if pre else None
+                phases["post_impact_steps_per_s"] = len(post) / (sum(post) * 1e-3) if post else None
+        result["phases"] = phases
+        del ph
+        torch.cuda.empty_cache()
+
     # the same workload with the other body coupling (the reference's voxel
-    # coupling when the line is quoted with face fractions, and vice versa)
+    # coupling when the line is quoted with face fractions, and vice versa),
+    # K steps from the initial state on a fresh handle
     if body is not None and not args.no_other_coupling:
         other = step_flags ^ FS_STEP_FACE_FRACTIONS
         log("other body coupling: " + ("SDF face fractions" if other & FS_STEP_FACE_FRACTIONS else "voxels"))
-        s2 = make_solver(hip, N, lab, body, ppc, seed, flags=1 if args.no_graph else 0, step_flags=other)
-        for _ in range(args.warmup):
-            s2.step(1)
+        s2 = make_solver(hip, N, lab, body, ppc, seed, flags=solver_flags, step_flags=other)
         torch.cuda.synchronize()
         a0 = s2.stage_times()
         t0 = time.perf_counter()
-        s2.step(args.steps)
+        s2.step(K)
         torch.cuda.synchronize()
         e2 = time.perf_counter() - t0
         a1 = s2.stage_times()
-        n_done = args.warmup + args.steps
-        t1 = time.perf_counter()
-        if args.full_steps > n_done:
-            s2.step(args.full_steps - n_done)
-            torch.cuda.synchronize()
-        e_rest = time.perf_counter() - t1
         result["other_coupling"] = {
             "coupling": "SDF face fractions" if other & FS_STEP_FACE_FRACTIONS else
                         "voxels at cell centres (the reference's)",
-            "value": args.steps / e2, "unit": "steps/s", "ms_per_step": 1e3 * e2 / args.steps,
-            "stage_ms": {k: round(v / args.steps, 4) for k, v in stage_delta(a0, a1).items()},
-            "full_run_steps_per_s": (max(args.full_steps, n_done) - args.warmup) / (e2 + e_rest),
-            "note": "same scene and seed on a fresh handle; full_run_steps_per_s over the steps after the "
-                    "warmup (fs_step batches, one synchronisation per batch)"}
+            "value": K / e2, "unit": "steps/s", "ms_per_step": 1e3 * e2 / K,
+            "stage_ms": {k: round(v / K, 4) for k, v in stage_delta(a0, a1).items()},
+            "note": f"same scene and seed on a fresh handle, steps 1..{K} in one fs_step call"}
         del s2
         torch.cuda.empty_cache()
 
@@ -388,7 +398,7 @@ def main():
             "pressure_rel_l2": float(np.linalg.norm(dp[fl]) / np.linalg.norm(pref[fl])),
             "velocity_rel_l2": float(scale * face_gradient_norm(dp, sys_lab) / u_norm) if u_norm > 0 else None,
             "reference": f"oracle FP64 MIC(0)-PCG to 1e-12 relative residual ({sref.iterations} iterations, "
-                         f"{tref:.1f} s); system of the last GPU step (step {len(step_ms)})",
+                         f"{tref:.1f} s); system of GPU step {K + 1} (the step after the timed ones)",
             "velocity_note": "dt/(rho dx) |grad(p_gpu - p_fp64)| over the faces the projection updates, "
                              "relative to |u| on the faces of fluid cells after projection",
         }
