@@ -278,13 +278,18 @@ def main():
     # learned vs plain, after the workload: the same run continued for a few
     # steps with each preconditioner, then one system solved by both from zero
     if not args.no_learned:
-        from fluidprjct_amd._abi import FS_PRECOND_MG
-        from fluidprjct_amd.learned import DEFAULT_PARAMS, use_learned
+        from fluidprjct_amd._abi import FS_PRECOND_HYBRID, FS_PRECOND_LEARNED, FS_PRECOND_MG
+        from fluidprjct_amd.learned import DEFAULT_PARAMS, HYBRID_PARAMS, load_params
         params = args.learned_params or DEFAULT_PARAMS
+        modes = [("plain", FS_PRECOND_MG, None), ("learned", FS_PRECOND_LEARNED, params)]
+        if os.path.exists(HYBRID_PARAMS):
+            modes.append(("hybrid", FS_PRECOND_HYBRID, HYBRID_PARAMS))
         legs = {}
-        for name in ("plain", "learned"):
-            if name == "learned":
-                use_learned(s, params, 5)
+        for name, pc, prm in modes:
+            if prm is not None:
+                s.set_learned_params(load_params(prm))
+            s.set_preconditioner(pc, 5)
+            if name != "plain":
                 s.step(1)
             torch.cuda.synchronize()
             t0s = s.stage_times()
@@ -299,17 +304,23 @@ def main():
             legs[name] = {"pressure_ms": tl["pressure"] / args.learned_steps, "iterations": iters, "fallbacks": fb}
             log(f"{name} steps: {legs[name]}")
         same = {}
-        for name, pc in (("plain", FS_PRECOND_MG), ("learned", 1)):
+        for name, pc, prm in modes:
+            if prm is not None:
+                s.set_learned_params(load_params(prm))
             s.set_preconditioner(pc, 5)
             s.solve_pressure_system(sys_lab, sys_b, 1e-6)
             _, sts = s.solve_pressure_system(sys_lab, sys_b, 1e-6)
             same[name] = {"ms": sts.solve_ms, "iterations": sts.iterations,
                           "fallback": sts.fallback, "rel_residual": sts.rel_residual}
         result["pressure_solve_ms_learned"] = legs["learned"]["pressure_ms"]
+        result["pressure_solve_ms_hybrid"] = legs["hybrid"]["pressure_ms"] if "hybrid" in legs else None
         result["learned"] = {
-            "params": os.path.relpath(params, ROOT), "steps": legs, "same_system": same,
+            "params": {name: os.path.relpath(prm, ROOT) for name, _, prm in modes if prm},
+            "steps": legs, "same_system": same,
             "note": f"steps: the workload continued after step {K + 1} for {args.learned_steps} steps with "
-                    f"each preconditioner (pressure stage mean); same_system: the system of step {K + 1} "
+                    f"each preconditioner (pressure stage mean; learned: the network alone in flexible PCG, hybrid: "
+                    f"a V-cycle corrected by the network, both with the multigrid fallback after 5 stalled "
+                    f"iterations); same_system: the system of step {K + 1} "
                     f"solved from p = 0 by both (the reference's matrix; the second solve's device time, without the host copies)"}
     del s
     torch.cuda.empty_cache()

@@ -14,7 +14,7 @@ import numpy as np
 FS_OK = 0
 FS_SOLID, FS_FLUID, FS_AIR, FS_BODY = 0, 1, 2, 3
 FS_FMT_PLY, FS_FMT_OFF, FS_FMT_PLY_BINARY = 0, 1, 2
-FS_PRECOND_MG, FS_PRECOND_LEARNED, FS_PRECOND_JACOBI = 0, 1, 2
+FS_PRECOND_MG, FS_PRECOND_LEARNED, FS_PRECOND_JACOBI, FS_PRECOND_HYBRID = 0, 1, 2, 3
 FS_SOLVER_NO_GRAPH = 1
 FS_SOLVER_NO_TAIL = 2
 FS_SOLVER_NO_FUSE = 4
@@ -115,6 +115,7 @@ _LEARNED_SIGS = {
     "fs_learned_train_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_float,
                                         C.POINTER(C.c_double)]),
     "fs_set_preconditioner": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "fs_mg_vcycle": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, C.c_int]),
 }
 LEARNED_SYMBOLS = tuple(_LEARNED_SIGS)
 
@@ -355,6 +356,16 @@ class Solver:
     def set_preconditioner(self, precond, stall_window=0):
         self._need_learned()
         self.fl.check(self.lib.fs_set_preconditioner(self.h, int(precond), int(stall_window)))
+
+    def mg_vcycle(self, labels, r):
+        """One multigrid V-cycle z = V(r) from zero (fs_mg_vcycle)."""
+        self._need_learned()
+        lab = np.ascontiguousarray(labels, dtype=np.uint8)
+        rr = np.ascontiguousarray(r, dtype=np.float32)
+        D, H, W = lab.shape
+        z = np.zeros((D, H, W), np.float32)
+        self.fl.check(self.lib.fs_mg_vcycle(self.h, _ptr(lab), _ptr(rr), _ptr(z), W, H, D))
+        return z
 
     def learned_init(self, seed):
         self._need_learned()

@@ -326,6 +326,7 @@ struct Solver {
     std::unique_ptr<Preconditioner> jacobi;
     std::unique_ptr<LNet> net;             // learned preconditioner (created on first use)
     std::unique_ptr<LearnedPre> learned;
+    std::unique_ptr<HybridPre> hybrid;
     fs_solve_stats stats{};
     double times[FS_NUM_STAGES] = {0};
     float dt_cur = 0.0f;        // dt of the current step
@@ -604,6 +605,14 @@ struct Solver {
                 learned.reset(new LearnedPre(&mg, n));
             }
             pre = learned.get();
+        }
+        if (cfg.precond == FS_PRECOND_HYBRID) {
+            if (!hybrid) {
+                LNet* n = get_net();
+                if (!n || n->ensure(W, H, D) != 0) return fail(FS_ERR_DEVICE, "learned preconditioner allocation failed");
+                hybrid.reset(new HybridPre(&mg, n));
+            }
+            pre = hybrid.get();
         }
         mg.stall_window = cfg.stall_window > 0 ? cfg.stall_window : 5;
         mg.use_graph = !(cfg.solver_flags & FS_SOLVER_NO_GRAPH);
@@ -1562,10 +1571,33 @@ int fs_learned_train_step(void* h, const uint8_t* labels, const float* b, const 
     });
 }
 
+int fs_mg_vcycle(void* h, const uint8_t* labels, const float* r, float* z, int W, int H, int D) {
+    return guarded([&]() -> int {
+        Solver& s = *(Solver*)h;
+        if (!labels || !r || !z || W <= 0 || H <= 0 || D <= 0) return fail(FS_ERR_ARG, "bad system");
+        if (s.W != W || s.H != H || s.D != D) {
+            std::vector<uint8_t> L(labels, labels + (size_t)W * H * D);
+            int rc = upload_scene(s, W, H, D, L);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
+        }
+        const size_t C = (size_t)W * H * D;
+        HIPCHK(hipMemcpy(s.d_rhs, r, sizeof(float) * C, hipMemcpyHostToDevice));
+        // the full-grid float pressure buffer receives z
+        int rc = s.mg.vcycle_system(s.d_labels, s.d_rhs, s.d_p, s.st);
+        s.p64_from_step = false;
+        if (rc == -5) return fail(FS_ERR_DEVICE, std::string("V-cycle: ") + hipGetErrorString(hipGetLastError()));
+        if (rc == -3) return fail(FS_ERR_NO_FLUID, "No fluid cells found!");
+        HIPCHK(hipMemcpy(z, s.d_p, sizeof(float) * C, hipMemcpyDeviceToHost));
+        return FS_OK;
+    });
+}
+
 int fs_set_preconditioner(void* h, int precond, int stall_window) {
     return guarded([&]() -> int {
         Solver& s = *(Solver*)h;
-        if (precond < FS_PRECOND_MG || precond > FS_PRECOND_JACOBI) return fail(FS_ERR_ARG, "unknown preconditioner");
+        if (precond < FS_PRECOND_MG || precond > FS_PRECOND_HYBRID) return fail(FS_ERR_ARG, "unknown preconditioner");
         s.cfg.precond = precond;
         if (stall_window > 0) s.cfg.stall_window = stall_window;
         return FS_OK;

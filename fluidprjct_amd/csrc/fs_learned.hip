@@ -222,10 +222,13 @@ __global__ __launch_bounds__(256) void k_layerA(LGrid g, const float* __restrict
 // when sc is given, else scale_host; the scale is kept for layer E
 __global__ __launch_bounds__(256) void k_normalise(long long n, const float* __restrict__ r,
                                                    const SolveScalars* __restrict__ sc, float scale_host,
+                                                   const float* __restrict__ scale_dev,
                                                    float* __restrict__ rn, float* __restrict__ scale_out) {
     if (sc && sc->done) return;
     float s = scale_host;
-    if (sc) {
+    if (scale_dev) {
+        s = scale_dev[0];
+    } else if (sc) {
         double v = sc->rr / (double)(sc->n_unknown > 0 ? sc->n_unknown : 1);
         s = v > 0.0 ? (float)sqrt(v) : 1.0f;
     }
@@ -572,12 +575,13 @@ int LNet::ensure(int W, int H, int D) {
 }
 
 void LNet::forward(const uint8_t* type, const float* r, float* z, const SolveScalars* sc, float scale_host, float* zn,
-                   hipStream_t st) {
+                   hipStream_t st, const float* scale_dev) {
     const int* done = sc ? &sc->done : nullptr;
     const float* nf = nullptr;
     const uint8_t* nt = nullptr;
     float* nz = nullptr;
-    hipLaunchKernelGGL(k_normalise, dim3(nblk(g0.n)), dim3(256), 0, st, g0.n, r, sc, scale_host, rn, d_scale);
+    hipLaunchKernelGGL(k_normalise, dim3(nblk(g0.n)), dim3(256), 0, st, g0.n, r, sc, scale_host, scale_dev, rn,
+                       d_scale);
     hipLaunchKernelGGL(k_layerA, dim3(ntiles(g0)), dim3(256), 0, st, g0, rn, type, d_params, a0, done);
     hipLaunchKernelGGL(k_pool16, dim3(nblk(g1.n * 16)), dim3(256), 0, st, g0, g1, a0, a1, done);
     hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g1)), dim3(256), 0, st, g1, a1, wB, d_params + LN_BB, 1, b1, nz, nz,
@@ -643,6 +647,44 @@ int LearnedPre::prepare_box() {
 void LearnedPre::apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) {
     MGLevel L0 = s_->fine();
     net_->forward(L0.type, r, z, sc, 1.0f, nullptr, st);
+    s_->dot_rz_zq(r, z, partial, sc, st);
+}
+
+HybridPre::~HybridPre() {
+    for (void* p : {(void*)r1_, (void*)z2_, (void*)scale_, (void*)part_})
+        if (p) hipFree(p);
+}
+
+int HybridPre::prepare_box() {
+    MGLevel L0 = s_->fine();
+    if (net_->ensure(L0.W, L0.H, L0.D) != 0) return -1;
+    if (L0.n <= cap_ && part_) return 0;
+    for (void* p : {(void*)r1_, (void*)z2_, (void*)scale_, (void*)part_})
+        if (p) hipFree(p);
+    r1_ = z2_ = scale_ = nullptr;
+    part_ = nullptr;
+    bool ok = hipMalloc((void**)&r1_, sizeof(float) * L0.n) == hipSuccess &&
+              hipMalloc((void**)&z2_, sizeof(float) * L0.n) == hipSuccess &&
+              hipMalloc((void**)&scale_, sizeof(float) * 2) == hipSuccess &&
+              hipMalloc((void**)&part_, sizeof(double) * 4 * 2048) == hipSuccess;
+    if (!ok) return -1;
+    cap_ = L0.n;
+    ++epoch_;
+    return 0;
+}
+
+void HybridPre::apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) {
+    MGLevel L0 = s_->fine();
+    double* p_scratch = part_;
+    double* p_rr = part_ + 2048;
+    double* p_num = part_ + 2 * 2048;
+    double* p_den = part_ + 3 * 2048;
+    s_->vcycle(r, z, p_scratch, sc, st);                  // z1 = V(r)
+    s_->resid_sq(r, z, r1_, p_rr, sc, st);                // r1 = r - A z1
+    s_->partial_rms(p_rr, scale_, sc, st);
+    net_->forward(L0.type, r1_, z2_, sc, 1.0f, nullptr, st, scale_);   // z2 = N(r1)
+    s_->line_dots(z2_, r1_, p_num, p_den, sc, st);
+    s_->add_scaled(z, z2_, p_num, p_den, sc, st);         // z = z1 + c z2
     s_->dot_rz_zq(r, z, partial, sc, st);
 }
 

@@ -35,8 +35,9 @@ public:
     // scale = rms(r) over the unknowns, sqrt(sc->rr / sc->n_unknown) when sc is
     // given (every kernel then returns at once when sc->done is set), else
     // scale_host.  zn (optional): the unscaled output (training).
+    // scale_dev (optional): the scale as a device float, in place of the two above.
     void forward(const uint8_t* type, const float* r, float* z, const SolveScalars* sc, float scale_host, float* zn,
-                 hipStream_t st);
+                 hipStream_t st, const float* scale_dev = nullptr);
     // gradient of the A-norm loss 0.5 (zn - target).A(zn - target) / n_fluid into
     // d_grad (zeroed first); the forward pass must have run with zn.  The loss
     // -> d_loss[0].
@@ -80,6 +81,29 @@ public:
 private:
     MGPCG* s_;
     LNet* net_;
+};
+
+// Hybrid preconditioner on the fine level of an MGPCG box (flexible PCG):
+// z1 = V(r) (one multigrid V-cycle), r1 = r - A z1, z2 = N(r1) (the network on
+// the residual the V-cycle leaves), z = z1 + c z2 with the exact line search
+// c = (z2.r1) / (z2.A z2), which minimises the A-norm of the error of z1 + c z2
+// as an approximation of A^-1 r.  The network is trained on pairs (r1, A^-1 r1)
+// of V-cycle residuals.
+class HybridPre : public Preconditioner {
+public:
+    HybridPre(MGPCG* s, LNet* net) : s_(s), net_(net) {}
+    ~HybridPre();
+    void apply(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st) override;
+    bool flexible() const override { return true; }
+    int prepare_box() override;
+    long long graph_epoch() const override { return net_->epoch * 1000003LL + epoch_; }
+
+private:
+    MGPCG* s_;
+    LNet* net_;
+    float *r1_ = nullptr, *z2_ = nullptr, *scale_ = nullptr;
+    double* part_ = nullptr;   // 4 partial-sum arrays of RED_BLOCKS doubles
+    long long cap_ = 0, epoch_ = 0;
 };
 
 }  // namespace fsk

@@ -2260,6 +2260,117 @@ void MGPCG::dot_rz_zq(const float* rr, const float* zz, double* pdot, SolveScala
     hipLaunchKernelGGL(k_dot_rz_zq, dim3(nred), dim3(256), 0, st, n0, rr, zz, (const float*)q, pdot, slot(PS_ZQ), s);
 }
 
+// ---------------------------------------------------------------- hybrid preconditioner pieces
+__global__ __launch_bounds__(256) void k_resid_sq(MGLevel L, const float* __restrict__ r, const float* __restrict__ z,
+                                                  float* __restrict__ r1, double* __restrict__ p_rr,
+                                                  const SolveScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    __shared__ double sm[12];
+    const long long WH = (long long)L.W * L.H;
+    double acc = 0.0;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const uint32_t code = L.code[c];
+        const float v = (code & SELF) ? r[c] - stencil_Ax_L(L, z, c, code, WH) : 0.0f;
+        r1[c] = v;
+        acc += (double)v * (double)v;
+    }
+    store_partial(acc, sm, p_rr);
+}
+
+__global__ __launch_bounds__(256) void k_partial_rms(const double* __restrict__ p_rr, int np, float* __restrict__ scale,
+                                                     const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[12];
+    double o[1];
+    sum_partials_k<1>({p_rr}, np, sm, o);
+    if (threadIdx.x != 0) return;
+    const double v = o[0] / (double)(sc->n_unknown > 0 ? sc->n_unknown : 1);
+    float s = v > 0.0 ? (float)sqrt(v) : 1.0f;
+    if (!(s > 0.0f) || !isfinite(s)) s = 1.0f;
+    scale[0] = s;
+}
+
+__global__ __launch_bounds__(256) void k_line_dots(MGLevel L, const float* __restrict__ z2, const float* __restrict__ r1,
+                                                   double* __restrict__ p_num, double* __restrict__ p_den,
+                                                   const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[12];
+    const long long WH = (long long)L.W * L.H;
+    double a = 0.0, b = 0.0;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < L.n;
+         c += (long long)gridDim.x * blockDim.x) {
+        const uint32_t code = L.code[c];
+        if (!(code & SELF)) continue;
+        const double zc = z2[c];
+        a += zc * (double)r1[c];
+        b += zc * (double)stencil_Ax_L(L, z2, c, code, WH);
+    }
+    store_partial(a, sm, p_num);
+    store_partial(b, sm, p_den);
+}
+
+__global__ __launch_bounds__(256) void k_add_scaled(long long n, float* __restrict__ z, const float* __restrict__ z2,
+                                                    const double* __restrict__ p_num, const double* __restrict__ p_den,
+                                                    int np, const SolveScalars* __restrict__ sc) {
+    if (sc->done) return;
+    __shared__ double sm[12];
+    double o[2];
+    sum_partials_k<2>({p_num, p_den}, np, sm, o);
+    const float cf = o[1] > 0.0 ? (float)(o[0] / o[1]) : 0.0f;
+    for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < n;
+         c += (long long)gridDim.x * blockDim.x)
+        z[c] = fmaf(cf, z2[c], z[c]);
+}
+
+__global__ __launch_bounds__(256) void k_box_to_full(long long n, int W, int H, MGLevel L, int ox, int oy, int oz,
+                                                     const float* __restrict__ zb, float* __restrict__ z) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int i, j, k;
+    split3(c, W, H, i, j, k);
+    const int bi = i - ox, bj = j - oy, bk = k - oz;
+    float v = 0.0f;
+    if (bi >= 0 && bi < L.W && bj >= 0 && bj < L.H && bk >= 0 && bk < L.D) {
+        const long long q = (long long)bi + (long long)bj * L.W + (long long)bk * L.W * L.H;
+        if (L.code[q] & SELF) v = zb[q];
+    }
+    z[c] = v;
+}
+
+void MGPCG::resid_sq(const float* rr, const float* zz, float* r1, double* p_rr, SolveScalars* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_resid_sq, dim3(nred), dim3(256), 0, st, lv[0], rr, zz, r1, p_rr, (const SolveScalars*)s);
+}
+
+void MGPCG::partial_rms(const double* p_rr, float* scale, SolveScalars* s, hipStream_t st) {
+    hipLaunchKernelGGL(k_partial_rms, dim3(1), dim3(256), 0, st, p_rr, nred, scale, (const SolveScalars*)s);
+}
+
+void MGPCG::line_dots(const float* z2, const float* r1, double* p_num, double* p_den, SolveScalars* s,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_line_dots, dim3(nred), dim3(256), 0, st, lv[0], z2, r1, p_num, p_den, (const SolveScalars*)s);
+}
+
+void MGPCG::add_scaled(float* zz, const float* z2, const double* p_num, const double* p_den, SolveScalars* s,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(k_add_scaled, dim3(nred), dim3(256), 0, st, n0, zz, z2, p_num, p_den, nred,
+                       (const SolveScalars*)s);
+}
+
+int MGPCG::vcycle_system(const uint8_t* labels, const float* b, float* zfull, hipStream_t st) {
+    face = fsd::FaceFrac{};
+    int rc = prepare(labels, b, st);
+    if (rc) {
+        if (rc == -3) hipMemsetAsync(zfull, 0, sizeof(float) * (size_t)N, st);
+        return rc;
+    }
+    hipMemsetAsync(sc, 0, sizeof(SolveScalars), st);
+    vcycle(bb, z, slot(PS_RES), sc, st);
+    hipLaunchKernelGGL(k_box_to_full, dim3(nblk(N)), dim3(256), 0, st, N, W, H, lv[0], box[0], box[1], box[2],
+                       (const float*)z, zfull);
+    return hipStreamSynchronize(st) == hipSuccess ? 0 : -5;
+}
+
 class JacobiPre : public Preconditioner {
 public:
     explicit JacobiPre(MGPCG* s) : s_(s) {}

@@ -99,6 +99,23 @@ public:
     void vcycle(const float* r, float* z, double* partial, SolveScalars* sc, hipStream_t st);
     // y = A x on the fine grid (FP32), for tests and the learned path.
     void apply_A(const float* x, float* y, hipStream_t st);
+    // Pieces of the hybrid preconditioner z = V(r) + c N(r - A V(r)) over the
+    // solve box (N the learned network, c the exact line search
+    // (z2.r1) / (z2.A z2) along its output z2):
+    // r1 = r - A z on the unknowns (0 elsewhere) and partial sums of r1.r1
+    void resid_sq(const float* r, const float* z, float* r1, double* p_rr, SolveScalars* sc, hipStream_t st);
+    // scale = sqrt(sum p_rr / n_unknown), 1 when that is not a positive finite number
+    void partial_rms(const double* p_rr, float* scale, SolveScalars* sc, hipStream_t st);
+    // partial sums of z2.r1 and z2.(A z2)
+    void line_dots(const float* z2, const float* r1, double* p_num, double* p_den, SolveScalars* sc, hipStream_t st);
+    // z += c z2 with c = sum p_num / sum p_den (0 unless the denominator is positive)
+    void add_scaled(float* z, const float* z2, const double* p_num, const double* p_den, SolveScalars* sc,
+                    hipStream_t st);
+    // One V-cycle from zero for the system (labels, b) of the full grid (the
+    // reference's matrix), z on the full grid (0 off the unknowns).  Returns -3
+    // without fluid, -5 on a device error.
+    int vcycle_system(const uint8_t* labels, const float* b, float* zfull, hipStream_t st);
+    int red_blocks() const { return nred; }
 
     int W = 0, H = 0, D = 0;
     long long N = 0;               // full grid cells

@@ -14,6 +14,14 @@ learns the residuals it meets inside flexible PCG.
 
     python -m fluidprjct_amd.learned --pairs 2000 --grid 64 --iters 3000 \
         --out fluidprjct_amd/learned_params.npy
+
+The hybrid preconditioner (FS_PRECOND_HYBRID: a multigrid V-cycle corrected by
+the network applied to the residual the V-cycle leaves) trains the same
+network on pairs (r1 = b - A V(b), e1 = A^-1 b - V(b)), then on rounds of the
+V-cycle residuals that flexible PCG with the hybrid preconditioner meets:
+
+    python -m fluidprjct_amd.learned --hybrid --pairs 800 --grid 64 --iters 4000 \
+        --rounds 3 --round-pairs 200 --out fluidprjct_amd/hybrid_params.npy
 """
 from __future__ import annotations
 
@@ -28,6 +36,7 @@ from ._abi import FS_GRID_LABELS, FS_GRID_P64, FS_GRID_RHS, FS_PRECOND_LEARNED, 
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_PARAMS = os.path.join(HERE, "learned_params.npy")
+HYBRID_PARAMS = os.path.join(HERE, "hybrid_params.npy")
 
 
 def load_params(path: str = DEFAULT_PARAMS) -> np.ndarray:
@@ -126,6 +135,61 @@ def residual_pairs(solver, pairs, n, seed=0):
     return out
 
 
+def vcycle_pairs(solver, pairs):
+    """(labels, r1, e1) for the hybrid preconditioner: the residual r1 = b - A V(b)
+    that one multigrid V-cycle from zero leaves, and its error e1 = p - V(b)."""
+    out = []
+    for lab, b, p in pairs:
+        z = solver.mg_vcycle(lab, b).astype(np.float64)
+        f = lab == 1
+        r1 = np.where(f, b - apply_A(lab, z), 0.0)
+        e1 = np.where(f, p.astype(np.float64) - z, 0.0)
+        if np.abs(r1).max() > 0:
+            out.append((lab, r1.astype(np.float32), e1.astype(np.float32)))
+    return out
+
+
+def hybrid_direction(solver, lab, r):
+    """The hybrid preconditioner's z for r on the host (the same steps as the
+    device path): z1 = V(r), r1 = r - A z1, z2 = N(r1), z = z1 + c z2 with the
+    line search c = (z2.r1) / (z2.A z2); returns (z, r1)."""
+    z1 = solver.mg_vcycle(lab, r).astype(np.float64)
+    r1 = np.where(lab == 1, r - apply_A(lab, z1), 0.0)
+    z2 = solver.learned_apply(lab, r1.astype(np.float32)).astype(np.float64)
+    den = float((z2 * apply_A(lab, z2)).sum())
+    c = float((z2 * r1).sum()) / den if den > 0 else 0.0
+    return z1 + c * z2, r1
+
+
+def hybrid_inloop_pairs(solver, exact, systems, kmax: int = 12, tol: float = 1e-6):
+    """(labels, r1, e1) for the residuals that flexible PCG with the hybrid
+    preconditioner meets: the iteration runs on the host (beta = -z.q_old /
+    p_old.q_old, alpha = r.p / p.q), and at every iteration the V-cycle's
+    residual r1 of the current r is paired with its error e1 = A^-1 r1 (a
+    tight multigrid solve on `exact`, another handle).  Returns the pairs and
+    the mean iteration count."""
+    out, its = [], []
+    for lab, b, _ in systems:
+        f = lab == 1
+        b = np.where(f, b.astype(np.float64), 0.0)
+        bn = np.linalg.norm(b)
+        r = b.copy()
+        p_old = q_old = None
+        k = 0
+        for k in range(kmax):
+            z, r1 = hybrid_direction(solver, lab, r.astype(np.float32))
+            e1, _ = exact.solve_pressure_system(lab, r1.astype(np.float32), 1e-8)
+            out.append((lab, r1.astype(np.float32), np.where(f, e1, 0.0).astype(np.float32)))
+            p = z if p_old is None else z - (float((z * q_old).sum()) / float((p_old * q_old).sum())) * p_old
+            q = apply_A(lab, p)
+            r -= (float((r * p).sum()) / float((p * q).sum())) * q
+            if np.linalg.norm(r) <= tol * bn:
+                break
+            p_old, q_old = p, q
+        its.append(k + 1)
+    return out, float(np.mean(its)) if its else 0.0
+
+
 def train(solver, pairs, iters: int = 3000, lr: float = 2e-3, seed: int = 0, log=print, log_every: int = 250):
     """Adam with cosine decay over random pairs; returns the loss history."""
     rng = np.random.default_rng(seed)
@@ -152,7 +216,12 @@ def main():
     ap.add_argument("--init", default="", help="start from these parameters instead of a random draw")
     ap.add_argument("--rounds", type=int, default=3, help="rounds of residual pairs after the first training")
     ap.add_argument("--round-pairs", type=int, default=300)
+    ap.add_argument("--hybrid", action="store_true",
+                    help="train the correction network of FS_PRECOND_HYBRID: V-cycle residual pairs, then rounds "
+                         "of the residuals that flexible PCG with the hybrid preconditioner meets")
     args = ap.parse_args()
+    if args.hybrid:
+        return main_hybrid(args)
     from . import load_library
     lib = load_library()
     pairs = generate_pairs(lib, args.pairs, args.grid, args.seed)
@@ -170,6 +239,30 @@ def main():
     np.save(args.out, s.learned_params())
     print(f"[learned] saved {args.out}; loss first 100 {hist[:100].mean():.5f} last 100 {hist[-100:].mean():.5f}",
           flush=True)
+
+
+def main_hybrid(args):
+    from . import load_library
+    lib = load_library()
+    N = args.grid
+    pairs = generate_pairs(lib, args.pairs, N, args.seed)
+    s = lib.solver(dx=1.0 / N, seed=args.seed)
+    exact = lib.solver(dx=1.0 / N, seed=args.seed + 1)
+    if args.init:
+        s.set_learned_params(load_params(args.init))
+    else:
+        s.learned_init(args.seed)
+    train_set = vcycle_pairs(s, pairs)
+    hist = train(s, train_set, args.iters, args.lr)
+    rng = np.random.default_rng(args.seed)
+    for rd in range(args.rounds):
+        idx = rng.choice(len(pairs), size=min(args.round_pairs, len(pairs)), replace=False)
+        new, mean_it = hybrid_inloop_pairs(s, exact, [pairs[int(i)] for i in idx])
+        train_set += new
+        print(f"[learned] round {rd + 1}: {len(new)} in-loop pairs (host FPCG {mean_it:.1f} iterations)", flush=True)
+        hist = train(s, train_set, args.iters, args.lr * 0.5, seed=rd + 1)
+    np.save(args.out, s.learned_params())
+    print(f"[learned] saved {args.out}; loss last 100 {hist[-100:].mean():.5f}", flush=True)
 
 
 if __name__ == "__main__":

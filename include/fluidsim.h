@@ -52,6 +52,7 @@ extern "C" {
 #define FS_PRECOND_MG       0   /* multigrid V-cycle inside PCG (default)            */
 #define FS_PRECOND_LEARNED  1   /* learned conv net inside flexible PCG, MG fallback  */
 #define FS_PRECOND_JACOBI   2   /* diagonal; plain PCG                                */
+#define FS_PRECOND_HYBRID   3   /* V-cycle + learned correction, flexible PCG, MG fallback */
 
 /* grids readable / writable through fs_get_grid / fs_set_grid */
 #define FS_GRID_U        0   /* float (W+1)*H*D  */

@@ -55,8 +55,18 @@ int fs_learned_loss_grad(void* s, const uint8_t* labels, const float* b, const d
 int fs_learned_train_step(void* s, const uint8_t* labels, const float* b, const double* p, int W, int H, int D,
                           float lr, double* loss);
 
+/* One multigrid V-cycle z = V(r) from zero for the pressure system of the
+ * labels (the reference's matrix; the plain solver's preconditioner), z on the
+ * W x H x D grid (0 off the unknowns).  With r = b it gives the residual
+ * b - A V(b) and the error A^-1 b - V(b) that FS_PRECOND_HYBRID's network is
+ * trained on.  The handle's pressure grid is overwritten. */
+int fs_mg_vcycle(void* s, const uint8_t* labels, const float* r, float* z, int W, int H, int D);
+
 /* Select the pressure preconditioner of later solves (FS_PRECOND_*) and, when
- * stall_window > 0, the learned path's stall window. */
+ * stall_window > 0, the learned path's stall window.  FS_PRECOND_HYBRID:
+ * z = V(r) + c N(r - A V(r)), a V-cycle corrected by the network applied to
+ * the residual it leaves, c = (z2.r1) / (z2.A z2) the exact line search along
+ * the network's output z2 (flexible PCG; multigrid fallback on a stall). */
 int fs_set_preconditioner(void* s, int precond, int stall_window);
 
 #ifdef __cplusplus
