@@ -2155,9 +2155,13 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     }
     const unsigned nb = nblk(L.n);
     const unsigned ng = (l == 0) ? (unsigned)nred : nb;   // grid of the r.z-accumulating sweep
+    // a fine level above the tail's size: its own sweep count and polynomial
+    const bool fine = l == 0 && L.n > TAIL_MAX && nu_pre == SMOOTH_STEPS && nu_post == SMOOTH_STEPS;
+    const float* om = fine ? fine_omega : sweep_omega;
+    const int msteps = fine ? NU_FINE : SMOOTH_STEPS;
     // Buffer writes after the first: the extra pre sweeps plus the post sweeps;
     // the first buffer is chosen so that the last sweep lands in L.x.
-    const int npre = std::max(nu_pre, 1), npost = std::max(nu_post, 1);
+    const int npre = fine ? NU_FINE : std::max(nu_pre, 1), npost = fine ? NU_FINE : std::max(nu_post, 1);
     // the fused prolongation+sweep writes the other buffer; the plain
     // prolongation adds in place, so either way npost sweeps follow the pre-smoothing
     const int extra = (npre >= 2 ? npre - 2 : 0) + npost;
@@ -2169,17 +2173,17 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     const bool v4 = use_vec && L.W % 4 == 0;
     const unsigned nb4 = nblk(L.n / 4);
     if (v4)
-        hipLaunchKernelGGL(k_jacobi2_v4, dim3(nb4), dim3(256), 0, st, L, L.b, cur, sweep_omega[0],
-                           npre >= 2 ? sweep_omega[1] : 0.0f, s);
+        hipLaunchKernelGGL(k_jacobi2_v4, dim3(nb4), dim3(256), 0, st, L, L.b, cur, om[0],
+                           npre >= 2 ? om[1 % msteps] : 0.0f, s);
     else
-        hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, sweep_omega[0],
-                           npre >= 2 ? sweep_omega[1] : 0.0f, s);
+        hipLaunchKernelGGL(k_jacobi2, dim3(nb), dim3(256), 0, st, L, L.b, cur, om[0],
+                           npre >= 2 ? om[1 % msteps] : 0.0f, s);
     for (int k = 2; k < npre; ++k) {
         if (v4)
-            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
+            hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth, om[k % msteps],
                                (double*)nullptr, s);
         else
-            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, om[k % msteps],
                                (double*)nullptr, s);
         std::swap(cur, oth);
     }
@@ -2202,10 +2206,10 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     int k0;
     if (fuse) {
         if (pdot && npost == 1)
-            hipLaunchKernelGGL(k_prolong_smooth<true>, dim3(ng), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, sweep_omega[0],
+            hipLaunchKernelGGL(k_prolong_smooth<true>, dim3(ng), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, om[0],
                                pdot, s);
         else
-            hipLaunchKernelGGL(k_prolong_smooth<false>, dim3(nb), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, sweep_omega[0],
+            hipLaunchKernelGGL(k_prolong_smooth<false>, dim3(nb), dim3(256), 0, st, L, Cn, L.b, cur, xc, oth, om[0],
                                (double*)nullptr, s);
         std::swap(cur, oth);
         k0 = 1;
@@ -2221,16 +2225,16 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
         const bool dot = k == npost - 1 && pdot;
         if (v4) {
             if (dot)
-                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
+                hipLaunchKernelGGL(k_jacobi_v4<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, om[k % msteps],
                                    pdot, s);
             else
                 hipLaunchKernelGGL(k_jacobi_v4<false>, dim3(nb4), dim3(256), 0, st, L, L.b, cur, oth,
-                                   sweep_omega[k % SMOOTH_STEPS], (double*)nullptr, s);
+                                   om[k % msteps], (double*)nullptr, s);
         } else if (dot) {
-            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS], pdot,
+            hipLaunchKernelGGL(k_jacobi<true>, dim3(ng), dim3(256), 0, st, L, L.b, cur, oth, om[k % msteps], pdot,
                                s);
         } else {
-            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, sweep_omega[k % SMOOTH_STEPS],
+            hipLaunchKernelGGL(k_jacobi<false>, dim3(nb), dim3(256), 0, st, L, L.b, cur, oth, om[k % msteps],
                                (double*)nullptr, s);
         }
         std::swap(cur, oth);

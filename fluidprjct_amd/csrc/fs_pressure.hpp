@@ -138,6 +138,16 @@ public:
     // iterations, within 3% in time); V(2,2), V(5,5) and other intervals
     // were slower.  Equal steps give damped Jacobi.
     float sweep_omega[SMOOTH_STEPS] = {0.5192694f, 0.7152481f, 1.5340161f, 8.0503983f};
+    // A fine level larger than the single-workgroup tail smooths with a
+    // degree-2 Chebyshev polynomial on [b/10, b] (V(2,2)): its sweeps are the
+    // expensive HBM passes, the coarse levels' sweeps are cheap.  Measured over
+    // steps 61-90 of the 256^3 sphere drop (face fractions): V(4,4) everywhere
+    // 12.0 iterations, 546 us each, 6.54 ms per solve; fine V(2,2) on
+    // [b/10, b] 13.2 iterations, 432 us, 5.71 ms; on [b/20, b] 13.4, 5.75 ms;
+    // [b/7, b] 13.7, 5.90 ms; [b/5, b] 14.0, 5.96 ms; fine V(3,3) on [b/40, b]
+    // 12.6, 6.16 ms; fine V(1,1) 17.2 iterations, 7.11 ms.
+    static constexpr int NU_FINE = 2;
+    float fine_omega[NU_FINE] = {0.57590544f, 2.157014f};
     int batch = 8;              // inner iterations launched between host checks (even)
     double inner_rtol = 0.0;    // inner relative reduction per refinement pass (0: straight to the
                                 // tolerance; FP64 A p sums and solution keep r with the true residual)
