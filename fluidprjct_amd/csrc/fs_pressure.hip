@@ -2155,8 +2155,9 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
     }
     const unsigned nb = nblk(L.n);
     const unsigned ng = (l == 0) ? (unsigned)nred : nb;   // grid of the r.z-accumulating sweep
-    // a fine level above the tail's size: its own sweep count and polynomial
-    const bool fine = l == 0 && L.n > TAIL_MAX && nu_pre == SMOOTH_STEPS && nu_post == SMOOTH_STEPS;
+    // a fine level whose sweeps are bandwidth-bound passes (above the fused
+    // transfer kernels' size): its own sweep count and polynomial
+    const bool fine = l == 0 && L.n > FUSE_MAX && nu_pre == SMOOTH_STEPS && nu_post == SMOOTH_STEPS;
     const float* om = fine ? fine_omega : sweep_omega;
     const int msteps = fine ? NU_FINE : SMOOTH_STEPS;
     // Buffer writes after the first: the extra pre sweeps plus the post sweeps;

@@ -138,9 +138,11 @@ public:
     // iterations, within 3% in time); V(2,2), V(5,5) and other intervals
     // were slower.  Equal steps give damped Jacobi.
     float sweep_omega[SMOOTH_STEPS] = {0.5192694f, 0.7152481f, 1.5340161f, 8.0503983f};
-    // A fine level larger than the single-workgroup tail smooths with a
-    // degree-2 Chebyshev polynomial on [b/10, b] (V(2,2)): its sweeps are the
-    // expensive HBM passes, the coarse levels' sweeps are cheap.  Measured over
+    // A fine level of more than 2^16 cells smooths with a degree-2 Chebyshev
+    // polynomial on [b/10, b] (V(2,2)): its sweeps are the expensive HBM
+    // passes, the coarse levels' sweeps are cheap (on a smaller, launch-bound
+    // fine level fewer sweeps only add iterations: the 128^2 slab took 19
+    // instead of 16 per solve and 1.27 instead of 1.17 ms).  Measured over
     // steps 61-90 of the 256^3 sphere drop (face fractions): V(4,4) everywhere
     // 12.0 iterations, 546 us each, 6.54 ms per solve; fine V(2,2) on
     // [b/10, b] 13.2 iterations, 432 us, 5.71 ms; on [b/20, b] 13.4, 5.75 ms;
