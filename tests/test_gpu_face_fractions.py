@@ -27,9 +27,10 @@ VEL = (FS_GRID_U, FS_GRID_V, FS_GRID_W)
 
 
 def body_scene(lib, N, body, level=0.6, angles=(0.2, -0.3, 0.4), vel=(0.3, -0.5, 0.2), omega=(1.0, -2.0, 0.5),
-               tol=1e-8):
+               tol=1e-8, solver_flags=0):
     dx = 1.0 / N
-    s = lib.solver(dx=dx, dt=1.0 / 120, pic_weight=0.05, solver_tol=tol, step_flags=FS_STEP_FACE_FRACTIONS)
+    s = lib.solver(dx=dx, dt=1.0 / 120, pic_weight=0.05, solver_tol=tol, step_flags=FS_STEP_FACE_FRACTIONS,
+                   solver_flags=solver_flags)
     s.set_labels(scenes.water_column(N, level))
     if body == "sphere":
         r = 0.17
@@ -105,6 +106,30 @@ def test_pressure_with_face_fractions_vs_oracle(hip, oracle, level):
     (ph, sh), (po, so) = res
     assert sh.converged == 1 and sh.singular == so.singular == (1 if level == 1.0 else 0)
     assert np.linalg.norm(ph - po) / np.linalg.norm(po) <= 1e-6, np.linalg.norm(ph - po) / np.linalg.norm(po)
+
+
+def test_blocked_smoothing_bitexact_with_face_fractions(hip):
+    """The temporally blocked smoothing of the small coarse levels against one
+    launch per sweep (FS_SOLVER_NO_FUSE) with the fraction-weighted fine
+    level of a 72^3 tank (about 230K cells; the sphere gives it partly open
+    faces): the same solution bits and iteration count."""
+    from fluidprjct_amd._abi import FS_SOLVER_NO_FUSE
+    N = 72
+    res = []
+    for flags in (0, FS_SOLVER_NO_FUSE):
+        s = body_scene(hip, N, "sphere", tol=1e-8, solver_flags=flags)
+        lab = s.get_grid(FS_GRID_LABELS)
+        b = (np.random.default_rng(29).standard_normal(lab.shape) * 1e3).astype(np.float32) * (lab == 1)
+        s.set_grid(FS_GRID_RHS, b)
+        s.stage("pressure")
+        res.append((s.get_grid(FS_GRID_P64), s.solve_stats()))
+        partial = sum(int(((g > 0) & (g < 1)).sum()) for g in (s.get_grid(o) for o in OPEN))
+        assert partial > 500, partial
+        s.close()
+    (pf, sf), (pn, sn) = res
+    assert sf.converged == 1 and sn.converged == 1
+    assert sf.iterations == sn.iterations
+    assert np.array_equal(pf.view(np.uint64), pn.view(np.uint64))
 
 
 def test_falling_sphere_with_face_fractions(hip, oracle):
