@@ -39,31 +39,58 @@ FS_HD uint16_t f2bf(float f) {
 }
 FS_DEV float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-FS_DEV void tile_origin(const LGrid& g, int& x0, int& y0, int& z0) {
+FS_DEV void tile_origin_of(const LGrid& g, int b, int& x0, int& y0, int& z0) {
     const int ntx = (g.W + TX - 1) / TX, nty = (g.H + TY - 1) / TY;
-    const int b = blockIdx.x;
     x0 = (b % ntx) * TX;
     y0 = ((b / ntx) % nty) * TY;
     z0 = (b / (ntx * nty)) * TZ;
 }
+FS_DEV void tile_origin(const LGrid& g, int& x0, int& y0, int& z0) { tile_origin_of(g, blockIdx.x, x0, y0, z0); }
 static unsigned ntiles(const LGrid& g) {
     return (unsigned)(((g.W + TX - 1) / TX) * ((g.H + TY - 1) / TY) * ((g.D + TZ - 1) / TZ));
 }
 static unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
-// stage the 16-channel halo of a tile into LDS (uint4 = 8 channels per load)
-FS_DEV void stage_halo16(const LGrid& g, const uint16_t* __restrict__ in, int x0, int y0, int z0, uint16_t* sh) {
-    for (int q = threadIdx.x; q < HALO * 2; q += blockDim.x) {
-        int hv = q >> 1, half = q & 1;
-        int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-        int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-        uint4 val = make_uint4(0, 0, 0, 0);
-        if (x >= 0 && x < g.W && y >= 0 && y < g.H && z >= 0 && z < g.D) {
-            long long v = (long long)x + (long long)y * g.W + (long long)z * g.W * g.H;
-            val = reinterpret_cast<const uint4*>(in + v * 16)[half];
+// LDS layout of the staged halo: 16 BF16 channels (32 bytes) per voxel.
+// Offset in uint16 of channel group `grp` (0..3, 4 channels) of halo voxel hv
+// (affine, so that the 108 reads of a tile are one base plus immediate
+// offsets; permuting the groups per voxel against bank conflicts measured no
+// faster)
+FS_DEV int lds_grp(int hv, int grp) { return hv * 16 + grp * 4; }
+
+// stage the 16-channel halo of a tile into LDS (uint4 = 8 channels per load);
+// a 256-thread block issues all of a thread's loads before its first store
+constexpr int STAGE16_U = (HALO * 2 + 255) / 256;
+FS_DEV void halo16_load(const LGrid& g, const uint16_t* __restrict__ in, int x0, int y0, int z0,
+                        uint4 (&val)[STAGE16_U]) {
+#pragma unroll
+    for (int u = 0; u < STAGE16_U; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        const int hv = q >> 1, half = q & 1;
+        const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+        const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+        val[u] = make_uint4(0, 0, 0, 0);
+        if (q < HALO * 2 && x >= 0 && x < g.W && y >= 0 && y < g.H && z >= 0 && z < g.D) {
+            const long long v = (long long)x + (long long)y * g.W + (long long)z * g.W * g.H;
+            val[u] = reinterpret_cast<const uint4*>(in + v * 16)[half];
         }
-        reinterpret_cast<uint4*>(sh + hv * 16)[half] = val;
     }
+}
+FS_DEV void halo16_store(uint16_t* sh, const uint4 (&val)[STAGE16_U]) {
+#pragma unroll
+    for (int u = 0; u < STAGE16_U; ++u) {
+        const int q = threadIdx.x + 256 * u;
+        if (q < HALO * 2) {
+            const int hv = q >> 1, g0 = 2 * (q & 1);
+            *reinterpret_cast<uint2*>(sh + lds_grp(hv, g0)) = make_uint2(val[u].x, val[u].y);
+            *reinterpret_cast<uint2*>(sh + lds_grp(hv, g0 + 1)) = make_uint2(val[u].z, val[u].w);
+        }
+    }
+}
+FS_DEV void stage_halo16(const LGrid& g, const uint16_t* __restrict__ in, int x0, int y0, int z0, uint16_t* sh) {
+    uint4 val[STAGE16_U];
+    halo16_load(g, in, x0, y0, z0, val);
+    halo16_store(sh, val);
 }
 
 // ---------------------------------------------------------------- MFMA conv
@@ -74,28 +101,49 @@ __global__ __launch_bounds__(256) void k_conv16(LGrid g, const uint16_t* __restr
                                                 const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                 int relu, uint16_t* __restrict__ out, float* __restrict__ z,
                                                 float* __restrict__ zn, const uint8_t* __restrict__ type,
-                                                const float* __restrict__ scale, const int* __restrict__ done) {
+                                                const float* __restrict__ scale, const int* __restrict__ done,
+                                                int ntile) {
     if (done && *done) return;
-    __shared__ uint16_t sh[HALO * 16];
+    __shared__ __attribute__((aligned(16))) uint16_t sh[HALO * 16];
+    // a block walks tiles blockIdx.x, + gridDim.x, ...: the weights are loaded
+    // once, and the next tile's halo is loaded into registers while the
+    // current one is computed
+    int tile = blockIdx.x;
+    if (tile >= ntile) return;
     int x0, y0, z0;
-    tile_origin(g, x0, y0, z0);
-    stage_halo16(g, in, x0, y0, z0, sh);
+    tile_origin_of(g, tile, x0, y0, z0);
+    uint4 hal[STAGE16_U];
+    halo16_load(g, in, x0, y0, z0, hal);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int col = lane & 15, kg = lane >> 4;
     v4s wreg[LN_TAPS];
 #pragma unroll
     for (int t = 0; t < LN_TAPS; ++t) wreg[t] = *reinterpret_cast<const v4s*>(w + t * 256 + col * 16 + kg * 4);
+    const float bcol = bias ? bias[col] : 0.0f;
+    for (;;) {
+    __syncthreads();   // the previous tile's MFMAs have read sh
+    halo16_store(sh, hal);
     __syncthreads();
+    const int next = tile + (int)gridDim.x;
+    int nx0 = 0, ny0 = 0, nz0 = 0;
+    if (next < ntile) {
+        tile_origin_of(g, next, nx0, ny0, nz0);
+        halo16_load(g, in, nx0, ny0, nz0, hal);
+    }
     v4f acc[TY];
 #pragma unroll
     for (int r = 0; r < TY; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
+    // opaque per tile: keeps the 108 LDS addresses below from being hoisted
+    // out of the tile loop and held in registers
+    int hv0 = col + 1 + HX * (1 + HY * (wv + 1));
+    asm volatile("" : "+v"(hv0));
 #pragma unroll
     for (int t = 0; t < LN_TAPS; ++t) {
         const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
 #pragma unroll
         for (int r = 0; r < TY; ++r) {
-            const int hv = (col + 1 + dx) + HX * ((r + 1 + dy) + HY * (wv + 1 + dz));
-            v4s a = *reinterpret_cast<const v4s*>(sh + hv * 16 + kg * 4);
+            const int hv = hv0 + dx + HX * (r + dy + HY * dz);
+            v4s a = *reinterpret_cast<const v4s*>(sh + lds_grp(hv, kg));
             acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wreg[t], acc[r], 0, 0, 0);
         }
     }
@@ -108,7 +156,7 @@ __global__ __launch_bounds__(256) void k_conv16(LGrid g, const uint16_t* __restr
             const int xx = x0 + kg * 4 + e;
             if (xx >= g.W || yy >= g.H || zz >= g.D) continue;
             const long long v = (long long)xx + (long long)yy * g.W + (long long)zz * g.W * g.H;
-            float val = acc[r][e] + (bias ? bias[col] : 0.0f);
+            float val = acc[r][e] + bcol;
             if (OUT == 0) {
                 if (relu) val = fmaxf(val, 0.0f);
                 out[v * 16 + col] = f2bf(val);
@@ -118,13 +166,19 @@ __global__ __launch_bounds__(256) void k_conv16(LGrid g, const uint16_t* __restr
             }
         }
     }
+    if (next >= ntile) break;
+    tile = next;
+    x0 = nx0; y0 = ny0; z0 = nz0;
+    }
 }
+// blocks of a k_conv16 launch: at most 3 per CU (its registers allow 3 waves per SIMD)
+static unsigned conv_blocks(const LGrid& g) { return std::min(ntiles(g), 768u); }
 
 // weight gradient dW[t][ci][co] += sum_v in(v+t)[ci] dout(v)[co]; dbias[co] += sum_v dout(v)[co]
 __global__ __launch_bounds__(256) void k_wgrad16(LGrid g, const uint16_t* __restrict__ in,
                                                  const uint16_t* __restrict__ dout, float* __restrict__ dw,
                                                  float* __restrict__ db) {
-    __shared__ uint16_t sh[HALO * 16];
+    __shared__ __attribute__((aligned(16))) uint16_t sh[HALO * 16];
     __shared__ uint16_t sd[TX * TY * TZ * 16];
     int x0, y0, z0;
     tile_origin(g, x0, y0, z0);
@@ -152,7 +206,7 @@ __global__ __launch_bounds__(256) void k_wgrad16(LGrid g, const uint16_t* __rest
                 const int tv = chunk * 16 + kg * 4 + e;          // voxel of the tile (K index)
                 const int tx = tv % TX, ty = (tv / TX) % TY, tz = tv / (TX * TY);
                 const int hv = (tx + 1 + dx) + HX * ((ty + 1 + dy) + HY * (tz + 1 + dz));
-                a[e] = (short)sh[hv * 16 + col];                  // A[ci = col][k]
+                a[e] = (short)sh[lds_grp(hv, col >> 2) + (col & 3)];   // A[ci = col][k]
                 b[e] = (short)sd[tv * 16 + col];                  // B[k][co = col]
             }
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -173,49 +227,106 @@ __global__ __launch_bounds__(256) void k_wgrad16(LGrid g, const uint16_t* __rest
 // ---------------------------------------------------------------- VALU layers
 // layer A: a0 = ReLU(bA + conv([rn, fluid, air], wA)), wA [tap][ci 3][co 16] FP32
 // (type: 1 unknown (FLUID), 2 Dirichlet (AIR), anything else Neumann/none)
+// layer A on the FP32 MFMA (v_mfma_f32_16x16x4_f32): per row of 16 voxels an
+// implicit GEMM with M = 16 voxels along x, N = 16 output channels, K = 27
+// taps x 3 inputs (81, padded to 84 with zero weights) in 21 steps of 4.
+// Operand layout: A lane l row l % 16, k = l / 16; B lane l column l % 16,
+// k = l / 16; D lane l column l % 16, rows 4 (l / 16) + e.  The products are
+// FP32 as in a VALU evaluation; the sum runs in the MFMA's order, and the
+// bias is added last.
+constexpr int A_KSTEPS = 21;
 __global__ __launch_bounds__(256) void k_layerA(LGrid g, const float* __restrict__ rn,
                                                 const uint8_t* __restrict__ type, const float* __restrict__ prm,
-                                                uint16_t* __restrict__ out, const int* __restrict__ done) {
+                                                uint16_t* __restrict__ out, const int* __restrict__ done, int ntile) {
     if (done && *done) return;
     __shared__ float sx[HALO * 3];
-    int x0, y0, z0;
-    tile_origin(g, x0, y0, z0);
-    for (int hv = threadIdx.x; hv < HALO; hv += blockDim.x) {
-        int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
-        int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
-        float r = 0.0f, mf = 0.0f, ma = 0.0f;
-        if (x >= 0 && x < g.W && y >= 0 && y < g.H && z >= 0 && z < g.D) {
-            long long v = (long long)x + (long long)y * g.W + (long long)z * g.W * g.H;
-            r = rn[v];
-            mf = type[v] == 1 ? 1.0f : 0.0f;
-            ma = type[v] == 2 ? 1.0f : 0.0f;
+    // a block walks tiles blockIdx.x, + gridDim.x, ...: the weights are loaded
+    // once, the next tile's inputs while the current one is computed
+    int tile = blockIdx.x;
+    if (tile >= ntile) return;
+    constexpr int U = (HALO + 255) / 256;
+    float hr[U];
+    uint8_t ht[U];
+    auto load = [&](int x0, int y0, int z0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int hv = threadIdx.x + 256 * u;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int x = x0 + hx - 1, y = y0 + hy - 1, z = z0 + hz - 1;
+            hr[u] = 0.0f;
+            ht[u] = 0;
+            if (hv < HALO && x >= 0 && x < g.W && y >= 0 && y < g.H && z >= 0 && z < g.D) {
+                const long long v = (long long)x + (long long)y * g.W + (long long)z * g.W * g.H;
+                hr[u] = rn[v];
+                ht[u] = type[v];
+            }
         }
-        sx[hv * 3] = r;
-        sx[hv * 3 + 1] = mf;
-        sx[hv * 3 + 2] = ma;
-    }
-    __syncthreads();
-    const int tx = threadIdx.x % TX, ty = (threadIdx.x / TX) % TY, tz = threadIdx.x / (TX * TY);
-    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
-    if (x >= g.W || y >= g.H || z >= g.D) return;
+    };
+    int x0, y0, z0;
+    tile_origin_of(g, tile, x0, y0, z0);
+    load(x0, y0, z0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int col = lane & 15, kg = lane >> 4;
+    // this lane's (tap, input) of each K step: LDS offset relative to the
+    // row's centre voxel, and the weight of output channel col
+    int koff[A_KSTEPS];
+    float wk[A_KSTEPS];
     const float* wA = prm + LN_AW;
-    float acc[16];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) acc[c] = prm[LN_AB + c];
-    for (int t = 0; t < LN_TAPS; ++t) {
+    for (int j = 0; j < A_KSTEPS; ++j) {
+        const int k = 4 * j + kg;
+        const int t = k / 3, ci = k - 3 * (k / 3);
         const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-        const int hv = (tx + 1 + dx) + HX * ((ty + 1 + dy) + HY * (tz + 1 + dz));
-        const float r = sx[hv * 3], mf = sx[hv * 3 + 1], ma = sx[hv * 3 + 2];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) acc[c] += wA[t * 48 + c] * r + wA[t * 48 + 16 + c] * mf + wA[t * 48 + 32 + c] * ma;
+        koff[j] = k < 81 ? (dx + HX * (dy + HY * dz)) * 3 + ci : 0;
+        wk[j] = k < 81 ? wA[t * 48 + ci * 16 + col] : 0.0f;
     }
-    const long long v = (long long)x + (long long)y * g.W + (long long)z * g.W * g.H;
-    uint16_t o[16];
+    const float bcol = prm[LN_AB + col];
+    for (;;) {
+        __syncthreads();   // the previous tile's MFMAs have read sx
 #pragma unroll
-    for (int c = 0; c < 16; ++c) o[c] = f2bf(fmaxf(acc[c], 0.0f));
-    uint4* dst = reinterpret_cast<uint4*>(out + v * 16);
-    dst[0] = *reinterpret_cast<uint4*>(&o[0]);
-    dst[1] = *reinterpret_cast<uint4*>(&o[8]);
+        for (int u = 0; u < U; ++u) {
+            const int hv = threadIdx.x + 256 * u;
+            if (hv >= HALO) break;
+            sx[hv * 3] = hr[u];
+            sx[hv * 3 + 1] = ht[u] == 1 ? 1.0f : 0.0f;
+            sx[hv * 3 + 2] = ht[u] == 2 ? 1.0f : 0.0f;
+        }
+        __syncthreads();
+        const int next = tile + (int)gridDim.x;
+        int nx0 = 0, ny0 = 0, nz0 = 0;
+        if (next < ntile) {
+            tile_origin_of(g, next, nx0, ny0, nz0);
+            load(nx0, ny0, nz0);
+        }
+        v4f acc[TY];
+#pragma unroll
+        for (int r = 0; r < TY; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
+        // opaque per tile, so that the LDS addresses are not hoisted out of the loop
+        int c0 = ((col + 1) + HX * (1 + HY * (wv + 1))) * 3;
+        asm volatile("" : "+v"(c0));
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            const int c3 = c0 + HX * 3 * r;
+#pragma unroll
+            for (int j = 0; j < A_KSTEPS; ++j)
+                acc[r] = __builtin_amdgcn_mfma_f32_16x16x4f32(sx[c3 + koff[j]], wk[j], acc[r], 0, 0, 0);
+        }
+        const int zz = z0 + wv;
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+            const int yy = y0 + r;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int xx = x0 + kg * 4 + e;
+                if (xx >= g.W || yy >= g.H || zz >= g.D) continue;
+                const long long v = (long long)xx + (long long)yy * g.W + (long long)zz * g.W * g.H;
+                out[v * 16 + col] = f2bf(fmaxf(acc[r][e] + bcol, 0.0f));
+            }
+        }
+        if (next >= ntile) break;
+        tile = next;
+        x0 = nx0; y0 = ny0; z0 = nz0;
+    }
 }
 
 // rn = r / scale with scale = rms of r over the unknowns, sqrt(sc->rr / sc->n_unknown)
@@ -239,40 +350,73 @@ __global__ __launch_bounds__(256) void k_normalise(long long n, const float* __r
     rn[c] = r[c] / s;
 }
 
-// average pooling 2x (zero padding), 16 channels
+// eight BF16 channels of a uint4 as floats, and back
+FS_DEV void bf8_unpack(const uint4& u, float (&f)[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        f[2 * e] = __uint_as_float(w[e] << 16);
+        f[2 * e + 1] = __uint_as_float(w[e] & 0xffff0000u);
+    }
+}
+FS_DEV uint4 bf8_pack(const float (&f)[8]) {
+    uint32_t w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// average pooling 2x (zero padding), 16 channels; a thread takes 8 channels
+// of one coarse voxel (16-byte loads and stores), each summed in the order
+// dk, dj, di
 __global__ __launch_bounds__(256) void k_pool16(LGrid f, LGrid c, const uint16_t* __restrict__ in,
                                                 uint16_t* __restrict__ out, const int* __restrict__ done) {
     if (done && *done) return;
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= c.n * 16) return;
-    long long V = q >> 4;
-    int ch = (int)(q & 15);
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= c.n * 2) return;
+    const long long V = q >> 1;
+    const int half = (int)(q & 1);
     int I, J, K;
     split3(V, c.W, c.H, I, J, K);
-    float s = 0.0f;
+    float s[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] = 0.0f;
     for (int dk = 0; dk < 2; ++dk)
         for (int dj = 0; dj < 2; ++dj)
             for (int di = 0; di < 2; ++di) {
-                int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
-                if (i < f.W && j < f.H && k < f.D)
-                    s += bf2f(in[((long long)i + (long long)j * f.W + (long long)k * f.W * f.H) * 16 + ch]);
+                const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                if (i < f.W && j < f.H && k < f.D) {
+                    float v[8];
+                    bf8_unpack(reinterpret_cast<const uint4*>(
+                                   in + ((long long)i + (long long)j * f.W + (long long)k * f.W * f.H) * 16)[half], v);
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) s[e] += v[e];
+                }
             }
-    out[q] = f2bf(0.125f * s);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s[e] = 0.125f * s[e];
+    reinterpret_cast<uint4*>(out + V * 16)[half] = bf8_pack(s);
 }
 
-// out = fine + up(coarse) (nearest), 16 channels
+// out = fine + up(coarse) (nearest), 16 channels; a thread takes 8 channels of
+// one fine voxel
 __global__ __launch_bounds__(256) void k_upadd16(LGrid f, LGrid c, const uint16_t* __restrict__ fine,
                                                  const uint16_t* __restrict__ coarse, uint16_t* __restrict__ out,
                                                  const int* __restrict__ done) {
     if (done && *done) return;
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= f.n * 16) return;
-    long long v = q >> 4;
-    int ch = (int)(q & 15);
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= f.n * 2) return;
+    const long long v = q >> 1;
+    const int half = (int)(q & 1);
     int i, j, k;
     split3(v, f.W, f.H, i, j, k);
-    long long V = (long long)(i >> 1) + (long long)(j >> 1) * c.W + (long long)(k >> 1) * c.W * c.H;
-    out[q] = f2bf(bf2f(fine[q]) + bf2f(coarse[V * 16 + ch]));
+    const long long V = (long long)(i >> 1) + (long long)(j >> 1) * c.W + (long long)(k >> 1) * c.W * c.H;
+    float a[8], b[8];
+    bf8_unpack(reinterpret_cast<const uint4*>(fine + v * 16)[half], a);
+    bf8_unpack(reinterpret_cast<const uint4*>(coarse + V * 16)[half], b);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[e] = a[e] + b[e];
+    reinterpret_cast<uint4*>(out + v * 16)[half] = bf8_pack(a);
 }
 
 // ---------------------------------------------------------------- backward
@@ -582,19 +726,20 @@ void LNet::forward(const uint8_t* type, const float* r, float* z, const SolveSca
     float* nz = nullptr;
     hipLaunchKernelGGL(k_normalise, dim3(nblk(g0.n)), dim3(256), 0, st, g0.n, r, sc, scale_host, scale_dev, rn,
                        d_scale);
-    hipLaunchKernelGGL(k_layerA, dim3(ntiles(g0)), dim3(256), 0, st, g0, rn, type, d_params, a0, done);
-    hipLaunchKernelGGL(k_pool16, dim3(nblk(g1.n * 16)), dim3(256), 0, st, g0, g1, a0, a1, done);
-    hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g1)), dim3(256), 0, st, g1, a1, wB, d_params + LN_BB, 1, b1, nz, nz,
-                       nt, nf, done);
-    hipLaunchKernelGGL(k_pool16, dim3(nblk(g2.n * 16)), dim3(256), 0, st, g1, g2, b1, b2, done);
-    hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g2)), dim3(256), 0, st, g2, b2, wC, d_params + LN_CB, 1, c2, nz, nz,
-                       nt, nf, done);
-    hipLaunchKernelGGL(k_upadd16, dim3(nblk(g1.n * 16)), dim3(256), 0, st, g1, g2, b1, c2, s1, done);
-    hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g1)), dim3(256), 0, st, g1, s1, wD, d_params + LN_DB, 1, d1, nz, nz,
-                       nt, nf, done);
-    hipLaunchKernelGGL(k_upadd16, dim3(nblk(g0.n * 16)), dim3(256), 0, st, g0, g1, a0, d1, s0, done);
-    hipLaunchKernelGGL(k_conv16<1>, dim3(ntiles(g0)), dim3(256), 0, st, g0, s0, wE, d_params + LN_EB, 0,
-                       (uint16_t*)nullptr, z, zn, type, (const float*)d_scale, done);
+    hipLaunchKernelGGL(k_layerA, dim3(conv_blocks(g0)), dim3(256), 0, st, g0, rn, type, d_params, a0, done,
+                       (int)ntiles(g0));
+    hipLaunchKernelGGL(k_pool16, dim3(nblk(g1.n * 2)), dim3(256), 0, st, g0, g1, a0, a1, done);
+    hipLaunchKernelGGL(k_conv16<0>, dim3(conv_blocks(g1)), dim3(256), 0, st, g1, a1, wB, d_params + LN_BB, 1, b1, nz,
+                       nz, nt, nf, done, (int)ntiles(g1));
+    hipLaunchKernelGGL(k_pool16, dim3(nblk(g2.n * 2)), dim3(256), 0, st, g1, g2, b1, b2, done);
+    hipLaunchKernelGGL(k_conv16<0>, dim3(conv_blocks(g2)), dim3(256), 0, st, g2, b2, wC, d_params + LN_CB, 1, c2, nz,
+                       nz, nt, nf, done, (int)ntiles(g2));
+    hipLaunchKernelGGL(k_upadd16, dim3(nblk(g1.n * 2)), dim3(256), 0, st, g1, g2, b1, c2, s1, done);
+    hipLaunchKernelGGL(k_conv16<0>, dim3(conv_blocks(g1)), dim3(256), 0, st, g1, s1, wD, d_params + LN_DB, 1, d1, nz,
+                       nz, nt, nf, done, (int)ntiles(g1));
+    hipLaunchKernelGGL(k_upadd16, dim3(nblk(g0.n * 2)), dim3(256), 0, st, g0, g1, a0, d1, s0, done);
+    hipLaunchKernelGGL(k_conv16<1>, dim3(conv_blocks(g0)), dim3(256), 0, st, g0, s0, wE, d_params + LN_EB, 0,
+                       (uint16_t*)nullptr, z, zn, type, (const float*)d_scale, done, (int)ntiles(g0));
 }
 
 void LNet::backward(const uint8_t* type, const float* target, const float* zn, int n_fluid, hipStream_t st) {
@@ -609,21 +754,21 @@ void LNet::backward(const uint8_t* type, const float* target, const float* zn, i
     // D: gd1 = relu'(d1) * upT(ds0)
     hipLaunchKernelGGL(k_upT_relu16, dim3(nblk(g1.n * 16)), dim3(256), 0, st, g0, g1, ga0, d1, gd1);
     hipLaunchKernelGGL(k_wgrad16, dim3(ntiles(g1)), dim3(256), 0, st, g1, s1, gd1, d_grad + LN_DW, d_grad + LN_DB);
-    hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g1)), dim3(256), 0, st, g1, gd1, wDt, (const float*)nullptr, 0, gs1,
-                       (float*)nullptr, (float*)nullptr, (const uint8_t*)nullptr, (const float*)nullptr,
-                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_conv16<0>, dim3(conv_blocks(g1)), dim3(256), 0, st, g1, gd1, wDt, (const float*)nullptr, 0,
+                       gs1, (float*)nullptr, (float*)nullptr, (const uint8_t*)nullptr, (const float*)nullptr,
+                       (const int*)nullptr, (int)ntiles(g1));
     // C: gc2 = relu'(c2) * upT(gs1)
     hipLaunchKernelGGL(k_upT_relu16, dim3(nblk(g2.n * 16)), dim3(256), 0, st, g1, g2, gs1, c2, gc2);
     hipLaunchKernelGGL(k_wgrad16, dim3(ntiles(g2)), dim3(256), 0, st, g2, b2, gc2, d_grad + LN_CW, d_grad + LN_CB);
-    hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g2)), dim3(256), 0, st, g2, gc2, wCt, (const float*)nullptr, 0, gb2,
-                       (float*)nullptr, (float*)nullptr, (const uint8_t*)nullptr, (const float*)nullptr,
-                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_conv16<0>, dim3(conv_blocks(g2)), dim3(256), 0, st, g2, gc2, wCt, (const float*)nullptr, 0,
+                       gb2, (float*)nullptr, (float*)nullptr, (const uint8_t*)nullptr, (const float*)nullptr,
+                       (const int*)nullptr, (int)ntiles(g2));
     // B: gb1 = relu'(b1) * (gs1 + poolT(gb2))
     hipLaunchKernelGGL(k_poolT_add_relu16, dim3(nblk(g1.n * 16)), dim3(256), 0, st, g1, g2, gs1, gb2, b1, gb1);
     hipLaunchKernelGGL(k_wgrad16, dim3(ntiles(g1)), dim3(256), 0, st, g1, a1, gb1, d_grad + LN_BW, d_grad + LN_BB);
-    hipLaunchKernelGGL(k_conv16<0>, dim3(ntiles(g1)), dim3(256), 0, st, g1, gb1, wBt, (const float*)nullptr, 0, ga1,
-                       (float*)nullptr, (float*)nullptr, (const uint8_t*)nullptr, (const float*)nullptr,
-                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_conv16<0>, dim3(conv_blocks(g1)), dim3(256), 0, st, g1, gb1, wBt, (const float*)nullptr, 0,
+                       ga1, (float*)nullptr, (float*)nullptr, (const uint8_t*)nullptr, (const float*)nullptr,
+                       (const int*)nullptr, (int)ntiles(g1));
     // A: ga0 = relu'(a0) * (ds0 + poolT(ga1)) (in place over ds0)
     hipLaunchKernelGGL(k_poolT_add_relu16, dim3(nblk(g0.n * 16)), dim3(256), 0, st, g0, g1, ga0, ga1, a0, ga0);
     hipLaunchKernelGGL((k_wgrad_valu<1>), dim3(ntiles(g0)), dim3(256), 0, st, g0, (const uint16_t*)nullptr, rn, type,
