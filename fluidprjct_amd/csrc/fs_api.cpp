@@ -301,7 +301,11 @@ struct Solver {
     int W = 0, H = 0, D = 0;
     long long C = 0;
     hipStream_t st = nullptr;
-    hipEvent_t ev[8] = {};
+    // stage events of two steps (8 each): a step's times are collected once
+    // its events completed, during the next step or when the stream drains
+    hipEvent_t ev[16] = {};
+    int ev_set = 0;             // the set the current step records into
+    int times_pending = -1;     // the set whose times are not collected yet
     // grids
     uint8_t *d_solid = nullptr, *d_labels = nullptr;
     float *d_u = nullptr, *d_v = nullptr, *d_w = nullptr, *d_us = nullptr, *d_vs = nullptr, *d_ws = nullptr;
@@ -337,6 +341,9 @@ struct Solver {
     bool p64_from_step = false; // d_p64 holds the last step's pressure (the next solve's initial guess)
     unsigned* d_umax = nullptr; // max |v| of the particle velocities (float bits)
     bool umax_valid = false;
+    unsigned* h_umax = nullptr; // pinned copy of d_umax
+    hipEvent_t umax_ev = nullptr;
+    bool umax_copied = false;   // h_umax receives d_umax of the last G2P launch (umax_ev after the copy)
     // face fractions: box tables of the current body pose (FaceFrac::th)
     float* d_theta[3] = {nullptr, nullptr, nullptr};
     long long theta_cap[3] = {0, 0, 0};
@@ -355,6 +362,8 @@ struct Solver {
         dfree(d_umax);
         for (float*& t : d_theta) dfree(t);
         if (h_bacc) hipHostFree(h_bacc);
+        if (h_umax) hipHostFree(h_umax);
+        if (umax_ev) hipEventDestroy(umax_ev);
         if (body_done) hipEventDestroy(body_done);
         for (auto& e : ev) if (e) hipEventDestroy(e);
         if (st) hipStreamDestroy(st);
@@ -447,8 +456,15 @@ struct Solver {
             launch_max_speed(PA, (int)np, d_umax, st);
         }
         unsigned bits = 0;
-        HIPCHK(hipMemcpyAsync(&bits, d_umax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        if (umax_valid && umax_copied) {
+            // copied after the last G2P launch; the work queued since then keeps running
+            HIPCHK(hipEventSynchronize(umax_ev));
+            bits = *h_umax;
+        } else {
+            HIPCHK(hipMemcpyAsync(h_umax, d_umax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            bits = *h_umax;
+        }
         float umax;
         std::memcpy(&umax, &bits, sizeof(float));
         if (umax > 0.0f) dt_cur = std::fmin(cfg.dt, (cfg.cfl * cfg.dx) / umax);
@@ -710,6 +726,7 @@ struct Solver {
         a.umax = d_umax;
         launch_g2p_advect(a, d_start, st);
         umax_valid = true;
+        umax_copied = false;
         keys_valid = true;
         count_clean = false;
         sorted_valid = false;
@@ -724,15 +741,52 @@ struct Solver {
                                               std::to_string(nonconv_res) + ")");
     }
 
+    // max |v| after the G2P launch, to pinned memory: the next step's dt waits
+    // for this copy only
+    int copy_umax() {
+        HIPCHK(hipMemcpyAsync(h_umax, d_umax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(umax_ev, st));
+        umax_copied = true;
+        return FS_OK;
+    }
+
+    // add the stage times of the last completed step (waits for its last event)
+    int collect_times() {
+        if (times_pending < 0) return FS_OK;
+        hipEvent_t* e = ev + 8 * times_pending;
+        HIPCHK(hipEventSynchronize(e[6]));
+        float ms;
+        const int slot[6] = {FS_STAGE_LABEL, FS_STAGE_P2G, FS_STAGE_PRESSURE, FS_STAGE_PROJECT, FS_STAGE_BODY,
+                             FS_STAGE_G2P};
+        for (int i = 0; i < 6; ++i) {
+            HIPCHK(hipEventElapsedTime(&ms, e[i], e[i + 1]));
+            times[slot[i]] += ms;
+        }
+        times_pending = -1;
+        return FS_OK;
+    }
+
+    // wait for every queued step and collect its times
+    int drain() {
+        HIPCHK(hipStreamSynchronize(st));
+        return collect_times();
+    }
+
+    // One step, launched without draining the stream: the label/sort stage of
+    // the step is queued behind the previous step's G2P before the host waits
+    // for that G2P's max speed (which sets dt), so the GPU does not idle
+    // between steps.  Callers drain() before reading device state.
     int step_once() {
         if (C == 0) return fail(FS_ERR_STATE, "no scene loaded");
         int rc;
-        if ((rc = choose_dt())) return rc;
-        HIPCHK(hipEventRecord(ev[0], st));
+        hipEvent_t* e = ev + 8 * ev_set;
+        HIPCHK(hipEventRecord(e[0], st));
         if ((rc = stage_sort_label())) return rc;
-        HIPCHK(hipEventRecord(ev[1], st));
+        HIPCHK(hipEventRecord(e[1], st));
+        if ((rc = choose_dt())) return rc;
+        if ((rc = collect_times())) return rc;
         if ((rc = stage_p2g())) return rc;
-        HIPCHK(hipEventRecord(ev[2], st));
+        HIPCHK(hipEventRecord(e[2], st));
         if ((rc = stage_rhs())) return rc;
         rc = stage_pressure(true);
         // a solve that misses the tolerance still completes the step (the
@@ -743,22 +797,17 @@ struct Solver {
         } else if (rc && rc != FS_ERR_NO_FLUID) {
             return rc;
         }
-        HIPCHK(hipEventRecord(ev[3], st));
+        HIPCHK(hipEventRecord(e[3], st));
         if ((rc = stage_project())) return rc;
-        HIPCHK(hipEventRecord(ev[4], st));
+        HIPCHK(hipEventRecord(e[4], st));
         if ((rc = stage_body_launch())) return rc;
-        HIPCHK(hipEventRecord(ev[5], st));
+        HIPCHK(hipEventRecord(e[5], st));
         if ((rc = stage_g2p_advect())) return rc;
-        HIPCHK(hipEventRecord(ev[6], st));
+        HIPCHK(hipEventRecord(e[6], st));
+        if ((rc = copy_umax())) return rc;
         if ((rc = stage_body_finish())) return rc;
-        HIPCHK(hipEventSynchronize(ev[6]));
-        float ms;
-        const int slot[6] = {FS_STAGE_LABEL, FS_STAGE_P2G, FS_STAGE_PRESSURE, FS_STAGE_PROJECT, FS_STAGE_BODY,
-                             FS_STAGE_G2P};
-        for (int i = 0; i < 6; ++i) {
-            HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-            times[slot[i]] += ms;
-        }
+        times_pending = ev_set;
+        ev_set ^= 1;
         sim_time += dt_cur;
         nsteps += 1;
         return FS_OK;
@@ -981,6 +1030,8 @@ int fs_create(const fs_config* cfg, void** out) {
         s->cfg = *cfg;
         HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
         for (auto& e : s->ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreateWithFlags(&s->umax_ev, hipEventDisableTiming));
+        HIPCHK(hipHostMalloc((void**)&s->h_umax, sizeof(unsigned)));
         HIPCHK(hipMalloc(&s->d_bacc, sizeof(double) * 8));
         HIPCHK(hipHostMalloc((void**)&s->h_bacc, sizeof(double) * 8));
         HIPCHK(hipEventCreateWithFlags(&s->body_done, hipEventDisableTiming));
@@ -1185,8 +1236,12 @@ int fs_step(void* h, int nsteps) {
         const int64_t before = s.nonconv;
         for (int i = 0; i < nsteps; ++i) {
             int rc = s.step_once();
-            if (rc) return rc;
+            if (rc) {
+                s.drain();
+                return rc;
+            }
         }
+        if (int rc = s.drain()) return rc;
         return s.report_nonconv(before, nsteps);
     });
 }
@@ -1215,6 +1270,7 @@ int fs_run(void* h, int frames) {
             rc = s.step_once();
             if (rc) break;
             if (i % ipf == 0) {
+                if ((rc = s.drain())) break;
                 if ((rc = join())) break;
                 if ((rc = gather_positions(s, frame_pos))) break;
                 const std::string fn = std::string(s.cfg.output_template) + std::to_string(i / ipf) +
@@ -1229,8 +1285,10 @@ int fs_run(void* h, int frames) {
                         << s.body.theta << "," << s.body.psi << "," << s.body.phi << "\n";
             }
         }
+        const int drc = s.drain();
         const int wrc = join();
         if (rc) return rc;
+        if (drc) return drc;
         if (wrc) return fail(wrc, writer_err);
         return s.report_nonconv(before, (int64_t)frames * ipf + 1);
     });
