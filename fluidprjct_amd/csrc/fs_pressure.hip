@@ -1722,11 +1722,19 @@ __global__ __launch_bounds__(256) void k_setup0(const uint8_t* __restrict__ labe
             if (FF) {
 #pragma unroll
                 for (int q = 0; q < 6; ++q) part |= th[q] > 0.0f && th[q] < 1.0f;
-                // every FLUID cell's plus faces (a WBIT neighbour reads its
-                // minus faces there), the diagonal of the WBIT cells
-                const_cast<float*>(L.wx)[c] = th[0];
-                const_cast<float*>(L.wy)[c] = th[2];
-                const_cast<float*>(L.wz)[c] = th[4];
+                // the plus faces of the FLUID cells that a WBIT row reads (its
+                // own and its minus neighbours'), the diagonal of the WBIT cells.
+                // A WBIT cell has a face in the fraction box, so it and its
+                // neighbours lie within two cells of the box: only those are written.
+                const int gi = i + ox, gj = j + oy, gk = k + oz;
+                const bool near = (unsigned)(gi - ff.lo[0] + 2) < (unsigned)(ff.n[0] + 4) &&
+                                  (unsigned)(gj - ff.lo[1] + 2) < (unsigned)(ff.n[1] + 4) &&
+                                  (unsigned)(gk - ff.lo[2] + 2) < (unsigned)(ff.n[2] + 4);
+                if (near) {
+                    const_cast<float*>(L.wx)[c] = th[0];
+                    const_cast<float*>(L.wy)[c] = th[2];
+                    const_cast<float*>(L.wz)[c] = th[4];
+                }
                 if (part) const_cast<float*>(L.wd)[c] = ((((th[0] + th[1]) + th[2]) + th[3]) + th[4]) + th[5];
             }
             // an empty row (no FLUID or AIR neighbour) is no unknown
