@@ -117,22 +117,40 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const int* __restrict__ 
 }
 
 // single block: exclusive scan of nb block sums in place; total -> *total_out
+// Exclusive scan of the nb block sums in place (one 1024-thread block): chunks
+// of SCAN_TOP_CHUNK sums staged in LDS with coalesced loads, each thread
+// scanning SCAN_TOP_PER consecutive ones, the chunk totals carried over.
+constexpr int SCAN_TOP_PER = 8;
+constexpr int SCAN_TOP_CHUNK = 1024 * SCAN_TOP_PER;
 __global__ __launch_bounds__(1024) void k_scan_top(int* __restrict__ bsum, int nb, int* __restrict__ total_out) {
     __shared__ int sm[16];
-    const int T = blockDim.x;
-    int per = (nb + T - 1) / T;
-    int b0 = threadIdx.x * per, b1 = min(b0 + per, nb);
-    int local = 0;
-    for (int b = b0; b < b1; ++b) local += bsum[b];
-    int tot;
-    int pre = block_excl_scan(local, sm, &tot);
-    int run = pre;
-    for (int b = b0; b < b1; ++b) {
-        int v = bsum[b];
-        bsum[b] = run;
-        run += v;
+    __shared__ int chunk[SCAN_TOP_CHUNK];
+    int carry = 0;
+    for (int base = 0; base < nb; base += SCAN_TOP_CHUNK) {
+        const int m = min(SCAN_TOP_CHUNK, nb - base);
+        for (int q = threadIdx.x; q < m; q += 1024) chunk[q] = bsum[base + q];
+        __syncthreads();
+        const int b0 = threadIdx.x * SCAN_TOP_PER;
+        int v[SCAN_TOP_PER];
+        int local = 0;
+#pragma unroll
+        for (int e = 0; e < SCAN_TOP_PER; ++e) {
+            v[e] = b0 + e < m ? chunk[b0 + e] : 0;
+            local += v[e];
+        }
+        int tot;
+        int run = carry + block_excl_scan(local, sm, &tot);
+#pragma unroll
+        for (int e = 0; e < SCAN_TOP_PER; ++e) {
+            if (b0 + e < m) chunk[b0 + e] = run;
+            run += v[e];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < m; q += 1024) bsum[base + q] = chunk[q];
+        __syncthreads();
+        carry += tot;
     }
-    if (threadIdx.x == 0) *total_out = tot;
+    if (threadIdx.x == 0) *total_out = carry;
 }
 
 // Final pass.  MODE 0 (cell counts): writes start[], labels (labelGrid :340-397

@@ -300,9 +300,8 @@ __global__ __launch_bounds__(256) void k_scatter_out(const uint8_t* __restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
-    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (c >= Cc.n) return;
+// type of coarse cell c: Dirichlet if a child is, else unknown if a child is
+FS_DEV void type_coarsen_at(const MGLevel& F, const MGLevel& Cc, long long c) {
     int I, J, K;
     split3(c, Cc.W, Cc.H, I, J, K);
     int any1 = 0, any2 = 0;
@@ -324,10 +323,22 @@ __global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
     Cc.type[c] = any2 ? 2 : (any1 ? 1 : 0);
 }
 
+__global__ __launch_bounds__(256) void k_type_coarsen(MGLevel F, MGLevel Cc) {
+    long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (c >= Cc.n) return;
+    type_coarsen_at(F, Cc, c);
+}
+
 // per-cell stencil code; counters[0] += unknowns, counters[1] |= Dirichlet contact
-// (grid-strided, one pair of global atomics per block)
-__global__ __launch_bounds__(256) void k_code(MGLevel L, int* __restrict__ counters) {
+// (grid-strided, one pair of global atomics per block).  COARSEN: the types of
+// the next level Cc in the same launch (both read only this level's types).
+template <bool COARSEN>
+__global__ __launch_bounds__(256) void k_code(MGLevel L, MGLevel Cc, int* __restrict__ counters) {
     __shared__ int s_cnt[2];
+    if (COARSEN)
+        for (long long c = (long long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; c < Cc.n;
+             c += (long long)gridDim.x * blockDim.x)
+            type_coarsen_at(L, Cc, c);
     if (threadIdx.x == 0) { s_cnt[0] = 0; s_cnt[1] = 0; }
     __syncthreads();
     int unk = 0, dir = 0;
@@ -2104,11 +2115,15 @@ void MGPCG::setup(const uint8_t* labels, const float* b, hipStream_t st) {
     else
         hipLaunchKernelGGL(k_setup0<false>, dim3(nred), dim3(256), 0, st, labels, b, L0, W, H, box[0], box[1],
                            box[2], bb, slot(PS_B2), slot(PS_B1), counters + 8, face);
-    for (int l = 0; l + 1 < nlev; ++l)
-        hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[l + 1].n)), dim3(256), 0, st, lv[l], lv[l + 1]);
-    for (int l = 1; l < nlev; ++l)
-        hipLaunchKernelGGL(k_code, dim3((unsigned)std::min<long long>(1024, nblk(lv[l].n))), dim3(256), 0, st, lv[l],
-                           counters + 10);
+    // level 1's types, then per level its codes and the next level's types
+    if (nlev > 1) hipLaunchKernelGGL(k_type_coarsen, dim3(nblk(lv[1].n)), dim3(256), 0, st, lv[0], lv[1]);
+    for (int l = 1; l < nlev; ++l) {
+        const dim3 g((unsigned)std::min<long long>(1024, nblk(lv[l].n)));
+        if (l + 1 < nlev)
+            hipLaunchKernelGGL(k_code<true>, g, dim3(256), 0, st, lv[l], lv[l + 1], counters + 10);
+        else
+            hipLaunchKernelGGL(k_code<false>, g, dim3(256), 0, st, lv[l], lv[l], counters + 10);
+    }
 }
 
 MGLevel MGPCG::fine() const { return lv[0]; }
