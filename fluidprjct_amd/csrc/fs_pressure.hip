@@ -42,7 +42,7 @@ constexpr uint32_t WBIT = 0x4000u;   // level 0: the cell has a partly open face
 // partial-sum slots (each RED_BLOCKS doubles) in MGPCG::partial
 enum { PS_RZ0 = 0, PS_RZ1, PS_PQ0, PS_PQ1, PS_RR, PS_B2, PS_B1, PS_RES, PS_ZQ, PS_RP, PS_MEAN, PS_COUNT };
 
-enum { FIN_BNORM = 0, FIN_OUTER, FIN_OUTER_REL };
+enum { FIN_BNORM = 0, FIN_OUTER, FIN_OUTER_REL, FIN_CHECK };
 
 // 1/d for the smoothers (d = 1..6): the hardware reciprocal (1 ulp) instead of
 // a correctly rounded division.  The preconditioner only needs one fixed
@@ -1824,7 +1824,11 @@ __global__ __launch_bounds__(256) void k_resid64_x0(MGLevel L, const float* __re
 // ZERO_X: the first pass, which also sets x64 = 0, so r = b - shift
 template <bool ZERO_X>
 __global__ __launch_bounds__(256) void k_resid64(MGLevel L, const float* __restrict__ b, double* __restrict__ x64,
-                                                 double shift, float* __restrict__ r, double* __restrict__ partial) {
+                                                 double shift, float* __restrict__ r, double* __restrict__ partial,
+                                                 const SolveScalars* __restrict__ gate) {
+    // gate: run only once the inner iterations have converged (the check
+    // that closes an inner batch), as every block reads the same flag
+    if (gate && gate->done != 1) return;
     __shared__ double sm[12];
     const long long WH = (long long)L.W * L.H;
     double acc = 0.0;
@@ -1925,6 +1929,9 @@ __global__ __launch_bounds__(256) void k_fill0(long long n, double* __restrict__
 __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __restrict__ p1,
                                                   const double* __restrict__ p2, int np, SolveScalars* sc,
                                                   double tol2b) {
+    // FIN_CHECK: FIN_OUTER with the tolerance of the solve (sc->tol2b), only
+    // after inner iterations that converged
+    if (mode == FIN_CHECK && sc->done != 1) return;
     __shared__ double sm[12];
     double s1 = sum_partials(p1, np, sm);
     double s2 = p2 ? sum_partials(p2, np, sm) : 0.0;
@@ -1935,6 +1942,7 @@ __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __rest
     } else {
         // FIN_OUTER_REL: tol2b holds tol^2, relative to the b norm of FIN_BNORM
         if (mode == FIN_OUTER_REL) tol2b *= sc->bnorm2;
+        if (mode == FIN_CHECK) tol2b = sc->tol2b;
         sc->rr_true = s1;
         sc->tol2b = tol2b;
         if (s1 <= tol2b) {
@@ -2464,6 +2472,16 @@ void MGPCG::record_batch(Preconditioner* pre, int nb, hipStream_t st) {
         std::swap(pold, pnew);
         cur ^= 1;
     }
+    // Iterations that converged are checked here against the FP64 residual
+    // (done = 2, or a restart from the true residual), saving the host round
+    // trip of a separate check.  Not for singular systems (their check
+    // shifts b by its mean, a value of the solve).
+    if (!singular_mode) {
+        hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, st, L0, (const float*)bb, xb, 0.0, r,
+                           slot(PS_RES), (const SolveScalars*)sc);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_CHECK, (const double*)slot(PS_RES),
+                           (const double*)nullptr, nred, sc, 0.0);
+    }
 }
 
 // The batch as a HIP graph, cached per solve-box shape and preconditioner
@@ -2556,9 +2574,11 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     // r = (b - shift) - A x64 (zero: x64 = 0 first)
     auto launch_resid64 = [&](bool zero, double sh, hipStream_t s) {
         if (zero)
-            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, s, lv[0], bb, xb, sh, r, slot(PS_RES));
+            hipLaunchKernelGGL(k_resid64<true>, dim3(nred), dim3(256), 0, s, lv[0], bb, xb, sh, r, slot(PS_RES),
+                               (const SolveScalars*)nullptr);
         else
-            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, s, lv[0], bb, xb, sh, r, slot(PS_RES));
+            hipLaunchKernelGGL(k_resid64<false>, dim3(nred), dim3(256), 0, s, lv[0], bb, xb, sh, r, slot(PS_RES),
+                               (const SolveScalars*)nullptr);
     };
     int rc = prepare(labels, b, st);
     if (rc == -5) return -5;
@@ -2609,10 +2629,13 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     const bool first_done = !singular && bn2 > 0.0 && !(x0 && sc_host->rr_true >= bn2);
     int total_iters = 0, outer = 0;
     bool converged = false;
+    bool checked = false;   // the last batch ran the FP64 check (sc_host current)
     double rr_true = 0.0, rr_first = 0.0;
     for (outer = 0;; ++outer) {
         if (outer == 0 && first_done) {
             // the speculative first pass stands
+        } else if (checked) {
+            // the batch closed with the check (done = 2: converged)
         } else if (outer == 0 && x0 && bn2 > 0.0 && singular) {
             // warm start: the initial guess on the unknowns of the box (0 elsewhere)
             hipLaunchKernelGGL(k_gather_x0, dim3(nblk(n0)), dim3(256), 0, st, L0, x0, W, H, box[0], box[1], box[2],
@@ -2628,7 +2651,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             launch_resid64(true, shift, st);
         else
             launch_resid64(false, shift, st);
-        if (!(outer == 0 && first_done)) {
+        if (!(outer == 0 && first_done) && !checked) {
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER, slot(PS_RES),
                                (const double*)nullptr, nred, sc, tol2b);
             hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
@@ -2690,6 +2713,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             rr_prev = sc_host->rr;
             it_prev = sc_host->iters;
         }
+        checked = sc_host->done == 2;
         // a flexible preconditioner that stalls or breaks down hands over to the V-cycle
         if (pre->flexible() && (sc_host->done == 5 || sc_host->done == 3)) {
             pre = &mgp;

@@ -144,6 +144,25 @@ def test_deterministic_runs_repeat(hip, tmp_path):
     assert np.abs(pd - a.particles()[0]).max() <= 2e-5
 
 
+def test_queued_steps_match_single_steps(hip):
+    """fs_step(n) queues each step's label/sort behind the previous G2P before
+    waiting for its max speed and collects stage times one step late: in the
+    deterministic order, 6 steps in one call give the bits of 6 calls of one
+    step (body included), and every stage's time is accounted."""
+    a, b = make(hip, flags=FS_STEP_DETERMINISTIC), make(hip, flags=FS_STEP_DETERMINISTIC)
+    a.step(6)
+    for _ in range(6):
+        b.step(1)
+    pa, va = particles_bits(a)
+    pb, vb = particles_bits(b)
+    assert np.array_equal(pa, pb) and np.array_equal(va, vb)
+    assert bytes(a.body_state()) == bytes(b.body_state())
+    assert a.clock() == b.clock()
+    ta, tb = a.stage_times(), b.stage_times()
+    for k in ("label", "p2g", "pressure", "project", "g2p_advect"):
+        assert ta[k] > 0.0 and tb[k] > 0.0, (k, ta[k], tb[k])
+
+
 def test_checkpoints_cross_load(hip, oracle, tmp_path):
     """A HIP checkpoint continued by the oracle, and an oracle checkpoint
     continued by the HIP library, against the run that saved it."""
