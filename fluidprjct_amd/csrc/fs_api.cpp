@@ -30,6 +30,7 @@
 using fsd::f3;
 using fsd::mk;
 using fsd::FaceFrac;
+using fsd::BodyParams;
 using namespace fsk;
 
 namespace {
@@ -325,6 +326,12 @@ struct Solver {
     float* d_sdf = nullptr;
     double* d_bacc = nullptr;
     double* h_bacc = nullptr;   // pinned copy of d_bacc, read after body_done
+    double* d_bpart = nullptr;  // per-block partial sums of the body forces
+    long long bpart_cap = 0;
+    // the pose of the label pass that marked the current BODY cells (false:
+    // labels written from elsewhere, BODY cells anywhere)
+    bool body_box_pose = false;
+    BodyParams box_pose{};
     hipEvent_t body_done = nullptr;
     MGPCG mg;
     std::unique_ptr<Preconditioner> jacobi;
@@ -359,6 +366,7 @@ struct Solver {
         free_particles();
         dfree(d_sdf);
         dfree(d_bacc);
+        dfree(d_bpart);
         dfree(d_umax);
         for (float*& t : d_theta) dfree(t);
         if (h_bacc) hipHostFree(h_bacc);
@@ -540,6 +548,8 @@ struct Solver {
         if (rc) return rc;
         LabelArgs la = label_args();
         launch_scan_label(d_count, d_solid, d_labels, C, d_bsum, d_total, d_start, la, st);
+        box_pose = la.body;
+        body_box_pose = true;
         return finish_sort();
     }
 
@@ -667,8 +677,16 @@ struct Solver {
             int rc = sort_only();
             if (rc) return rc;
         }
-        HIPCHK(hipMemsetAsync(d_bacc, 0, sizeof(double) * 8, st));
-        launch_body_forces(grid_args(), d_sdf, d_start, PA, d_bacc, st);
+        const GridArgs ga = grid_args();
+        const BodyParams* pose = body_box_pose ? &box_pose : nullptr;
+        const long long need = body_force_partials(ga, pose);
+        if (need > bpart_cap) {
+            dfree(d_bpart);
+            bpart_cap = 0;
+            HIPCHK(hipMalloc(&d_bpart, sizeof(double) * (size_t)need));
+            bpart_cap = need;
+        }
+        launch_body_forces(ga, pose, d_sdf, d_start, PA, d_bpart, d_bacc, st);
         HIPCHK(hipMemcpyAsync(h_bacc, d_bacc, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
         HIPCHK(hipEventRecord(body_done, st));
         return FS_OK;
@@ -871,9 +889,11 @@ int upload_scene(Solver& s, int W, int H, int D, const std::vector<uint8_t>& L) 
     std::vector<uint8_t> solid(L.size());
     for (size_t c = 0; c < L.size(); ++c) solid[c] = (L[c] == FS_SOLID);
     HIPCHK(hipMemcpy(s.d_labels, L.data(), L.size(), hipMemcpyHostToDevice));
+    s.body_box_pose = false;   // BODY cells not (only) from a label pass
     HIPCHK(hipMemcpy(s.d_solid, solid.data(), L.size(), hipMemcpyHostToDevice));
     if (s.body.active) {
         launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
+        s.body_box_pose = false;   // BODY cells not (only) from a label pass
         HIPCHK(hipStreamSynchronize(s.st));
     }
     return FS_OK;
@@ -1111,6 +1131,7 @@ int fs_set_body_sdf(void* h, const float* sdf, int w, int hh, int d, float ox, f
         HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
         if (s.C > 0) {
             launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
+            s.body_box_pose = false;   // BODY cells not (only) from a label pass
             HIPCHK(hipStreamSynchronize(s.st));
         }
         return FS_OK;
@@ -1414,6 +1435,7 @@ int fs_set_grid(void* h, int which, const void* in, int64_t count) {
         if (count != n) return fail(FS_ERR_ARG, "size mismatch");
         HIPCHK(hipStreamSynchronize(s.st));
         if (n) HIPCHK(hipMemcpy(dst, in, esz * n, hipMemcpyHostToDevice));
+        if (dst == s.d_labels) s.body_box_pose = false;   // BODY cells anywhere
         return FS_OK;
     });
 }
@@ -1459,6 +1481,7 @@ int fs_solve_pressure_system(void* h, const uint8_t* labels, const float* b, dou
             if (rc) return rc;
         } else {
             HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
+            s.body_box_pose = false;   // BODY cells not (only) from a label pass
         }
         HIPCHK(hipMemcpy(s.d_rhs, b, sizeof(float) * (size_t)W * H * D, hipMemcpyHostToDevice));
         double save = s.cfg.solver_tol;
@@ -1639,6 +1662,7 @@ int fs_mg_vcycle(void* h, const uint8_t* labels, const float* r, float* z, int W
             if (rc) return rc;
         } else {
             HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
+            s.body_box_pose = false;   // BODY cells not (only) from a label pass
         }
         const size_t C = (size_t)W * H * D;
         HIPCHK(hipMemcpy(s.d_rhs, r, sizeof(float) * C, hipMemcpyHostToDevice));
@@ -1850,6 +1874,7 @@ int fs_load_checkpoint(void* h, const char* path) {
         if (rc) return rc;
         HIPCHK(hipMemcpy(s.d_solid, solid.data(), C, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(s.d_labels, labels.data(), C, hipMemcpyHostToDevice));
+        s.body_box_pose = false;   // BODY cells not (only) from a label pass
         rc = s.alloc_particles(np);
         if (rc) return rc;
         float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};

@@ -8,6 +8,9 @@
 // Label reads that the reference performs past a row/slab end (:1406-1455,
 // :2109-2110) use the same wrapped linear index; an index outside the array
 // reads SOLID with p = 0.
+#include <algorithm>
+#include <cmath>
+
 #include "fs_kernels.hpp"
 
 using namespace fsd;
@@ -224,19 +227,28 @@ __global__ __launch_bounds__(256) void k_project(GridArgs a, FaceFrac ff, const 
 
 // Drag and lift on BODY-FLUID faces.  The lift term is skipped when
 // |n x u_rel| == 0, where the reference divides by zero (:2774-2775).
-__global__ __launch_bounds__(256) void k_drag_lift(GridArgs a, double* __restrict__ out) {
+// Over the box of cells [lo, lo + n) that holds every BODY cell (the body's
+// bounding sphere): each block writes the sums of its cells, component q at
+// partial[q * gridDim.x + block], for a fixed-order final sum (k_body_sum).
+__global__ __launch_bounds__(256) void k_drag_lift(GridArgs a, int3 lo, int3 n, double* __restrict__ partial) {
     __shared__ double sm[4];
     const int W = a.W, H = a.H, D = a.D;
-    const long long C = (long long)W * H * D;
-    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nbox = (long long)n.x * n.y * n.z;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    const bool body = idx < C && a.labels[idx] == FS_BODY;
-    // most blocks hold no body cell: skip their six block reductions
-    if (!__syncthreads_or(body)) return;
+    int ci = 0, cj = 0, ck = 0;
+    if (t < nbox) {
+        split3(t, n.x, n.y, ci, cj, ck);
+        ci += lo.x; cj += lo.y; ck += lo.z;
+    }
+    const bool body = t < nbox && a.labels[(size_t)ci + (size_t)cj * W + (size_t)ck * W * H] == FS_BODY;
+    // blocks without a body cell skip their six block reductions
+    if (!__syncthreads_or(body)) {
+        if (threadIdx.x < 6) partial[threadIdx.x * gridDim.x + blockIdx.x] = 0.0;
+        return;
+    }
     if (body) {
         const float dx = a.dx, rho = a.density;
-        int ci, cj, ck;
-        split3(idx, W, H, ci, cj, ck);
         f3 cc = mk((ci + 0.5f) * dx, (cj + 0.5f) * dx, (ck + 0.5f) * dx);
         const float area = dx * dx;
         const int NUc = (W + 1) * H * D, NVc = W * (H + 1) * D, NWc = W * H * (D + 1);
@@ -274,16 +286,17 @@ __global__ __launch_bounds__(256) void k_drag_lift(GridArgs a, double* __restric
         }
     }
     for (int q = 0; q < 6; ++q) {
-        double t = block_sum(acc[q], sm);
-        if (threadIdx.x == 0 && t != 0.0) atomicAdd(&out[q], t);
+        double s = block_sum(acc[q], sm);
+        if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = s;
     }
 }
 
 // Submerged SDF volume: a voxel with phi <= 0 counts (partially when phi > -cs)
 // if its cell is FLUID or a particle binned into that cell lies within 2dx.
+// partial[block]: the block's submerged volume (k_body_sum adds them in order)
 __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __restrict__ sdf,
                                                   const int* __restrict__ start, ParticlesView P,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ partial) {
     __shared__ double sm[4];
     const BodyParams& b = a.body;
     long long nvox = (long long)b.sw * b.sh * b.sd;
@@ -318,8 +331,25 @@ __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __res
             if (subm) vol = v;
         }
     }
-    double t = block_sum(vol, sm);
-    if (threadIdx.x == 0 && t != 0.0) atomicAdd(&out[6], t);
+    double s = block_sum(vol, sm);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[0..5] = drag/lift force and torque, out[6] = submerged volume, out[7] = 0:
+// the per-block partial sums added in a fixed order (one block), so that the
+// body's forces are the same bits from run to run
+__global__ __launch_bounds__(256) void k_body_sum(const double* __restrict__ pdrag, int nd,
+                                                  const double* __restrict__ pbuoy, int nbu, double* __restrict__ out) {
+    __shared__ double sm[4];
+    for (int q = 0; q < 7; ++q) {
+        const double* p = q < 6 ? pdrag + (size_t)q * nd : pbuoy;
+        const int n = q < 6 ? nd : nbu;
+        double v = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+        const double s = block_sum(v, sm);
+        if (threadIdx.x == 0) out[q] = s;
+    }
+    if (threadIdx.x == 0) out[7] = 0.0;
 }
 
 // z cells per column thread (4 measured faster than 1, 2, 8 and 16 at 256^3:
@@ -385,12 +415,49 @@ void launch_face_open(const GridArgs& a, const FaceFrac& ff, int comp, float* ou
     hipLaunchKernelGGL(k_face_open, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, ff, comp, out, n);
 }
 
-void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P, double* out,
-                        hipStream_t st) {
-    long long C = (long long)a.W * a.H * a.D;
-    hipLaunchKernelGGL(k_drag_lift, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, st, a, out);
-    long long nv = (long long)a.body.sw * a.body.sh * a.body.sd;
-    hipLaunchKernelGGL(k_buoyancy, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, a, sdf, start, P, out);
+// the cells whose centres lie in the bounding sphere of the pose that marked the
+// BODY cells (the label pass marks only cells there), one cell of margin,
+// clamped to the grid; n = 0 when none; the whole grid without a pose
+static void body_box(const GridArgs& a, const BodyParams* pose, int3& lo, int3& n) {
+    if (!pose) {
+        lo = make_int3(0, 0, 0);
+        n = make_int3(a.W, a.H, a.D);
+        return;
+    }
+    const float r = sqrtf(pose->brad2);
+    const float c[3] = {pose->bcen.x, pose->bcen.y, pose->bcen.z};
+    const int dim[3] = {a.W, a.H, a.D};
+    int l[3], m[3];
+    for (int q = 0; q < 3; ++q) {
+        // centre (i + 1/2) dx within [c - r, c + r]
+        const int i0 = (int)std::floor((c[q] - r) / a.dx - 0.5f) - 1, i1 = (int)std::ceil((c[q] + r) / a.dx - 0.5f) + 1;
+        l[q] = std::max(i0, 0);
+        m[q] = std::max(0, std::min(i1, dim[q] - 1) - l[q] + 1);
+    }
+    lo = make_int3(l[0], l[1], l[2]);
+    n = make_int3(m[0], m[1], m[2]);
+    if (!(r >= 0.0f) || m[0] == 0 || m[1] == 0 || m[2] == 0) n = make_int3(0, 0, 0);
+}
+
+long long body_force_partials(const GridArgs& a, const BodyParams* pose) {
+    int3 lo, n;
+    body_box(a, pose, lo, n);
+    const long long nb = ((long long)n.x * n.y * n.z + 255) / 256;
+    const long long nv = (long long)a.body.sw * a.body.sh * a.body.sd;
+    return 6 * nb + (nv + 255) / 256;
+}
+
+void launch_body_forces(const GridArgs& a, const BodyParams* pose, const float* sdf, const int* start,
+                        const ParticlesView& P, double* partial, double* out, hipStream_t st) {
+    int3 lo, n;
+    body_box(a, pose, lo, n);
+    const unsigned nd = (unsigned)(((long long)n.x * n.y * n.z + 255) / 256);
+    const long long nv = (long long)a.body.sw * a.body.sh * a.body.sd;
+    const unsigned nbu = (unsigned)((nv + 255) / 256);
+    if (nd) hipLaunchKernelGGL(k_drag_lift, dim3(nd), dim3(256), 0, st, a, lo, n, partial);
+    if (nbu) hipLaunchKernelGGL(k_buoyancy, dim3(nbu), dim3(256), 0, st, a, sdf, start, P, partial + 6 * (size_t)nd);
+    hipLaunchKernelGGL(k_body_sum, dim3(1), dim3(256), 0, st, (const double*)partial, (int)nd,
+                       (const double*)(partial + 6 * (size_t)nd), (int)nbu, out);
 }
 
 }  // namespace fsk

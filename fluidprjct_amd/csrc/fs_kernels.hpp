@@ -120,9 +120,13 @@ void launch_rhs(const GridArgs& a, const fsd::FaceFrac& ff, float* rhs, hipStrea
 void launch_project(const GridArgs& a, const fsd::FaceFrac& ff, const float* p, hipStream_t st);
 void launch_face_open(const GridArgs& a, const fsd::FaceFrac& ff, int comp, float* out, hipStream_t st);
 void launch_face_theta(const fsd::FaceFrac& ff, float* const th[3], hipStream_t st);
-// drag/lift + buoyancy partial sums: out[0..2] force, [3..5] torque, [6] submerged volume (FP64)
-void launch_body_forces(const GridArgs& a, const float* sdf, const int* start, const ParticlesView& P,
-                        double* out, hipStream_t st);
+// drag/lift + buoyancy: out[0..2] force, [3..5] torque, [6] submerged volume (FP64), from
+// per-block partial sums (body_force_partials doubles of scratch) added in a fixed order.
+// pose: the pose whose label pass marked the BODY cells (drag/lift visits the cells of its
+// bounding sphere), null to visit every cell (labels from elsewhere)
+long long body_force_partials(const GridArgs& a, const fsd::BodyParams* pose);
+void launch_body_forces(const GridArgs& a, const fsd::BodyParams* pose, const float* sdf, const int* start,
+                        const ParticlesView& P, double* partial, double* out, hipStream_t st);
 void launch_mark_body(const LabelArgs& la, uint8_t* labels, long long C, hipStream_t st);
 void launch_cell_keys(const ParticlesView& P, int n, float dx, int W, int H, int D, int* out, hipStream_t st);
 
