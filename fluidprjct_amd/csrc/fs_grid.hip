@@ -341,12 +341,16 @@ __global__ __launch_bounds__(256) void k_buoyancy(GridArgs a, const float* __res
 __global__ __launch_bounds__(256) void k_body_sum(const double* __restrict__ pdrag, int nd,
                                                   const double* __restrict__ pbuoy, int nbu, double* __restrict__ out) {
     __shared__ double sm[4];
+    double v[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int nmax = nd > nbu ? nd : nbu;
+    for (int i = threadIdx.x; i < nmax; i += blockDim.x) {
+        if (i < nd)
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v[q] += pdrag[(size_t)q * nd + i];
+        if (i < nbu) v[6] += pbuoy[i];
+    }
     for (int q = 0; q < 7; ++q) {
-        const double* p = q < 6 ? pdrag + (size_t)q * nd : pbuoy;
-        const int n = q < 6 ? nd : nbu;
-        double v = 0.0;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
-        const double s = block_sum(v, sm);
+        const double s = block_sum(v[q], sm);
         if (threadIdx.x == 0) out[q] = s;
     }
     if (threadIdx.x == 0) out[7] = 0.0;
