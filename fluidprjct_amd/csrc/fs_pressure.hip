@@ -2610,6 +2610,17 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
         launch_resid64(true, 0.0, st);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, (int)FIN_OUTER_REL, slot(PS_RES),
                        (const double*)nullptr, nred, sc, tol * tol);
+    // The first batch of iterations is launched before the host sees the
+    // first pass (sized from the previous solve's count): it stands unless
+    // the system turns out singular or b zero, and is undone below in those
+    // cases.  Not for flexible
+    // preconditioners (their fallback needs the host between batches).
+    const bool spec = use_graph && !singular_mode && !pre->flexible() && last_iters > 0 && maxit > 0;
+    if (spec) {
+        const int nbs = std::max(2, std::min(batch, 2 * ((last_iters + 1) / 2)));
+        hipGraphExec_t ex = batch_graph(pre, nbs, st);
+        if (!ex || hipGraphLaunch(ex, st) != hipSuccess) return -5;
+    }
     hipMemcpyAsync(host_counters + 8, counters + 8, sizeof(int) * 2, hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return -5;
@@ -2626,7 +2637,19 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     // the first pass is redone when the system is singular, when b is zero
     // (the solution is then 0, whatever the initial guess), and when the
     // initial guess is no better than 0
-    const bool first_done = !singular && bn2 > 0.0 && !(x0 && sc_host->rr_true >= bn2);
+    // (after a speculative batch a start worse than 0 stands too: the batch
+    // iterated from it, and starting over would cost more than it saves)
+    const bool first_done = !singular && bn2 > 0.0 && (spec || !(x0 && sc_host->rr_true >= bn2));
+    // the speculative batch stands when the first pass does
+    bool spec_pending = spec && first_done;
+    if (spec && !spec_pending) {
+        // undo it: the passes below restart the iteration (x, r and the
+        // scalars are rewritten; the directions are cleared, since beta = 0
+        // times a non-finite direction would not clear it; the count restarts)
+        hipMemcpyAsync(&sc->iters, &zero_count, sizeof(int), hipMemcpyHostToDevice, st);
+        hipMemsetAsync(pa, 0, sizeof(float) * (size_t)n0, st);
+        hipMemsetAsync(pb, 0, sizeof(float) * (size_t)n0, st);
+    }
     int total_iters = 0, outer = 0;
     bool converged = false;
     bool checked = false;   // the last batch ran the FP64 check (sc_host current)
@@ -2666,7 +2689,8 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             if (hipStreamSynchronize(st) != hipSuccess) return -5;
         }
         rr_true = sc_host->rr_true;
-        if (outer == 0) rr_first = rr_true;
+        // (after the speculative batch, rr_true may already be its final check)
+        if (outer == 0) rr_first = spec_pending ? sc_host->rr_start : rr_true;
         total_iters = sc_host->iters;
         if (sc_host->done == 2 || bn2 == 0.0) { converged = true; break; }
         if (outer >= max_outer || total_iters >= maxit) break;
@@ -2688,21 +2712,27 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
             const int n = 2 * (int)std::ceil(need / 2.0);
             return std::max(2, std::min(batch, n));
         };
-        double rr_prev = sc_host->rr;
-        int it_prev = sc_host->iters;
+        // after the speculative batch (sc_host already current) the rate is
+        // measured from the first pass
+        double rr_prev = spec_pending ? sc_host->rr_start : sc_host->rr;
+        int it_prev = spec_pending ? 0 : sc_host->iters;
         int nb = batch;
         if (rate_per_iter < 0.0 && sc_host->rr > 0.0 && sc_host->target > 0.0)
             nb = batch_for(std::log(sc_host->target / sc_host->rr) / rate_per_iter);
         while (true) {
-            if (use_graph) {
-                hipGraphExec_t ex = batch_graph(pre, nb, st);
-                if (!ex) return -5;
-                if (hipGraphLaunch(ex, st) != hipSuccess) return -5;
+            if (spec_pending) {
+                spec_pending = false;
             } else {
-                record_batch(pre, nb, st);
+                if (use_graph) {
+                    hipGraphExec_t ex = batch_graph(pre, nb, st);
+                    if (!ex) return -5;
+                    if (hipGraphLaunch(ex, st) != hipSuccess) return -5;
+                } else {
+                    record_batch(pre, nb, st);
+                }
+                hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
+                if (hipStreamSynchronize(st) != hipSuccess) return -5;
             }
-            hipMemcpyAsync(sc_host, sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, st);
-            if (hipStreamSynchronize(st) != hipSuccess) return -5;
             if (sc_host->done) break;
             nb = batch;
             const int dit = sc_host->iters - it_prev;

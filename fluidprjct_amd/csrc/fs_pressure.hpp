@@ -168,6 +168,7 @@ public:
     SolveScalars* sc_host = nullptr;  // pinned
     int nred = 0;   // number of partial-sum slots used by reducing kernels
     int last_iters = 0, last_outer = 0;
+    int zero_count = 0;           // host source of the iteration count's reset
     double rate_per_iter = 0.0;   // log(||r||^2 reduction) per iteration of the last converged solve (< 0)
     bool use_graph = true;      // replay inner batches as HIP graphs
     bool use_tail = true;
