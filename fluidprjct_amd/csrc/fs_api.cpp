@@ -332,6 +332,11 @@ struct Solver {
     // labels written from elsewhere, BODY cells anywhere)
     bool body_box_pose = false;
     BodyParams box_pose{};
+    // labels written from the host: BODY cells anywhere, a queued bounding box stale
+    void labels_from_outside() {
+        body_box_pose = false;
+        mg.forget_bbox();
+    }
     hipEvent_t body_done = nullptr;
     MGPCG mg;
     std::unique_ptr<Preconditioner> jacobi;
@@ -547,6 +552,7 @@ struct Solver {
         int rc = ensure_keys();
         if (rc) return rc;
         LabelArgs la = label_args();
+        mg.forget_bbox();   // the labels change
         launch_scan_label(d_count, d_solid, d_labels, C, d_bsum, d_total, d_start, la, st);
         box_pose = la.body;
         body_box_pose = true;
@@ -800,6 +806,8 @@ struct Solver {
         hipEvent_t* e = ev + 8 * ev_set;
         HIPCHK(hipEventRecord(e[0], st));
         if ((rc = stage_sort_label())) return rc;
+        // the solve's bounding box of these labels, copied while P2G and RHS run
+        if (mg.queue_bbox(d_labels, st) != 0) return fail(FS_ERR_DEVICE, "bounding box launch failed");
         HIPCHK(hipEventRecord(e[1], st));
         if ((rc = choose_dt())) return rc;
         if ((rc = collect_times())) return rc;
@@ -889,11 +897,11 @@ int upload_scene(Solver& s, int W, int H, int D, const std::vector<uint8_t>& L) 
     std::vector<uint8_t> solid(L.size());
     for (size_t c = 0; c < L.size(); ++c) solid[c] = (L[c] == FS_SOLID);
     HIPCHK(hipMemcpy(s.d_labels, L.data(), L.size(), hipMemcpyHostToDevice));
-    s.body_box_pose = false;   // BODY cells not (only) from a label pass
+    s.labels_from_outside();
     HIPCHK(hipMemcpy(s.d_solid, solid.data(), L.size(), hipMemcpyHostToDevice));
     if (s.body.active) {
         launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
-        s.body_box_pose = false;   // BODY cells not (only) from a label pass
+        s.labels_from_outside();
         HIPCHK(hipStreamSynchronize(s.st));
     }
     return FS_OK;
@@ -1131,7 +1139,7 @@ int fs_set_body_sdf(void* h, const float* sdf, int w, int hh, int d, float ox, f
         HIPCHK(hipMemcpy(s.d_sdf, b.sdf.data(), sizeof(float) * b.sdf.size(), hipMemcpyHostToDevice));
         if (s.C > 0) {
             launch_mark_body(s.label_args(), s.d_labels, s.C, s.st);
-            s.body_box_pose = false;   // BODY cells not (only) from a label pass
+            s.labels_from_outside();
             HIPCHK(hipStreamSynchronize(s.st));
         }
         return FS_OK;
@@ -1435,7 +1443,7 @@ int fs_set_grid(void* h, int which, const void* in, int64_t count) {
         if (count != n) return fail(FS_ERR_ARG, "size mismatch");
         HIPCHK(hipStreamSynchronize(s.st));
         if (n) HIPCHK(hipMemcpy(dst, in, esz * n, hipMemcpyHostToDevice));
-        if (dst == s.d_labels) s.body_box_pose = false;   // BODY cells anywhere
+        if (dst == s.d_labels) s.labels_from_outside();
         return FS_OK;
     });
 }
@@ -1481,7 +1489,7 @@ int fs_solve_pressure_system(void* h, const uint8_t* labels, const float* b, dou
             if (rc) return rc;
         } else {
             HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
-            s.body_box_pose = false;   // BODY cells not (only) from a label pass
+            s.labels_from_outside();
         }
         HIPCHK(hipMemcpy(s.d_rhs, b, sizeof(float) * (size_t)W * H * D, hipMemcpyHostToDevice));
         double save = s.cfg.solver_tol;
@@ -1662,7 +1670,7 @@ int fs_mg_vcycle(void* h, const uint8_t* labels, const float* r, float* z, int W
             if (rc) return rc;
         } else {
             HIPCHK(hipMemcpy(s.d_labels, labels, (size_t)W * H * D, hipMemcpyHostToDevice));
-            s.body_box_pose = false;   // BODY cells not (only) from a label pass
+            s.labels_from_outside();
         }
         const size_t C = (size_t)W * H * D;
         HIPCHK(hipMemcpy(s.d_rhs, r, sizeof(float) * C, hipMemcpyHostToDevice));
@@ -1874,7 +1882,7 @@ int fs_load_checkpoint(void* h, const char* path) {
         if (rc) return rc;
         HIPCHK(hipMemcpy(s.d_solid, solid.data(), C, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(s.d_labels, labels.data(), C, hipMemcpyHostToDevice));
-        s.body_box_pose = false;   // BODY cells not (only) from a label pass
+        s.labels_from_outside();
         rc = s.alloc_particles(np);
         if (rc) return rc;
         float* dst[6] = {s.PA.px, s.PA.py, s.PA.pz, s.PA.vx, s.PA.vy, s.PA.vz};

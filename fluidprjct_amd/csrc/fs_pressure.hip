@@ -1962,7 +1962,10 @@ __global__ __launch_bounds__(256) void k_finalize(int mode, const double* __rest
 // ---------------------------------------------------------------- host side
 static inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
-MGPCG::~MGPCG() { free_all(); }
+MGPCG::~MGPCG() {
+    free_all();
+    if (bbox_ev) hipEventDestroy(bbox_ev);
+}
 
 void MGPCG::free_all() {
     clear_graphs();
@@ -2517,9 +2520,7 @@ void MGPCG::clear_graphs() {
     graphs.clear();
 }
 
-int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
-    // fluid bounding box, expanded by one cell so that every neighbour of a
-    // FLUID cell (AIR/SOLID/BODY or out of domain) is inside the solve box
+int MGPCG::launch_bbox(const uint8_t* labels, hipStream_t st) {
     const int nbb = (int)std::min<long long>(BBOX_BLOCKS, (N + 16 * 256 - 1) / (16 * 256));
     int* part = counters + COUNTERS_BBOX;
     if (W % 16 == 0)
@@ -2527,7 +2528,34 @@ int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
     else
         hipLaunchKernelGGL(k_bbox<false>, dim3(nbb), dim3(256), 0, st, labels, W, H, N, part);
     hipMemcpyAsync(host_counters + COUNTERS_BBOX, part, sizeof(int) * 6 * nbb, hipMemcpyDeviceToHost, st);
-    if (hipStreamSynchronize(st) != hipSuccess) return -5;
+    return nbb;
+}
+
+int MGPCG::queue_bbox(const uint8_t* labels, hipStream_t st) {
+    if (!owns_memory) return 0;
+    if (!bbox_ev && hipEventCreateWithFlags(&bbox_ev, hipEventDisableTiming) != hipSuccess) {
+        bbox_ev = nullptr;
+        return -5;
+    }
+    launch_bbox(labels, st);
+    if (hipEventRecord(bbox_ev, st) != hipSuccess) return -5;
+    bbox_labels = labels;
+    return 0;
+}
+
+int MGPCG::prepare(const uint8_t* labels, const float* b, hipStream_t st) {
+    // fluid bounding box, expanded by one cell so that every neighbour of a
+    // FLUID cell (AIR/SOLID/BODY or out of domain) is inside the solve box;
+    // queued earlier by queue_bbox (the host then waits for that copy only),
+    // else computed here
+    const int nbb = (int)std::min<long long>(BBOX_BLOCKS, (N + 16 * 256 - 1) / (16 * 256));
+    if (bbox_labels && bbox_labels == labels) {
+        if (hipEventSynchronize(bbox_ev) != hipSuccess) return -5;
+    } else {
+        launch_bbox(labels, st);
+        if (hipStreamSynchronize(st) != hipSuccess) return -5;
+    }
+    bbox_labels = nullptr;
     int hb[6] = {0x7fffffff, -1, 0x7fffffff, -1, 0x7fffffff, -1};
     for (int blk = 0; blk < nbb; ++blk)
         for (int q = 0; q < 3; ++q) {

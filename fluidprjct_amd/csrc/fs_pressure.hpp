@@ -169,6 +169,12 @@ public:
     int nred = 0;   // number of partial-sum slots used by reducing kernels
     int last_iters = 0, last_outer = 0;
     int zero_count = 0;           // host source of the iteration count's reset
+    // The fluid bounding box of `labels` queued ahead of prepare() (the
+    // host then waits for its copy only, while the work queued since runs);
+    // the caller clears bbox_labels (forget_bbox) when the labels change
+    // before the solve.
+    int queue_bbox(const uint8_t* labels, hipStream_t st);
+    void forget_bbox() { bbox_labels = nullptr; }
     double rate_per_iter = 0.0;   // log(||r||^2 reduction) per iteration of the last converged solve (< 0)
     bool use_graph = true;      // replay inner batches as HIP graphs
     bool use_tail = true;
@@ -198,6 +204,9 @@ private:
     bool owns_memory = true;   // buffers from hipMalloc (else carved from a caller's workspace)
     float* fw = nullptr;       // face weights of the solve box, 4 x N floats (allocated on first use)
     float* vcycle_level(int l, hipStream_t st, double* partial_dot, SolveScalars* sc);
+    int launch_bbox(const uint8_t* labels, hipStream_t st);
+    hipEvent_t bbox_ev = nullptr;
+    const uint8_t* bbox_labels = nullptr;   // labels of the queued bounding box (null: none)
 };
 
 Preconditioner* make_jacobi_preconditioner(MGPCG* s);
