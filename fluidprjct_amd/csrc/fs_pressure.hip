@@ -2645,7 +2645,7 @@ int MGPCG::solve(const uint8_t* labels, const float* b, double* x64, float* p, d
     // preconditioners (their fallback needs the host between batches).
     const bool spec = use_graph && !singular_mode && !pre->flexible() && last_iters > 0 && maxit > 0;
     if (spec) {
-        const int nbs = std::max(2, std::min(batch, 2 * ((last_iters + 1) / 2)));
+        const int nbs = std::max(2, std::min(2 * batch, 2 * ((last_iters + 1) / 2)));
         hipGraphExec_t ex = batch_graph(pre, nbs, st);
         if (!ex || hipGraphLaunch(ex, st) != hipSuccess) return -5;
     }
