@@ -35,7 +35,8 @@ namespace fsk {
 constexpr int RED_BLOCKS = 2048;      // partial-sum slots of every reducing kernel
 constexpr int COARSE_MAX = 4096;      // cells of the LDS-resident coarsest level
 constexpr long long FUSE_MAX = 1LL << 16;  // levels up to this size use the fused (recomputing) transfer kernels
-constexpr long long TB_MAX = 1LL << 18;    // coarse levels up to this size use the temporally blocked smoothers
+constexpr long long TB_MAX = 1LL << 18;
+constexpr long long RESTRICT_V2_MIN = 1LL << 18;   // coarse levels above this restrict two cells per thread    // coarse levels up to this size use the temporally blocked smoothers
 constexpr uint32_t SELF = 0x8000u;
 constexpr uint32_t WBIT = 0x4000u;   // level 0: the cell has a partly open face (weighted stencil)
 
@@ -2231,7 +2232,9 @@ float* MGPCG::vcycle_level(int l, hipStream_t st, double* pdot, SolveScalars* s)
                            (const SolveScalars*)s);
     else
         hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, L, L.b, cur, L.r, s);
-    if (v4)
+    // two coarse cells per thread when the coarse level is large; four lanes
+    // per coarse cell (more threads in flight) when it is small (same bits)
+    if (v4 && Cn.n > RESTRICT_V2_MIN)
         hipLaunchKernelGGL(k_restrict_v2, dim3(nblk(Cn.n / 2)), dim3(256), 0, st, L, Cn, (const float*)L.r, Cn.b,
                            (const SolveScalars*)s);
     else
